@@ -1,0 +1,204 @@
+"""Benchmark of the hot path on one MI355X.
+
+Workload: incoherent extend rays (8,388,608 rays, origins uniform in the scene box, directions uniform
+on the sphere, seed 0) against the 1024 x 1024-cell procedural heightfield (2,097,152 triangles).
+One step = one closest-hit launch over all rays, rays and BVH resident in HBM. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "incoherent Mrays/s (2M tris) + equal-time relMSE vs 4096-spp ref, plain vs cache"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, rays, seconds_target=12.0):
+    """The oracle's restatement of the reference CPU path (two-level BVH without tmax culling,
+    Moller-Trumbore, one task per chunk over all host threads), timed on a bounded sample."""
+    from oracle import refcpu
+
+    threads = min(os.cpu_count() or 1, 16)
+    o = refcpu.OracleScene(scene)
+    sample = rays[: 1 << 14]
+    t0 = time.perf_counter()
+    o.trace(sample, refcpu.TRACE_REF_MT, threads)
+    dt = time.perf_counter() - t0
+    # grow the sample to roughly seconds_target of CPU work
+    n = int(min(rays.shape[0], max(1 << 14, (1 << 14) * seconds_target / max(dt, 1e-3) / 2)))
+    sample = rays[:n]
+    runs = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        o.trace(sample, refcpu.TRACE_REF_MT, threads)
+        runs.append(time.perf_counter() - t0)
+    mt_rate = n / min(runs) / 1e6
+    t0 = time.perf_counter()
+    o.trace(sample[: n // 2], refcpu.TRACE_WT, threads)
+    wt_rate = (n // 2) / (time.perf_counter() - t0) / 1e6
+    return {"value": round(mt_rate, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of the 8,388,608 incoherent rays, reference Moller-Trumbore two-level BVH "
+                      f"without tmax culling, best of 2; watertight FP32 variant {wt_rate:.3f} Mrays/s",
+            "watertight_value": round(wt_rate, 3)}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "trace_incoherent_2m_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch"), d.get("source")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=8 * 1024 * 1024)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by image tiles or scenes, "
+              f"see the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import sparkle_amd as spk
+    from sparkle_amd import scenes
+
+    log("generating heightfield and rays")
+    sc = scenes.heightfield_scene(cells=args.cells)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=args.rays, seed=0)
+    n = rays.shape[0]
+
+    ctx = spk.Context(0)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+    log("building BVH and uploading")
+    st = ctx.upload_scene(sc)
+    d_rays = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()
+    d_hits = torch.empty(n * 8, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    tune = None
+    if not args.no_autotune:
+        log("autotuning traversal variants")
+        sub = min(n, 1 << 21)
+        cfg, ms = ctx.autotune(d_rays, sub, tune_width=True)
+        tune = {"bvh_width": cfg[0], "stack": cfg[1], "block": cfg[2]}
+        st = ctx.stats()
+
+    # traversal counts for the algorithmic-bytes model
+    cnt_nodes = torch.empty(n, dtype=torch.int32, device="cuda")
+    cnt_tris = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.trace_count(d_rays, cnt_nodes, cnt_tris, n)
+    mean_nodes = cnt_nodes.double().mean().item()
+    mean_tris = cnt_tris.double().mean().item()
+    bytes_per_ray = 32 + 8 + st.node_bytes * mean_nodes + st.triangle_bytes * mean_tris
+    bytes_per_launch = bytes_per_ray * n
+
+    for _ in range(args.warmup):
+        ctx.trace(d_rays, d_hits, n, sync=False)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        rc = ctx.trace(d_rays, d_hits, n, sync=False)
+        if rc != spk.SPK_OK:
+            raise RuntimeError(f"trace failed {rc}")
+        ends[k].record(stream)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_ms = float(np.mean(kernel_ms))
+    region_ms = starts[0].elapsed_time(ends[-1])
+    value = n * args.steps / (region_ms / 1e3) / 1e6
+
+    hits = d_hits.cpu().numpy().view(spk.HIT_DTYPE)
+    hit_frac = float((hits["prim"] != spk.SPK_MISS).mean())
+
+    # coherent 1080p primary rays on the same scene (reported beside the metric)
+    rays_a = scenes.coherent_rays()
+    d_rays_a = torch.from_numpy(rays_a.view(np.uint8).reshape(-1)).cuda()
+    d_hits_a = torch.empty(rays_a.shape[0] * 8, dtype=torch.uint8, device="cuda")
+    for _ in range(2):
+        ctx.trace(d_rays_a, d_hits_a, rays_a.shape[0], sync=False)
+    torch.cuda.synchronize()
+    ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ea.record(stream)
+    for _ in range(10):
+        ctx.trace(d_rays_a, d_hits_a, rays_a.shape[0], sync=False)
+    eb.record(stream)
+    torch.cuda.synchronize()
+    coherent_mrays = rays_a.shape[0] * 10 / (ea.elapsed_time(eb) / 1e3) / 1e6
+
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    traffic, traffic_src = load_traffic()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(region_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"incoherent extend rays: {n} rays (origins uniform in the scene box, directions uniform "
+                        f"on the sphere, seed 0) vs a {args.cells}x{args.cells}-cell value-noise heightfield "
+                        f"({st.num_triangles} triangles), closest hit with id and t",
+            "bvh": {"width": st.bvh_width, "nodes": st.num_nodes, "depth": st.depth, "node_bytes": st.node_bytes,
+                    "triangle_bytes": st.triangle_bytes, "sah": round(st.sah_cost, 3),
+                    "build_s": round(st.build_seconds, 3)},
+            "autotuned": tune,
+            "hit_fraction": round(hit_frac, 4),
+            "coherent_1080p_mrays": round(coherent_mrays, 1),
+            "mean_nodes_per_ray": round(mean_nodes, 3),
+            "mean_tris_per_ray": round(mean_tris, 3),
+            "wall_s": round(wall, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_ray": round(bytes_per_ray, 1),
+            "kernel_avg_ms": round(avg_ms, 4),
+            "traffic_source": traffic_src,
+        },
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(sc, rays)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,239 @@
+/*
+ * spk_pt.h — C ABI of the MI355X path-tracer hot path.
+ *
+ * A C++ host that keeps Sparkle's scene, material, camera and renderer semantics swaps its
+ * per-pixel sample loop for these calls. Nothing C++ crosses the boundary: plain structs,
+ * pointers and sizes. Every call returns SPK_OK (0) or a negative SPK_ERR_* code; none aborts
+ * or throws; spk_last_error() gives the message of the last failing call on a context.
+ * A context is used by one host thread at a time; separate contexts are independent.
+ *
+ * Reference interfaces replaced (paths relative to the Sparkle repository):
+ *   spk_scene_upload  SceneRenderProxy::UpdateBVH / TLAS::Build / BLAS::Build
+ *                     (libraries/source/renderer/proxy/SceneRenderProxy.cpp:269-297, 31-69;
+ *                      libraries/source/renderer/proxy/MeshRenderProxy.cpp:31-65, 310-315)
+ *   spk_trace_closest SceneRenderProxy::Intersect<false> (SceneRenderProxy.h:119,
+ *                     SceneRenderProxy.cpp:299-309) down to MeshRenderProxy::IntersectTriangle
+ *                     (MeshRenderProxy.cpp:128-194)
+ *   spk_set_camera    CameraRenderProxy posture + focus plane (CameraRenderProxy.h:18-57,
+ *                     CameraRenderProxy.cpp:113-136)
+ *   spk_render_frame  CPURenderer::BasePass -> RenderPixel -> SamplePixel
+ *                     (libraries/include/renderer/renderer/CPURenderer.h:37,
+ *                      libraries/source/renderer/renderer/CPURenderer.cpp:240-450)
+ *   spk_accumulate    CPURenderer::DenoisePass, temporal part (CPURenderer.cpp:506-552)
+ *   spk_tonemap       CPURenderer::ToneMappingPass (CPURenderer.cpp:554-576)
+ * Materials mirror MaterialRenderProxy::MaterialRenderData (MaterialRenderProxy.h:39-58);
+ * the camera mirrors CameraRenderProxy::UniformBufferData (CameraRenderProxy.h:18-28).
+ *
+ * Device pointers: arguments named *_dev are HIP device pointers on the context's device.
+ * Everything else is host memory, copied by the call.
+ */
+#ifndef SPK_PT_H
+#define SPK_PT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPK_OK 0
+#define SPK_ERR_INVALID_ARGUMENT (-1)
+#define SPK_ERR_HIP (-2)
+#define SPK_ERR_OUT_OF_MEMORY (-3)
+#define SPK_ERR_NO_SCENE (-4)
+#define SPK_ERR_INVALID_SCENE (-5)
+#define SPK_ERR_STACK_DEPTH (-6)
+#define SPK_ERR_INVALID_RAY (-7) /* some rays were NaN/inf/zero-length; they were reported as misses */
+#define SPK_ERR_NO_DEVICE (-8)
+#define SPK_ERR_UNSUPPORTED (-9)
+
+#define SPK_MISS 0xFFFFFFFFu
+
+typedef struct spk_context spk_context;
+
+/* 32-byte ray record: origin, tmin, direction (need not be normalised), tmax. A hit needs
+   tmin < t and t <= tmax. */
+typedef struct spk_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} spk_ray;
+
+/* 8-byte hit record: global triangle id (SPK_MISS when nothing is hit) and distance t. */
+typedef struct spk_hit {
+    uint32_t prim;
+    float t;
+} spk_hit;
+
+/* One triangle mesh. Global triangle ids number the triangles of every instance in the order
+   the instances are given: id = (triangles of earlier instances) + face index. */
+typedef struct spk_mesh_desc {
+    const float *positions; /* 3 floats per vertex */
+    uint32_t num_vertices;
+    const uint32_t *indices; /* 3 per triangle */
+    uint32_t num_triangles;
+    const float *normals;  /* 3 per vertex, or NULL */
+    const float *tangents; /* 4 per vertex (xyz, handedness), or NULL -> (1,0,0,1) */
+    const float *uvs;      /* 2 per vertex, or NULL */
+} spk_mesh_desc;
+
+/* Instance of a mesh with a material. world_from_local is row-major 3x4. Instances are baked
+   into world space at upload (the hot path works on flattened geometry). */
+typedef struct spk_instance_desc {
+    uint32_t mesh;
+    uint32_t material;
+    float world_from_local[12];
+} spk_instance_desc;
+
+#define SPK_MATERIAL_PBR 0u
+#define SPK_MATERIAL_DIELECTRIC 1u
+
+/* Field set of MaterialRenderData; textures are not part of the hot path. */
+typedef struct spk_material_desc {
+    uint32_t type; /* SPK_MATERIAL_* */
+    float base_color[3];
+    float emissive[3];
+    float metallic;
+    float roughness;
+    float eta;
+} spk_material_desc;
+
+typedef struct spk_build_options {
+    uint32_t bvh_width;     /* 8 (default) or 4 */
+    uint32_t max_leaf_tris; /* triangles per leaf slot, 1..3 (default 3) */
+    uint32_t num_threads;   /* host build threads, 0 = all */
+} spk_build_options;
+
+typedef struct spk_scene_stats {
+    uint64_t num_triangles;
+    uint64_t num_nodes;
+    uint32_t bvh_width;
+    uint32_t depth;           /* deepest node level (root = 1) */
+    uint64_t node_bytes;      /* bytes per node record */
+    uint64_t triangle_bytes;  /* bytes per triangle record */
+    uint64_t device_bytes;    /* scene bytes resident on the device */
+    uint64_t degenerate_tris; /* zero-area triangles (never hit) */
+    double sah_cost;          /* SAH cost of the wide tree (node traversal = 1, triangle = 1) */
+    double build_seconds;
+} spk_scene_stats;
+
+/* Camera exactly as SetupViewRay consumes it (CPURenderer.cpp:220-238). */
+typedef struct spk_camera_desc {
+    float position[3];
+    float right[3];
+    float up[3];
+    float front[3];
+    float lower_left[3];
+    float max_u[3];
+    float max_v[3];
+    float lens_radius;
+    float far_plane;
+} spk_camera_desc;
+
+#define SPK_DEBUG_COLOR 0u
+#define SPK_DEBUG_NORMAL 3u
+#define SPK_DEBUG_RAY_DIRECTION 4u
+#define SPK_DEBUG_ALBEDO 9u
+#define SPK_DEBUG_DEPTH 11u
+
+#define SPK_FRAME_SEED64 1u /* compose the pixel seed in 64 bits (no uint32 wrap) */
+#define SPK_FRAME_NEE 2u    /* next-event estimation on emissive triangles (not in the reference) */
+
+typedef struct spk_frame_desc {
+    uint32_t width;
+    uint32_t height;
+    uint32_t frame_seed; /* dispatched sample count, CPURenderer.cpp:432 */
+    uint32_t max_bounce;
+    uint32_t debug_mode;
+    uint32_t sky_enabled;
+    float sky_color[3];
+    uint32_t flags; /* SPK_FRAME_* */
+} spk_frame_desc;
+
+typedef struct spk_frame_stats {
+    uint64_t rays_traced; /* extend rays (one per path segment) */
+    uint64_t shadow_rays;
+    double milliseconds;  /* device time of the frame */
+} spk_frame_stats;
+
+/* ---- context ------------------------------------------------------------------------------ */
+int spk_create(int device, spk_context **out);
+int spk_destroy(spk_context *ctx);
+const char *spk_last_error(const spk_context *ctx);
+/* Run all work on this HIP stream (hipStream_t); NULL restores the context's own stream. */
+int spk_set_stream(spk_context *ctx, void *hip_stream);
+void *spk_get_stream(spk_context *ctx);
+int spk_synchronize(spk_context *ctx);
+int spk_device_alloc(spk_context *ctx, size_t bytes, void **out_dev);
+int spk_device_free(spk_context *ctx, void *dev);
+int spk_copy_to_device(spk_context *ctx, void *dst_dev, const void *src, size_t bytes);
+int spk_copy_to_host(spk_context *ctx, void *dst, const void *src_dev, size_t bytes);
+
+/* ---- scene -------------------------------------------------------------------------------- */
+int spk_scene_upload(spk_context *ctx, const spk_mesh_desc *meshes, uint32_t num_meshes,
+                     const spk_instance_desc *instances, uint32_t num_instances,
+                     const spk_material_desc *materials, uint32_t num_materials,
+                     const spk_build_options *options);
+int spk_scene_stats_get(spk_context *ctx, spk_scene_stats *out);
+
+/* Host-only check of the BVH builder (no device needed): flattens and builds exactly as
+   spk_scene_upload does, verifies the traversal invariants (every triangle referenced once and
+   inside each dequantised ancestor box, indices in range, depth as reported) and fills stats.
+   Returns SPK_OK or SPK_ERR_INVALID_SCENE; errmsg (may be NULL) receives the reason. */
+int spk_build_check(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_instance_desc *instances,
+                    uint32_t num_instances, const spk_build_options *options, spk_scene_stats *stats,
+                    char *errmsg, size_t errmsg_size);
+
+/* ---- closest hit -------------------------------------------------------------------------- */
+/* Closest hit of n rays (device arrays). Blocks until done. Exact-t ties go to the smaller id. */
+int spk_trace_closest(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev);
+/* Same, enqueued on the context stream without waiting; invalid-ray counts are reported by the
+   next blocking call. */
+int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev);
+/* Host-array convenience wrapper (copies in and out). */
+int spk_trace_closest_host(spk_context *ctx, const spk_ray *rays, uint64_t n, spk_hit *hits);
+/* Traversal statistics for n rays: per ray node visits and triangle tests (device arrays of
+   uint32, either may be NULL). */
+int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t *nodes_dev,
+                    uint32_t *tris_dev);
+
+/* Traversal kernel configuration: BVH width (4 or 8), LDS stack entries per lane, threads per
+   block. The stack must hold the tree depth (spk_scene_stats.depth). */
+typedef struct spk_trace_config {
+    uint32_t bvh_width;
+    uint32_t stack;
+    uint32_t block;
+} spk_trace_config;
+
+#define SPK_TUNE_WIDTH 1u /* also try the other BVH width (rebuilds the tree on the host) */
+
+int spk_get_trace_config(spk_context *ctx, spk_trace_config *out);
+/* Selects a compiled variant; a different bvh_width rebuilds and re-uploads the BVH. */
+int spk_set_trace_config(spk_context *ctx, const spk_trace_config *config);
+/* Lists the compiled variants valid for the current scene depth (up to max_out). */
+int spk_list_trace_configs(spk_context *ctx, spk_trace_config *out, uint32_t max_out, uint32_t *count);
+/* Autotuner: times every valid variant on the given rays (device array) and keeps the fastest.
+   best_ms receives the fastest time per launch. */
+int spk_autotune(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t flags, spk_trace_config *chosen,
+                 double *best_ms);
+
+/* ---- path tracing ------------------------------------------------------------------------- */
+int spk_set_camera(spk_context *ctx, const spk_camera_desc *camera);
+/* One BasePass: one path per pixel; rgba = clamped colour and valid flag (+1/-1); normal =
+   first-hit world normal (may be NULL). Device arrays of width*height*4 / *3 floats. */
+int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, float *normal_dev,
+                     spk_frame_stats *stats);
+/* accum = rgba + (accum - rgba) * cumulated / (cumulated + spp), on the device. */
+int spk_accumulate(spk_context *ctx, const float *rgba_dev, float *accum_dev, uint64_t num_pixels,
+                   uint32_t cumulated, uint32_t spp);
+/* ACES film curve with exposure, vertically flipped, to RGBA float (CPURenderer.cpp:554-576). */
+int spk_tonemap(spk_context *ctx, const float *accum_dev, float *out_dev, uint32_t width, uint32_t height,
+                float exposure);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPK_PT_H */

@@ -1,0 +1,1385 @@
+// refcpu — CPU restatement of Sparkle's `cpu` pipeline hot path (test infrastructure only).
+// Built with -ffp-contract=off and without fast-math so every float operation rounds
+// exactly once, in the order written; see refcpu.h for the list of restated files.
+#include "refcpu.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <thread>
+#include <vector>
+
+namespace orcl {
+
+// ---------------------------------------------------------------------------------------------
+// math: a float3 whose reductions run ((x + y) + z), matching a non-vectorised Eigen Vector3
+// (libraries/include/core/math/Types.h:12-43)
+// ---------------------------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    explicit V3(const float *p) : x(p[0]), y(p[1]), z(p[2]) {}
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+static inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+static inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+static inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+static inline V3 cmul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+static inline float sqnorm(V3 a) { return dot(a, a); }
+static inline float norm(V3 a) { return std::sqrt(sqnorm(a)); }
+// Eigen normalized(): divides by the norm, returns the input when the norm is zero
+static inline V3 normalized(V3 a) {
+    float n = sqnorm(a);
+    if (n > 0.f) {
+        float s = std::sqrt(n);
+        return {a.x / s, a.y / s, a.z / s};
+    }
+    return a;
+}
+static inline float maxc(V3 a) { return std::max(std::max(a.x, a.y), a.z); }
+
+static const V3 kUp(0, 0, 1), kRight(1, 0, 0), kOnes(1, 1, 1), kZeros(0, 0, 0);
+static const float kPi = 3.14159265358979323846f;  // std::numbers::pi_v<float>
+static const float kEps = 1e-6f;                    // Types.h:54
+static const float kTolerance = 1e-4f;              // Types.h:55
+static const float kOutputLimit = 6.f;              // CameraRenderProxy.h:16
+
+// row-major 3x4 affine transform
+struct Xf {
+    float m[12];
+    V3 point(V3 p) const {
+        return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+                m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+    }
+    V3 dir(V3 d) const {
+        return {m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+                m[8] * d.x + m[9] * d.y + m[10] * d.z};
+    }
+    bool identity() const {
+        static const float I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        return std::memcmp(m, I, sizeof(I)) == 0;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// RNG: xoshiro128+ 1.0 (Blackman & Vigna), state seeded by SplitMix64 (Sampler.h:16-41).
+// Xoshiro-cpp seeds each 32-bit state word from one SplitMix64 output; we take the low 32 bits
+// of four consecutive outputs (documented choice; the exact expansion is unpinned here).
+// ---------------------------------------------------------------------------------------------
+struct Rng {
+    uint32_t s[4];
+    explicit Rng(uint64_t seed) {
+        uint64_t x = seed;
+        for (int k = 0; k < 4; ++k) {
+            x += 0x9e3779b97f4a7c15ULL;
+            uint64_t z = x;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            z = z ^ (z >> 31);
+            s[k] = static_cast<uint32_t>(z);
+        }
+    }
+    static inline uint32_t rotl(uint32_t v, int k) { return (v << k) | (v >> (32 - k)); }
+    uint32_t next() {
+        const uint32_t result = s[0] + s[3];
+        const uint32_t t = s[1] << 9;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 11);
+        return result;
+    }
+    // RandomUnit(): float(u32) / float(UINT32_MAX); the denominator rounds to 2^32 (Sampler.h:40)
+    float unit() { return static_cast<float>(next()) / static_cast<float>(std::numeric_limits<uint32_t>::max()); }
+    // UnitDisk() (Sampler.h:43-53)
+    void disk(float &x, float &y) {
+        float rand = unit();
+        float theta = unit() * 2.0f * kPi;
+        float r = std::sqrt(rand);
+        x = r * std::cos(theta);
+        y = r * std::sin(theta);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// triangle tests
+// ---------------------------------------------------------------------------------------------
+// Moller-Trumbore as in MeshRenderProxy.cpp:128-170 (local-space t, before the world_t step)
+static inline bool tri_mt(V3 o, V3 d, V3 p0, V3 e1, V3 e2, float &t, float &u, float &v) {
+    const V3 h = cross(d, e2);
+    float a = dot(e1, h);
+    if (std::abs(a) < kEps) return false;
+    const V3 s = o - p0;
+    float f = 1.f / a;
+    u = f * dot(s, h);
+    if (u < 0.f || u > 1.f) return false;
+    const V3 q = cross(s, e1);
+    v = f * dot(d, q);
+    if (v < 0.f || u + v > 1.f) return false;
+    t = f * dot(e2, q);
+    return t > kEps;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), FP32, no FMA, no FP64 fallback:
+// an edge value of exactly zero counts as inside, so a ray through a shared edge hits both
+// triangles and the closest-hit tie rule (smaller t, then smaller id) decides.
+struct WtRay {
+    V3 o;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+static inline WtRay wt_setup(V3 o, V3 d) {
+    WtRay r;
+    r.o = o;
+    float ax = std::abs(d.x), ay = std::abs(d.y), az = std::abs(d.z);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > std::max(ax, ay)) kz = 2;
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    if (d[kz] < 0.f) std::swap(kx, ky);
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.Sx = d[kx] / d[kz];
+    r.Sy = d[ky] / d[kz];
+    r.Sz = 1.0f / d[kz];
+    return r;
+}
+static inline bool tri_wt(const WtRay &r, V3 v0, V3 v1, V3 v2, float &t, float &U_, float &V_, float &W_,
+                          float &det_) {
+    const V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    const float Ax = A[r.kx] - r.Sx * A[r.kz];
+    const float Ay = A[r.ky] - r.Sy * A[r.kz];
+    const float Bx = B[r.kx] - r.Sx * B[r.kz];
+    const float By = B[r.ky] - r.Sy * B[r.kz];
+    const float Cx = C[r.kx] - r.Sx * C[r.kz];
+    const float Cy = C[r.ky] - r.Sy * C[r.kz];
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    const float det = U + V + W;
+    if (det == 0.f) return false;
+    const float Az = r.Sz * A[r.kz];
+    const float Bz = r.Sz * B[r.kz];
+    const float Cz = r.Sz * C[r.kz];
+    const float T = U * Az + V * Bz + W * Cz;
+    if ((det < 0.f && T > 0.f) || (det > 0.f && T < 0.f)) return false;
+    t = T / det;
+    U_ = U;
+    V_ = V;
+    W_ = W;
+    det_ = det;
+    return true;
+}
+
+// Hit distance of the selected triangle, recomputed as n.(v0 - o) / n.d with v0 - o split into
+// an exact hi+lo pair (TwoDiff) and both dot products compensated (Dot2, Ogita-Rump-Oishi 2005).
+static inline void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+static inline void two_prod(float a, float b, float &p, float &e) {
+    p = a * b;
+    e = std::fma(a, b, -p);
+}
+static inline float dot2(V3 a, V3 b, V3 blo) {
+    float p, s, h, r, q;
+    two_prod(a.x, b.x, p, s);
+    two_prod(a.y, b.y, h, r);
+    two_sum(p, h, p, q);
+    s = s + (q + r);
+    two_prod(a.z, b.z, h, r);
+    two_sum(p, h, p, q);
+    s = s + (q + r);
+    s = s + ((a.x * blo.x + a.y * blo.y) + a.z * blo.z);
+    return p + s;
+}
+static inline float refine_t(V3 o, V3 d, V3 v0, V3 v1, V3 v2, float t_fallback) {
+    const V3 e1 = v1 - v0, e2 = v2 - v0;
+    const V3 n = cross(e1, e2);
+    float hx, lx, hy, ly, hz, lz;
+    two_sum(v0.x, -o.x, hx, lx);
+    two_sum(v0.y, -o.y, hy, ly);
+    two_sum(v0.z, -o.z, hz, lz);
+    const float num = dot2(n, V3(hx, hy, hz), V3(lx, ly, lz));
+    const float den = dot2(n, d, V3(0.f, 0.f, 0.f));
+    if (den == 0.f) return t_fallback;
+    const float t = num / den;
+    if (!(t > 0.f) || !std::isfinite(t)) return t_fallback;
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// binary SAH BVH (binned), used for both the TLAS over instances and one BLAS per mesh.
+// The reference uses madmann91/bvh v2 DefaultBuilder<High>; tree shape does not change closest
+// hits under the traversal below (no tmax culling), only visit order for exact-t ties.
+// ---------------------------------------------------------------------------------------------
+struct Box {
+    float lo[3], hi[3];
+    Box() {
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::numeric_limits<float>::max();
+            hi[i] = -std::numeric_limits<float>::max();
+        }
+    }
+    void grow(const Box &b) {
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::min(lo[i], b.lo[i]);
+            hi[i] = std::max(hi[i], b.hi[i]);
+        }
+    }
+    void grow(V3 p) {
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::min(lo[i], p[i]);
+            hi[i] = std::max(hi[i], p[i]);
+        }
+    }
+    double area() const {
+        double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+        if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct BNode {
+    Box box;
+    uint32_t first;  // inner: index of left child (right = first + 1); leaf: first prim slot
+    uint32_t count;  // 0 = inner
+};
+
+struct Bvh {
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> prim_ids;
+
+    void build(const std::vector<Box> &boxes, uint32_t max_leaf) {
+        const uint32_t n = static_cast<uint32_t>(boxes.size());
+        prim_ids.resize(n);
+        for (uint32_t i = 0; i < n; ++i) prim_ids[i] = i;
+        nodes.clear();
+        if (n == 0) return;
+        std::vector<float> cen(3 * (size_t)n);
+        for (uint32_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) cen[3 * (size_t)i + a] = 0.5f * (boxes[i].lo[a] + boxes[i].hi[a]);
+        nodes.reserve(2 * (size_t)n);
+        nodes.push_back(BNode());
+        struct Task {
+            uint32_t node, begin, end;
+        };
+        std::vector<Task> stack;
+        stack.push_back({0, 0, n});
+        const int kBins = 16;
+        while (!stack.empty()) {
+            Task tk = stack.back();
+            stack.pop_back();
+            Box nb, cb;
+            for (uint32_t i = tk.begin; i < tk.end; ++i) {
+                nb.grow(boxes[prim_ids[i]]);
+                cb.grow(V3(&cen[3 * (size_t)prim_ids[i]]));
+            }
+            BNode &node = nodes[tk.node];
+            node.box = nb;
+            const uint32_t cnt = tk.end - tk.begin;
+            auto make_leaf = [&]() {
+                nodes[tk.node].first = tk.begin;
+                nodes[tk.node].count = cnt;
+            };
+            if (cnt <= 1) {
+                make_leaf();
+                continue;
+            }
+            int best_axis = -1, best_split = -1;
+            double best_cost = std::numeric_limits<double>::max();
+            for (int a = 0; a < 3; ++a) {
+                float ext = cb.hi[a] - cb.lo[a];
+                if (!(ext > 0.f)) continue;
+                Box bb[kBins];
+                uint32_t bc[kBins] = {0};
+                for (uint32_t i = tk.begin; i < tk.end; ++i) {
+                    uint32_t p = prim_ids[i];
+                    int b = (int)((cen[3 * (size_t)p + a] - cb.lo[a]) / ext * kBins);
+                    b = std::min(std::max(b, 0), kBins - 1);
+                    bb[b].grow(boxes[p]);
+                    bc[b]++;
+                }
+                double rarea[kBins];
+                uint32_t rcnt[kBins];
+                Box acc;
+                uint32_t c = 0;
+                for (int b = kBins - 1; b > 0; --b) {
+                    acc.grow(bb[b]);
+                    c += bc[b];
+                    rarea[b] = acc.area();
+                    rcnt[b] = c;
+                }
+                Box lacc;
+                uint32_t lc = 0;
+                for (int b = 0; b < kBins - 1; ++b) {
+                    lacc.grow(bb[b]);
+                    lc += bc[b];
+                    if (lc == 0 || rcnt[b + 1] == 0) continue;
+                    double cost = lacc.area() * lc + rarea[b + 1] * rcnt[b + 1];
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_axis = a;
+                        best_split = b;
+                    }
+                }
+            }
+            double leaf_cost = nb.area() * cnt;
+            uint32_t mid;
+            if (best_axis < 0) {
+                if (cnt <= max_leaf) {
+                    make_leaf();
+                    continue;
+                }
+                mid = tk.begin + cnt / 2;  // all centroids equal: split in the middle
+            } else {
+                if (cnt <= max_leaf && best_cost * 1.0 + nb.area() >= leaf_cost) {
+                    make_leaf();
+                    continue;
+                }
+                float ext = cb.hi[best_axis] - cb.lo[best_axis];
+                auto it = std::partition(prim_ids.begin() + tk.begin, prim_ids.begin() + tk.end, [&](uint32_t p) {
+                    int b = (int)((cen[3 * (size_t)p + best_axis] - cb.lo[best_axis]) / ext * kBins);
+                    b = std::min(std::max(b, 0), kBins - 1);
+                    return b <= best_split;
+                });
+                mid = static_cast<uint32_t>(it - prim_ids.begin());
+                if (mid == tk.begin || mid == tk.end) mid = tk.begin + cnt / 2;
+            }
+            uint32_t left = static_cast<uint32_t>(nodes.size());
+            nodes.push_back(BNode());
+            nodes.push_back(BNode());
+            nodes[tk.node].first = left;
+            nodes[tk.node].count = 0;
+            stack.push_back({left + 1, mid, tk.end});
+            stack.push_back({left, tk.begin, mid});
+        }
+    }
+};
+
+// slab test of bvh v2's non-robust mode: tmin=0, tmax=+inf, never shrunk
+struct SlabRay {
+    float o[3], inv[3];
+};
+static inline SlabRay slab_setup(V3 o, V3 d) {
+    SlabRay r;
+    for (int a = 0; a < 3; ++a) {
+        r.o[a] = o[a];
+        float da = d[a];
+        // bvh v2 replaces a zero direction component by a tiny value before inverting
+        if (da == 0.f) da = std::numeric_limits<float>::epsilon();
+        r.inv[a] = 1.f / da;
+    }
+    return r;
+}
+static inline bool slab_hit(const SlabRay &r, const Box &b, float tmax, float &tnear) {
+    float t0 = 0.f, t1 = tmax;
+    for (int a = 0; a < 3; ++a) {
+        float ta = (b.lo[a] - r.o[a]) * r.inv[a];
+        float tb = (b.hi[a] - r.o[a]) * r.inv[a];
+        if (ta > tb) std::swap(ta, tb);
+        t0 = std::max(t0, ta);
+        t1 = std::min(t1, tb);
+    }
+    tnear = t0;
+    return t0 <= t1;
+}
+// conservative box test in double with a relative widening, used by the checker modes so that
+// box rounding can never drop a triangle the test below would accept
+static inline bool slab_hit_robust(V3 o, V3 d, const Box &b, double tmax) {
+    double t0 = 0.0, t1 = tmax;
+    for (int a = 0; a < 3; ++a) {
+        double ext = std::max(1e-30, ((double)b.hi[a] - (double)b.lo[a]) * 1e-6);
+        double lo = (double)b.lo[a] - ext, hi = (double)b.hi[a] + ext;
+        double da = d[a], oa = o[a];
+        if (da == 0.0) {
+            if (oa < lo || oa > hi) return false;
+            continue;
+        }
+        double ta = (lo - oa) / da, tb = (hi - oa) / da;
+        if (ta > tb) std::swap(ta, tb);
+        t0 = std::max(t0, ta);
+        t1 = std::min(t1, tb);
+    }
+    return t0 <= t1 * (1.0 + 1e-9);
+}
+
+// ---------------------------------------------------------------------------------------------
+// scene: one BLAS per mesh over local-space triangles, TLAS over instance world boxes
+// (MeshRenderProxy.cpp:31-65, SceneRenderProxy.cpp:31-69)
+// ---------------------------------------------------------------------------------------------
+struct Mesh {
+    std::vector<float> pos, nrm, tan, uv;
+    std::vector<uint32_t> idx;
+    uint32_t ntri = 0;
+    // MeshRenderProxy::BLAS::Triangle {p0, e1, e2, n} (MeshRenderProxy.cpp:78-92)
+    std::vector<V3> p0, e1, e2;
+    Bvh bvh;
+    Box local_box;
+    V3 vtx(uint32_t i) const { return V3(&pos[3 * (size_t)i]); }
+    void tri(uint32_t f, V3 &a, V3 &b, V3 &c) const {
+        a = vtx(idx[3 * (size_t)f]);
+        b = vtx(idx[3 * (size_t)f + 1]);
+        c = vtx(idx[3 * (size_t)f + 2]);
+    }
+};
+
+struct Instance {
+    uint32_t mesh, material;
+    Xf to_world, to_local;
+    uint64_t prim_offset;
+};
+
+}  // namespace orcl
+
+struct orcl_scene {
+    std::vector<orcl::Mesh> meshes;
+    std::vector<orcl::Instance> instances;  // in TLAS leaf order
+    std::vector<orcl_material> materials;
+    orcl::Bvh tlas;
+    uint64_t total_tris = 0;
+};
+
+namespace orcl {
+
+struct Candidate {
+    float t = std::numeric_limits<float>::max();  // IntersectionCandidate (Intersection.h:91-104)
+    float u = 0, v = 0;
+    uint32_t face = 0;
+    int inst = -1;
+    uint64_t gid = ~0ull;
+};
+
+// BLAS traversal + MT + world_t (MeshRenderProxy.cpp:94-194), shared candidate across meshes
+static void blas_intersect_ref(const orcl_scene &sc, int ii, V3 wo, V3 wd, Candidate &cand) {
+    const Instance &ins = sc.instances[ii];
+    const Mesh &m = sc.meshes[ins.mesh];
+    V3 lo = ins.to_local.point(wo), ld = ins.to_local.dir(wd);
+    SlabRay sr = slab_setup(lo, ld);
+    if (m.bvh.nodes.empty()) return;
+    std::vector<uint32_t> stack;
+    stack.reserve(64);
+    // bvh v2 pushes the root and tests children boxes only (ordered traversal)
+    uint32_t top = 0;
+    stack.push_back(top);
+    while (!stack.empty()) {
+        uint32_t ni = stack.back();
+        stack.pop_back();
+        bool leaf_reached = true;
+        while (m.bvh.nodes[ni].count == 0) {
+            const BNode &L = m.bvh.nodes[m.bvh.nodes[ni].first];
+            const BNode &R = m.bvh.nodes[m.bvh.nodes[ni].first + 1];
+            float tl, tr;
+            bool hl = slab_hit(sr, L.box, std::numeric_limits<float>::infinity(), tl);
+            bool hr = slab_hit(sr, R.box, std::numeric_limits<float>::infinity(), tr);
+            uint32_t li = m.bvh.nodes[ni].first, ri = li + 1;
+            if (hl) {
+                uint32_t nearI = li;
+                if (hr) {
+                    uint32_t farI = ri;
+                    if (tl > tr) std::swap(nearI, farI);
+                    stack.push_back(farI);
+                }
+                ni = nearI;
+            } else if (hr) {
+                ni = ri;
+            } else {
+                leaf_reached = false;
+                break;
+            }
+        }
+        if (!leaf_reached) continue;
+        const BNode &leaf = m.bvh.nodes[ni];
+        for (uint32_t k = leaf.first; k < leaf.first + leaf.count; ++k) {
+            uint32_t f = m.bvh.prim_ids[k];
+            float t, u, v;
+            if (!tri_mt(lo, ld, m.p0[f], m.e1[f], m.e2[f], t, u, v)) continue;
+            // world_p = transform(local.At(t)); world_t = InverseAt(world_p) (Ray.h:43-47)
+            V3 lp = lo + ld * t;
+            V3 wp = ins.to_world.point(lp);
+            float world_t = (wp - wo).x / wd.x;
+            if (world_t > 0 && world_t < cand.t) {
+                cand.t = world_t;
+                cand.u = u;
+                cand.v = v;
+                cand.face = f;
+                cand.inst = ii;
+            }
+        }
+    }
+}
+
+// TLAS traversal (SceneRenderProxy.cpp:82-126)
+static void scene_intersect_ref(const orcl_scene &sc, V3 o, V3 d, Candidate &cand) {
+    if (sc.tlas.nodes.empty()) return;
+    SlabRay sr = slab_setup(o, d);
+    std::vector<uint32_t> stack;
+    stack.push_back(0);
+    while (!stack.empty()) {
+        uint32_t ni = stack.back();
+        stack.pop_back();
+        bool leaf_reached = true;
+        while (sc.tlas.nodes[ni].count == 0) {
+            uint32_t li = sc.tlas.nodes[ni].first, ri = li + 1;
+            float tl, tr;
+            bool hl = slab_hit(sr, sc.tlas.nodes[li].box, std::numeric_limits<float>::infinity(), tl);
+            bool hr = slab_hit(sr, sc.tlas.nodes[ri].box, std::numeric_limits<float>::infinity(), tr);
+            if (hl) {
+                uint32_t nearI = li;
+                if (hr) {
+                    uint32_t farI = ri;
+                    if (tl > tr) std::swap(nearI, farI);
+                    stack.push_back(farI);
+                }
+                ni = nearI;
+            } else if (hr) {
+                ni = ri;
+            } else {
+                leaf_reached = false;
+                break;
+            }
+        }
+        if (!leaf_reached) continue;
+        const BNode &leaf = sc.tlas.nodes[ni];
+        for (uint32_t k = leaf.first; k < leaf.first + leaf.count; ++k) blas_intersect_ref(sc, (int)k, o, d, cand);
+    }
+    if (cand.inst >= 0) cand.gid = sc.instances[cand.inst].prim_offset + cand.face;
+}
+
+// watertight closest hit, order-independent: smaller t wins, exact ties go to the smaller id
+struct WtBest {
+    float t;
+    uint64_t gid = ~0ull;
+    int inst = -1;
+    uint32_t face = 0;
+    float U = 0, V = 0, W = 0, det = 1;
+};
+static inline void wt_consider(WtBest &b, float t, float tmin, uint64_t gid, int inst, uint32_t face, float U,
+                               float V, float W, float det) {
+    if (!(t > tmin)) return;
+    if (t < b.t || (t == b.t && gid < b.gid)) {
+        b.t = t;
+        b.gid = gid;
+        b.inst = inst;
+        b.face = face;
+        b.U = U;
+        b.V = V;
+        b.W = W;
+        b.det = det;
+    }
+}
+
+static void blas_intersect_wt(const orcl_scene &sc, int ii, V3 wo, V3 wd, float tmin, WtBest &best, bool brute) {
+    const Instance &ins = sc.instances[ii];
+    const Mesh &m = sc.meshes[ins.mesh];
+    const bool ident = ins.to_local.identity();
+    V3 lo = ident ? wo : ins.to_local.point(wo), ld = ident ? wd : ins.to_local.dir(wd);
+    WtRay wr = wt_setup(lo, ld);
+    auto test = [&](uint32_t f) {
+        V3 a, b, c;
+        m.tri(f, a, b, c);
+        float t, U, V, W, det;
+        if (tri_wt(wr, a, b, c, t, U, V, W, det))
+            wt_consider(best, t, tmin, ins.prim_offset + f, ii, f, U, V, W, det);
+    };
+    if (brute) {
+        for (uint32_t f = 0; f < m.ntri; ++f) test(f);
+        return;
+    }
+    if (m.bvh.nodes.empty()) return;
+    std::vector<uint32_t> stack;
+    stack.push_back(0);
+    while (!stack.empty()) {
+        uint32_t ni = stack.back();
+        stack.pop_back();
+        const BNode &nd = m.bvh.nodes[ni];
+        double tmax = best.gid == ~0ull ? std::numeric_limits<double>::infinity() : (double)best.t * (1.0 + 1e-5);
+        if (!slab_hit_robust(lo, ld, nd.box, tmax)) continue;
+        if (nd.count == 0) {
+            stack.push_back(nd.first + 1);
+            stack.push_back(nd.first);
+        } else {
+            for (uint32_t k = nd.first; k < nd.first + nd.count; ++k) test(m.bvh.prim_ids[k]);
+        }
+    }
+}
+
+static void scene_intersect_wt(const orcl_scene &sc, V3 o, V3 d, float tmin, float tmax, WtBest &best, bool brute) {
+    best.t = tmax;
+    // every instance is visited; each BLAS walk culls robustly against the best hit so far
+    for (size_t ii = 0; ii < sc.instances.size(); ++ii) blas_intersect_wt(sc, (int)ii, o, d, tmin, best, brute);
+}
+
+// FP64 closest hit with flags
+static double g_eps_edge = 1e-4, g_eps_tie = 1e-6, g_eps_graze = 1e-3;
+
+struct F64Best {
+    double t = std::numeric_limits<double>::infinity();
+    uint64_t gid = ~0ull;
+    double minb = 0, cosn = 1;
+    double t2 = std::numeric_limits<double>::infinity();     // second-best t among accepted hits
+    double t_out = std::numeric_limits<double>::infinity();  // closest near-edge rejected candidate
+};
+
+static void f64_tri(const double o[3], const double d[3], const double a[3], const double b[3], const double c[3],
+                    double tmin, uint64_t gid, F64Best &best) {
+    double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double h[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    double det = e1[0] * h[0] + e1[1] * h[1] + e1[2] * h[2];
+    if (det == 0.0) return;
+    double s[3] = {o[0] - a[0], o[1] - a[1], o[2] - a[2]};
+    double u = (s[0] * h[0] + s[1] * h[1] + s[2] * h[2]) / det;
+    double q[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2], s[0] * e1[1] - s[1] * e1[0]};
+    double v = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) / det;
+    double w = 1.0 - u - v;
+    double t = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) / det;
+    if (!(t > tmin)) return;
+    double minb = std::min(u, std::min(v, w));
+    if (minb < -g_eps_edge) return;
+    if (minb < 0.0) {
+        // just outside: the FP32 test may legitimately accept it
+        best.t_out = std::min(best.t_out, t);
+        return;
+    }
+    if (t < best.t || (t == best.t && gid < best.gid)) {
+        best.t2 = std::min(best.t2, best.t);
+        best.t = t;
+        best.gid = gid;
+        best.minb = minb;
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        double dd = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        best.cosn = nn > 0 ? std::abs(n[0] * d[0] + n[1] * d[1] + n[2] * d[2]) / (nn * dd) : 0.0;
+    } else {
+        best.t2 = std::min(best.t2, t);
+    }
+}
+
+static void scene_intersect_f64(const orcl_scene &sc, V3 o, V3 d, float tmin, float tmax, F64Best &best) {
+    best.t = tmax;
+    for (size_t ii = 0; ii < sc.instances.size(); ++ii) {
+        const Instance &ins = sc.instances[ii];
+        const Mesh &m = sc.meshes[ins.mesh];
+        // instance transforms are applied in double
+        double od[3], dd[3];
+        const float *M = ins.to_local.m;
+        for (int r = 0; r < 3; ++r) {
+            od[r] = (double)M[4 * r] * o.x + (double)M[4 * r + 1] * o.y + (double)M[4 * r + 2] * o.z + M[4 * r + 3];
+            dd[r] = (double)M[4 * r] * d.x + (double)M[4 * r + 1] * d.y + (double)M[4 * r + 2] * d.z;
+        }
+        V3 lo((float)od[0], (float)od[1], (float)od[2]), ld((float)dd[0], (float)dd[1], (float)dd[2]);
+        if (m.bvh.nodes.empty()) continue;
+        std::vector<uint32_t> stack;
+        stack.push_back(0);
+        while (!stack.empty()) {
+            uint32_t ni = stack.back();
+            stack.pop_back();
+            const BNode &nd = m.bvh.nodes[ni];
+            const double cull = best.t * (1.0 + 1e-4);
+            // widen by the float rounding of the local ray, then test
+            Box wb = nd.box;
+            for (int a = 0; a < 3; ++a) {
+                float pad = 1e-5f * std::max(1.f, std::max(std::abs(wb.lo[a]), std::abs(wb.hi[a])));
+                wb.lo[a] -= pad;
+                wb.hi[a] += pad;
+            }
+            if (!slab_hit_robust(lo, ld, wb, cull)) continue;
+            if (nd.count == 0) {
+                stack.push_back(nd.first + 1);
+                stack.push_back(nd.first);
+                continue;
+            }
+            for (uint32_t k = nd.first; k < nd.first + nd.count; ++k) {
+                uint32_t f = m.bvh.prim_ids[k];
+                V3 a, b, c;
+                m.tri(f, a, b, c);
+                double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, C[3] = {c.x, c.y, c.z};
+                f64_tri(od, dd, A, B, C, (double)tmin, ins.prim_offset + f, best);
+            }
+        }
+    }
+}
+
+}  // namespace orcl
+
+using namespace orcl;
+
+extern "C" {
+
+orcl_scene *orcl_scene_create(const orcl_mesh *meshes, uint32_t num_meshes, const orcl_instance *instances,
+                              uint32_t num_instances, const orcl_material *materials, uint32_t num_materials) {
+    auto *sc = new orcl_scene();
+    sc->meshes.resize(num_meshes);
+    for (uint32_t mi = 0; mi < num_meshes; ++mi) {
+        const orcl_mesh &src = meshes[mi];
+        Mesh &m = sc->meshes[mi];
+        m.pos.assign(src.positions, src.positions + 3 * (size_t)src.num_vertices);
+        m.idx.assign(src.indices, src.indices + 3 * (size_t)src.num_triangles);
+        if (src.normals) m.nrm.assign(src.normals, src.normals + 3 * (size_t)src.num_vertices);
+        if (src.tangents) m.tan.assign(src.tangents, src.tangents + 4 * (size_t)src.num_vertices);
+        if (src.uvs) m.uv.assign(src.uvs, src.uvs + 2 * (size_t)src.num_vertices);
+        m.ntri = src.num_triangles;
+        m.p0.resize(m.ntri);
+        m.e1.resize(m.ntri);
+        m.e2.resize(m.ntri);
+        std::vector<Box> boxes(m.ntri);
+        for (uint32_t f = 0; f < m.ntri; ++f) {
+            V3 a, b, c;
+            m.tri(f, a, b, c);
+            m.p0[f] = a;
+            m.e1[f] = b - a;
+            m.e2[f] = c - a;
+            boxes[f].grow(a);
+            boxes[f].grow(b);
+            boxes[f].grow(c);
+            m.local_box.grow(boxes[f]);
+        }
+        m.bvh.build(boxes, 8);
+    }
+    // TLAS over instance world boxes (AABB::TransformTo, AABB.h:57-79)
+    std::vector<Instance> inst(num_instances);
+    std::vector<Box> wboxes(num_instances);
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        inst[i].mesh = instances[i].mesh;
+        inst[i].material = instances[i].material;
+        std::memcpy(inst[i].to_world.m, instances[i].world_from_local, sizeof(float) * 12);
+        std::memcpy(inst[i].to_local.m, instances[i].local_from_world, sizeof(float) * 12);
+        const Box &lb = sc->meshes[inst[i].mesh].local_box;
+        for (int c = 0; c < 8; ++c) {
+            V3 p((c & 1) ? lb.hi[0] : lb.lo[0], (c & 2) ? lb.hi[1] : lb.lo[1], (c & 4) ? lb.hi[2] : lb.lo[2]);
+            wboxes[i].grow(inst[i].to_world.point(p));
+        }
+    }
+    // global triangle ids follow the caller's instance order
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        inst[i].prim_offset = off;
+        off += sc->meshes[inst[i].mesh].ntri;
+    }
+    sc->total_tris = off;
+    sc->tlas.build(wboxes, 1);
+    // reorder primitives by TLAS leaf order (SceneRenderProxy.cpp:59-68)
+    sc->instances.resize(num_instances);
+    for (uint32_t i = 0; i < num_instances; ++i) sc->instances[i] = inst[sc->tlas.prim_ids[i]];
+    sc->materials.assign(materials, materials + num_materials);
+    return sc;
+}
+
+void orcl_scene_destroy(orcl_scene *scene) { delete scene; }
+uint64_t orcl_scene_num_triangles(const orcl_scene *scene) { return scene->total_tris; }
+
+void orcl_set_fp64_eps(double eps_edge, double eps_tie, double eps_graze) {
+    g_eps_edge = eps_edge;
+    g_eps_tie = eps_tie;
+    g_eps_graze = eps_graze;
+}
+
+static void parallel_for(uint64_t n, int num_threads, const std::function<void(uint64_t, uint64_t)> &fn);
+
+int orcl_trace(const orcl_scene *scene, const orcl_ray *rays, uint64_t num_rays, int mode, int num_threads,
+               orcl_hit *hits, uint32_t *flags, double *t64) {
+    if (!scene || (!rays && num_rays) || (!hits && num_rays)) return -1;
+    const orcl_scene &sc = *scene;
+    parallel_for(num_rays, num_threads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) {
+            const orcl_ray &r = rays[i];
+            V3 o(r.org), d(r.dir);
+            hits[i].prim = 0xFFFFFFFFu;
+            hits[i].t = std::numeric_limits<float>::infinity();
+            if (mode == ORCL_TRACE_REF_MT) {
+                Candidate c;
+                scene_intersect_ref(sc, o, d, c);
+                if (c.inst >= 0) {
+                    hits[i].prim = (uint32_t)c.gid;
+                    hits[i].t = c.t;
+                }
+            } else if (mode == ORCL_TRACE_WT || mode == ORCL_TRACE_WT_BRUTE) {
+                WtBest best;
+                bool ok = std::isfinite(o.x) && std::isfinite(o.y) && std::isfinite(o.z) && std::isfinite(d.x) &&
+                          std::isfinite(d.y) && std::isfinite(d.z) && (d.x != 0.f || d.y != 0.f || d.z != 0.f);
+                if (!ok) continue;
+                if (std::isnan(r.tmin) || std::isnan(r.tmax)) continue;
+                // a negative tmin is treated as 0, as on the device
+                scene_intersect_wt(sc, o, d, std::max(r.tmin, 0.f), r.tmax, best, mode == ORCL_TRACE_WT_BRUTE);
+                if (best.gid != ~0ull) {
+                    const Instance &ins = sc.instances[best.inst];
+                    const Mesh &m = sc.meshes[ins.mesh];
+                    V3 a, bb, c;
+                    m.tri(best.face, a, bb, c);
+                    const bool ident = ins.to_local.identity();
+                    V3 lo = ident ? o : ins.to_local.point(o), ld = ident ? d : ins.to_local.dir(d);
+                    hits[i].prim = (uint32_t)best.gid;
+                    hits[i].t = refine_t(lo, ld, a, bb, c, best.t);
+                }
+            } else if (mode == ORCL_TRACE_FP64) {
+                F64Best best;
+                scene_intersect_f64(sc, o, d, r.tmin, r.tmax, best);
+                uint32_t fl = 0;
+                if (best.gid != ~0ull) {
+                    hits[i].prim = (uint32_t)best.gid;
+                    hits[i].t = (float)best.t;
+                    if (best.minb < g_eps_edge) fl |= ORCL_FLAG_EDGE;
+                    if (best.t2 <= best.t * (1.0 + g_eps_tie)) fl |= ORCL_FLAG_TIE;
+                    if (best.cosn < g_eps_graze) fl |= ORCL_FLAG_GRAZE;
+                }
+                if (std::isfinite(best.t_out) && (best.gid == ~0ull || best.t_out <= best.t * (1.0 + g_eps_tie)))
+                    fl |= ORCL_FLAG_EDGE;
+                if (flags) flags[i] = fl;
+                if (t64) t64[i] = best.gid != ~0ull ? best.t : std::numeric_limits<double>::infinity();
+            } else {
+                continue;
+            }
+        }
+    });
+    return 0;
+}
+
+int orcl_tri_mt(const float *org, const float *dir, const float *v0, const float *v1, const float *v2, float *t,
+                float *u, float *v) {
+    V3 a(v0), b(v1), c(v2);
+    return tri_mt(V3(org), V3(dir), a, b - a, c - a, *t, *u, *v) ? 1 : 0;
+}
+
+int orcl_tri_wt(const float *org, const float *dir, float tmin, const float *v0, const float *v1, const float *v2,
+                float *t, float *u, float *v) {
+    WtRay r = wt_setup(V3(org), V3(dir));
+    float tt, U, V, W, det;
+    if (!tri_wt(r, V3(v0), V3(v1), V3(v2), tt, U, V, W, det)) return 0;
+    if (!(tt > tmin)) return 0;
+    *t = tt;
+    *u = V / det;  // weight of v1, as Moller-Trumbore's u
+    *v = W / det;  // weight of v2
+    return 1;
+}
+
+float orcl_refine_t(const float *org, const float *dir, const float *v0, const float *v1, const float *v2) {
+    return refine_t(V3(org), V3(dir), V3(v0), V3(v1), V3(v2), std::numeric_limits<float>::quiet_NaN());
+}
+
+void orcl_xoshiro_stream(uint64_t seed, uint32_t *out, uint32_t n) {
+    Rng r(seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
+}
+
+void orcl_random_unit_stream(uint64_t seed, float *out, uint32_t n) {
+    Rng r(seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.unit();
+}
+
+}  // extern "C"
+
+static void parallel_for(uint64_t n, int num_threads, const std::function<void(uint64_t, uint64_t)> &fn) {
+    if (num_threads <= 1 || n < 256) {
+        fn(0, n);
+        return;
+    }
+    // dynamic chunks so that uneven rays do not serialise on one thread
+    const uint64_t chunk = std::max<uint64_t>(64, n / ((uint64_t)num_threads * 64));
+    std::atomic<uint64_t> next(0);
+    std::vector<std::thread> pool;
+    for (int k = 0; k < num_threads; ++k)
+        pool.emplace_back([&]() {
+            for (;;) {
+                uint64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                fn(b, std::min(n, b + chunk));
+            }
+        });
+    for (auto &th : pool) th.join();
+}
+
+// =============================================================================================
+// integrator: BasePass / RenderPixel / SamplePixel (CPURenderer.cpp:220-450)
+// =============================================================================================
+namespace orcl {
+
+struct Hit {
+    bool hit = false;
+    float t = 0;
+    V3 location, normal, tangent;
+    float uvx = 0, uvy = 0;
+    const Instance *inst = nullptr;
+};
+
+template <class T>
+static inline T lerp3(const T &v0, const T &v1, const T &v2, float u, float v) {
+    return v0 + (v1 - v0) * u + (v2 - v0) * v;  // Utilities.h:114-117
+}
+
+// SceneRenderProxy::Intersect<false> + MeshRenderProxy::GetIntersection (MeshRenderProxy.cpp:340-361)
+static Hit scene_hit(const orcl_scene &sc, V3 o, V3 d) {
+    Candidate c;
+    scene_intersect_ref(sc, o, d, c);
+    Hit h;
+    if (c.inst < 0) return h;
+    const Instance &ins = sc.instances[c.inst];
+    const Mesh &m = sc.meshes[ins.mesh];
+    const uint32_t i0 = m.idx[3 * (size_t)c.face], i1 = m.idx[3 * (size_t)c.face + 1],
+                   i2 = m.idx[3 * (size_t)c.face + 2];
+    if (!m.uv.empty()) {
+        h.uvx = m.uv[2 * i0] + (m.uv[2 * i1] - m.uv[2 * i0]) * c.u + (m.uv[2 * i2] - m.uv[2 * i0]) * c.v;
+        h.uvy = m.uv[2 * i0 + 1] + (m.uv[2 * i1 + 1] - m.uv[2 * i0 + 1]) * c.u +
+                (m.uv[2 * i2 + 1] - m.uv[2 * i0 + 1]) * c.v;
+    }
+    V3 n(0, 0, 0);
+    if (!m.nrm.empty()) n = lerp3(V3(&m.nrm[3 * i0]), V3(&m.nrm[3 * i1]), V3(&m.nrm[3 * i2]), c.u, c.v);
+    V3 tg = kRight;
+    if (!m.tan.empty())
+        tg = lerp3(V3(&m.tan[4 * i0]), V3(&m.tan[4 * i1]), V3(&m.tan[4 * i2]), c.u, c.v);
+    // TransformDirectionTangentSpace on the inverse = the forward 3x3 (Transform.h:105-108)
+    h.normal = normalized(ins.to_world.dir(n));
+    h.tangent = ins.to_world.dir(tg);
+    h.t = c.t;
+    h.location = o + d * c.t;  // Intersection::Update -> ray.At(t)
+    h.inst = &ins;
+    h.hit = true;
+    return h;
+}
+
+// GetLocalAxisFromNormal(normal, major_axis) (Utilities.h:236-244)
+static inline void local_axes(V3 n, V3 major, V3 &u, V3 &v, V3 &w) {
+    u = normalized(cross(n, major));
+    v = cross(n, u);
+    w = n;
+}
+
+static inline float schlick(float c, float r0) { return r0 + (1.f - r0) * std::pow((1.f - c), 5.f); }
+static inline V3 schlick3(float c, V3 r0) {
+    float p = std::pow((1.f - c), 5.f);
+    return r0 + (kOnes - r0) * p;
+}
+
+static inline float smith_ggx_correlated(float cos_o, float cos_i, float roughness) {
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float ggx_i = cos_o * std::sqrt((-cos_i * a2 + cos_i) * cos_i + a2);
+    float ggx_o = cos_i * std::sqrt((-cos_o * a2 + cos_o) * cos_o + a2);
+    return 2.f * cos_o * cos_i / (ggx_o + ggx_i);
+}
+
+static inline float geometry_schlick_ggx(float cos_theta, float roughness) {
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float tan2_v = (1.f - cos_theta * cos_theta) / (cos_theta * cos_theta);
+    return 2.f / (1.f + std::sqrt(1.f + a2 * tan2_v));
+}
+
+// DistributionVn::Sample (Sampler.h:133-165)
+static V3 sample_vndf(Rng &rng, V3 w_o, float roughness) {
+    float a = roughness * roughness;
+    float u1 = rng.unit();
+    float u2 = rng.unit();
+    V3 v = normalized(V3(w_o.x * a, w_o.y * a, w_o.z));
+    V3 t1 = (v.z < 0.999f) ? normalized(cross(v, kUp)) : kRight;
+    V3 t2 = cross(v, t1);
+    float r = std::sqrt(u1);
+    float phi = 2.f * kPi * u2;
+    float p1 = r * std::cos(phi);
+    float p2 = r * std::sin(phi);
+    float s = 0.5f * (1.f + v.z);
+    p2 = (1.f - s) * std::sqrt(1.f - p1 * p1) + s * p2;
+    V3 n = p1 * t1 + p2 * t2 + std::sqrt(std::max(0.f, 1.f - p1 * p1 - p2 * p2)) * v;
+    return normalized(V3(a * n.x, a * n.y, std::max(0.f, n.z)));
+}
+
+struct Sample {
+    bool valid = false;
+    V3 throughput, wi;
+};
+
+// SpecularBxDF::Sample (BxDF.h:48-109)
+static Sample specular_sample(Rng &rng, V3 wo, V3 base, float roughness, float metallic) {
+    Sample s;
+    V3 wm = sample_vndf(rng, wo, roughness);
+    V3 wi = wm * (dot(wo, wm) * 2.f) - wo;  // Reflect (Utilities.h:206-209)
+    float cos_o = std::max(std::abs(wo.z), 0.f);
+    float cos_i = std::max(std::abs(wi.z), 0.f);
+    float cos_m = std::max(std::abs(wm.z), 0.f);
+    float local_cos_i = dot(wi, wm);
+    V3 fres;
+    if (cos_o > kEps && cos_i > kEps && cos_m > kEps) {
+        if (rng.unit() < metallic) {
+            fres = schlick3(local_cos_i, base);
+            s.valid = true;
+        } else {
+            float f = schlick(local_cos_i, 0.04f);
+            if (rng.unit() < f) {
+                fres = kOnes;
+                s.valid = true;
+            }
+        }
+    }
+    if (!s.valid) return s;
+    float occlusion = smith_ggx_correlated(cos_o, cos_i, roughness);
+    float normalizer = geometry_schlick_ggx(cos_o, roughness) + kEps;
+    s.throughput = fres * occlusion / normalizer;
+    s.wi = wi;
+    return s;
+}
+
+// LambertianBxDF::Sample (BxDF.h:32-46) with CosineWeightedHemiSphere (Sampler.h:79-95)
+static Sample lambert_sample(Rng &rng, V3 base) {
+    Sample s;
+    s.valid = true;
+    float x, y;
+    rng.disk(x, y);
+    float z = std::sqrt(std::max(0.f, 1.0f - (x * x + y * y)));
+    s.wi = V3(x, y, z);
+    s.throughput = base;
+    return s;
+}
+
+// FrDielectric (Utilities.h:301-324)
+static float fr_dielectric(float cos_i, float eta_i, float eta_t) {
+    if (!(cos_i > 0.f)) {
+        std::swap(eta_i, eta_t);
+        cos_i = -cos_i;
+    }
+    float sin_i = std::sqrt(1 - cos_i * cos_i);
+    float ref_idx = eta_i / eta_t;
+    float sin_t = ref_idx * sin_i;
+    if (sin_t >= 1.f) return 1.f;
+    float r0 = (1.f - ref_idx) / (1.f + ref_idx);
+    r0 = r0 * r0;
+    return schlick(cos_i, r0);
+}
+
+// DieletricBxDF::Sample (BxDF.h:112-141)
+static Sample dielectric_sample(Rng &rng, V3 wo, V3 base, float eta) {
+    Sample s;
+    float fres = fr_dielectric(wo.z, 1.0f, eta);
+    s.valid = true;
+    if (rng.unit() < fres) {
+        s.wi = V3(-wo.x, -wo.y, wo.z);
+    } else {
+        bool entering = wo.z > 0;
+        float e = entering ? 1.0f / eta : eta / 1.0f;
+        float cos_i = wo.z;
+        float sin_i2 = 1.f - cos_i * cos_i;
+        float sin_t2 = e * e * sin_i2;
+        float cos_t = std::sqrt(1 - sin_t2) * (cos_i > 0 ? -1.f : 1.f);
+        s.wi = normalized(V3(-e * wo.x, -e * wo.y, cos_t));
+    }
+    s.throughput = base;
+    return s;
+}
+
+// MaterialRenderProxy::SampleSurface for Pbr / Dielectric proxies
+static V3 sample_surface(Rng &rng, const orcl_material &mat, V3 ray_dir, V3 normal, V3 tangent, V3 &w_i) {
+    V3 base(mat.base_color);
+    V3 wo = -ray_dir;
+    V3 u, v, w;
+    local_axes(normal, tangent, u, v, w);
+    V3 lwo(dot(wo, u), dot(wo, v), dot(wo, w));
+    Sample s;
+    if (mat.type == ORCL_MAT_DIELECTRIC) {
+        s = dielectric_sample(rng, lwo, base, mat.eta);
+    } else {
+        s = specular_sample(rng, lwo, base, mat.roughness, mat.metallic);
+        if (!s.valid) s = lambert_sample(rng, base);
+    }
+    V3 u2, v2, w2;
+    local_axes(normal, tangent, u2, v2, w2);
+    w_i = normalized(u2 * s.wi.x + v2 * s.wi.y + w2 * s.wi.z);
+    return s.throughput;
+}
+
+struct PixelResult {
+    V3 color;
+    V3 normal;
+    float valid = 1.f;
+};
+
+static std::atomic<uint64_t> g_rays{0};
+
+static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, const orcl_frame &fr, Rng &rng,
+                                float u, float v, uint64_t &nrays) {
+    // SetupViewRay (CPURenderer.cpp:220-238)
+    float nx, ny;
+    rng.disk(nx, ny);
+    nx = nx * cam.lens_radius;
+    ny = ny * cam.lens_radius;
+    V3 lens = V3(cam.right) * nx + V3(cam.up) * ny;
+    V3 o = V3(cam.position) + lens;
+    V3 focus = V3(cam.lower_left) + V3(cam.max_u) * u + V3(cam.max_v) * v;
+    V3 d = normalized(focus - o);
+
+    V3 T = kOnes;
+    PixelResult res;
+    unsigned bounce = 0;
+    for (; bounce < fr.max_bounce; bounce++) {
+        Hit h = scene_hit(sc, o, d);
+        nrays++;
+        if (!h.hit) {
+            if (fr.sky_enabled) {
+                float t = 0.5f * d.z + 0.5f;
+                V3 sky = kOnes + (V3(fr.sky_color) - kOnes) * t;
+                res.color = res.color + cmul(sky, T);
+            }
+            break;
+        }
+        const orcl_material &mat = sc.materials[h.inst->material];
+        if (bounce == 0) res.normal = h.normal;
+        V3 E(mat.emissive);
+        if (norm(E) > kEps) {
+            res.color = res.color + cmul(E, T);
+            break;
+        }
+        V3 wi;
+        V3 f = sample_surface(rng, mat, d, h.normal, h.tangent, wi);
+        switch (fr.debug_mode) {
+            case 3:  // Normal
+                res.color = (h.normal + kOnes) * 0.5f;
+                return res;
+            case 4:  // RayDirection
+                res.color = (wi + kOnes) * 0.5f;
+                return res;
+            case 7:
+                res.color = kOnes * mat.metallic;
+                return res;
+            case 8:
+                res.color = kOnes * mat.roughness;
+                return res;
+            case 9:
+                res.color = V3(mat.base_color);
+                return res;
+            case 10:
+                res.color = V3(mat.emissive);
+                return res;
+            case 11:
+                res.color = kOnes * (dot(h.location - V3(cam.position), V3(cam.front)) / cam.far_plane);
+                return res;
+            default:
+                break;
+        }
+        T = cmul(T, f);
+        if (sqnorm(T) < kEps) {
+            res.valid = -1.f;
+            break;
+        }
+        if (bounce >= 3) {
+            float p = maxc(T);
+            p = std::min(std::max(p, 0.05f), 1.0f);
+            if (rng.unit() > p) break;
+            T = T / p;
+        }
+        o = h.location + wi * kTolerance;
+        d = wi;
+    }
+    return res;
+}
+
+}  // namespace orcl
+
+extern "C" {
+
+int orcl_render_frame(const orcl_scene *scene, const orcl_camera *camera, const orcl_frame *frame, int num_threads,
+                      float *rgba, float *normal) {
+    if (!scene || !camera || !frame || !rgba) return -1;
+    const orcl_frame fr = *frame;
+    const orcl_camera cam = *camera;
+    const float pw = 1.f / static_cast<float>(fr.width - 1);
+    const float ph = 1.f / static_cast<float>(fr.height - 1);
+    g_rays = 0;
+    // one task per row (CPURenderer.cpp:435-449)
+    parallel_for(fr.height, num_threads, [&](uint64_t jb, uint64_t je) {
+        uint64_t nrays = 0;
+        for (uint64_t j = jb; j < je; ++j) {
+            for (uint32_t i = 0; i < fr.width; ++i) {
+                const uint32_t seed32 = (uint32_t)j * fr.width + i + fr.frame_seed * fr.width * fr.height;
+                uint64_t seed = seed32;
+                if (fr.seed_hi) {
+                    seed = (uint64_t)j * fr.width + i + (uint64_t)fr.frame_seed * fr.width * fr.height;
+                }
+                Rng rng(seed);
+                float u = (static_cast<float>(i) + rng.unit()) * pw;
+                float v = (static_cast<float>(j) + rng.unit()) * ph;
+                PixelResult r = sample_pixel(*scene, cam, fr, rng, u, v, nrays);
+                const size_t p = (size_t)j * fr.width + i;
+                rgba[4 * p + 0] = std::min(r.color.x, kOutputLimit);
+                rgba[4 * p + 1] = std::min(r.color.y, kOutputLimit);
+                rgba[4 * p + 2] = std::min(r.color.z, kOutputLimit);
+                rgba[4 * p + 3] = r.valid;
+                if (normal) {
+                    normal[3 * p + 0] = r.normal.x;
+                    normal[3 * p + 1] = r.normal.y;
+                    normal[3 * p + 2] = r.normal.z;
+                }
+            }
+        }
+        g_rays += nrays;
+    });
+    return 0;
+}
+
+uint64_t orcl_last_render_rays(void) { return g_rays.load(); }
+
+// CPURenderer::DenoisePass temporal part (CPURenderer.cpp:527-542): acc = new + (acc - new) * n/(n+spp)
+void orcl_accumulate(const float *rgba, float *accum, uint64_t num_pixels, uint32_t cumulated, uint32_t spp) {
+    const float ma = static_cast<float>(cumulated) / static_cast<float>(cumulated + spp);
+    for (uint64_t i = 0; i < 4 * num_pixels; ++i) accum[i] = rgba[i] + (accum[i] - rgba[i]) * ma;
+}
+
+void orcl_camera_setup(const float position[3], const float right[3], const float front[3], const float up[3],
+                       float vertical_fov, float focus_distance, float aperture_radius, float aspect,
+                       orcl_camera *c) {
+    float h = std::tan(vertical_fov / 2.f);
+    float height = 2.f * h * focus_distance;
+    float width = aspect * height;
+    V3 R(right), F(front), U(up), P(position);
+    V3 mu = R * width, mv = U * height;
+    V3 ll = P + F * focus_distance - mu * .5f - mv * .5f;
+    for (int a = 0; a < 3; ++a) {
+        c->position[a] = P[a];
+        c->right[a] = R[a];
+        c->up[a] = U[a];
+        c->front[a] = F[a];
+        c->max_u[a] = mu[a];
+        c->max_v[a] = mv[a];
+        c->lower_left[a] = ll[a];
+    }
+    c->lens_radius = aperture_radius;
+    c->far_plane = 1000.f;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// learned radiance cache, FP64 (no reference counterpart): multiresolution hash grid
+// (Mueller et al. 2022, primes 1, 2654435761, 805459861) and a bias-free ReLU MLP trained with
+// a relative L2 loss (Mueller et al. 2021) and Adam (Kingma & Ba 2015).
+// =============================================================================================
+namespace orcl {
+static inline uint32_t level_resolution(const orcl_cache_config *cfg, uint32_t level) {
+    return (uint32_t)std::floor((double)cfg->base_resolution * std::pow((double)cfg->per_level_scale, (double)level));
+}
+}  // namespace orcl
+
+extern "C" {
+
+uint32_t orcl_hashgrid_index(const orcl_cache_config *cfg, uint32_t level, uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t res = orcl::level_resolution(cfg, level);
+    const uint64_t T = 1ull << cfg->log2_table;
+    const uint64_t side = (uint64_t)res + 1;
+    if (side * side * side <= T) return (uint32_t)(x + side * (y + side * z));
+    const uint32_t h = (x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u);
+    return h & (uint32_t)(T - 1);
+}
+
+void orcl_hashgrid_encode(const orcl_cache_config *cfg, const float *table, const float *points, uint32_t n,
+                          double *features) {
+    const uint32_t L = cfg->levels, F = cfg->features;
+    const uint64_t T = 1ull << cfg->log2_table;
+    for (uint32_t i = 0; i < n; ++i) {
+        for (uint32_t l = 0; l < L; ++l) {
+            const uint32_t res = orcl::level_resolution(cfg, l);
+            uint32_t cell[3];
+            double frac[3];
+            for (int a = 0; a < 3; ++a) {
+                // cell selection in float, as the device computes it
+                float p = std::min(std::max(points[3 * (size_t)i + a], 0.f), 1.f);
+                float pos = p * (float)res;
+                float c = std::floor(pos);
+                c = std::min(std::max(c, 0.f), (float)(res - 1));
+                cell[a] = (uint32_t)c;
+                frac[a] = (double)(pos - c);
+            }
+            for (uint32_t f = 0; f < F; ++f) features[(size_t)i * L * F + l * F + f] = 0.0;
+            for (int k = 0; k < 8; ++k) {
+                uint32_t cx = cell[0] + (k & 1), cy = cell[1] + ((k >> 1) & 1), cz = cell[2] + ((k >> 2) & 1);
+                double w = ((k & 1) ? frac[0] : 1.0 - frac[0]) * (((k >> 1) & 1) ? frac[1] : 1.0 - frac[1]) *
+                           (((k >> 2) & 1) ? frac[2] : 1.0 - frac[2]);
+                uint32_t idx = orcl_hashgrid_index(cfg, l, cx, cy, cz);
+                for (uint32_t f = 0; f < F; ++f)
+                    features[(size_t)i * L * F + l * F + f] += w * (double)table[((size_t)l * T + idx) * F + f];
+            }
+        }
+    }
+}
+
+double orcl_mlp_loss_grad(const orcl_cache_config *cfg, const double *weights, const double *features,
+                          const double *targets, uint32_t n, double *outputs, double *dweights, double *dfeatures) {
+    const uint32_t in = cfg->levels * cfg->features, H = cfg->hidden, NL = cfg->hidden_layers, O = cfg->outputs;
+    // layer shapes: [H x in], (NL-1) x [H x H], [O x H]
+    std::vector<uint32_t> rows, cols;
+    rows.push_back(H);
+    cols.push_back(in);
+    for (uint32_t k = 1; k < NL; ++k) {
+        rows.push_back(H);
+        cols.push_back(H);
+    }
+    rows.push_back(O);
+    cols.push_back(H);
+    const size_t nlayers = rows.size();
+    std::vector<size_t> off(nlayers + 1, 0);
+    for (size_t k = 0; k < nlayers; ++k) off[k + 1] = off[k] + (size_t)rows[k] * cols[k];
+    if (dweights) std::fill(dweights, dweights + off[nlayers], 0.0);
+    double loss = 0.0;
+    const double inv = 1.0 / ((double)n * O);
+    std::vector<std::vector<double>> act(nlayers + 1);
+    for (uint32_t s = 0; s < n; ++s) {
+        act[0].assign(features + (size_t)s * in, features + (size_t)s * in + in);
+        for (size_t k = 0; k < nlayers; ++k) {
+            act[k + 1].assign(rows[k], 0.0);
+            for (uint32_t r = 0; r < rows[k]; ++r) {
+                double acc = 0.0;
+                for (uint32_t c = 0; c < cols[k]; ++c) acc += weights[off[k] + (size_t)r * cols[k] + c] * act[k][c];
+                act[k + 1][r] = (k + 1 < nlayers) ? std::max(acc, 0.0) : acc;
+            }
+        }
+        std::vector<double> g(O);
+        for (uint32_t o = 0; o < O; ++o) {
+            double y = act[nlayers][o], tg = targets[(size_t)s * O + o];
+            if (outputs) outputs[(size_t)s * O + o] = y;
+            double den = y * y + 0.01;  // the denominator is treated as a constant (stop-gradient)
+            loss += (y - tg) * (y - tg) / den * inv;
+            g[o] = 2.0 * (y - tg) / den * inv;
+        }
+        for (size_t k = nlayers; k-- > 0;) {
+            std::vector<double> gin(cols[k], 0.0);
+            for (uint32_t r = 0; r < rows[k]; ++r) {
+                for (uint32_t c = 0; c < cols[k]; ++c) {
+                    if (dweights) dweights[off[k] + (size_t)r * cols[k] + c] += g[r] * act[k][c];
+                    gin[c] += weights[off[k] + (size_t)r * cols[k] + c] * g[r];
+                }
+            }
+            if (k > 0)
+                for (uint32_t c = 0; c < cols[k]; ++c) gin[c] = act[k][c] > 0.0 ? gin[c] : 0.0;
+            g.swap(gin);
+        }
+        if (dfeatures)
+            for (uint32_t c = 0; c < in; ++c) dfeatures[(size_t)s * in + c] = g[c];
+    }
+    return loss;
+}
+
+void orcl_adam(double *param, const double *grad, double *m, double *v, uint64_t count, double lr, double beta1,
+               double beta2, double eps, uint32_t step) {
+    const double c1 = 1.0 - std::pow(beta1, (double)step), c2 = 1.0 - std::pow(beta2, (double)step);
+    for (uint64_t i = 0; i < count; ++i) {
+        m[i] = beta1 * m[i] + (1.0 - beta1) * grad[i];
+        v[i] = beta2 * v[i] + (1.0 - beta2) * grad[i] * grad[i];
+        const double mh = m[i] / c1, vh = v[i] / c2;
+        param[i] -= lr * mh / (std::sqrt(vh) + eps);
+    }
+}
+
+}  // extern "C"

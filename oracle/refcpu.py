@@ -1,0 +1,230 @@
+"""ctypes wrapper of liborcl.so (refcpu.cpp): the CPU restatement of the reference path.
+
+Test infrastructure only (see refcpu.h for what each function restates and how it is pinned).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborcl.so")
+
+TRACE_REF_MT = 0
+TRACE_WT = 1
+TRACE_WT_BRUTE = 2
+TRACE_FP64 = 3
+FLAG_EDGE, FLAG_TIE, FLAG_GRAZE = 1, 2, 4
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("t", "<f4")])
+
+
+class OMesh(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("num_vertices", ctypes.c_uint32), ("indices", ctypes.c_void_p),
+                ("num_triangles", ctypes.c_uint32), ("normals", ctypes.c_void_p), ("tangents", ctypes.c_void_p),
+                ("uvs", ctypes.c_void_p)]
+
+
+class OInstance(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_uint32), ("material", ctypes.c_uint32), ("world_from_local", ctypes.c_float * 12),
+                ("local_from_world", ctypes.c_float * 12)]
+
+
+class OMaterial(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("base_color", ctypes.c_float * 3), ("emissive", ctypes.c_float * 3),
+                ("metallic", ctypes.c_float), ("roughness", ctypes.c_float), ("eta", ctypes.c_float)]
+
+
+class OCamera(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("front", ctypes.c_float * 3), ("lower_left", ctypes.c_float * 3), ("max_u", ctypes.c_float * 3),
+                ("max_v", ctypes.c_float * 3), ("lens_radius", ctypes.c_float), ("far_plane", ctypes.c_float)]
+
+
+class OFrame(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32),
+                ("max_bounce", ctypes.c_uint32), ("debug_mode", ctypes.c_uint32), ("sky_enabled", ctypes.c_uint32),
+                ("sky_color", ctypes.c_float * 3), ("seed_hi", ctypes.c_uint32)]
+
+
+class OCacheConfig(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_uint32), ("features", ctypes.c_uint32), ("log2_table", ctypes.c_uint32),
+                ("base_resolution", ctypes.c_uint32), ("per_level_scale", ctypes.c_float),
+                ("hidden", ctypes.c_uint32), ("hidden_layers", ctypes.c_uint32), ("outputs", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P, U32, U64, I, F, D = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float,
+                                ctypes.c_double)
+        sig = {
+            "orcl_scene_create": (P, [P, U32, P, U32, P, U32]),
+            "orcl_scene_destroy": (None, [P]),
+            "orcl_scene_num_triangles": (U64, [P]),
+            "orcl_trace": (I, [P, P, U64, I, I, P, P, P]),
+            "orcl_set_fp64_eps": (None, [D, D, D]),
+            "orcl_tri_mt": (I, [P, P, P, P, P, P, P, P]),
+            "orcl_tri_wt": (I, [P, P, F, P, P, P, P, P, P]),
+            "orcl_refine_t": (F, [P, P, P, P, P]),
+            "orcl_xoshiro_stream": (None, [U64, P, U32]),
+            "orcl_random_unit_stream": (None, [U64, P, U32]),
+            "orcl_render_frame": (I, [P, P, P, I, P, P]),
+            "orcl_accumulate": (None, [P, P, U64, U32, U32]),
+            "orcl_last_render_rays": (U64, []),
+            "orcl_camera_setup": (None, [P, P, P, P, F, F, F, F, P]),
+            "orcl_hashgrid_encode": (None, [P, P, P, U32, P]),
+            "orcl_hashgrid_index": (U32, [P, U32, U32, U32, U32]),
+            "orcl_mlp_loss_grad": (D, [P, P, P, P, U32, P, P, P]),
+            "orcl_adam": (None, [P, P, P, P, U64, D, D, D, D, U32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _inverse34(M):
+    M = np.asarray(M, dtype=np.float64).reshape(3, 4)
+    A = M[:, :3]
+    Ai = np.linalg.inv(A)
+    return np.concatenate([Ai, -(Ai @ M[:, 3])[:, None]], axis=1)
+
+
+class OracleScene:
+    """Two-level scene of the oracle built from a sparkle_amd.Scene-like description
+    (dicts with numpy arrays; the arrays are copied by the library)."""
+
+    def __init__(self, scene):
+        L = lib()
+        self._meshes = scene.meshes
+        ms = (OMesh * len(scene.meshes))()
+        for k, m in enumerate(scene.meshes):
+            ms[k].positions = _p(m["positions"])
+            ms[k].num_vertices = m["positions"].shape[0]
+            ms[k].indices = _p(m["indices"])
+            ms[k].num_triangles = m["indices"].shape[0]
+            ms[k].normals = _p(m["normals"])
+            ms[k].tangents = _p(m["tangents"])
+            ms[k].uvs = _p(m["uvs"])
+        ins = (OInstance * len(scene.instances))()
+        for k, i in enumerate(scene.instances):
+            ins[k].mesh = i["mesh"]
+            ins[k].material = i["material"]
+            ins[k].world_from_local[:] = [float(x) for x in np.asarray(i["M"], np.float32).reshape(-1)]
+            ins[k].local_from_world[:] = [float(x) for x in _inverse34(i["M"]).astype(np.float32).reshape(-1)]
+        mats = (OMaterial * max(1, len(scene.materials)))()
+        for k, m in enumerate(scene.materials):
+            mats[k].type = m["type"]
+            mats[k].base_color[:] = m["base_color"]
+            mats[k].emissive[:] = m["emissive"]
+            mats[k].metallic = m["metallic"]
+            mats[k].roughness = m["roughness"]
+            mats[k].eta = m["eta"]
+        self.h = L.orcl_scene_create(ms, len(scene.meshes), ins, len(scene.instances), mats, len(scene.materials))
+        if not self.h:
+            raise RuntimeError("orcl_scene_create failed")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orcl_scene_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def trace(self, rays, mode=TRACE_WT, threads=None):
+        rays = np.ascontiguousarray(rays)
+        n = rays.shape[0]
+        hits = np.zeros(n, dtype=HIT_DTYPE)
+        flags = np.zeros(n, dtype=np.uint32)
+        t64 = np.zeros(n, dtype=np.float64)
+        threads = threads or os.cpu_count() or 1
+        rc = lib().orcl_trace(self.h, rays.ctypes.data, n, mode, threads, hits.ctypes.data, flags.ctypes.data,
+                              t64.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"orcl_trace failed: {rc}")
+        if mode == TRACE_FP64:
+            return hits, flags, t64
+        return hits
+
+    def render_frame(self, cam, width, height, frame_seed=0, max_bounce=4, debug_mode=0, sky=None, seed64=False,
+                     threads=None):
+        fr = OFrame(width, height, frame_seed, max_bounce, debug_mode, 1 if sky is not None else 0,
+                    (ctypes.c_float * 3)(*(sky if sky is not None else (0, 0, 0))), 1 if seed64 else 0)
+        oc = OCamera()
+        for f in ("position", "right", "up", "front", "lower_left", "max_u", "max_v"):
+            getattr(oc, f)[:] = list(getattr(cam, f))
+        oc.lens_radius = cam.lens_radius
+        oc.far_plane = cam.far_plane
+        rgba = np.zeros((height, width, 4), dtype=np.float32)
+        nrm = np.zeros((height, width, 3), dtype=np.float32)
+        rc = lib().orcl_render_frame(self.h, ctypes.byref(oc), ctypes.byref(fr), threads or os.cpu_count() or 1,
+                                     rgba.ctypes.data, nrm.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("orcl_render_frame failed")
+        return rgba, nrm
+
+    @staticmethod
+    def last_render_rays():
+        return int(lib().orcl_last_render_rays())
+
+
+def accumulate(rgba, accum, cumulated, spp=1):
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    assert accum.flags["C_CONTIGUOUS"] and accum.dtype == np.float32
+    lib().orcl_accumulate(rgba.ctypes.data, accum.ctypes.data, rgba.size // 4, cumulated, spp)
+
+
+def tri_mt(o, d, v0, v1, v2):
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    o, d, v0, v1, v2 = map(f, (o, d, v0, v1, v2))
+    t, u, v = (np.zeros(1, np.float32) for _ in range(3))
+    hit = lib().orcl_tri_mt(o.ctypes.data, d.ctypes.data, v0.ctypes.data, v1.ctypes.data, v2.ctypes.data,
+                            t.ctypes.data, u.ctypes.data, v.ctypes.data)
+    return bool(hit), float(t[0]), float(u[0]), float(v[0])
+
+
+def tri_wt(o, d, v0, v1, v2, tmin=0.0):
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    o, d, v0, v1, v2 = map(f, (o, d, v0, v1, v2))
+    t, u, v = (np.zeros(1, np.float32) for _ in range(3))
+    hit = lib().orcl_tri_wt(o.ctypes.data, d.ctypes.data, float(tmin), v0.ctypes.data, v1.ctypes.data,
+                            v2.ctypes.data, t.ctypes.data, u.ctypes.data, v.ctypes.data)
+    return bool(hit), float(t[0]), float(u[0]), float(v[0])
+
+
+def refine_t(o, d, v0, v1, v2):
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    o, d, v0, v1, v2 = map(f, (o, d, v0, v1, v2))
+    return float(lib().orcl_refine_t(o.ctypes.data, d.ctypes.data, v0.ctypes.data, v1.ctypes.data, v2.ctypes.data))
+
+
+def xoshiro_stream(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orcl_xoshiro_stream(seed, out.ctypes.data, n)
+    return out
+
+
+def random_unit_stream(seed, n):
+    out = np.zeros(n, dtype=np.float32)
+    lib().orcl_random_unit_stream(seed, out.ctypes.data, n)
+    return out

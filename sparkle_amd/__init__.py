@@ -1,0 +1,378 @@
+"""sparkle_amd — MI355X-native hot path of Sparkle's `cpu` path tracer.
+
+Python mirror of the C ABI in include/spk_pt.h (ctypes). The compute runs in the HIP library
+sparkle_amd/lib/libspk_pt.so; there is no CPU fallback: importing the package on a machine without
+the built library raises, and every call that needs a GPU fails with the library's error code.
+
+Reference interfaces mirrored (paths relative to the Sparkle repository):
+    Context.upload_scene   SceneRenderProxy::UpdateBVH (SceneRenderProxy.cpp:269-297)
+    Context.trace          SceneRenderProxy::Intersect<false> (SceneRenderProxy.cpp:299-309)
+    Context.render_frame   CPURenderer::BasePass (CPURenderer.cpp:419-450)
+    Context.accumulate     CPURenderer::DenoisePass, temporal part (CPURenderer.cpp:506-552)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libspk_pt.so")
+
+SPK_OK = 0
+SPK_ERR_INVALID_ARGUMENT = -1
+SPK_ERR_HIP = -2
+SPK_ERR_OUT_OF_MEMORY = -3
+SPK_ERR_NO_SCENE = -4
+SPK_ERR_INVALID_SCENE = -5
+SPK_ERR_STACK_DEPTH = -6
+SPK_ERR_INVALID_RAY = -7
+SPK_ERR_NO_DEVICE = -8
+SPK_ERR_UNSUPPORTED = -9
+SPK_MISS = 0xFFFFFFFF
+
+MATERIAL_PBR = 0
+MATERIAL_DIELECTRIC = 1
+
+DEBUG_COLOR = 0
+DEBUG_NORMAL = 3
+DEBUG_RAY_DIRECTION = 4
+DEBUG_ALBEDO = 9
+DEBUG_DEPTH = 11
+
+FRAME_SEED64 = 1
+FRAME_NEE = 2
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("t", "<f4")])
+
+
+class SpkError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"spk error {code}: {message}")
+        self.code = code
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("num_vertices", ctypes.c_uint32),
+                ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint32),
+                ("normals", ctypes.c_void_p), ("tangents", ctypes.c_void_p), ("uvs", ctypes.c_void_p)]
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_uint32), ("material", ctypes.c_uint32), ("world_from_local", ctypes.c_float * 12)]
+
+
+class MaterialDesc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("base_color", ctypes.c_float * 3), ("emissive", ctypes.c_float * 3),
+                ("metallic", ctypes.c_float), ("roughness", ctypes.c_float), ("eta", ctypes.c_float)]
+
+
+class BuildOptions(ctypes.Structure):
+    _fields_ = [("bvh_width", ctypes.c_uint32), ("max_leaf_tris", ctypes.c_uint32), ("num_threads", ctypes.c_uint32)]
+
+
+class SceneStats(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_uint64), ("num_nodes", ctypes.c_uint64), ("bvh_width", ctypes.c_uint32),
+                ("depth", ctypes.c_uint32), ("node_bytes", ctypes.c_uint64), ("triangle_bytes", ctypes.c_uint64),
+                ("device_bytes", ctypes.c_uint64), ("degenerate_tris", ctypes.c_uint64), ("sah_cost", ctypes.c_double),
+                ("build_seconds", ctypes.c_double)]
+
+
+class TraceConfig(ctypes.Structure):
+    _fields_ = [("bvh_width", ctypes.c_uint32), ("stack", ctypes.c_uint32), ("block", ctypes.c_uint32)]
+
+    def as_tuple(self):
+        return (self.bvh_width, self.stack, self.block)
+
+
+TUNE_WIDTH = 1
+
+
+class CameraDesc(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("front", ctypes.c_float * 3), ("lower_left", ctypes.c_float * 3), ("max_u", ctypes.c_float * 3),
+                ("max_v", ctypes.c_float * 3), ("lens_radius", ctypes.c_float), ("far_plane", ctypes.c_float)]
+
+
+class FrameDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32),
+                ("max_bounce", ctypes.c_uint32), ("debug_mode", ctypes.c_uint32), ("sky_enabled", ctypes.c_uint32),
+                ("sky_color", ctypes.c_float * 3), ("flags", ctypes.c_uint32)]
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [("rays_traced", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libspk_pt.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"HIP extension not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, U32, U64, I, F = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
+    sig = {
+        "spk_create": (I, [I, ctypes.POINTER(P)]),
+        "spk_destroy": (I, [P]),
+        "spk_last_error": (ctypes.c_char_p, [P]),
+        "spk_set_stream": (I, [P, P]),
+        "spk_get_stream": (P, [P]),
+        "spk_synchronize": (I, [P]),
+        "spk_device_alloc": (I, [P, ctypes.c_size_t, ctypes.POINTER(P)]),
+        "spk_device_free": (I, [P, P]),
+        "spk_copy_to_device": (I, [P, P, P, ctypes.c_size_t]),
+        "spk_copy_to_host": (I, [P, P, P, ctypes.c_size_t]),
+        "spk_scene_upload": (I, [P, P, U32, P, U32, P, U32, P]),
+        "spk_scene_stats_get": (I, [P, ctypes.POINTER(SceneStats)]),
+        "spk_build_check": (I, [P, U32, P, U32, P, ctypes.POINTER(SceneStats), ctypes.c_char_p, ctypes.c_size_t]),
+        "spk_trace_closest": (I, [P, P, U64, P]),
+        "spk_trace_closest_async": (I, [P, P, U64, P]),
+        "spk_trace_closest_host": (I, [P, P, U64, P]),
+        "spk_trace_count": (I, [P, P, U64, P, P]),
+        "spk_get_trace_config": (I, [P, ctypes.POINTER(TraceConfig)]),
+        "spk_set_trace_config": (I, [P, ctypes.POINTER(TraceConfig)]),
+        "spk_list_trace_configs": (I, [P, P, U32, ctypes.POINTER(U32)]),
+        "spk_autotune": (I, [P, P, U64, U32, ctypes.POINTER(TraceConfig), ctypes.POINTER(ctypes.c_double)]),
+        "spk_set_camera": (I, [P, ctypes.POINTER(CameraDesc)]),
+        "spk_render_frame": (I, [P, ctypes.POINTER(FrameDesc), P, P, ctypes.POINTER(FrameStats)]),
+        "spk_accumulate": (I, [P, P, P, U64, U32, U32]),
+        "spk_tonemap": (I, [P, P, P, U32, U32, F]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return int(a.data_ptr())  # torch tensor
+
+
+class Scene:
+    """Host description of a scene: meshes (numpy arrays), instances and materials."""
+
+    def __init__(self):
+        self.meshes = []
+        self.instances = []
+        self.materials = []
+
+    def add_mesh(self, positions, indices, normals=None, tangents=None, uvs=None):
+        m = dict(positions=np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3),
+                 indices=np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3),
+                 normals=None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3),
+                 tangents=None if tangents is None else np.ascontiguousarray(tangents, dtype=np.float32).reshape(-1, 4),
+                 uvs=None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2))
+        self.meshes.append(m)
+        return len(self.meshes) - 1
+
+    def add_material(self, base_color=(1, 1, 1), emissive=(0, 0, 0), metallic=1.0, roughness=1.0, eta=0.0,
+                     type=MATERIAL_PBR):
+        self.materials.append(dict(type=type, base_color=tuple(base_color), emissive=tuple(emissive),
+                                   metallic=float(metallic), roughness=float(roughness), eta=float(eta)))
+        return len(self.materials) - 1
+
+    def add_instance(self, mesh, material, world_from_local=None):
+        M = np.eye(3, 4, dtype=np.float32) if world_from_local is None else np.asarray(world_from_local,
+                                                                                       dtype=np.float32)
+        self.instances.append(dict(mesh=mesh, material=material, M=M.reshape(3, 4)))
+        return len(self.instances) - 1
+
+    @property
+    def num_triangles(self):
+        return sum(self.meshes[i["mesh"]]["indices"].shape[0] for i in self.instances)
+
+    def c_arrays(self):
+        ms = (MeshDesc * len(self.meshes))()
+        for k, m in enumerate(self.meshes):
+            ms[k].positions = _ptr(m["positions"])
+            ms[k].num_vertices = m["positions"].shape[0]
+            ms[k].indices = _ptr(m["indices"])
+            ms[k].num_triangles = m["indices"].shape[0]
+            ms[k].normals = _ptr(m["normals"])
+            ms[k].tangents = _ptr(m["tangents"])
+            ms[k].uvs = _ptr(m["uvs"])
+        ins = (InstanceDesc * len(self.instances))()
+        for k, i in enumerate(self.instances):
+            ins[k].mesh = i["mesh"]
+            ins[k].material = i["material"]
+            ins[k].world_from_local[:] = [float(x) for x in i["M"].reshape(-1)]
+        mats = (MaterialDesc * max(1, len(self.materials)))()
+        for k, m in enumerate(self.materials):
+            mats[k].type = m["type"]
+            mats[k].base_color[:] = m["base_color"]
+            mats[k].emissive[:] = m["emissive"]
+            mats[k].metallic = m["metallic"]
+            mats[k].roughness = m["roughness"]
+            mats[k].eta = m["eta"]
+        return ms, ins, mats
+
+
+def build_check(scene, bvh_width=8, max_leaf_tris=3, num_threads=0):
+    """Host-only BVH build + invariant check (no GPU). Returns (rc, stats, message)."""
+    lib = load_library()
+    ms, ins, _ = scene.c_arrays()
+    opts = BuildOptions(bvh_width, max_leaf_tris, num_threads)
+    st = SceneStats()
+    msg = ctypes.create_string_buffer(512)
+    rc = lib.spk_build_check(ms, len(scene.meshes), ins, len(scene.instances), ctypes.byref(opts),
+                             ctypes.byref(st), msg, 512)
+    return rc, st, msg.value.decode()
+
+
+class Context:
+    """One spk_context on a device; all arrays passed to device calls are torch CUDA tensors."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.spk_create(device, ctypes.byref(h))
+        if rc != SPK_OK:
+            raise SpkError(rc, "spk_create failed")
+        self.h = h
+        self.device = device
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            self.lib.spk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != SPK_OK:
+            raise SpkError(rc, self.lib.spk_last_error(self.h).decode())
+        return rc
+
+    def set_stream(self, stream_ptr):
+        return self._check(self.lib.spk_set_stream(self.h, stream_ptr))
+
+    def stream(self):
+        return self.lib.spk_get_stream(self.h)
+
+    def synchronize(self):
+        return self._check(self.lib.spk_synchronize(self.h))
+
+    def upload_scene(self, scene, bvh_width=8, max_leaf_tris=3, num_threads=0):
+        ms, ins, mats = scene.c_arrays()
+        opts = BuildOptions(bvh_width, max_leaf_tris, num_threads)
+        self._check(self.lib.spk_scene_upload(self.h, ms, len(scene.meshes), ins, len(scene.instances), mats,
+                                              len(scene.materials), ctypes.byref(opts)))
+        return self.stats()
+
+    def stats(self):
+        st = SceneStats()
+        self._check(self.lib.spk_scene_stats_get(self.h, ctypes.byref(st)))
+        return st
+
+    def trace(self, rays_dev, hits_dev, n=None, sync=True):
+        """rays_dev: torch uint8/float tensor holding n 32-byte records; hits_dev: n 8-byte records."""
+        if n is None:
+            n = rays_dev.numel() * rays_dev.element_size() // 32
+        fn = self.lib.spk_trace_closest if sync else self.lib.spk_trace_closest_async
+        return fn(self.h, _ptr(rays_dev), n, _ptr(hits_dev))
+
+    def trace_checked(self, rays_dev, hits_dev, n=None):
+        return self._check(self.trace(rays_dev, hits_dev, n))
+
+    def trace_host(self, rays):
+        """rays: numpy RAY_DTYPE array. Returns (rc, hits numpy HIT_DTYPE array)."""
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        rc = self.lib.spk_trace_closest_host(self.h, rays.ctypes.data, rays.shape[0], hits.ctypes.data)
+        return rc, hits
+
+    def trace_count(self, rays_dev, nodes_dev, tris_dev, n=None):
+        if n is None:
+            n = rays_dev.numel() * rays_dev.element_size() // 32
+        return self._check(self.lib.spk_trace_count(self.h, _ptr(rays_dev), n, _ptr(nodes_dev), _ptr(tris_dev)))
+
+    def trace_config(self):
+        c = TraceConfig()
+        self._check(self.lib.spk_get_trace_config(self.h, ctypes.byref(c)))
+        return c.as_tuple()
+
+    def set_trace_config(self, bvh_width, stack, block):
+        c = TraceConfig(bvh_width, stack, block)
+        return self._check(self.lib.spk_set_trace_config(self.h, ctypes.byref(c)))
+
+    def list_trace_configs(self):
+        arr = (TraceConfig * 64)()
+        cnt = ctypes.c_uint32()
+        self._check(self.lib.spk_list_trace_configs(self.h, arr, 64, ctypes.byref(cnt)))
+        return [arr[i].as_tuple() for i in range(min(cnt.value, 64))]
+
+    def autotune(self, rays_dev, n=None, tune_width=False):
+        if n is None:
+            n = rays_dev.numel() * rays_dev.element_size() // 32
+        c = TraceConfig()
+        ms = ctypes.c_double()
+        self._check(self.lib.spk_autotune(self.h, _ptr(rays_dev), n, TUNE_WIDTH if tune_width else 0,
+                                          ctypes.byref(c), ctypes.byref(ms)))
+        return c.as_tuple(), ms.value
+
+    def set_camera(self, cam):
+        return self._check(self.lib.spk_set_camera(self.h, ctypes.byref(cam)))
+
+    def render_frame(self, frame, rgba_dev, normal_dev=None):
+        st = FrameStats()
+        self._check(self.lib.spk_render_frame(self.h, ctypes.byref(frame), _ptr(rgba_dev), _ptr(normal_dev),
+                                              ctypes.byref(st)))
+        return st
+
+    def accumulate(self, rgba_dev, accum_dev, num_pixels, cumulated, spp=1):
+        return self._check(self.lib.spk_accumulate(self.h, _ptr(rgba_dev), _ptr(accum_dev), num_pixels,
+                                                   cumulated, spp))
+
+    def tonemap(self, accum_dev, out_dev, width, height, exposure=1.0):
+        return self._check(self.lib.spk_tonemap(self.h, _ptr(accum_dev), _ptr(out_dev), width, height, exposure))
+
+
+def camera_desc(position, front, up_hint=(0.0, 0.0, 1.0), vertical_fov=None, focus_distance=1.0,
+                aperture_radius=0.0, aspect=1.0, far_plane=1000.0):
+    """Builds the camera record SetupViewRay consumes (CameraRenderProxy.cpp:24-27, 113-136) from a
+    position, a viewing direction and the physical attributes, in float32 in the reference's order."""
+    f32 = np.float32
+    P = np.asarray(position, dtype=f32)
+    F = np.asarray(front, dtype=np.float64)
+    F = (F / np.linalg.norm(F)).astype(f32)
+    R = np.cross(F.astype(np.float64), np.asarray(up_hint, dtype=np.float64))
+    R = (R / np.linalg.norm(R)).astype(f32)
+    U = np.cross(R.astype(np.float64), F.astype(np.float64))
+    U = (U / np.linalg.norm(U)).astype(f32)
+    if vertical_fov is None:
+        # CameraComponent defaults: 35 mm focal length, 24 mm sensor height (CameraComponent.h:18-22)
+        vertical_fov = 2.0 * np.arctan(0.024 / (2.0 * 0.035))
+    h = f32(np.tan(f32(vertical_fov) / f32(2.0)))
+    height = f32(2.0) * h * f32(focus_distance)
+    width = f32(aspect) * height
+    mu = R * width
+    mv = U * height
+    ll = P + F * f32(focus_distance) - mu * f32(0.5) - mv * f32(0.5)
+    c = CameraDesc()
+    c.position[:] = [float(x) for x in P]
+    c.right[:] = [float(x) for x in R]
+    c.up[:] = [float(x) for x in U]
+    c.front[:] = [float(x) for x in F]
+    c.lower_left[:] = [float(x) for x in ll]
+    c.max_u[:] = [float(x) for x in mu]
+    c.max_v[:] = [float(x) for x in mv]
+    c.lens_radius = float(aperture_radius)
+    c.far_plane = float(far_plane)
+    return c

@@ -1,0 +1,81 @@
+// Internal state behind the opaque spk_context handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/spk_pt.h"
+#include "bvh_build.h"
+#include "spk_kernels.h"
+
+namespace spk {
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer();
+    void release();
+    hipError_t ensure(size_t want);
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(ptr);
+    }
+};
+
+struct ShadeSource {
+    uint32_t inst, face;
+};
+
+// Host copy of the flattened scene and its BVH (kept for statistics and re-validation).
+struct HostScene {
+    std::vector<float> verts;
+    std::vector<uint32_t> global_ids, materials, local_ids;
+    std::vector<ShadeSource> shade_src;
+    std::vector<TriShade> shade_in;  // per input triangle
+    BuildInput input;
+    BuildOutput bvh;
+};
+
+struct DeviceScene {
+    DeviceBuffer nodes, tris, shade, materials, emitters;
+    uint32_t num_tris = 0, num_nodes = 0, num_materials = 0, num_emitters = 0;
+    int width = 8, depth = 0;
+    TraceVariant variant{8, 16, 128};
+    bool valid = false;
+    std::vector<uint32_t> host_emitters;
+};
+
+struct PathState;  // path-tracer buffers, defined with the path-tracer kernels
+
+}  // namespace spk
+
+struct spk_context {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string error;
+    spk::DeviceBuffer counters;  // [0] invalid rays
+    spk::DeviceBuffer tmp_rays, tmp_hits;
+    spk::DeviceScene scene;
+    spk::HostScene host_scene;
+    std::vector<spk::MaterialGpu> host_mats;
+    spk_scene_stats stats{};
+    spk_camera_desc camera{};
+    bool has_camera = false;
+    spk::PathState *paths = nullptr;
+
+    uint32_t *invalid_counter() { return static_cast<uint32_t *>(counters.ptr); }
+    int check_invalid_rays();
+    void on_scene_changed();
+    void release_all();
+};
+
+namespace spk {
+int set_error(spk_context *ctx, int code, const char *fmt, ...);
+int hip_fail(spk_context *ctx, hipError_t e, const char *what);
+}  // namespace spk

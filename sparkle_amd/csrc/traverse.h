@@ -1,0 +1,347 @@
+// Device-side closest-hit traversal of the quantised wide BVH, shared by the closest-hit kernel
+// and the path tracer's extend stage.
+//
+// Replaces SceneRenderProxy::Intersect<false> -> TLAS/BLAS bvh::v2 traversal ->
+// MeshRenderProxy::IntersectTriangle (SceneRenderProxy.cpp:82-126, MeshRenderProxy.cpp:94-194)
+// for flattened world-space geometry, with two deliberate differences:
+//   * the triangle test is the watertight FP32 test of Woop, Benthin and Wald (JCGT 2013)
+//     instead of Moller-Trumbore with an absolute |det| < 1e-6 cutoff, which rejects every cell of
+//     a 1024^2 heightfield over the unit square;
+//   * exact-t ties go to the smaller global triangle id rather than to the first leaf visited,
+//     so the result does not depend on tree shape or visit order.
+// The arithmetic of the triangle test and of the final hit distance is written operation for
+// operation like the CPU oracle and compiled without FMA contraction, so hit ids and t are
+// bit-identical to it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spk_layout.h"
+
+namespace spk {
+
+struct WtSetup {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ float vsel(int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ __forceinline__ WtSetup wt_setup(float dx, float dy, float dz) {
+    WtSetup r;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > fmaxf(ax, ay)) kz = 2;
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    const float dkz = vsel(kz, dx, dy, dz);
+    if (dkz < 0.f) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.Sx = vsel(kx, dx, dy, dz) / dkz;
+    r.Sy = vsel(ky, dx, dy, dz) / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+
+// Ray translated and sheared so that the test per triangle needs only selects, no divisions
+// until a hit is accepted.
+struct TriTest {
+    bool hit;
+    float t, U, V, W, det;
+};
+
+__device__ __forceinline__ TriTest tri_wt(const WtSetup &r, float ox, float oy, float oz, const float4 a,
+                                          const float4 b, const float4 c) {
+    TriTest res;
+    res.hit = false;
+    const float Ax0 = a.x - ox, Ay0 = a.y - oy, Az0 = a.z - oz;
+    const float Bx0 = b.x - ox, By0 = b.y - oy, Bz0 = b.z - oz;
+    const float Cx0 = c.x - ox, Cy0 = c.y - oy, Cz0 = c.z - oz;
+    const float Akz = vsel(r.kz, Ax0, Ay0, Az0), Bkz = vsel(r.kz, Bx0, By0, Bz0), Ckz = vsel(r.kz, Cx0, Cy0, Cz0);
+    const float Ax = vsel(r.kx, Ax0, Ay0, Az0) - r.Sx * Akz;
+    const float Ay = vsel(r.ky, Ax0, Ay0, Az0) - r.Sy * Akz;
+    const float Bx = vsel(r.kx, Bx0, By0, Bz0) - r.Sx * Bkz;
+    const float By = vsel(r.ky, Bx0, By0, Bz0) - r.Sy * Bkz;
+    const float Cx = vsel(r.kx, Cx0, Cy0, Cz0) - r.Sx * Ckz;
+    const float Cy = vsel(r.ky, Cx0, Cy0, Cz0) - r.Sy * Ckz;
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return res;
+    const float det = U + V + W;
+    if (det == 0.f) return res;
+    const float Az = r.Sz * Akz;
+    const float Bz = r.Sz * Bkz;
+    const float Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    if ((det < 0.f && T > 0.f) || (det > 0.f && T < 0.f)) return res;
+    res.hit = true;
+    res.t = T / det;
+    res.U = U;
+    res.V = V;
+    res.W = W;
+    res.det = det;
+    return res;
+}
+
+// Hit distance of the selected triangle, n.(v0 - o) / n.d, with v0 - o kept exactly as a hi+lo
+// pair (TwoSum) and both dot products compensated (Dot2, Ogita, Rump and Oishi 2005).
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void two_prod(float a, float b, float &p, float &e) {
+    p = a * b;
+    e = __builtin_fmaf(a, b, -p);
+}
+__device__ __forceinline__ float dot2(float ax, float ay, float az, float bx, float by, float bz, float lx, float ly,
+                                      float lz) {
+    float p, s, h, r, q;
+    two_prod(ax, bx, p, s);
+    two_prod(ay, by, h, r);
+    two_sum(p, h, p, q);
+    s = s + (q + r);
+    two_prod(az, bz, h, r);
+    two_sum(p, h, p, q);
+    s = s + (q + r);
+    s = s + ((ax * lx + ay * ly) + az * lz);
+    return p + s;
+}
+__device__ __forceinline__ float refine_t(float ox, float oy, float oz, float dx, float dy, float dz, const float4 v0,
+                                          const float4 v1, const float4 v2, float t_fallback) {
+    const float e1x = v1.x - v0.x, e1y = v1.y - v0.y, e1z = v1.z - v0.z;
+    const float e2x = v2.x - v0.x, e2y = v2.y - v0.y, e2z = v2.z - v0.z;
+    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    float hx, lx, hy, ly, hz, lz;
+    two_sum(v0.x, -ox, hx, lx);
+    two_sum(v0.y, -oy, hy, ly);
+    two_sum(v0.z, -oz, hz, lz);
+    const float num = dot2(nx, ny, nz, hx, hy, hz, lx, ly, lz);
+    const float den = dot2(nx, ny, nz, dx, dy, dz, 0.f, 0.f, 0.f);
+    if (den == 0.f) return t_fallback;
+    const float t = num / den;
+    if (!(t > 0.f) || !isfinite(t)) return t_fallback;
+    return t;
+}
+
+__device__ __forceinline__ float exp2i(int e) { return __int_as_float((e + 127) << 23); }
+__device__ __forceinline__ float ubyte(uint32_t w, int k) {
+    return (float)((w >> (8 * k)) & 0xffu);  // v_cvt_f32_ubyte{k}
+}
+
+// Slack factor of the slab test: covers the rounding of t = q*scale + (p - o)*invd at any
+// distance; the builder adds one quantisation step of slack in space on each side of every box.
+#define SPK_BOX_SLACK 1.00000095367431640625f /* 1 + 2^-20 */
+
+struct RayIn {
+    float ox, oy, oz, tmin, dx, dy, dz, tmax;
+};
+
+struct BestHit {
+    float t;           // Woop t of the best hit so far (selection and culling)
+    uint32_t prim;     // global triangle id, 0xFFFFFFFF when none
+    uint32_t tri;      // leaf-order index of the best triangle
+    float U, V, W, det;
+};
+
+struct TraverseCounters {
+    uint32_t nodes;
+    uint32_t tris;
+};
+
+__device__ __forceinline__ bool ray_valid(const RayIn &r) {
+    const bool finite = isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) && isfinite(r.dy) &&
+                        isfinite(r.dz) && !isnan(r.tmin) && !isnan(r.tmax);
+    return finite && (r.dx != 0.f || r.dy != 0.f || r.dz != 0.f);
+}
+
+__device__ __forceinline__ float safe_inv(float d) {
+    const float dd = fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d;
+    return __builtin_amdgcn_rcpf(dd);
+}
+
+// One closest-hit query over a wide BVH. STACK entries per lane live in LDS at
+// stack[k * stride]; each entry is a node group {child_base, imask << 8 | pending-hit bits}.
+template <int WIDTH, int STACK, bool COUNT>
+__device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const uint4 *__restrict__ tris,
+                                         uint32_t root, const RayIn &ray, uint2 *stack, int stride, BestHit &best,
+                                         TraverseCounters &cnt) {
+    const float ox = ray.ox, oy = ray.oy, oz = ray.oz;
+    const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
+    const WtSetup wt = wt_setup(dx, dy, dz);
+    const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+    const uint32_t oct = (dx < 0.f ? 1u : 0u) | (dy < 0.f ? 2u : 0u) | (dz < 0.f ? 4u : 0u);
+    const float box_tmin = fmaxf(ray.tmin, 0.f);
+
+    // current node group: slot s of the group's node was hit iff bit (s ^ oct) of `hits` is set
+    uint32_t g_base = root, g_imask = 1u, g_hits;
+    if (WIDTH == 8) {
+        g_hits = 1u << oct;
+    } else {
+        g_hits = 1u;  // 4-wide: rank order is resolved per node below
+    }
+    uint32_t g_oct = WIDTH == 8 ? oct : 0u;
+    int sp = 0;
+
+    for (;;) {
+        if (g_hits == 0u) {
+            if (sp == 0) break;
+            --sp;
+            const uint2 e = stack[sp * stride];
+            g_base = e.x;
+            g_imask = (e.y >> 8) & 0xffu;
+            g_hits = (e.y >> 16) & 0xffu;
+            g_oct = e.y & 0xffu;
+            continue;
+        }
+        const uint32_t r = __builtin_ctz(g_hits);
+        g_hits &= g_hits - 1u;
+        const uint32_t slot = r ^ g_oct;
+        const uint32_t child = g_base + __builtin_popcount(g_imask & ((1u << slot) - 1u));
+        if (g_hits != 0u) {
+            stack[sp * stride] = make_uint2(g_base, (g_imask << 8) | (g_hits << 16) | g_oct);
+            ++sp;
+        }
+        if (COUNT) cnt.nodes++;
+
+        const float tcull = best.t * SPK_BOX_SLACK;
+        uint32_t new_hits = 0u, tri_bits = 0u;
+        uint32_t child_base, tri_base, imask, node_oct;
+        if (WIDTH == 8) {
+            const uint4 *np = nodes + (size_t)child * 5u;
+            const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+            const float px = __uint_as_float(n0.x), py = __uint_as_float(n0.y), pz = __uint_as_float(n0.z);
+            const int ex = (int)(int8_t)(n0.w & 0xffu), ey = (int)(int8_t)((n0.w >> 8) & 0xffu),
+                      ez = (int)(int8_t)((n0.w >> 16) & 0xffu);
+            imask = n0.w >> 24;
+            child_base = n1.x;
+            tri_base = n1.y;
+            const float sx = exp2i(ex) * ix, sy = exp2i(ey) * iy, sz = exp2i(ez) * iz;
+            const float ax = (px - ox) * ix, ay = (py - oy) * iy, az = (pz - oz) * iz;
+            // near planes are qlo for a positive direction, qhi for a negative one
+            const uint32_t nx0 = dx < 0.f ? n3.z : n2.x, nx1 = dx < 0.f ? n3.w : n2.y;
+            const uint32_t fx0 = dx < 0.f ? n2.x : n3.z, fx1 = dx < 0.f ? n2.y : n3.w;
+            const uint32_t ny0 = dy < 0.f ? n4.x : n2.z, ny1 = dy < 0.f ? n4.y : n2.w;
+            const uint32_t fy0 = dy < 0.f ? n2.z : n4.x, fy1 = dy < 0.f ? n2.w : n4.y;
+            const uint32_t nz0 = dz < 0.f ? n4.z : n3.x, nz1 = dz < 0.f ? n4.w : n3.y;
+            const uint32_t fz0 = dz < 0.f ? n3.x : n4.z, fz1 = dz < 0.f ? n3.y : n4.w;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int k = s & 3;
+                const uint32_t wnx = s < 4 ? nx0 : nx1, wfx = s < 4 ? fx0 : fx1;
+                const uint32_t wny = s < 4 ? ny0 : ny1, wfy = s < 4 ? fy0 : fy1;
+                const uint32_t wnz = s < 4 ? nz0 : nz1, wfz = s < 4 ? fz0 : fz1;
+                const float tnx = __builtin_fmaf(ubyte(wnx, k), sx, ax);
+                const float tny = __builtin_fmaf(ubyte(wny, k), sy, ay);
+                const float tnz = __builtin_fmaf(ubyte(wnz, k), sz, az);
+                const float tfx = __builtin_fmaf(ubyte(wfx, k), sx, ax);
+                const float tfy = __builtin_fmaf(ubyte(wfy, k), sy, ay);
+                const float tfz = __builtin_fmaf(ubyte(wfz, k), sz, az);
+                const float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), box_tmin);
+                const float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * SPK_BOX_SLACK, tcull);
+                const uint32_t meta = ((s < 4 ? n1.z : n1.w) >> (8 * k)) & 0xffu;
+                const bool inner = (imask >> s) & 1u;
+                const bool hit = tnear <= tfar && (inner || meta != 0u);
+                if (hit) {
+                    if (inner) {
+                        new_hits |= 1u << (s ^ oct);
+                    } else {
+                        tri_bits |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
+                    }
+                }
+            }
+            node_oct = oct;
+        } else {
+            const uint4 *np = nodes + (size_t)child * 4u;
+            const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            const float px = __uint_as_float(n0.x), py = __uint_as_float(n0.y), pz = __uint_as_float(n0.z);
+            const int ex = (int)(int8_t)(n0.w & 0xffu), ey = (int)(int8_t)((n0.w >> 8) & 0xffu),
+                      ez = (int)(int8_t)((n0.w >> 16) & 0xffu);
+            imask = n0.w >> 24;
+            child_base = n1.x;
+            tri_base = n1.y;
+            const uint32_t meta_w = n1.z;
+            // n1.w: qlo_x, n2.x: qlo_y, n2.y: qlo_z, n2.z: qhi_x, n2.w: qhi_y, n3.x: qhi_z, n3.y: axis
+            const float sx = exp2i(ex) * ix, sy = exp2i(ey) * iy, sz = exp2i(ez) * iz;
+            const float ax = (px - ox) * ix, ay = (py - oy) * iy, az = (pz - oz) * iz;
+            const uint32_t wnx = dx < 0.f ? n2.z : n1.w, wfx = dx < 0.f ? n1.w : n2.z;
+            const uint32_t wny = dy < 0.f ? n2.w : n2.x, wfy = dy < 0.f ? n2.x : n2.w;
+            const uint32_t wnz = dz < 0.f ? n3.x : n2.y, wfz = dz < 0.f ? n2.y : n3.x;
+            const uint32_t axis = n3.y;
+            const float dax = axis == 0u ? dx : (axis == 1u ? dy : dz);
+            node_oct = dax < 0.f ? 3u : 0u;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const float tnx = __builtin_fmaf(ubyte(wnx, s), sx, ax);
+                const float tny = __builtin_fmaf(ubyte(wny, s), sy, ay);
+                const float tnz = __builtin_fmaf(ubyte(wnz, s), sz, az);
+                const float tfx = __builtin_fmaf(ubyte(wfx, s), sx, ax);
+                const float tfy = __builtin_fmaf(ubyte(wfy, s), sy, ay);
+                const float tfz = __builtin_fmaf(ubyte(wfz, s), sz, az);
+                const float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), box_tmin);
+                const float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * SPK_BOX_SLACK, tcull);
+                const uint32_t meta = (meta_w >> (8 * s)) & 0xffu;
+                const bool inner = (imask >> s) & 1u;
+                const bool hit = tnear <= tfar && (inner || meta != 0u);
+                if (hit) {
+                    if (inner) {
+                        new_hits |= 1u << (s ^ node_oct);
+                    } else {
+                        tri_bits |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
+                    }
+                }
+            }
+        }
+        g_base = child_base;
+        g_imask = imask;
+        g_hits = new_hits;
+        g_oct = node_oct;
+
+        // leaf triangles of this node
+        while (tri_bits != 0u) {
+            const uint32_t k = __builtin_ctz(tri_bits);
+            tri_bits &= tri_bits - 1u;
+            const uint32_t ti = tri_base + k;
+            const uint4 *tp = tris + (size_t)ti * 3u;
+            const uint4 a = tp[0], b = tp[1], c = tp[2];
+            if (COUNT) cnt.tris++;
+            const float4 fa = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), 0.f);
+            const float4 fb = make_float4(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), 0.f);
+            const float4 fc = make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), 0.f);
+            const TriTest tt = tri_wt(wt, ox, oy, oz, fa, fb, fc);
+            if (tt.hit && tt.t > box_tmin) {
+                const uint32_t gid = a.w;
+                if (tt.t < best.t || (tt.t == best.t && gid < best.prim)) {
+                    best.t = tt.t;
+                    best.prim = gid;
+                    best.tri = ti;
+                    best.U = tt.U;
+                    best.V = tt.V;
+                    best.W = tt.W;
+                    best.det = tt.det;
+                }
+            }
+        }
+    }
+}
+
+// refined distance of the final hit (bit-identical to the oracle's)
+__device__ __forceinline__ float final_t(const uint4 *__restrict__ tris, const RayIn &ray, const BestHit &best) {
+    const uint4 *tp = tris + (size_t)best.tri * 3u;
+    const uint4 a = tp[0], b = tp[1], c = tp[2];
+    const float4 fa = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), 0.f);
+    const float4 fb = make_float4(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), 0.f);
+    const float4 fc = make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), 0.f);
+    return refine_t(ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, fa, fb, fc, best.t);
+}
+
+}  // namespace spk

@@ -1,0 +1,282 @@
+"""Seeded synthetic workloads: the procedural heightfield with its coherent and incoherent ray sets,
+the Cornell-style room, and the many-instance scene. All numbers are generated in float64 with
+numpy's PCG64 and rounded once to float32, so every run and every machine builds identical inputs.
+"""
+import math
+
+import numpy as np
+
+from . import RAY_DTYPE, Scene, camera_desc, MATERIAL_PBR, MATERIAL_DIELECTRIC
+
+
+# ---------------------------------------------------------------------------------------------
+# heightfield: 1024 x 1024 cells over the unit square, 6-octave value noise, amplitude 0.2
+# ---------------------------------------------------------------------------------------------
+def value_noise(n_vert, octaves=6, base_freq=4, seed=0):
+    """Value noise on an n_vert x n_vert grid over [0,1]^2: lattice values U[0,1), smoothstep
+    interpolation, lacunarity 2, gain 0.5, normalised to [0, 1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.linspace(0.0, 1.0, n_vert)
+    total = np.zeros((n_vert, n_vert))
+    amp = 1.0
+    for o in range(octaves):
+        f = base_freq * (2 ** o)
+        lattice = rng.random((f + 1, f + 1))
+        g = x * f
+        i0 = np.minimum(np.floor(g).astype(np.int64), f - 1)
+        t = g - i0
+        s = t * t * (3.0 - 2.0 * t)
+        # rows: y, cols: x
+        v00 = lattice[np.ix_(i0, i0)]
+        v10 = lattice[np.ix_(i0, i0 + 1)]
+        v01 = lattice[np.ix_(i0 + 1, i0)]
+        v11 = lattice[np.ix_(i0 + 1, i0 + 1)]
+        sx = s[None, :]
+        sy = s[:, None]
+        total += amp * ((v00 * (1 - sx) + v10 * sx) * (1 - sy) + (v01 * (1 - sx) + v11 * sx) * sy)
+        amp *= 0.5
+    return (total - total.min()) / (total.max() - total.min())
+
+
+def heightfield_mesh(cells=1024, amplitude=0.2, seed=0):
+    """Vertices (i/cells, j/cells, amplitude*N); two triangles per cell, fixed diagonal;
+    triangle id = 2*(j*cells + i) + k."""
+    nv = cells + 1
+    h = value_noise(nv, seed=seed) * amplitude
+    jj, ii = np.meshgrid(np.arange(nv), np.arange(nv), indexing="ij")
+    pos = np.stack([ii / cells, jj / cells, h], axis=-1).reshape(-1, 3)
+    # normals from central differences, tangents along +x
+    dzdx = np.gradient(h, 1.0 / cells, axis=1)
+    dzdy = np.gradient(h, 1.0 / cells, axis=0)
+    nrm = np.stack([-dzdx, -dzdy, np.ones_like(h)], axis=-1).reshape(-1, 3)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    tan = np.stack([np.ones_like(h), np.zeros_like(h), dzdx], axis=-1).reshape(-1, 3)
+    tan /= np.linalg.norm(tan, axis=1, keepdims=True)
+    tan = np.concatenate([tan, np.ones((tan.shape[0], 1))], axis=1)
+    uv = pos[:, :2].copy()
+    ci, cj = np.meshgrid(np.arange(cells), np.arange(cells), indexing="xy")
+    a = (cj * nv + ci).reshape(-1)
+    b = a + 1
+    c = a + nv + 1
+    d = a + nv
+    idx = np.empty((cells * cells * 2, 3), dtype=np.uint32)
+    idx[0::2] = np.stack([a, b, c], axis=1)
+    idx[1::2] = np.stack([a, c, d], axis=1)
+    return (pos.astype(np.float32), idx, nrm.astype(np.float32), tan.astype(np.float32), uv.astype(np.float32))
+
+
+def heightfield_scene(cells=1024, amplitude=0.2, seed=0):
+    pos, idx, nrm, tan, uv = heightfield_mesh(cells, amplitude, seed)
+    sc = Scene()
+    m = sc.add_mesh(pos, idx, nrm, tan, uv)
+    mat = sc.add_material(base_color=(0.6, 0.6, 0.6), metallic=0.0, roughness=1.0)
+    sc.add_instance(m, mat)
+    return sc
+
+
+def scene_aabb(scene):
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    for ins in scene.instances:
+        p = scene.meshes[ins["mesh"]]["positions"].astype(np.float64)
+        M = ins["M"].astype(np.float64)
+        w = p @ M[:, :3].T + M[:, 3]
+        lo = np.minimum(lo, w.min(0))
+        hi = np.maximum(hi, w.max(0))
+    return lo, hi
+
+
+def incoherent_rays(aabb_lo, aabb_hi, n=8 * 1024 * 1024, seed=0):
+    """Origins uniform in the AABB, directions uniform on the sphere (z = 1 - 2u, phi = 2*pi*v),
+    tmin = 0, tmax = +inf."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo = np.asarray(aabb_lo, dtype=np.float64)
+    ext = np.asarray(aabb_hi, dtype=np.float64) - lo
+    rays = np.empty(n, dtype=RAY_DTYPE)
+    chunk = 1 << 22
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        m = e - s
+        o = lo + rng.random((m, 3)) * ext
+        u = rng.random(m)
+        v = rng.random(m)
+        z = 1.0 - 2.0 * u
+        r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+        phi = 2.0 * math.pi * v
+        d = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+        rays["org"][s:e] = o.astype(np.float32)
+        rays["dir"][s:e] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def coherent_rays(width=1920, height=1080, vfov_deg=60.0, pitch_deg=35.0, target=(0.5, 0.5, 0.1), distance=1.35):
+    """Pixel-centre pinhole rays from a camera pitched down at the heightfield centre."""
+    p = math.radians(pitch_deg)
+    front = np.array([0.0, math.cos(p), -math.sin(p)])
+    eye = np.asarray(target, dtype=np.float64) - distance * front
+    right = np.array([1.0, 0.0, 0.0])
+    up = np.cross(right, front)
+    th = math.tan(math.radians(vfov_deg) / 2.0)
+    aspect = width / height
+    i = (np.arange(width) + 0.5) / width
+    j = (np.arange(height) + 0.5) / height
+    jj, ii = np.meshgrid(j, i, indexing="ij")
+    d = (front[None, None, :] + ((2 * ii - 1) * th * aspect)[..., None] * right + ((1 - 2 * jj) * th)[..., None] * up)
+    d = d.reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.empty(width * height, dtype=RAY_DTYPE)
+    rays["org"] = eye.astype(np.float32)
+    rays["dir"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+# ---------------------------------------------------------------------------------------------
+# meshes for the room scenes
+# ---------------------------------------------------------------------------------------------
+def _quad(p0, p1, p2, p3, normal, tangent):
+    pos = np.array([p0, p1, p2, p3], dtype=np.float64)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32)
+    nrm = np.tile(np.asarray(normal, dtype=np.float64), (4, 1))
+    tan = np.tile(np.concatenate([np.asarray(tangent, dtype=np.float64), [1.0]]), (4, 1))
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float64)
+    return pos, idx, nrm, tan, uv
+
+
+def box_mesh(center, half, angle_z):
+    """Axis-aligned box rotated about z, 12 triangles with flat normals."""
+    c, s = math.cos(angle_z), math.sin(angle_z)
+    R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
+    faces = []
+    hx, hy, hz = half
+    corners = lambda sx, sy, sz: np.array([sx * hx, sy * hy, sz * hz])
+    # +x, -x, +y, -y, +z, -z
+    defs = [((1, 0, 0), (0, 1, 0)), ((-1, 0, 0), (0, -1, 0)), ((0, 1, 0), (-1, 0, 0)), ((0, -1, 0), (1, 0, 0)),
+            ((0, 0, 1), (1, 0, 0)), ((0, 0, -1), (1, 0, 0))]
+    pos_all, idx_all, nrm_all, tan_all, uv_all = [], [], [], [], []
+    for n, t in defs:
+        n = np.array(n, dtype=np.float64)
+        t = np.array(t, dtype=np.float64)
+        b = np.cross(n, t)
+        ext = np.array([hx, hy, hz])
+        ctr = n * ext
+        ut = t * ext
+        vb = b * ext
+        quad = [ctr - ut - vb, ctr + ut - vb, ctr + ut + vb, ctr - ut + vb]
+        base = len(pos_all) * 4
+        p, i, nn, tt, uv = _quad(*quad, normal=n, tangent=t)
+        pos_all.append(p)
+        idx_all.append(i + 4 * len(idx_all))
+        nrm_all.append(nn)
+        tan_all.append(tt)
+        uv_all.append(uv)
+    pos = np.concatenate(pos_all) @ R.T + np.asarray(center)
+    nrm = np.concatenate(nrm_all) @ R.T
+    tan = np.concatenate(tan_all)
+    tan[:, :3] = tan[:, :3] @ R.T
+    return pos, np.concatenate(idx_all), nrm, tan, np.concatenate(uv_all)
+
+
+def icosphere(subdiv, radius=1.0, center=(0, 0, 0)):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in verts]
+    f = faces
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                cache[key] = len(v) - 1
+            return cache[key]
+
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    V = np.array(v)
+    nrm = V.copy()
+    # tangent: d/dphi direction, fall back to +x at the poles
+    tan = np.cross(np.array([0.0, 0.0, 1.0]), nrm)
+    ln = np.linalg.norm(tan, axis=1, keepdims=True)
+    tan = np.where(ln > 1e-6, tan / np.maximum(ln, 1e-12), np.array([1.0, 0.0, 0.0]))
+    tan = np.concatenate([tan, np.ones((tan.shape[0], 1))], axis=1)
+    uv = np.stack([np.arctan2(V[:, 1], V[:, 0]) / (2 * math.pi) + 0.5, np.arccos(np.clip(V[:, 2], -1, 1)) / math.pi],
+                  axis=1)
+    return V * radius + np.asarray(center), np.array(f, dtype=np.uint32), nrm, tan, uv
+
+
+def torus(R, r, n_major, n_minor, center=(0, 0, 0)):
+    u = np.linspace(0, 2 * math.pi, n_major, endpoint=False)
+    v = np.linspace(0, 2 * math.pi, n_minor, endpoint=False)
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    x = (R + r * np.cos(vv)) * np.cos(uu)
+    y = (R + r * np.cos(vv)) * np.sin(uu)
+    z = r * np.sin(vv)
+    pos = np.stack([x, y, z], -1).reshape(-1, 3) + np.asarray(center)
+    nrm = np.stack([np.cos(vv) * np.cos(uu), np.cos(vv) * np.sin(uu), np.sin(vv)], -1).reshape(-1, 3)
+    tan = np.stack([-np.sin(uu), np.cos(uu), np.zeros_like(uu)], -1).reshape(-1, 3)
+    tan = np.concatenate([tan, np.ones((tan.shape[0], 1))], axis=1)
+    uv = np.stack([uu / (2 * math.pi), vv / (2 * math.pi)], -1).reshape(-1, 2)
+    i = np.arange(n_major)[:, None]
+    j = np.arange(n_minor)[None, :]
+    a = (i * n_minor + j)
+    b = (((i + 1) % n_major) * n_minor + j)
+    c = (((i + 1) % n_major) * n_minor + (j + 1) % n_minor)
+    d = (i * n_minor + (j + 1) % n_minor)
+    idx = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    return pos, idx.astype(np.uint32), nrm, tan, uv
+
+
+def _add(sc, mesh, material):
+    pos, idx, nrm, tan, uv = mesh
+    m = sc.add_mesh(pos, idx, nrm, tan, uv)
+    return sc.add_instance(m, material)
+
+
+def cornell_scene(seed=0):
+    """Room of 5 walls (10 triangles), a 1x1 area light (2 triangles, radiance 15) below the ceiling,
+    two boxes rotated by seed-drawn angles in +-30 degrees (24 triangles) and a 3x subdivided
+    icosphere of radius 0.3 (1280 triangles). Lambertian albedos from U[0.2, 0.8]^3 and one GGX metal
+    with roughness 0.3. The room spans [-1, 1]^2 x [0, 2]; z is up, the camera looks along +y."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+
+    def lambert():
+        return sc.add_material(base_color=tuple(rng.uniform(0.2, 0.8, 3)), metallic=0.0, roughness=1.0)
+
+    m_floor, m_ceil, m_back, m_left, m_right = lambert(), lambert(), lambert(), lambert(), lambert()
+    m_box1, m_box2 = lambert(), lambert()
+    m_metal = sc.add_material(base_color=(0.9, 0.75, 0.5), metallic=1.0, roughness=0.3)
+    m_light = sc.add_material(base_color=(1, 1, 1), emissive=(15.0, 15.0, 15.0), metallic=0.0, roughness=1.0)
+    L = 1.0
+    _add(sc, _quad((-L, -L, 0), (L, -L, 0), (L, L, 0), (-L, L, 0), (0, 0, 1), (1, 0, 0)), m_floor)
+    _add(sc, _quad((-L, -L, 2), (-L, L, 2), (L, L, 2), (L, -L, 2), (0, 0, -1), (1, 0, 0)), m_ceil)
+    _add(sc, _quad((-L, L, 0), (L, L, 0), (L, L, 2), (-L, L, 2), (0, -1, 0), (1, 0, 0)), m_back)
+    _add(sc, _quad((-L, -L, 0), (-L, L, 0), (-L, L, 2), (-L, -L, 2), (1, 0, 0), (0, 1, 0)), m_left)
+    _add(sc, _quad((L, -L, 0), (L, -L, 2), (L, L, 2), (L, L, 0), (-1, 0, 0), (0, 1, 0)), m_right)
+    _add(sc, _quad((-0.5, -0.5, 1.99), (-0.5, 0.5, 1.99), (0.5, 0.5, 1.99), (0.5, -0.5, 1.99), (0, 0, -1),
+                   (1, 0, 0)), m_light)
+    a1, a2 = np.radians(rng.uniform(-30, 30, 2))
+    _add(sc, box_mesh((-0.4, 0.3, 0.6), (0.3, 0.3, 0.6), a1), m_box1)
+    _add(sc, box_mesh((0.45, -0.2, 0.3), (0.3, 0.3, 0.3), a2), m_box2)
+    _add(sc, icosphere(3, radius=0.3, center=(0.4, -0.25, 0.9)), m_metal)
+    return sc
+
+
+def cornell_camera(width=256, height=256):
+    # camera outside the open front of the room, looking along +y
+    # CameraComponent defaults: 35 mm lens on a 24 mm sensor at f/22 -> 37.85 deg, radius 7.95e-4
+    return camera_desc(position=(0.0, -3.4, 1.0), front=(0.0, 1.0, 0.0), vertical_fov=None, focus_distance=3.4,
+                       aperture_radius=0.035 / 22.0 * 0.5, aspect=width / height)

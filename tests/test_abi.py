@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every symbol include/spk_pt.h declares (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import sparkle_amd as spk
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "spk_pt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(spk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_all_declared_symbols():
+    lib = spk.load_library()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_header_has_no_cpp_or_torch_types():
+    text = open(os.path.join(ROOT, "include", "spk_pt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    for bad in ("std::", "torch", "at::", "hipStream_t", "class "):
+        assert bad not in text
+
+
+def test_struct_sizes_match_header_layout():
+    assert ctypes.sizeof(spk.MeshDesc) == 56
+    assert ctypes.sizeof(spk.InstanceDesc) == 56
+    assert ctypes.sizeof(spk.MaterialDesc) == 40
+    assert ctypes.sizeof(spk.CameraDesc) == 92
+    assert ctypes.sizeof(spk.FrameDesc) == 40
+    assert spk.RAY_DTYPE.itemsize == 32 and spk.HIT_DTYPE.itemsize == 8
+
+
+def test_create_without_device_fails_cleanly():
+    # On a machine without a GPU spk_create must return an error code, never crash.
+    import torch
+
+    if torch.cuda.is_available():
+        return
+    lib = spk.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.spk_create(0, ctypes.byref(h))
+    assert rc in (spk.SPK_ERR_NO_DEVICE, spk.SPK_ERR_HIP)
+
+
+def test_null_arguments_are_rejected():
+    lib = spk.load_library()
+    assert lib.spk_destroy(None) == spk.SPK_ERR_INVALID_ARGUMENT
+    assert lib.spk_set_camera(None, None) == spk.SPK_ERR_INVALID_ARGUMENT
+    assert lib.spk_create(0, None) == spk.SPK_ERR_INVALID_ARGUMENT

@@ -1,0 +1,216 @@
+"""Closest-hit parity of the HIP traversal against the CPU oracle (runs on the MI355X).
+
+Bar: hit ids and t bit-identical to the oracle's watertight FP32 twin on every ray; against the
+FP64 oracle, ids equal on every ray it does not flag (edge within 1e-4 barycentric, t-tie within
+1e-6, grazing below 1e-3) and t within 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import sparkle_amd as spk
+from sparkle_amd import scenes
+from oracle import refcpu
+from tests.conftest import print_case
+from tests.helpers import empty_dev, gpu_trace, hits_from_dev, multi_mesh_scene, soup_scene, to_dev
+
+pytestmark = pytest.mark.gpu
+
+T_REL_TOL = 1e-6
+
+
+def check_against_oracle(ctx, sc, rays, width=8, label=""):
+    print_case("trace parity", label, "width", width, "rays", rays.shape[0])
+    ctx.upload_scene(sc, bvh_width=width)
+    rc, h = gpu_trace(ctx, rays)
+    assert rc == spk.SPK_OK, ctx.lib.spk_last_error(ctx.h)
+    o = refcpu.OracleScene(sc)
+    wt = o.trace(rays, refcpu.TRACE_WT)
+    np.testing.assert_array_equal(h["prim"], wt["prim"])
+    np.testing.assert_array_equal(h["t"].view(np.uint32), wt["t"].view(np.uint32))
+    h64, flags, t64 = o.trace(rays, refcpu.TRACE_FP64)
+    clean = flags == 0
+    np.testing.assert_array_equal(h["prim"][clean], h64["prim"][clean])
+    hit = clean & (h64["prim"] != spk.SPK_MISS)
+    rel = np.abs(h["t"][hit].astype(np.float64) - t64[hit]) / t64[hit]
+    assert rel.size == 0 or rel.max() <= T_REL_TOL, rel.max()
+    return h, flags
+
+
+@pytest.mark.parametrize("width", [8, 4])
+def test_heightfield_incoherent(gpu_ctx, width):
+    sc = scenes.heightfield_scene(cells=128)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=200_000, seed=3)
+    h, _ = check_against_oracle(gpu_ctx, sc, rays, width, "heightfield128 incoherent")
+    assert 0.3 < (h["prim"] != spk.SPK_MISS).mean() < 0.7
+
+
+def test_heightfield_coherent(gpu_ctx):
+    sc = scenes.heightfield_scene(cells=256)
+    rays = scenes.coherent_rays(320, 180)
+    h, _ = check_against_oracle(gpu_ctx, sc, rays, 8, "heightfield256 coherent")
+    assert (h["prim"] != spk.SPK_MISS).mean() > 0.5
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_triangle_soup(gpu_ctx, seed):
+    sc = soup_scene(5000, seed=seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 10))
+    n = 100_000
+    rays = np.zeros(n, dtype=spk.RAY_DTYPE)
+    rays["org"] = rng.random((n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    rays["dir"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    check_against_oracle(gpu_ctx, sc, rays, 8, f"soup seed {seed}")
+
+
+def test_multi_mesh_ids(gpu_ctx):
+    sc = multi_mesh_scene()
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=50_000, seed=5)
+    h, _ = check_against_oracle(gpu_ctx, sc, rays, 8, "multi mesh")
+    assert h["prim"][h["prim"] != spk.SPK_MISS].max() < sc.num_triangles
+
+
+def test_shared_edge_ties_go_to_smaller_id(gpu_ctx):
+    # rays through the shared diagonal of a unit quad: both triangles report the same t
+    pos = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], dtype=np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32)
+    sc = spk.Scene()
+    sc.add_instance(sc.add_mesh(pos, idx), sc.add_material())
+    s = np.linspace(0.05, 0.95, 64, dtype=np.float32)
+    rays = np.zeros(64, dtype=spk.RAY_DTYPE)
+    rays["org"] = np.stack([s, s, np.ones_like(s)], 1)
+    rays["dir"] = np.array([0, 0, -1], dtype=np.float32)
+    rays["tmax"] = np.inf
+    gpu_ctx.upload_scene(sc)
+    rc, h = gpu_trace(gpu_ctx, rays)
+    assert rc == spk.SPK_OK
+    assert (h["prim"] == 0).all() and (h["t"] == 1.0).all()
+
+
+def test_tmin_tmax_and_empty(gpu_ctx):
+    sc = scenes.heightfield_scene(cells=64)
+    gpu_ctx.upload_scene(sc)
+    rays = np.zeros(3, dtype=spk.RAY_DTYPE)
+    rays["org"] = [[0.5, 0.5, 1.0]] * 3
+    rays["dir"] = [[0, 0, -1.0]] * 3
+    rays["tmin"] = [0.0, 0.0, 2.0]
+    rays["tmax"] = [np.inf, 0.5, np.inf]
+    rc, h = gpu_trace(gpu_ctx, rays)
+    assert rc == spk.SPK_OK
+    assert h["prim"][0] != spk.SPK_MISS and 0.5 < h["t"][0] < 1.0
+    assert h["prim"][1] == spk.SPK_MISS and h["prim"][2] == spk.SPK_MISS
+    # zero rays is a no-op
+    assert gpu_ctx.trace(empty_dev(32), empty_dev(8), 0) == spk.SPK_OK
+
+
+def test_invalid_rays_reported_as_misses(gpu_ctx):
+    sc = scenes.heightfield_scene(cells=64)
+    gpu_ctx.upload_scene(sc)
+    rays = np.zeros(4, dtype=spk.RAY_DTYPE)
+    rays["org"] = [[0.5, 0.5, 1.0], [np.nan, 0.5, 1.0], [0.5, 0.5, 1.0], [0.2, 0.3, 1.0]]
+    rays["dir"] = [[0, 0, -1.0], [0, 0, -1.0], [0, 0, 0], [0, 0, -1.0]]
+    rays["tmax"] = np.inf
+    rc, h = gpu_trace(gpu_ctx, rays)
+    assert rc == spk.SPK_ERR_INVALID_RAY
+    assert h["prim"][1] == spk.SPK_MISS and h["prim"][2] == spk.SPK_MISS
+    assert h["prim"][0] != spk.SPK_MISS and h["prim"][3] != spk.SPK_MISS
+    # the error is reported once; the next call is clean
+    rc, _ = gpu_trace(gpu_ctx, rays[[0, 3]])
+    assert rc == spk.SPK_OK
+
+
+def test_axis_parallel_rays(gpu_ctx):
+    # d.x == 0 rays: the reference's InverseAt gives NaN and drops them; the device test does not
+    sc = scenes.heightfield_scene(cells=64)
+    n = 4096
+    rng = np.random.Generator(np.random.PCG64(9))
+    rays = np.zeros(n, dtype=spk.RAY_DTYPE)
+    rays["org"] = np.concatenate([rng.random((n, 2)), np.full((n, 1), 0.5)], 1).astype(np.float32)
+    d = np.zeros((n, 3))
+    d[:, 1] = rng.normal(size=n)
+    d[:, 2] = -np.abs(rng.normal(size=n))
+    rays["dir"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmax"] = np.inf
+    h, _ = check_against_oracle(gpu_ctx, sc, rays, 8, "axis-parallel")
+    assert (h["prim"] != spk.SPK_MISS).mean() > 0.9
+
+
+def test_all_kernel_variants_agree(gpu_ctx):
+    """Every compiled (width, stack, block) variant gives identical results."""
+    sc = scenes.heightfield_scene(cells=128)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=50_000, seed=11)
+    ref = None
+    seen = 0
+    for width in (8, 4):
+        gpu_ctx.upload_scene(sc, bvh_width=width)
+        for cfg in gpu_ctx.list_trace_configs():
+            print_case("variant", cfg)
+            gpu_ctx.set_trace_config(*cfg)
+            assert gpu_ctx.trace_config() == cfg
+            rc, h = gpu_trace(gpu_ctx, rays)
+            assert rc == spk.SPK_OK
+            if ref is None:
+                ref = h
+            np.testing.assert_array_equal(h["prim"], ref["prim"])
+            np.testing.assert_array_equal(h["t"].view(np.uint32), ref["t"].view(np.uint32))
+            seen += 1
+    assert seen >= 10
+
+
+def test_autotune_keeps_results(gpu_ctx):
+    sc = scenes.heightfield_scene(cells=256)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=100_000, seed=12)
+    gpu_ctx.upload_scene(sc)
+    rc, before = gpu_trace(gpu_ctx, rays)
+    d = to_dev(rays)
+    print_case("autotune")
+    cfg, ms = gpu_ctx.autotune(d, tune_width=True)
+    assert ms > 0 and cfg[0] in (4, 8)
+    assert gpu_ctx.trace_config() == cfg
+    rc, after = gpu_trace(gpu_ctx, rays)
+    np.testing.assert_array_equal(after["prim"], before["prim"])
+    np.testing.assert_array_equal(after["t"].view(np.uint32), before["t"].view(np.uint32))
+
+
+def test_full_heightfield_2m_subsampled(gpu_ctx):
+    """1024^2-cell heightfield (2,097,152 triangles), 8,388,608 incoherent rays traced on the GPU;
+    every 64th ray checked against the oracle; every ray checked for range properties."""
+    sc = scenes.heightfield_scene(cells=1024)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=8 * 1024 * 1024, seed=0)
+    print_case("full heightfield upload")
+    st = gpu_ctx.upload_scene(sc)
+    assert st.num_triangles == 2 * 1024 * 1024
+    print_case("full heightfield trace 8M")
+    rc, h = gpu_trace(gpu_ctx, rays)
+    assert rc == spk.SPK_OK
+    hit = h["prim"] != spk.SPK_MISS
+    assert (h["prim"][hit] < 2 * 1024 * 1024).all()
+    assert (h["t"][hit] > 0).all() and np.isfinite(h["t"][hit]).all()
+    assert np.isinf(h["t"][~hit]).all()
+    sub = np.arange(0, rays.shape[0], 64)
+    o = refcpu.OracleScene(sc)
+    wt = o.trace(rays[sub], refcpu.TRACE_WT)
+    np.testing.assert_array_equal(h["prim"][sub], wt["prim"])
+    np.testing.assert_array_equal(h["t"][sub].view(np.uint32), wt["t"].view(np.uint32))
+    h64, flags, t64 = o.trace(rays[sub], refcpu.TRACE_FP64)
+    clean = flags == 0
+    np.testing.assert_array_equal(h["prim"][sub][clean], h64["prim"][clean])
+    m = clean & (h64["prim"] != spk.SPK_MISS)
+    rel = np.abs(h["t"][sub][m].astype(np.float64) - t64[m]) / t64[m]
+    assert rel.max() <= T_REL_TOL
+    # the coherent 1080p set on the same scene, every 16th ray
+    rays_a = scenes.coherent_rays()
+    print_case("full heightfield trace coherent 1080p")
+    rc, ha = gpu_trace(gpu_ctx, rays_a)
+    assert rc == spk.SPK_OK
+    sub = np.arange(0, rays_a.shape[0], 16)
+    wa = o.trace(rays_a[sub], refcpu.TRACE_WT)
+    np.testing.assert_array_equal(ha["prim"][sub], wa["prim"])
+    np.testing.assert_array_equal(ha["t"][sub].view(np.uint32), wa["t"].view(np.uint32))
