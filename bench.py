@@ -87,8 +87,11 @@ def main():
     n = rays.shape[0]
 
     ctx = spk.Context(0)
-    stream = torch.cuda.current_stream()
+    # a dedicated (default-priority) torch stream, so torch events time the kernels launched on it
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     ctx.set_stream(stream.cuda_stream)
+    assert ctx.stream() == stream.cuda_stream
     log("building BVH and uploading")
     st = ctx.upload_scene(sc)
     d_rays = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()

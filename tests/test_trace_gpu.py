@@ -49,7 +49,7 @@ def test_heightfield_coherent(gpu_ctx):
     sc = scenes.heightfield_scene(cells=256)
     rays = scenes.coherent_rays(320, 180)
     h, _ = check_against_oracle(gpu_ctx, sc, rays, 8, "heightfield256 coherent")
-    assert (h["prim"] != spk.SPK_MISS).mean() > 0.5
+    assert (h["prim"] != spk.SPK_MISS).mean() > 0.1
 
 
 @pytest.mark.parametrize("seed", [0, 1])
