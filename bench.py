@@ -70,7 +70,22 @@ def main():
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-coherent", action="store_true")
+    ap.add_argument("--config", type=str, default=None, help="traversal variant width,stack,block")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run N device copies of 256 MiB (counter calibration for the traffic figure)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.copy_calibration):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel(), "copy_bytes_written": b.numel(),
+                          "copies": args.copy_calibration}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by image tiles or scenes, "
               f"see the design notes", file=sys.stderr)
@@ -99,7 +114,12 @@ def main():
     torch.cuda.synchronize()
 
     tune = None
-    if not args.no_autotune:
+    if args.config:
+        w_, s_, b_ = (int(x) for x in args.config.split(","))
+        ctx.set_trace_config(w_, s_, b_)
+        st = ctx.stats()
+        tune = {"bvh_width": w_, "stack": s_, "block": b_, "fixed": True}
+    elif not args.no_autotune:
         log("autotuning traversal variants")
         sub = min(n, 1 << 21)
         cfg, ms = ctx.autotune(d_rays, sub, tune_width=True)
@@ -141,19 +161,21 @@ def main():
     hit_frac = float((hits["prim"] != spk.SPK_MISS).mean())
 
     # coherent 1080p primary rays on the same scene (reported beside the metric)
-    rays_a = scenes.coherent_rays()
-    d_rays_a = torch.from_numpy(rays_a.view(np.uint8).reshape(-1)).cuda()
-    d_hits_a = torch.empty(rays_a.shape[0] * 8, dtype=torch.uint8, device="cuda")
-    for _ in range(2):
-        ctx.trace(d_rays_a, d_hits_a, rays_a.shape[0], sync=False)
-    torch.cuda.synchronize()
-    ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ea.record(stream)
-    for _ in range(10):
-        ctx.trace(d_rays_a, d_hits_a, rays_a.shape[0], sync=False)
-    eb.record(stream)
-    torch.cuda.synchronize()
-    coherent_mrays = rays_a.shape[0] * 10 / (ea.elapsed_time(eb) / 1e3) / 1e6
+    coherent_mrays = None
+    if not args.no_coherent:
+        rays_a = scenes.coherent_rays()
+        d_rays_a = torch.from_numpy(rays_a.view(np.uint8).reshape(-1)).cuda()
+        d_hits_a = torch.empty(rays_a.shape[0] * 8, dtype=torch.uint8, device="cuda")
+        for _ in range(2):
+            ctx.trace(d_rays_a, d_hits_a, rays_a.shape[0], sync=False)
+        torch.cuda.synchronize()
+        ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ea.record(stream)
+        for _ in range(10):
+            ctx.trace(d_rays_a, d_hits_a, rays_a.shape[0], sync=False)
+        eb.record(stream)
+        torch.cuda.synchronize()
+        coherent_mrays = round(rays_a.shape[0] * 10 / (ea.elapsed_time(eb) / 1e3) / 1e6, 1)
 
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     traffic, traffic_src = load_traffic()
@@ -179,7 +201,7 @@ def main():
                     "build_s": round(st.build_seconds, 3)},
             "autotuned": tune,
             "hit_fraction": round(hit_frac, 4),
-            "coherent_1080p_mrays": round(coherent_mrays, 1),
+            "coherent_1080p_mrays": coherent_mrays,
             "mean_nodes_per_ray": round(mean_nodes, 3),
             "mean_tris_per_ray": round(mean_tris, 3),
             "wall_s": round(wall, 4),

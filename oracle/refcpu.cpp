@@ -603,20 +603,33 @@ static inline void wt_consider(WtBest &b, float t, float tmin, uint64_t gid, int
     }
 }
 
+// The watertight twin works on world-space triangles, as the device does: an instance with a
+// transform has its vertices baked with the same float operations as the device upload, and its
+// BVH is walked with the local-space ray only for culling (parametric t is the same in both spaces).
+static inline void world_tri(const Instance &ins, const Mesh &m, uint32_t f, V3 &a, V3 &b, V3 &c) {
+    m.tri(f, a, b, c);
+    if (!ins.to_world.identity()) {
+        a = ins.to_world.point(a);
+        b = ins.to_world.point(b);
+        c = ins.to_world.point(c);
+    }
+}
+
 static void blas_intersect_wt(const orcl_scene &sc, int ii, V3 wo, V3 wd, float tmin, WtBest &best, bool brute) {
     const Instance &ins = sc.instances[ii];
     const Mesh &m = sc.meshes[ins.mesh];
     const bool ident = ins.to_local.identity();
     V3 lo = ident ? wo : ins.to_local.point(wo), ld = ident ? wd : ins.to_local.dir(wd);
-    WtRay wr = wt_setup(lo, ld);
+    WtRay wr = wt_setup(wo, wd);
     auto test = [&](uint32_t f) {
         V3 a, b, c;
-        m.tri(f, a, b, c);
+        world_tri(ins, m, f, a, b, c);
         float t, U, V, W, det;
         if (tri_wt(wr, a, b, c, t, U, V, W, det))
             wt_consider(best, t, tmin, ins.prim_offset + f, ii, f, U, V, W, det);
     };
-    if (brute) {
+    // a transformed instance is tested triangle by triangle (the rounded local ray could cull)
+    if (brute || !ident) {
         for (uint32_t f = 0; f < m.ntri; ++f) test(f);
         return;
     }
@@ -840,11 +853,9 @@ int orcl_trace(const orcl_scene *scene, const orcl_ray *rays, uint64_t num_rays,
                     const Instance &ins = sc.instances[best.inst];
                     const Mesh &m = sc.meshes[ins.mesh];
                     V3 a, bb, c;
-                    m.tri(best.face, a, bb, c);
-                    const bool ident = ins.to_local.identity();
-                    V3 lo = ident ? o : ins.to_local.point(o), ld = ident ? d : ins.to_local.dir(d);
+                    world_tri(ins, m, best.face, a, bb, c);
                     hits[i].prim = (uint32_t)best.gid;
-                    hits[i].t = refine_t(lo, ld, a, bb, c, best.t);
+                    hits[i].t = refine_t(o, d, a, bb, c, best.t);
                 }
             } else if (mode == ORCL_TRACE_FP64) {
                 F64Best best;

@@ -1,16 +1,597 @@
-// Wavefront path tracer (BasePass on the device). Filled in below the closest-hit path.
+// Wavefront path tracer: CPURenderer::BasePass (CPURenderer.cpp:419-450) on the device.
+//
+// One path per pixel per frame, as in the reference. Per bounce the live paths pass through
+//   extend  - closest hit of every queued ray (shared traversal code), plus a material histogram
+//   sort    - counting sort of the queue by material id (scan + scatter)
+//   shade   - SamplePixel's loop body (CPURenderer.cpp:259-368) for each hit, in material order;
+//             survivors are appended to the next bounce's queue
+//   shadow  - occlusion rays of next-event estimation (opt-in, not in the reference)
+// All kernels are persistent: the grid is sized to the device, waves fetch work from counters in
+// device memory, so the whole frame is captured once as a hipGraph and replayed; per-frame
+// values (the frame seed) travel through a small device buffer written before each replay.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "../../include/spk_pt.h"
+#include "shade.h"
 #include "spk_context.h"
+#include "traverse.h"
 
 namespace spk {
-struct PathState {
-    int unused = 0;
+
+constexpr int kMaxBounce = 64;
+constexpr int kCtrStride = 8;     // counters per bounce
+constexpr int kMaxSortKeys = 1024;
+constexpr int kPtBlock = 128;
+
+enum : uint32_t {
+    CTR_QUEUE = 0,    // paths entering this bounce
+    CTR_EXTEND = 1,   // extend work fetch
+    CTR_SHADE = 2,    // shade work fetch
+    CTR_SHADOWS = 3,  // shadow rays emitted at this bounce
+    CTR_SHADOWW = 4,  // shadow work fetch
+    CTR_SCATTER = 5,  // scatter work fetch
 };
+
+struct FrameDev {
+    uint32_t frame_seed;
+    uint32_t pad[3];
+};
+
+struct PtArgs {
+    const uint4 *nodes;
+    const uint4 *tris;
+    const TriShade *shade;
+    const MaterialGpu *mats;
+    const uint32_t *emitters;
+    const float *emitter_cdf;
+    uint32_t num_emitters;
+    float emitter_total;
+    float4 *rays;      // 2 per path: (org, tmin), (dir, tmax)
+    float4 *thr;       // throughput rgb, w = 1 when the next emitter hit must be ignored (NEE)
+    float4 *col;       // colour rgb, w = valid flag
+    uint4 *rng;        // xoshiro128+ state per path
+    uint4 *nee_rng;    // second stream for light sampling
+    uint32_t *queue0;  // ping-pong path queues
+    uint32_t *queue1;
+    uint4 *hits;       // per queue slot: leaf triangle (or miss), t, u, v
+    uint32_t *sorted;  // queue slots in material order
+    float4 *shadow;    // 3 per path: (org, tmax), (dir, path), (contribution)
+    uint32_t *shadow_q;
+    uint32_t *counters;
+    uint32_t *hist;    // per bounce: counts[nkeys], offsets[nkeys]
+    uint32_t nkeys;
+    float *rgba_out;
+    float *normal_out;
+    const FrameDev *frame;
+    spk_camera_desc cam;
+    uint32_t width, height, npix, max_bounce, debug_mode, sky_enabled, flags, sort;
+    float sky[3];
+    uint32_t grid;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// one atomic per wave: the wave's lanes take consecutive items
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t *ctr) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr, 64u);
+    return __shfl(base, 0);
+}
+
+// wave-aggregated append: returns the slot for lanes with `want`
+__device__ __forceinline__ uint32_t wave_append(uint32_t *ctr, bool want) {
+    const uint64_t mask = __ballot(want);
+    const uint32_t cnt = __popcll(mask);
+    uint32_t base = 0;
+    if (lane_id() == 0 && cnt) base = atomicAdd(ctr, cnt);
+    base = __shfl(base, 0);
+    const uint64_t below = lane_id() == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane_id())));
+    return base + (uint32_t)__popcll(below);
+}
+
+// ---- generate: reseed per pixel, jitter, lens, camera ray (CPURenderer.cpp:220-238, 398-445) ----
+__global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
+    const uint32_t tid = blockIdx.x * kPtBlock + threadIdx.x;
+    const uint32_t stride = gridDim.x * kPtBlock;
+    // clear this frame's counters and histograms (no memset nodes in the graph)
+    for (uint32_t i = tid; i < (uint32_t)(kMaxBounce * kCtrStride); i += stride) a.counters[i] = 0u;
+    for (uint32_t i = tid; i < a.max_bounce * 2u * a.nkeys; i += stride) a.hist[i] = 0u;
+    const uint32_t frame_seed = a.frame->frame_seed;
+    const float pw = 1.f / (float)(a.width - 1);
+    const float ph = 1.f / (float)(a.height - 1);
+    for (uint32_t p = tid; p < a.npix; p += stride) {
+        const uint32_t j = p / a.width, i = p - j * a.width;
+        uint64_t seed;
+        if (a.flags & SPK_FRAME_SEED64)
+            seed = (uint64_t)j * a.width + i + (uint64_t)frame_seed * a.width * a.height;
+        else
+            seed = (uint32_t)(j * a.width + i + frame_seed * a.width * a.height);
+        Rng rng = rng_seed(seed);
+        const float u = ((float)i + rng_unit(rng)) * pw;
+        const float v = ((float)j + rng_unit(rng)) * ph;
+        float nx, ny;
+        rng_disk(rng, nx, ny);
+        nx = nx * a.cam.lens_radius;
+        ny = ny * a.cam.lens_radius;
+        const F3 lens = ld3(a.cam.right) * nx + ld3(a.cam.up) * ny;
+        const F3 o = ld3(a.cam.position) + lens;
+        const F3 focus = ld3(a.cam.lower_left) + ld3(a.cam.max_u) * u + ld3(a.cam.max_v) * v;
+        const F3 d = normalized(focus - o);
+        a.rays[2 * p] = make_float4(o.x, o.y, o.z, 0.f);
+        a.rays[2 * p + 1] = make_float4(d.x, d.y, d.z, INFINITY);
+        a.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+        a.col[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+        a.rng[p] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+        if (a.flags & SPK_FRAME_NEE) {
+            const Rng r2 = rng_seed(seed ^ 0x5bd1e9955bd1e995ull);
+            a.nee_rng[p] = make_uint4(r2.s0, r2.s1, r2.s2, r2.s3);
+        }
+        a.queue0[p] = p;
+        if (a.normal_out) {
+            a.normal_out[3 * (size_t)p] = 0.f;
+            a.normal_out[3 * (size_t)p + 1] = 0.f;
+            a.normal_out[3 * (size_t)p + 2] = 0.f;
+        }
+    }
+    if (tid == 0) a.counters[CTR_QUEUE] = a.npix;
+}
+
+// ---- extend: closest hit for every queued path + material histogram ----
+template <int WIDTH, int STACK>
+__global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) {
+    __shared__ uint2 stack[STACK * kPtBlock];
+    __shared__ uint32_t lhist[kMaxSortKeys];
+    uint32_t *ctr = a.counters + bounce * kCtrStride;
+    const uint32_t count = ctr[CTR_QUEUE];
+    const uint32_t *queue = (bounce & 1u) ? a.queue1 : a.queue0;
+    if (a.sort)
+        for (uint32_t k = threadIdx.x; k < a.nkeys; k += kPtBlock) lhist[k] = 0u;
+    __syncthreads();
+    for (;;) {
+        const uint32_t base = wave_fetch(ctr + CTR_EXTEND);
+        if (base >= count) break;
+        const uint32_t slot = base + lane_id();
+        if (slot >= count) continue;
+        const uint32_t path = queue[slot];
+        const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
+        RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
+        BestHit best;
+        best.t = r.tmax;
+        best.prim = 0xFFFFFFFFu;
+        best.tri = 0xFFFFFFFFu;
+        best.U = best.V = best.W = 0.f;
+        best.det = 1.f;
+        TraverseCounters c{0u, 0u};
+        traverse<WIDTH, STACK, false>(a.nodes, a.tris, 0u, r, stack + threadIdx.x, kPtBlock, best, c);
+        uint4 h;
+        uint32_t key = a.nkeys - 1u;  // misses sort last
+        if (best.prim != 0xFFFFFFFFu) {
+            const float t = final_t(a.tris, r, best);
+            h = make_uint4(best.tri, __float_as_uint(t), __float_as_uint(best.V / best.det),
+                           __float_as_uint(best.W / best.det));
+            key = a.shade[best.tri].material;
+            if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
+        } else {
+            h = make_uint4(0xFFFFFFFFu, __float_as_uint(INFINITY), 0u, 0u);
+        }
+        a.hits[slot] = h;
+        if (a.sort) atomicAdd(&lhist[key], 1u);
+    }
+    if (a.sort) {
+        __syncthreads();
+        uint32_t *gh = a.hist + (size_t)bounce * 2u * a.nkeys;
+        for (uint32_t k = threadIdx.x; k < a.nkeys; k += kPtBlock)
+            if (lhist[k]) atomicAdd(&gh[k], lhist[k]);
+    }
+}
+
+// ---- sort: exclusive scan of the histogram (one block) and scatter of queue slots ----
+__global__ __launch_bounds__(1024) void k_sort_scan(PtArgs a, uint32_t bounce) {
+    __shared__ uint32_t part[1024];
+    uint32_t *gh = a.hist + (size_t)bounce * 2u * a.nkeys;
+    const uint32_t k = threadIdx.x;
+    const uint32_t v = k < a.nkeys ? gh[k] : 0u;
+    part[k] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t add = k >= off ? part[k - off] : 0u;
+        __syncthreads();
+        part[k] += add;
+        __syncthreads();
+    }
+    if (k < a.nkeys) gh[a.nkeys + k] = part[k] - v;  // exclusive offsets
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_sort_scatter(PtArgs a, uint32_t bounce) {
+    uint32_t *ctr = a.counters + bounce * kCtrStride;
+    const uint32_t count = ctr[CTR_QUEUE];
+    uint32_t *offs = a.hist + (size_t)bounce * 2u * a.nkeys + a.nkeys;
+    for (;;) {
+        const uint32_t base = wave_fetch(ctr + CTR_SCATTER);
+        if (base >= count) break;
+        const uint32_t slot = base + lane_id();
+        if (slot >= count) continue;
+        const uint4 h = a.hits[slot];
+        uint32_t key = a.nkeys - 1u;
+        if (h.x != 0xFFFFFFFFu) {
+            key = a.shade[h.x].material;
+            if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
+        }
+        const uint32_t pos = atomicAdd(&offs[key], 1u);
+        a.sorted[pos] = slot;
+    }
+}
+
+__device__ __forceinline__ float4 lerp_attr(const float *p0, const float *p1, const float *p2, float u, float v) {
+    // v0 + (v1 - v0) * u + (v2 - v0) * v (Utilities.h:114-117)
+    return make_float4(p0[0] + (p1[0] - p0[0]) * u + (p2[0] - p0[0]) * v,
+                       p0[1] + (p1[1] - p0[1]) * u + (p2[1] - p0[1]) * v,
+                       p0[2] + (p1[2] - p0[2]) * u + (p2[2] - p0[2]) * v, 0.f);
+}
+
+// ---- shade: SamplePixel's loop body (CPURenderer.cpp:261-368) ----
+__global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
+    uint32_t *ctr = a.counters + bounce * kCtrStride;
+    const uint32_t count = ctr[CTR_QUEUE];
+    const uint32_t *queue = (bounce & 1u) ? a.queue1 : a.queue0;
+    uint32_t *next_queue = (bounce & 1u) ? a.queue0 : a.queue1;
+    const bool last = bounce + 1u >= a.max_bounce;
+    const bool nee = (a.flags & SPK_FRAME_NEE) && a.num_emitters > 0;
+    for (;;) {
+        const uint32_t base = wave_fetch(ctr + CTR_SHADE);
+        if (base >= count) break;
+        const uint32_t idx = base + lane_id();
+        bool cont = false, shadow = false;
+        uint32_t path = 0;
+        float4 sh0 = make_float4(0.f, 0.f, 0.f, 0.f), sh1 = sh0, sh2 = sh0;
+        if (idx < count) {
+            const uint32_t slot = a.sort ? a.sorted[idx] : idx;
+            path = queue[slot];
+            const uint4 h = a.hits[slot];
+            const float4 rd4 = a.rays[2 * path + 1], ro4 = a.rays[2 * path];
+            const F3 d = f3(rd4.x, rd4.y, rd4.z);
+            const float4 T4 = a.thr[path];
+            F3 T = f3(T4.x, T4.y, T4.z);
+            const bool skip_emission = T4.w != 0.f;
+            float4 C4 = a.col[path];
+            F3 C = f3(C4.x, C4.y, C4.z);
+            if (h.x == 0xFFFFFFFFu) {
+                if (a.sky_enabled) {
+                    const float t = 0.5f * d.z + 0.5f;
+                    const F3 sky = f3(1.f, 1.f, 1.f) + (f3(a.sky[0], a.sky[1], a.sky[2]) - f3(1.f, 1.f, 1.f)) * t;
+                    C = C + cmul(sky, T);
+                }
+            } else {
+                const TriShade &s = a.shade[h.x];
+                const float u = __uint_as_float(h.z), v = __uint_as_float(h.w), t = __uint_as_float(h.y);
+                const float4 n4 = lerp_attr(s.n0, s.n1, s.n2, u, v);
+                const float4 t4 = lerp_attr(s.t0, s.t1, s.t2, u, v);
+                const F3 normal = normalized(f3(n4.x, n4.y, n4.z));
+                const F3 tangent = f3(t4.x, t4.y, t4.z);
+                const F3 loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
+                const MaterialGpu mat = a.mats[s.material];
+                if (bounce == 0u && a.normal_out) {
+                    a.normal_out[3 * (size_t)path] = normal.x;
+                    a.normal_out[3 * (size_t)path + 1] = normal.y;
+                    a.normal_out[3 * (size_t)path + 2] = normal.z;
+                }
+                if (mat.emissive_flag) {
+                    if (!skip_emission) C = C + cmul(f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]), T);
+                } else {
+                    uint4 rs = a.rng[path];
+                    Rng rng{rs.x, rs.y, rs.z, rs.w};
+                    F3 wi;
+                    bool diffuse = false;
+                    const F3 f = sample_surface(rng, mat, d, normal, tangent, wi, diffuse);
+                    bool done = false;
+                    if (a.debug_mode != SPK_DEBUG_COLOR && bounce == 0u) {
+                        switch (a.debug_mode) {
+                            case 3: C = (normal + f3(1.f, 1.f, 1.f)) * 0.5f; break;
+                            case 4: C = (wi + f3(1.f, 1.f, 1.f)) * 0.5f; break;
+                            case 7: C = f3(1.f, 1.f, 1.f) * mat.metallic; break;
+                            case 8: C = f3(1.f, 1.f, 1.f) * mat.roughness; break;
+                            case 9: C = f3(mat.base[0], mat.base[1], mat.base[2]); break;
+                            case 10: C = f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]); break;
+                            case 11:
+                                C = f3(1.f, 1.f, 1.f) * (dot(loc - ld3(a.cam.position), ld3(a.cam.front)) /
+                                                         a.cam.far_plane);
+                                break;
+                            default: break;
+                        }
+                        done = true;
+                    }
+                    if (!done && nee && diffuse) {
+                        // next-event estimation for the Lambertian lobe: one area-sampled point on an
+                        // emissive triangle, Lambert BRDF base/pi, two-sided emitters
+                        uint4 ns = a.nee_rng[path];
+                        Rng nr{ns.x, ns.y, ns.z, ns.w};
+                        const float r0 = rng_unit(nr) * a.emitter_total;
+                        uint32_t lo = 0, hi = a.num_emitters - 1u;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (a.emitter_cdf[mid] < r0) lo = mid + 1u; else hi = mid;
+                        }
+                        const uint32_t et = a.emitters[lo];
+                        const float w_prev = lo ? a.emitter_cdf[lo - 1u] : 0.f;
+                        const float w_tri = a.emitter_cdf[lo] - w_prev;
+                        const float r1 = rng_unit(nr), r2 = rng_unit(nr);
+                        a.nee_rng[path] = make_uint4(nr.s0, nr.s1, nr.s2, nr.s3);
+                        const uint4 *tp = a.tris + (size_t)et * 3u;
+                        const uint4 va = tp[0], vb = tp[1], vc = tp[2];
+                        const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
+                        const F3 p1 = f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z));
+                        const F3 p2 = f3(__uint_as_float(vc.x), __uint_as_float(vc.y), __uint_as_float(vc.z));
+                        const float su = sqrtf(r1);
+                        const float b0 = 1.f - su, b1 = r2 * su;
+                        const F3 y = p0 * b0 + p1 * b1 + p2 * (1.f - b0 - b1);
+                        const F3 ng = cross(p1 - p0, p2 - p0);
+                        const float area2 = sqrtf(sqnorm(ng));
+                        const F3 dl = y - loc;
+                        const float dist2 = sqnorm(dl);
+                        const float dist = sqrtf(dist2);
+                        const F3 wl = dl / dist;
+                        const float cos_x = dot(normal, wl);
+                        const float cos_y = fabsf(dot(ng, wl)) / area2;
+                        const MaterialGpu &em = a.mats[vb.w];  // v1.w carries the material id
+                        if (cos_x > 0.f && cos_y > 0.f && dist > 2e-4f && w_tri > 0.f) {
+                            // pdf over area: (w_tri / total) / area; contribution f*cos_x*Le*cos_y/(d^2 pdf)
+                            const float area = 0.5f * area2;
+                            const float g = cos_x * cos_y / dist2 * area * a.emitter_total / w_tri / SPK_PI;
+                            const F3 base3 = f3(mat.base[0], mat.base[1], mat.base[2]);
+                            const F3 Le = f3(em.emissive[0], em.emissive[1], em.emissive[2]);
+                            const F3 contrib = cmul(cmul(T, base3), Le) * g;
+                            shadow = true;
+                            sh0 = make_float4(loc.x + wl.x * SPK_TOLERANCE, loc.y + wl.y * SPK_TOLERANCE,
+                                              loc.z + wl.z * SPK_TOLERANCE, dist - 2.f * SPK_TOLERANCE);
+                            sh1 = make_float4(wl.x, wl.y, wl.z, __uint_as_float(path));
+                            sh2 = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
+                        }
+                    }
+                    if (!done) {
+                        T = cmul(T, f);
+                        if (sqnorm(T) < SPK_EPS) {
+                            C4.w = -1.f;
+                        } else {
+                            bool alive = true;
+                            if (bounce >= 3u) {
+                                float p = fmaxf(fmaxf(T.x, T.y), T.z);
+                                p = fminf(fmaxf(p, 0.05f), 1.0f);
+                                if (rng_unit(rng) > p) alive = false; else T = T / p;
+                            }
+                            if (alive && !last) {
+                                cont = true;
+                                const F3 o2 = loc + wi * SPK_TOLERANCE;
+                                a.rays[2 * path] = make_float4(o2.x, o2.y, o2.z, SPK_EPS);
+                                a.rays[2 * path + 1] = make_float4(wi.x, wi.y, wi.z, INFINITY);
+                                a.thr[path] = make_float4(T.x, T.y, T.z, (nee && diffuse) ? 1.f : 0.f);
+                                a.rng[path] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+                            }
+                        }
+                    }
+                }
+            }
+            a.col[path] = make_float4(C.x, C.y, C.z, C4.w);
+        }
+        const uint32_t qs = wave_append(a.counters + (bounce + 1u) * kCtrStride + CTR_QUEUE, cont);
+        if (cont) next_queue[qs] = path;
+        const uint32_t ss = wave_append(ctr + CTR_SHADOWS, shadow);
+        if (shadow) {
+            a.shadow[3 * (size_t)ss] = sh0;
+            a.shadow[3 * (size_t)ss + 1] = sh1;
+            a.shadow[3 * (size_t)ss + 2] = sh2;
+        }
+    }
+}
+
+// ---- shadow: occlusion of the NEE segments; unoccluded contributions added to the path colour ----
+template <int WIDTH, int STACK>
+__global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) {
+    __shared__ uint2 stack[STACK * kPtBlock];
+    uint32_t *ctr = a.counters + bounce * kCtrStride;
+    const uint32_t count = ctr[CTR_SHADOWS];
+    for (;;) {
+        const uint32_t base = wave_fetch(ctr + CTR_SHADOWW);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        if (i >= count) continue;
+        const float4 s0 = a.shadow[3 * (size_t)i], s1 = a.shadow[3 * (size_t)i + 1], s2 = a.shadow[3 * (size_t)i + 2];
+        RayIn r{s0.x, s0.y, s0.z, SPK_EPS, s1.x, s1.y, s1.z, s0.w};
+        BestHit best;
+        best.t = r.tmax;
+        best.prim = 0xFFFFFFFFu;
+        best.tri = 0u;
+        best.U = best.V = best.W = 0.f;
+        best.det = 1.f;
+        TraverseCounters c{0u, 0u};
+        traverse<WIDTH, STACK, false, true>(a.nodes, a.tris, 0u, r, stack + threadIdx.x, kPtBlock, best, c);
+        if (best.prim == 0xFFFFFFFFu) {
+            const uint32_t path = __float_as_uint(s1.w);
+            float4 C = a.col[path];
+            C.x += s2.x;
+            C.y += s2.y;
+            C.z += s2.z;
+            a.col[path] = C;
+        }
+    }
+}
+
+// ---- finalize: per-sample clamp to 6 and the valid flag into the G-buffer (CPURenderer.cpp:408-411) ----
+__global__ __launch_bounds__(kPtBlock) void k_finalize(PtArgs a) {
+    const uint32_t stride = gridDim.x * kPtBlock;
+    for (uint32_t p = blockIdx.x * kPtBlock + threadIdx.x; p < a.npix; p += stride) {
+        const float4 C = a.col[p];
+        float4 o;
+        o.x = fminf(C.x, SPK_OUTPUT_LIMIT);
+        o.y = fminf(C.y, SPK_OUTPUT_LIMIT);
+        o.z = fminf(C.z, SPK_OUTPUT_LIMIT);
+        o.w = C.w;
+        reinterpret_cast<float4 *>(a.rgba_out)[p] = o;
+    }
+}
+
+// ---- accumulation and tone mapping (CPURenderer.cpp:527-576) ----
+__global__ void k_accumulate(const float4 *__restrict__ rgba, float4 *__restrict__ acc, uint64_t n, float ma) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = rgba[i];
+    float4 y = acc[i];
+    y.x = x.x + (y.x - x.x) * ma;
+    y.y = x.y + (y.y - x.y) * ma;
+    y.z = x.z + (y.z - x.z) * ma;
+    y.w = x.w + (y.w - x.w) * ma;
+    acc[i] = y;
+}
+
+__device__ __forceinline__ float aces(float x) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    const float v = (x * (x * a + b)) / (x * (x * c + d) + e);
+    return fminf(fmaxf(v, 0.f), 1.f);
+}
+
+__global__ void k_tonemap(const float4 *__restrict__ acc, float4 *__restrict__ out, uint32_t w, uint32_t h,
+                          float exposure) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t j = i / w, x = i - j * w;
+    const float4 c = acc[i];
+    out[(size_t)(h - 1u - j) * w + x] = make_float4(aces(c.x * exposure), aces(c.y * exposure), aces(c.z * exposure),
+                                                    1.f);
+}
+
+// ---------------------------------------------------------------------------------------------
+using ExtendFn = void (*)(PtArgs, uint32_t);
+struct PtEntry {
+    int width, stack;
+    ExtendFn extend, shadow;
+};
+const PtEntry kPtEntries[] = {
+    {8, 16, k_extend<8, 16>, k_shadow<8, 16>}, {8, 24, k_extend<8, 24>, k_shadow<8, 24>},
+    {8, 32, k_extend<8, 32>, k_shadow<8, 32>}, {8, 64, k_extend<8, 64>, k_shadow<8, 64>},
+    {4, 16, k_extend<4, 16>, k_shadow<4, 16>}, {4, 24, k_extend<4, 24>, k_shadow<4, 24>},
+    {4, 32, k_extend<4, 32>, k_shadow<4, 32>}, {4, 64, k_extend<4, 64>, k_shadow<4, 64>},
+};
+
+struct PathState {
+    DeviceBuffer rays, thr, col, rng, nee_rng, queue0, queue1, hits, sorted, shadow, shadow_q, counters, hist;
+    DeviceBuffer emitters, emitter_cdf, frame;
+    FrameDev *frame_host = nullptr;  // pinned
+    uint32_t capacity = 0;
+    uint32_t num_emitters = 0;
+    float emitter_total = 0.f;
+    bool emitters_ready = false;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    PtArgs graph_args{};
+    hipStream_t graph_stream = nullptr;
+    int graph_grid = 0;
+    ~PathState() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph_def) (void)hipGraphDestroy(graph_def);
+        if (frame_host) (void)hipHostFree(frame_host);
+    }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph_def) (void)hipGraphDestroy(graph_def);
+        graph = nullptr;
+        graph_def = nullptr;
+    }
+};
+
+static int device_grid(spk_context *ctx) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus * 8;  // 8 blocks of 128 threads per CU
+}
+
+static int ensure_paths(spk_context *ctx, uint32_t npix) {
+    if (!ctx->paths) ctx->paths = new PathState();
+    PathState &ps = *ctx->paths;
+    hipError_t e;
+    if (!ps.frame_host) {
+        if ((e = hipHostMalloc((void **)&ps.frame_host, sizeof(FrameDev), hipHostMallocDefault)) != hipSuccess)
+            return hip_fail(ctx, e, "pinned frame buffer");
+        if ((e = ps.frame.ensure(sizeof(FrameDev))) != hipSuccess) return hip_fail(ctx, e, "frame buffer");
+        if ((e = ps.counters.ensure(kMaxBounce * kCtrStride * sizeof(uint32_t))) != hipSuccess)
+            return hip_fail(ctx, e, "counters");
+        if ((e = ps.hist.ensure((size_t)kMaxBounce * 2 * kMaxSortKeys * sizeof(uint32_t))) != hipSuccess)
+            return hip_fail(ctx, e, "histograms");
+    }
+    if (npix > ps.capacity) {
+        ps.drop_graph();
+        const size_t n = npix;
+        if ((e = ps.rays.ensure(n * 32)) != hipSuccess || (e = ps.thr.ensure(n * 16)) != hipSuccess ||
+            (e = ps.col.ensure(n * 16)) != hipSuccess || (e = ps.rng.ensure(n * 16)) != hipSuccess ||
+            (e = ps.nee_rng.ensure(n * 16)) != hipSuccess || (e = ps.queue0.ensure(n * 4)) != hipSuccess ||
+            (e = ps.queue1.ensure(n * 4)) != hipSuccess || (e = ps.hits.ensure(n * 16)) != hipSuccess ||
+            (e = ps.sorted.ensure(n * 4)) != hipSuccess || (e = ps.shadow.ensure(n * 48)) != hipSuccess ||
+            (e = ps.shadow_q.ensure(n * 4)) != hipSuccess)
+            return hip_fail(ctx, e, "path buffers");
+        ps.capacity = npix;
+    }
+    if (!ps.emitters_ready) {
+        // emitter list weighted by area x mean emitted radiance
+        const std::vector<uint32_t> &em = ctx->scene.host_emitters;
+        std::vector<float> cdf(em.size());
+        double acc = 0.0;
+        for (size_t k = 0; k < em.size(); ++k) {
+            const TriRecord &r = ctx->host_scene.bvh.tris[em[k]];
+            const double e1[3] = {(double)r.v1[0] - r.v0[0], (double)r.v1[1] - r.v0[1], (double)r.v1[2] - r.v0[2]};
+            const double e2[3] = {(double)r.v2[0] - r.v0[0], (double)r.v2[1] - r.v0[1], (double)r.v2[2] - r.v0[2]};
+            const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                         cz = e1[0] * e2[1] - e1[1] * e2[0];
+            const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            uint32_t mat;
+            std::memcpy(&mat, &r.v1[3], 4);
+            const MaterialGpu &m = ctx->host_mats[mat];
+            const double lum = (m.emissive[0] + m.emissive[1] + m.emissive[2]) / 3.0;
+            acc += area * lum;
+            cdf[k] = (float)acc;
+        }
+        ps.num_emitters = (uint32_t)em.size();
+        ps.emitter_total = em.empty() ? 0.f : cdf.back();
+        if (!em.empty()) {
+            if ((e = ps.emitters.ensure(em.size() * 4)) != hipSuccess ||
+                (e = ps.emitter_cdf.ensure(em.size() * 4)) != hipSuccess)
+                return hip_fail(ctx, e, "emitter buffers");
+            if ((e = hipMemcpy(ps.emitters.ptr, em.data(), em.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(ps.emitter_cdf.ptr, cdf.data(), cdf.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+                return hip_fail(ctx, e, "emitter upload");
+        }
+        ps.emitters_ready = true;
+    }
+    return SPK_OK;
+}
+
+static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t s, int grid) {
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kPtBlock), 0, s, a);
+    for (uint32_t b = 0; b < a.max_bounce; ++b) {
+        hipLaunchKernelGGL(pe.extend, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+        if (a.sort) {
+            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, a, b);
+            hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+        }
+        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+        if ((a.flags & SPK_FRAME_NEE) && a.num_emitters) hipLaunchKernelGGL(pe.shadow, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(kPtBlock), 0, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace spk
 
-void spk_context::on_scene_changed() {}
+using namespace spk;
+
+void spk_context::on_scene_changed() {
+    if (paths) {
+        paths->drop_graph();
+        paths->emitters_ready = false;
+    }
+}
 
 void spk_context::release_all() {
     delete paths;
@@ -24,19 +605,167 @@ void spk_context::release_all() {
 }
 
 extern "C" {
+
 int spk_set_camera(spk_context *ctx, const spk_camera_desc *camera) {
     if (!ctx || !camera) return SPK_ERR_INVALID_ARGUMENT;
+    for (float v : {camera->position[0], camera->position[1], camera->position[2], camera->lens_radius})
+        if (!std::isfinite(v)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "camera has a non-finite value");
     ctx->camera = *camera;
     ctx->has_camera = true;
     return SPK_OK;
 }
-int spk_render_frame(spk_context *ctx, const spk_frame_desc *, float *, float *, spk_frame_stats *) {
-    return spk::set_error(ctx, SPK_ERR_UNSUPPORTED, "not built yet");
+
+int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, float *normal_dev,
+                     spk_frame_stats *stats) {
+    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
+    if (!ctx->has_camera) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "no camera set");
+    if (!rgba_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null output");
+    if (frame->width < 2 || frame->height < 2)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "frame must be at least 2x2 (pixel size 1/(W-1))");
+    if ((uint64_t)frame->width * frame->height >= (1ull << 31))
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "frame too large");
+    if (frame->max_bounce > (uint32_t)kMaxBounce)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "max_bounce above %d", kMaxBounce);
+    // modes that rewrite the colour after the loop (RayDepth, Indirect/DirectionalLighting) are not
+    // part of the device path
+    if (frame->debug_mode == 2u || frame->debug_mode == 5u || frame->debug_mode == 6u || frame->debug_mode > 11u)
+        return set_error(ctx, SPK_ERR_UNSUPPORTED, "debug mode %u is not supported on the device", frame->debug_mode);
+    const uint32_t npix = frame->width * frame->height;
+    int rc = ensure_paths(ctx, npix);
+    if (rc != SPK_OK) return rc;
+    PathState &ps = *ctx->paths;
+    const DeviceScene &ds = ctx->scene;
+    const PtEntry *pe = nullptr;
+    for (const PtEntry &e : kPtEntries)
+        if (e.width == ds.width && e.stack == pick_trace_stack(ds.depth)) pe = &e;
+    if (!pe) return set_error(ctx, SPK_ERR_STACK_DEPTH, "no path-tracer kernel for depth %d", ds.depth);
+
+    PtArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nodes = (const uint4 *)ds.nodes.ptr;
+    a.tris = (const uint4 *)ds.tris.ptr;
+    a.shade = (const TriShade *)ds.shade.ptr;
+    a.mats = (const MaterialGpu *)ds.materials.ptr;
+    a.emitters = (const uint32_t *)ps.emitters.ptr;
+    a.emitter_cdf = (const float *)ps.emitter_cdf.ptr;
+    a.num_emitters = ps.num_emitters;
+    a.emitter_total = ps.emitter_total;
+    a.rays = (float4 *)ps.rays.ptr;
+    a.thr = (float4 *)ps.thr.ptr;
+    a.col = (float4 *)ps.col.ptr;
+    a.rng = (uint4 *)ps.rng.ptr;
+    a.nee_rng = (uint4 *)ps.nee_rng.ptr;
+    a.queue0 = (uint32_t *)ps.queue0.ptr;
+    a.queue1 = (uint32_t *)ps.queue1.ptr;
+    a.hits = (uint4 *)ps.hits.ptr;
+    a.sorted = (uint32_t *)ps.sorted.ptr;
+    a.shadow = (float4 *)ps.shadow.ptr;
+    a.shadow_q = (uint32_t *)ps.shadow_q.ptr;
+    a.counters = (uint32_t *)ps.counters.ptr;
+    a.hist = (uint32_t *)ps.hist.ptr;
+    a.nkeys = ds.num_materials + 1u;
+    a.sort = a.nkeys <= (uint32_t)kMaxSortKeys && !(frame->flags & 0x80000000u) ? 1u : 0u;
+    if (!a.sort) a.nkeys = 2u;
+    a.rgba_out = rgba_dev;
+    a.normal_out = normal_dev;
+    a.frame = (const FrameDev *)ps.frame.ptr;
+    a.cam = ctx->camera;
+    a.width = frame->width;
+    a.height = frame->height;
+    a.npix = npix;
+    a.max_bounce = frame->max_bounce;
+    a.debug_mode = frame->debug_mode;
+    a.sky_enabled = frame->sky_enabled;
+    a.flags = frame->flags;
+    for (int c = 0; c < 3; ++c) a.sky[c] = frame->sky_color[c];
+    const int grid = device_grid(ctx);
+    a.grid = (uint32_t)grid;
+
+    hipError_t e;
+    ps.frame_host->frame_seed = frame->frame_seed;
+    if ((e = hipMemcpyAsync(ps.frame.ptr, ps.frame_host, sizeof(FrameDev), hipMemcpyHostToDevice, ctx->stream)) !=
+        hipSuccess)
+        return hip_fail(ctx, e, "frame parameters");
+    const bool reuse = ps.graph && ps.graph_stream == ctx->stream && std::memcmp(&ps.graph_args, &a, sizeof(a)) == 0;
+    if (!reuse && frame->max_bounce > 0) {
+        ps.drop_graph();
+        if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(ctx, e, "begin capture");
+        hipError_t le = enqueue_frame(a, *pe, ctx->stream, grid);
+        e = hipStreamEndCapture(ctx->stream, &ps.graph_def);
+        if (le != hipSuccess) return hip_fail(ctx, le, "frame launch during capture");
+        if (e != hipSuccess) return hip_fail(ctx, e, "end capture");
+        if ((e = hipGraphInstantiate(&ps.graph, ps.graph_def, nullptr, nullptr, 0)) != hipSuccess)
+            return hip_fail(ctx, e, "graph instantiate");
+        ps.graph_args = a;
+        ps.graph_stream = ctx->stream;
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (stats) {
+        (void)hipEventCreate(&t0);
+        (void)hipEventCreate(&t1);
+        (void)hipEventRecord(t0, ctx->stream);
+    }
+    if (frame->max_bounce == 0) {
+        // no segment is traced: the G-buffer is the untouched path state
+        hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kPtBlock), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(kPtBlock), 0, ctx->stream, a);
+        e = hipGetLastError();
+    } else {
+        e = hipGraphLaunch(ps.graph, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
+    if (stats) (void)hipEventRecord(t1, ctx->stream);
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "frame");
+    if (stats) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        std::vector<uint32_t> c(kMaxBounce * kCtrStride);
+        if ((e = hipMemcpy(c.data(), ps.counters.ptr, c.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(ctx, e, "frame counters");
+        uint64_t rays = 0, shadows = 0;
+        for (uint32_t b = 0; b < a.max_bounce; ++b) {
+            rays += c[b * kCtrStride + CTR_QUEUE];
+            shadows += c[b * kCtrStride + CTR_SHADOWS];
+        }
+        stats->rays_traced = rays;
+        stats->shadow_rays = shadows;
+        stats->milliseconds = ms;
+    }
+    return SPK_OK;
 }
-int spk_accumulate(spk_context *ctx, const float *, float *, uint64_t, uint32_t, uint32_t) {
-    return spk::set_error(ctx, SPK_ERR_UNSUPPORTED, "not built yet");
+
+int spk_accumulate(spk_context *ctx, const float *rgba_dev, float *accum_dev, uint64_t num_pixels,
+                   uint32_t cumulated, uint32_t spp) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (num_pixels && (!rgba_dev || !accum_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null buffer");
+    if (cumulated == 0 && spp == 0) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "spp must be positive");
+    if (num_pixels == 0) return SPK_OK;
+    const float ma = (float)cumulated / (float)(cumulated + spp);
+    const uint64_t blocks = (num_pixels + 255) / 256;
+    hipLaunchKernelGGL(k_accumulate, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, (const float4 *)rgba_dev,
+                       (float4 *)accum_dev, num_pixels, ma);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "accumulate");
+    return SPK_OK;
 }
-int spk_tonemap(spk_context *ctx, const float *, float *, uint32_t, uint32_t, float) {
-    return spk::set_error(ctx, SPK_ERR_UNSUPPORTED, "not built yet");
+
+int spk_tonemap(spk_context *ctx, const float *accum_dev, float *out_dev, uint32_t width, uint32_t height,
+                float exposure) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (!accum_dev || !out_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null buffer");
+    const uint64_t n = (uint64_t)width * height;
+    if (n == 0) return SPK_OK;
+    hipLaunchKernelGGL(k_tonemap, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float4 *)accum_dev, (float4 *)out_dev, width, height, exposure);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "tonemap");
+    return SPK_OK;
 }
-}
+
+}  // extern "C"

@@ -185,7 +185,7 @@ __device__ __forceinline__ float safe_inv(float d) {
 
 // One closest-hit query over a wide BVH. STACK entries per lane live in LDS at
 // stack[k * stride]; each entry is a node group {child_base, imask << 8 | pending-hit bits}.
-template <int WIDTH, int STACK, bool COUNT>
+template <int WIDTH, int STACK, bool COUNT, bool ANYHIT = false>
 __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const uint4 *__restrict__ tris,
                                          uint32_t root, const RayIn &ray, uint2 *stack, int stride, BestHit &best,
                                          TraverseCounters &cnt) {
@@ -342,6 +342,7 @@ __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const 
                     best.V = tt.V;
                     best.W = tt.W;
                     best.det = tt.det;
+                    if (ANYHIT) return;  // occlusion query: any hit in (tmin, tmax] ends it
                 }
             }
         }
