@@ -52,13 +52,20 @@ def cpu_baseline(scene, rays, seconds_target=12.0):
             "watertight_value": round(wt_rate, 3)}
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "trace_incoherent_2m_traffic.json")
+def load_traffic(config):
+    """HBM-side bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, when it
+    was taken with the kernel configuration this run uses."""
+    path = os.path.join(ROOT, "profiles", "trace_incoherent_2m_pmc.md")
     if not os.path.exists(path):
         return None, None
-    with open(path) as f:
-        d = json.load(f)
-    return d.get("hbm_bytes_per_launch"), d.get("source")
+    fields = {}
+    for line in open(path):
+        if ":" in line and line.split(":")[0] in ("config", "hbm_bytes_per_launch"):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if fields.get("config") != config:
+        return None, f"profiles/trace_incoherent_2m_pmc.md was taken with config {fields.get('config')}"
+    return int(float(fields["hbm_bytes_per_launch"])), "profiles/trace_incoherent_2m_pmc.md"
 
 
 def main():
@@ -178,7 +185,8 @@ def main():
         coherent_mrays = round(rays_a.shape[0] * 10 / (ea.elapsed_time(eb) / 1e3) / 1e6, 1)
 
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
-    traffic, traffic_src = load_traffic()
+    cur = ctx.trace_config()
+    traffic, traffic_src = load_traffic(f"{cur[0]},{cur[1]},{cur[2]}")
     out = {
         "metric": METRIC,
         "value": round(value, 2),

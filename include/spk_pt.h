@@ -140,6 +140,9 @@ typedef struct spk_camera_desc {
 
 #define SPK_FRAME_SEED64 1u /* compose the pixel seed in 64 bits (no uint32 wrap) */
 #define SPK_FRAME_NEE 2u    /* next-event estimation on emissive triangles (not in the reference) */
+#define SPK_FRAME_CACHE 4u  /* terminate paths into the learned radiance cache and train it (needs
+                               spk_cache_create; not in the reference) */
+#define SPK_FRAME_NO_SORT 0x80000000u /* shade in queue order instead of material order */
 
 typedef struct spk_frame_desc {
     uint32_t width;
@@ -150,6 +153,7 @@ typedef struct spk_frame_desc {
     uint32_t sky_enabled;
     float sky_color[3];
     uint32_t flags; /* SPK_FRAME_* */
+    float output_limit; /* per-sample clamp; 0 means the reference's 6 (CameraRenderProxy.h:16) */
 } spk_frame_desc;
 
 typedef struct spk_frame_stats {
@@ -231,6 +235,42 @@ int spk_accumulate(spk_context *ctx, const float *rgba_dev, float *accum_dev, ui
 /* ACES film curve with exposure, vertically flipped, to RGBA float (CPURenderer.cpp:554-576). */
 int spk_tonemap(spk_context *ctx, const float *accum_dev, float *out_dev, uint32_t width, uint32_t height,
                 float exposure);
+
+/* ---- learned radiance cache (no counterpart in the reference) ---------------------------- */
+/* A multiresolution hash-grid encoding (levels x 2 features, 2^log2_table FP16 entries per level)
+   feeding a bias-free ReLU MLP 32-64-64-64-64-3 on BF16 matrix cores with FP32 accumulation,
+   trained online with Adam on a relative L2 loss. Positions are normalised by the scene box. */
+typedef struct spk_cache_config {
+    uint32_t levels;           /* 16 */
+    uint32_t features;         /* 2 */
+    uint32_t log2_table;       /* 19 */
+    uint32_t base_resolution;  /* 16 */
+    float per_level_scale;     /* growth between levels, e.g. 1.3819 */
+    float learning_rate;       /* 1e-2 */
+    uint32_t batch;            /* training samples per frame (65536) */
+    uint32_t terminate_bounce; /* paths terminate into the cache at this hit index (2) */
+    uint32_t seed;             /* initialisation seed */
+    float train_fraction;      /* fraction of paths traced in full for training, 0 = batch/pixels */
+} spk_cache_config;
+
+int spk_cache_create(spk_context *ctx, const spk_cache_config *config);
+int spk_cache_destroy(spk_context *ctx);
+/* n points (device, 3 floats each; world space when world != 0, else in [0,1]^3) -> n x 3 radiance. */
+int spk_cache_infer(spk_context *ctx, const float *points_dev, uint32_t n, int world, float *out_dev);
+/* One training step (forward, backward, Adam) on n samples; loss (host, may be NULL) receives the
+   batch loss before the update. */
+int spk_cache_train(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n, int world,
+                    float *loss);
+/* Forward + backward without an update: loss, weight gradient (host, cache_num_weights floats) and
+   table gradient (host, levels*2^log2_table*2 floats; may be NULL). */
+int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n,
+                        int world, float *loss, float *dweights, float *dtable);
+/* FP32 master parameters to / from host arrays (either pointer may be NULL). */
+int spk_cache_get_params(spk_context *ctx, float *weights, float *table);
+int spk_cache_set_params(spk_context *ctx, const float *weights, const float *table);
+/* Number of MLP weights (14,528 for the default network); training steps taken; last frame's loss. */
+int spk_cache_info(spk_context *ctx, uint64_t *num_weights, uint64_t *num_table_params, uint32_t *steps,
+                   float *last_loss);
 
 #ifdef __cplusplus
 }

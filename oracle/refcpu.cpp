@@ -67,8 +67,11 @@ struct Xf {
                 m[8] * d.x + m[9] * d.y + m[10] * d.z};
     }
     bool identity() const {
+        // value comparison: an inverted identity carries -0.0 translations
         static const float I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-        return std::memcmp(m, I, sizeof(I)) == 0;
+        for (int k = 0; k < 12; ++k)
+            if (m[k] != I[k]) return false;
+        return true;
     }
 };
 
@@ -159,18 +162,40 @@ static inline WtRay wt_setup(V3 o, V3 d) {
     r.Sz = 1.0f / d[kz];
     return r;
 }
+// The FP32 edge functions are used only when a forward error bound certifies their signs; a
+// triangle whose signs are not certified (the ray passes within rounding distance of an edge, or
+// the triangle is seen edge-on) is re-evaluated in double-float arithmetic. This replaces the FP64
+// fallback of the paper and removes the false hits that exact-zero edge values would otherwise give.
+static bool tri_wt_precise(const WtRay &r, V3 v0, V3 v1, V3 v2, float &t, float &U_, float &V_, float &W_,
+                           float &det_);
+
+static const float kWtErr = 8.f * 5.9604644775390625e-08f;   // 8u, u = 2^-24
+static const float kWtErrAbs = 2.f * 5.9604644775390625e-08f;  // 2u
+
 static inline bool tri_wt(const WtRay &r, V3 v0, V3 v1, V3 v2, float &t, float &U_, float &V_, float &W_,
                           float &det_) {
     const V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
-    const float Ax = A[r.kx] - r.Sx * A[r.kz];
-    const float Ay = A[r.ky] - r.Sy * A[r.kz];
-    const float Bx = B[r.kx] - r.Sx * B[r.kz];
-    const float By = B[r.ky] - r.Sy * B[r.kz];
-    const float Cx = C[r.kx] - r.Sx * C[r.kz];
-    const float Cy = C[r.ky] - r.Sy * C[r.kz];
+    const float sAx = r.Sx * A[r.kz], sAy = r.Sy * A[r.kz];
+    const float sBx = r.Sx * B[r.kz], sBy = r.Sy * B[r.kz];
+    const float sCx = r.Sx * C[r.kz], sCy = r.Sy * C[r.kz];
+    const float Ax = A[r.kx] - sAx;
+    const float Ay = A[r.ky] - sAy;
+    const float Bx = B[r.kx] - sBx;
+    const float By = B[r.ky] - sBy;
+    const float Cx = C[r.kx] - sCx;
+    const float Cy = C[r.ky] - sCy;
     const float U = Cx * By - Cy * Bx;
     const float V = Ax * Cy - Ay * Cx;
     const float W = Bx * Ay - By * Ax;
+    // magnitudes before cancellation bound the rounding of the sheared coordinates and products
+    const float mAx = std::abs(A[r.kx]) + std::abs(sAx), mAy = std::abs(A[r.ky]) + std::abs(sAy);
+    const float mBx = std::abs(B[r.kx]) + std::abs(sBx), mBy = std::abs(B[r.ky]) + std::abs(sBy);
+    const float mCx = std::abs(C[r.kx]) + std::abs(sCx), mCy = std::abs(C[r.ky]) + std::abs(sCy);
+    const float eU = kWtErr * (mCx * mBy + mCy * mBx) + kWtErrAbs * std::abs(U);
+    const float eV = kWtErr * (mAx * mCy + mAy * mCx) + kWtErrAbs * std::abs(V);
+    const float eW = kWtErr * (mBx * mAy + mBy * mAx) + kWtErrAbs * std::abs(W);
+    if (!(std::abs(U) > eU) || !(std::abs(V) > eV) || !(std::abs(W) > eW))
+        return tri_wt_precise(r, v0, v1, v2, t, U_, V_, W_, det_);
     if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
     const float det = U + V + W;
     if (det == 0.f) return false;
@@ -226,6 +251,39 @@ static inline DD dd_add(DD a, DD b) {
     return r;
 }
 static inline DD dd_neg(DD a) { return DD{-a.h, -a.l}; }
+static inline DD dd_sel(int k, DD x, DD y, DD z) { return k == 0 ? x : (k == 1 ? y : z); }
+
+// The watertight test of tri_wt evaluated in double-float arithmetic: exact translation, shear and
+// edge functions carried to ~44 bits, signs decided on the rounded high parts.
+static bool tri_wt_precise(const WtRay &r, V3 v0, V3 v1, V3 v2, float &t, float &U_, float &V_, float &W_,
+                           float &det_) {
+    const DD A0 = dd_diff(v0.x, r.o.x), A1 = dd_diff(v0.y, r.o.y), A2 = dd_diff(v0.z, r.o.z);
+    const DD B0 = dd_diff(v1.x, r.o.x), B1 = dd_diff(v1.y, r.o.y), B2 = dd_diff(v1.z, r.o.z);
+    const DD C0 = dd_diff(v2.x, r.o.x), C1 = dd_diff(v2.y, r.o.y), C2 = dd_diff(v2.z, r.o.z);
+    const DD Akz = dd_sel(r.kz, A0, A1, A2), Bkz = dd_sel(r.kz, B0, B1, B2), Ckz = dd_sel(r.kz, C0, C1, C2);
+    const DD Sx{r.Sx, 0.f}, Sy{r.Sy, 0.f}, Sz{r.Sz, 0.f};
+    const DD Ax = dd_add(dd_sel(r.kx, A0, A1, A2), dd_neg(dd_mul(Sx, Akz)));
+    const DD Ay = dd_add(dd_sel(r.ky, A0, A1, A2), dd_neg(dd_mul(Sy, Akz)));
+    const DD Bx = dd_add(dd_sel(r.kx, B0, B1, B2), dd_neg(dd_mul(Sx, Bkz)));
+    const DD By = dd_add(dd_sel(r.ky, B0, B1, B2), dd_neg(dd_mul(Sy, Bkz)));
+    const DD Cx = dd_add(dd_sel(r.kx, C0, C1, C2), dd_neg(dd_mul(Sx, Ckz)));
+    const DD Cy = dd_add(dd_sel(r.ky, C0, C1, C2), dd_neg(dd_mul(Sy, Ckz)));
+    const DD U = dd_add(dd_mul(Cx, By), dd_neg(dd_mul(Cy, Bx)));
+    const DD V = dd_add(dd_mul(Ax, Cy), dd_neg(dd_mul(Ay, Cx)));
+    const DD W = dd_add(dd_mul(Bx, Ay), dd_neg(dd_mul(By, Ax)));
+    if ((U.h < 0.f || V.h < 0.f || W.h < 0.f) && (U.h > 0.f || V.h > 0.f || W.h > 0.f)) return false;
+    const DD det = dd_add(dd_add(U, V), W);
+    if (det.h == 0.f) return false;
+    const DD T = dd_add(dd_add(dd_mul(U, dd_mul(Sz, Akz)), dd_mul(V, dd_mul(Sz, Bkz))), dd_mul(W, dd_mul(Sz, Ckz)));
+    if ((det.h < 0.f && T.h > 0.f) || (det.h > 0.f && T.h < 0.f)) return false;
+    const float detf = det.h + det.l;
+    t = (T.h + T.l) / detf;
+    U_ = U.h;
+    V_ = V.h;
+    W_ = W.h;
+    det_ = detf;
+    return true;
+}
 static inline float refine_t(V3 o, V3 d, V3 v0, V3 v1, V3 v2, float t_fallback) {
     const DD e1x = dd_diff(v1.x, v0.x), e1y = dd_diff(v1.y, v0.y), e1z = dd_diff(v1.z, v0.z);
     const DD e2x = dd_diff(v2.x, v0.x), e2y = dd_diff(v2.y, v0.y), e2z = dd_diff(v2.z, v0.z);
@@ -708,6 +766,18 @@ static void scene_intersect_f64(const orcl_scene &sc, V3 o, V3 d, float tmin, fl
     for (size_t ii = 0; ii < sc.instances.size(); ++ii) {
         const Instance &ins = sc.instances[ii];
         const Mesh &m = sc.meshes[ins.mesh];
+        if (!ins.to_world.identity()) {
+            // a transformed instance is checked on its baked world-space triangles (the geometry the
+            // device intersects), triangle by triangle
+            const double ow[3] = {o.x, o.y, o.z}, dw[3] = {d.x, d.y, d.z};
+            for (uint32_t f = 0; f < m.ntri; ++f) {
+                V3 a, b, c;
+                world_tri(ins, m, f, a, b, c);
+                double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, C[3] = {c.x, c.y, c.z};
+                f64_tri(ow, dw, A, B, C, (double)tmin, ins.prim_offset + f, best);
+            }
+            continue;
+        }
         // instance transforms are applied in double
         double od[3], dd[3];
         const float *M = ins.to_local.m;

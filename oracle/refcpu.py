@@ -228,3 +228,86 @@ def random_unit_stream(seed, n):
     out = np.zeros(n, dtype=np.float32)
     lib().orcl_random_unit_stream(seed, out.ctypes.data, n)
     return out
+
+
+# ---- learned radiance cache (FP64) ----------------------------------------------------------
+def cache_cfg(log2_table=19, base_resolution=16, per_level_scale=1.3819):
+    return OCacheConfig(16, 2, log2_table, base_resolution, per_level_scale, 64, 4, 3)
+
+
+def level_resolutions(cfg):
+    import math
+
+    return [int(math.floor(cfg.base_resolution * cfg.per_level_scale ** l)) for l in range(cfg.levels)]
+
+
+def hashgrid_index(cfg, level, x, y, z):
+    return int(lib().orcl_hashgrid_index(ctypes.byref(cfg), level, x, y, z))
+
+
+def hashgrid_encode(cfg, table, points):
+    """table: float32 [levels * 2^log2_table * 2]; points [n, 3] in [0,1]; returns [n, 32] float64."""
+    table = np.ascontiguousarray(table, dtype=np.float32)
+    points = np.ascontiguousarray(points, dtype=np.float32)
+    n = points.shape[0]
+    out = np.zeros((n, cfg.levels * cfg.features), dtype=np.float64)
+    lib().orcl_hashgrid_encode(ctypes.byref(cfg), table.ctypes.data, points.ctypes.data, n, out.ctypes.data)
+    return out
+
+
+def mlp_loss_grad(cfg, weights, features, targets):
+    """Relative-L2 loss, outputs, dW and d(features) of the bias-free ReLU MLP in FP64."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    x = np.ascontiguousarray(features, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.float64)
+    n = x.shape[0]
+    out = np.zeros((n, cfg.outputs), dtype=np.float64)
+    dw = np.zeros_like(w)
+    dx = np.zeros_like(x)
+    loss = lib().orcl_mlp_loss_grad(ctypes.byref(cfg), w.ctypes.data, x.ctypes.data, t.ctypes.data, n,
+                                    out.ctypes.data, dw.ctypes.data, dx.ctypes.data)
+    return loss, out, dw, dx
+
+
+def adam(param, grad, m, v, lr, beta1, beta2, eps, step):
+    for a in (param, m, v):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    lib().orcl_adam(param.ctypes.data, g.ctypes.data, m.ctypes.data, v.ctypes.data, param.size, lr, beta1, beta2,
+                    eps, step)
+
+
+def table_backward(cfg, points, dfeatures):
+    """Scatter d(features) into a dense FP64 table gradient (trilinear weights), the adjoint of
+    hashgrid_encode. Pure numpy over the oracle's index function; for small n only."""
+    import math
+
+    T = 1 << cfg.log2_table
+    g = np.zeros(cfg.levels * T * 2, dtype=np.float64)
+    res = level_resolutions(cfg)
+    for i in range(points.shape[0]):
+        for l in range(cfg.levels):
+            r = res[l]
+            cell, frac = [], []
+            for a in range(3):
+                p = np.float32(min(max(points[i, a], 0.0), 1.0))
+                pos = np.float32(p * np.float32(r))
+                c = np.float32(math.floor(pos))
+                c = min(max(c, np.float32(0)), np.float32(r - 1))
+                cell.append(int(c))
+                frac.append(float(np.float32(pos - c)))
+            for k in range(8):
+                cx, cy, cz = cell[0] + (k & 1), cell[1] + ((k >> 1) & 1), cell[2] + ((k >> 2) & 1)
+                w = ((frac[0] if k & 1 else 1 - frac[0]) * (frac[1] if (k >> 1) & 1 else 1 - frac[1]) *
+                     (frac[2] if (k >> 2) & 1 else 1 - frac[2]))
+                idx = hashgrid_index(cfg, l, cx, cy, cz)
+                g[(l * T + idx) * 2] += w * dfeatures[i, 2 * l]
+                g[(l * T + idx) * 2 + 1] += w * dfeatures[i, 2 * l + 1]
+    return g
+
+
+def to_bf16(x):
+    """Round float32 values to bfloat16 (nearest even) and back to float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((a + 0x7FFF + ((a >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)

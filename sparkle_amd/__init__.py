@@ -97,7 +97,25 @@ class CameraDesc(ctypes.Structure):
 class FrameDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32),
                 ("max_bounce", ctypes.c_uint32), ("debug_mode", ctypes.c_uint32), ("sky_enabled", ctypes.c_uint32),
-                ("sky_color", ctypes.c_float * 3), ("flags", ctypes.c_uint32)]
+                ("sky_color", ctypes.c_float * 3), ("flags", ctypes.c_uint32), ("output_limit", ctypes.c_float)]
+
+
+class CacheConfig(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_uint32), ("features", ctypes.c_uint32), ("log2_table", ctypes.c_uint32),
+                ("base_resolution", ctypes.c_uint32), ("per_level_scale", ctypes.c_float),
+                ("learning_rate", ctypes.c_float), ("batch", ctypes.c_uint32), ("terminate_bounce", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("train_fraction", ctypes.c_float)]
+
+
+def cache_config(log2_table=19, base_resolution=16, per_level_scale=1.3819, learning_rate=1e-2, batch=1 << 16,
+                 terminate_bounce=2, seed=0, train_fraction=0.0):
+    """The configuration of the learned radiance cache: 16 levels x 2 features, FP16 table."""
+    return CacheConfig(16, 2, log2_table, base_resolution, per_level_scale, learning_rate, batch, terminate_bounce,
+                       seed, train_fraction)
+
+
+FRAME_CACHE = 4
+FRAME_NO_SORT = 0x80000000
 
 
 class FrameStats(ctypes.Structure):
@@ -142,6 +160,14 @@ def load_library():
         "spk_render_frame": (I, [P, ctypes.POINTER(FrameDesc), P, P, ctypes.POINTER(FrameStats)]),
         "spk_accumulate": (I, [P, P, P, U64, U32, U32]),
         "spk_tonemap": (I, [P, P, P, U32, U32, F]),
+        "spk_cache_create": (I, [P, ctypes.POINTER(CacheConfig)]),
+        "spk_cache_destroy": (I, [P]),
+        "spk_cache_infer": (I, [P, P, U32, I, P]),
+        "spk_cache_train": (I, [P, P, P, U32, I, ctypes.POINTER(F)]),
+        "spk_cache_gradients": (I, [P, P, P, U32, I, ctypes.POINTER(F), P, P]),
+        "spk_cache_get_params": (I, [P, P, P]),
+        "spk_cache_set_params": (I, [P, P, P]),
+        "spk_cache_info": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U32), ctypes.POINTER(F)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -342,6 +368,50 @@ class Context:
 
     def tonemap(self, accum_dev, out_dev, width, height, exposure=1.0):
         return self._check(self.lib.spk_tonemap(self.h, _ptr(accum_dev), _ptr(out_dev), width, height, exposure))
+
+    # ---- learned radiance cache ----
+    def cache_create(self, cfg=None):
+        cfg = cfg or cache_config()
+        return self._check(self.lib.spk_cache_create(self.h, ctypes.byref(cfg)))
+
+    def cache_destroy(self):
+        return self._check(self.lib.spk_cache_destroy(self.h))
+
+    def cache_info(self):
+        nw, nt, st, loss = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_float()
+        self._check(self.lib.spk_cache_info(self.h, ctypes.byref(nw), ctypes.byref(nt), ctypes.byref(st),
+                                            ctypes.byref(loss)))
+        return dict(num_weights=nw.value, num_table_params=nt.value, steps=st.value, last_loss=loss.value)
+
+    def cache_infer(self, points_dev, out_dev, n, world=False):
+        return self._check(self.lib.spk_cache_infer(self.h, _ptr(points_dev), n, 1 if world else 0, _ptr(out_dev)))
+
+    def cache_train(self, points_dev, targets_dev, n, world=False):
+        loss = ctypes.c_float()
+        self._check(self.lib.spk_cache_train(self.h, _ptr(points_dev), _ptr(targets_dev), n, 1 if world else 0,
+                                             ctypes.byref(loss)))
+        return loss.value
+
+    def cache_gradients(self, points_dev, targets_dev, n, world=False, table=False):
+        info = self.cache_info()
+        dw = np.zeros(info["num_weights"], dtype=np.float32)
+        dt = np.zeros(info["num_table_params"], dtype=np.float32) if table else None
+        loss = ctypes.c_float()
+        self._check(self.lib.spk_cache_gradients(self.h, _ptr(points_dev), _ptr(targets_dev), n, 1 if world else 0,
+                                                 ctypes.byref(loss), _ptr(dw), _ptr(dt)))
+        return loss.value, dw, dt
+
+    def cache_get_params(self):
+        info = self.cache_info()
+        w = np.zeros(info["num_weights"], dtype=np.float32)
+        t = np.zeros(info["num_table_params"], dtype=np.float32)
+        self._check(self.lib.spk_cache_get_params(self.h, _ptr(w), _ptr(t)))
+        return w, t
+
+    def cache_set_params(self, weights=None, table=None):
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        t = None if table is None else np.ascontiguousarray(table, dtype=np.float32)
+        return self._check(self.lib.spk_cache_set_params(self.h, _ptr(w), _ptr(t)))
 
 
 def camera_desc(position, front, up_hint=(0.0, 0.0, 1.0), vertical_fov=None, focus_distance=1.0,

@@ -1,0 +1,654 @@
+// Learned radiance cache (no counterpart in the reference): a 16-level hash-grid encoding
+// (Mueller et al., Instant NGP 2022) feeding a bias-free 32-64-64-64-64-3 ReLU MLP, trained online
+// with a relative L2 loss (Mueller et al., neural radiance caching 2021) and Adam (Kingma & Ba).
+//
+// Device layout:
+//   table      FP16 [levels][2^log2_table][2]   (+ FP32 master, Adam m/v, FP32 gradient)
+//   weights    FP32 master [out][in] per layer, concatenated (14,528 values for the default net)
+//              BF16 forward copy  Wf_l [out_pad16][in]  and backward copy Wb_l = W_l^T [in][out_pad32]
+// The MLP runs on CDNA4 matrix cores: v_mfma_f32_16x16x32_bf16 with neurons as rows (M), queries
+// as columns (N) and FP32 accumulation. One wave owns a tile of 64 queries; activations live in LDS
+// as BF16 rows [query][feature] so that every operand fetch is one 16-byte LDS read per lane.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/spk_pt.h"
+#include "spk_cache.h"
+#include "spk_context.h"
+
+namespace spk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kQ = 64;        // queries per wave tile
+constexpr int kIn = 32;       // encoding width (levels * features)
+constexpr int kH = 64;        // hidden width
+constexpr int kOutPad = 16;   // output rows padded to one MFMA tile
+constexpr int kStrideX = kIn + 8;   // LDS row strides (bf16), padded against bank conflicts
+constexpr int kStrideH = kH + 8;
+constexpr int kStrideG = kH + 8;
+
+struct CacheDev {
+    const __half2 *table;      // [L*T]
+    const __bf16 *wf[5];       // forward weights
+    const __bf16 *wb[5];       // backward (transposed) weights
+    uint32_t res[16];          // grid resolution per level
+    uint32_t dense[16];        // 1 when the level is indexed densely
+    uint32_t log2_table;
+    float lo[3], inv_ext[3];   // world -> [0,1]^3
+};
+
+__device__ __forceinline__ uint32_t grid_index(const CacheDev &c, uint32_t l, uint32_t x, uint32_t y, uint32_t z) {
+    if (c.dense[l]) {
+        const uint32_t side = c.res[l] + 1u;
+        return x + side * (y + side * z);
+    }
+    return ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & ((1u << c.log2_table) - 1u);
+}
+
+// trilinear hash-grid features of one point in [0,1]^3 (16 levels x 2 features)
+__device__ __forceinline__ void encode_point(const CacheDev &c, float px, float py, float pz, float *feat) {
+    const uint32_t T = 1u << c.log2_table;
+#pragma unroll 4
+    for (uint32_t l = 0; l < 16; ++l) {
+        const float res = (float)c.res[l];
+        float pos[3] = {px * res, py * res, pz * res};
+        uint32_t cell[3];
+        float fr[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float cf = floorf(pos[a]);
+            cf = fminf(fmaxf(cf, 0.f), res - 1.f);
+            cell[a] = (uint32_t)cf;
+            fr[a] = pos[a] - cf;
+        }
+        float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t cx = cell[0] + (k & 1), cy = cell[1] + ((k >> 1) & 1), cz = cell[2] + ((k >> 2) & 1);
+            const float w = ((k & 1) ? fr[0] : 1.f - fr[0]) * (((k >> 1) & 1) ? fr[1] : 1.f - fr[1]) *
+                            (((k >> 2) & 1) ? fr[2] : 1.f - fr[2]);
+            const __half2 v = c.table[(size_t)l * T + grid_index(c, l, cx, cy, cz)];
+            f0 += w * __low2float(v);
+            f1 += w * __high2float(v);
+        }
+        feat[2 * l] = f0;
+        feat[2 * l + 1] = f1;
+    }
+}
+
+// scatter d(loss)/d(features) of one point into the FP32 table gradient
+__device__ __forceinline__ void encode_backward(const CacheDev &c, float px, float py, float pz, const float *g,
+                                                float *gtable) {
+    const uint32_t T = 1u << c.log2_table;
+    for (uint32_t l = 0; l < 16; ++l) {
+        const float res = (float)c.res[l];
+        float pos[3] = {px * res, py * res, pz * res};
+        uint32_t cell[3];
+        float fr[3];
+        for (int a = 0; a < 3; ++a) {
+            float cf = floorf(pos[a]);
+            cf = fminf(fmaxf(cf, 0.f), res - 1.f);
+            cell[a] = (uint32_t)cf;
+            fr[a] = pos[a] - cf;
+        }
+        const float g0 = g[2 * l], g1 = g[2 * l + 1];
+        if (g0 == 0.f && g1 == 0.f) continue;
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t cx = cell[0] + (k & 1), cy = cell[1] + ((k >> 1) & 1), cz = cell[2] + ((k >> 2) & 1);
+            const float w = ((k & 1) ? fr[0] : 1.f - fr[0]) * (((k >> 1) & 1) ? fr[1] : 1.f - fr[1]) *
+                            (((k >> 2) & 1) ? fr[2] : 1.f - fr[2]);
+            const size_t e = ((size_t)l * T + grid_index(c, l, cx, cy, cz)) * 2u;
+            atomicAdd(&gtable[e], w * g0);
+            atomicAdd(&gtable[e + 1], w * g1);
+        }
+    }
+}
+
+__device__ __forceinline__ bf16x8 ld8(const __bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+// D[M_TILES*16 x 64] = W[M_TILES*16 x K_STEPS*32] * Hin^T, Hin rows [query][k] in LDS.
+// The result is written to Hout rows [query][neuron] as BF16, after ReLU when RELU. With MASK the
+// result is multiplied by (Mask[query][neuron] > 0) (the ReLU derivative of the forward pass).
+template <int M_TILES, int K_STEPS, bool RELU, bool MASK>
+__device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ W, const __bf16 *Hin, int sin, __bf16 *Hout,
+                                          int sout, const __bf16 *Mask, int smask, float *out_f32, int m_valid) {
+    const int lane = __lane_id();
+    const int r = lane & 15, g = lane >> 4;
+    bf16x8 a[M_TILES][K_STEPS];
+#pragma unroll
+    for (int mt = 0; mt < M_TILES; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < K_STEPS; ++ks) a[mt][ks] = ld8(W + (size_t)(16 * mt + r) * (32 * K_STEPS) + 32 * ks + 8 * g);
+#pragma unroll
+    for (int nt = 0; nt < kQ / 16; ++nt) {
+        const int q = 16 * nt + r;
+        bf16x8 b[K_STEPS];
+#pragma unroll
+        for (int ks = 0; ks < K_STEPS; ++ks) b[ks] = ld8(Hin + q * sin + 32 * ks + 8 * g);
+#pragma unroll
+        for (int mt = 0; mt < M_TILES; ++mt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < K_STEPS; ++ks)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][ks], b[ks], acc, 0, 0, 0);
+            // lane holds rows 4g..4g+3 of this tile for query q
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int n = 16 * mt + 4 * g + i;
+                float v = acc[i];
+                if (RELU) v = fmaxf(v, 0.f);
+                if (MASK) v = ((float)Mask[q * smask + n] > 0.f) ? v : 0.f;
+                if (out_f32) {
+                    if (n < m_valid) out_f32[q * m_valid + n] = v;
+                } else {
+                    Hout[q * sout + n] = (__bf16)v;
+                }
+            }
+        }
+    }
+}
+
+struct TrainDev {
+    const float *targets;  // [n][3]
+    float *loss;           // accumulated (sum of per-element losses / (n*3))
+    float *gtable;         // FP32 table gradient
+    __bf16 *act_t;         // transposed activations: 5 blocks [feature][n] (X is 32 rows, H 64 rows)
+    __bf16 *grad_t;        // transposed output gradients: 5 blocks [neuron][n] (dH1..dH4 64 rows, dY 16 rows)
+    uint32_t n_pad;        // column stride of act_t / grad_t
+};
+
+// Fused encode + MLP forward (and, when TRAIN, loss and backward to the table) for 64 queries per wave.
+template <bool TRAIN>
+__device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__restrict__ points, uint32_t n,
+                                           int points_are_world, float *__restrict__ out, const TrainDev &td,
+                                           uint32_t base) {
+    constexpr int kActs = TRAIN ? 4 : 2;
+    __shared__ __bf16 X[kQ * kStrideX];
+    __shared__ __bf16 H[kActs][kQ * kStrideH];
+    __shared__ __bf16 G[TRAIN ? 2 : 1][kQ * kStrideG];
+    __shared__ float Y[kQ * 3];
+    const int lane = __lane_id();
+    const uint32_t q = base + lane;
+    const bool valid = q < n;
+    float p[3] = {0.f, 0.f, 0.f};
+    if (valid) {
+        for (int a = 0; a < 3; ++a) {
+            float v = points[3 * (size_t)q + a];
+            if (points_are_world) v = (v - c.lo[a]) * c.inv_ext[a];
+            p[a] = fminf(fmaxf(v, 0.f), 1.f);
+        }
+    }
+    {
+        float feat[kIn];
+        encode_point(c, p[0], p[1], p[2], feat);
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) X[lane * kStrideX + k] = (__bf16)(valid ? feat[k] : 0.f);
+    }
+    __syncthreads();
+    // forward: X -> H0 -> H1 -> H2 -> H3 -> Y
+    __bf16 *h0 = H[0];
+    __bf16 *h1 = H[1];
+    __bf16 *h2 = TRAIN ? H[2] : H[0];
+    __bf16 *h3 = TRAIN ? H[3] : H[1];
+    mlp_layer<4, 1, true, false>(c.wf[0], X, kStrideX, h0, kStrideH, nullptr, 0, nullptr, 0);
+    __syncthreads();
+    mlp_layer<4, 2, true, false>(c.wf[1], h0, kStrideH, h1, kStrideH, nullptr, 0, nullptr, 0);
+    __syncthreads();
+    mlp_layer<4, 2, true, false>(c.wf[2], h1, kStrideH, h2, kStrideH, nullptr, 0, nullptr, 0);
+    __syncthreads();
+    mlp_layer<4, 2, true, false>(c.wf[3], h2, kStrideH, h3, kStrideH, nullptr, 0, nullptr, 0);
+    __syncthreads();
+    mlp_layer<1, 2, false, false>(c.wf[4], h3, kStrideH, nullptr, 0, nullptr, 0, Y, 3);
+    __syncthreads();
+    if (!TRAIN) {
+        if (valid)
+            for (int o = 0; o < 3; ++o) out[3 * (size_t)q + o] = Y[lane * 3 + o];
+        return;
+    }
+    // relative L2 loss, d/dy = 2 (y - t) / (y^2 + 0.01) / (n * 3) with the denominator held constant
+    const float inv = 1.f / ((float)n * 3.f);
+    float lsum = 0.f;
+    {
+        __bf16 *dy = G[0];
+        for (int o = 0; o < 32; ++o) dy[lane * kStrideG + o] = (__bf16)0.f;
+        if (valid) {
+            for (int o = 0; o < 3; ++o) {
+                const float y = Y[lane * 3 + o], t = td.targets[3 * (size_t)q + o];
+                const float den = y * y + 0.01f;
+                lsum += (y - t) * (y - t) / den * inv;
+                dy[lane * kStrideG + o] = (__bf16)(2.f * (y - t) / den * inv);
+                if (out) out[3 * (size_t)q + o] = y;
+            }
+        }
+    }
+    // wave reduction of the loss, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+    if (lane == 0) atomicAdd(td.loss, lsum);
+    __syncthreads();
+    // backward: dH3 = (W5^T dY) * relu'(H3), dH2, dH1, dH0, dX = W1^T dH0
+    __bf16 *gA = G[0], *gB = G[1];
+    // transposed stores for the weight-gradient GEMMs: activations X,H0..H3 and gradients dH0..dH3,dY
+    auto store_t = [&](const __bf16 *src, int stride, int rows, __bf16 *dst) {
+        for (int k = 0; k < rows; ++k) dst[(size_t)k * td.n_pad + base + lane] = src[lane * stride + k];
+    };
+    const size_t blk = (size_t)64 * td.n_pad;
+    store_t(X, kStrideX, 32, td.act_t);
+    store_t(h0, kStrideH, 64, td.act_t + blk);
+    store_t(h1, kStrideH, 64, td.act_t + 2 * blk);
+    store_t(h2, kStrideH, 64, td.act_t + 3 * blk);
+    store_t(h3, kStrideH, 64, td.act_t + 4 * blk);
+    store_t(gA, kStrideG, 16, td.grad_t + 4 * blk);  // dY (3 rows used, 16 stored)
+    mlp_layer<4, 1, false, true>(c.wb[4], gA, kStrideG, gB, kStrideG, h3, kStrideH, nullptr, 0);  // dH3
+    __syncthreads();
+    store_t(gB, kStrideG, 64, td.grad_t + 3 * blk);
+    mlp_layer<4, 2, false, true>(c.wb[3], gB, kStrideG, gA, kStrideG, h2, kStrideH, nullptr, 0);  // dH2
+    __syncthreads();
+    store_t(gA, kStrideG, 64, td.grad_t + 2 * blk);
+    mlp_layer<4, 2, false, true>(c.wb[2], gA, kStrideG, gB, kStrideG, h1, kStrideH, nullptr, 0);  // dH1
+    __syncthreads();
+    store_t(gB, kStrideG, 64, td.grad_t + 1 * blk);
+    mlp_layer<4, 2, false, true>(c.wb[1], gB, kStrideG, gA, kStrideG, h0, kStrideH, nullptr, 0);  // dH0
+    __syncthreads();
+    store_t(gA, kStrideG, 64, td.grad_t);
+    mlp_layer<2, 2, false, false>(c.wb[0], gA, kStrideG, gB, kStrideG, nullptr, 0, nullptr, 0);  // dX (32)
+    __syncthreads();
+    if (valid && td.gtable) {
+        float gx[kIn];
+        for (int k = 0; k < kIn; ++k) gx[k] = (float)gB[lane * kStrideG + k];
+        encode_backward(c, p[0], p[1], p[2], gx, td.gtable);
+    }
+}
+
+// Persistent over 64-query tiles; the query count comes from n_dev (device) when given, capped at n.
+template <bool TRAIN>
+__global__ __launch_bounds__(64) void k_cache_mlp(CacheDev c, const float *__restrict__ points, uint32_t n,
+                                                  const uint32_t *__restrict__ n_dev, int points_are_world,
+                                                  float *__restrict__ out, TrainDev td) {
+    const uint32_t count = n_dev ? min(*n_dev, n) : n;
+    for (uint32_t tile = blockIdx.x; tile * (uint32_t)kQ < count; tile += gridDim.x) {
+        cache_tile<TRAIN>(c, points, count, points_are_world, out, td, tile * (uint32_t)kQ);
+        __syncthreads();
+    }
+}
+
+// Weight gradients dW_l[out][in] = sum_q dH_l[out][q] * A_{l-1}[in][q] as a K = n GEMM on MFMA.
+// grid.x = K chunks of 256 queries, grid.y = layer; one wave per (16-row, 16-col) output tile pair set.
+__global__ __launch_bounds__(256) void k_cache_dw(const __bf16 *__restrict__ act_t, const __bf16 *__restrict__ grad_t,
+                                                  uint32_t n_pad, uint32_t n, float *__restrict__ dw) {
+    const int layer = blockIdx.y;
+    const int rows = layer == 4 ? 3 : 64;
+    const int cols = layer == 0 ? 32 : 64;
+    const size_t blk = (size_t)64 * n_pad;
+    const __bf16 *A = grad_t + layer * blk;  // [rows][n]
+    const __bf16 *B = act_t + layer * blk;   // [cols][n]
+    size_t woff = 0;
+    for (int l = 0; l < layer; ++l) woff += (size_t)(l == 0 ? 64 * 32 : 64 * 64);
+    const int wave = threadIdx.x >> 6;
+    const int lane = __lane_id();
+    const int r = lane & 15, g = lane >> 4;
+    const uint32_t k0 = blockIdx.x * 256u;
+    const int mtiles = (rows + 15) / 16, ntiles = cols / 16;
+    for (int tile = wave; tile < mtiles * ntiles; tile += 4) {
+        const int mt = tile / ntiles, nt = tile % ntiles;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t ks = 0; ks < 256; ks += 32) {
+            const uint32_t kk = k0 + ks + 8 * g;
+            // columns beyond n are zero in both operands (the buffers are cleared up to n_pad)
+            const bf16x8 a = ld8(A + (size_t)(16 * mt + r) * n_pad + kk);
+            const bf16x8 b = ld8(B + (size_t)(16 * nt + r) * n_pad + kk);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        }
+        // lane holds rows 4g..4g+3 (out) of column r (in) of this tile
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int o = 16 * mt + 4 * g + i, in_ = 16 * nt + r;
+            if (o < rows) atomicAdd(&dw[woff + (size_t)o * cols + in_], acc[i]);
+        }
+    }
+}
+
+// Adam (Kingma & Ba 2015) over FP32 master parameters, with bias correction; writes the low-precision
+// copies used by the kernels.
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ grad, float *__restrict__ m,
+                       float *__restrict__ v, uint64_t count, float lr, float b1, float b2, float eps, float c1,
+                       float c2, __half *__restrict__ half_copy) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float gi = grad[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (half_copy) half_copy[i] = __float2half(pi);
+}
+
+// FP32 master weights -> BF16 forward [out_pad][in] and backward [in][out_pad32] copies
+__global__ void k_pack_weights(const float *__restrict__ w, __bf16 *__restrict__ wf, __bf16 *__restrict__ wb) {
+    // layer geometry: in, out, forward out pad, backward out pad, offsets
+    const int ins[5] = {32, 64, 64, 64, 64}, outs[5] = {64, 64, 64, 64, 3};
+    const int fpad[5] = {64, 64, 64, 64, 16}, bpad[5] = {64, 64, 64, 64, 32};
+    int off = 0, foff = 0, boff = 0;
+    for (int l = 0; l < 5; ++l) {
+        const int total_f = fpad[l] * ins[l];
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total_f; e += gridDim.x * blockDim.x) {
+            const int o = e / ins[l], i = e % ins[l];
+            wf[foff + e] = (__bf16)(o < outs[l] ? w[off + o * ins[l] + i] : 0.f);
+        }
+        const int total_b = ins[l] * bpad[l];
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total_b; e += gridDim.x * blockDim.x) {
+            const int i = e / bpad[l], o = e % bpad[l];
+            wb[boff + e] = (__bf16)(o < outs[l] ? w[off + o * ins[l] + i] : 0.f);
+        }
+        off += outs[l] * ins[l];
+        foff += total_f;
+        boff += total_b;
+    }
+}
+
+__global__ void k_half_to_f32(const __half *__restrict__ h, float *__restrict__ f, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = __half2float(h[i]);
+}
+__global__ void k_f32_to_half(const float *__restrict__ f, __half *__restrict__ h, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) h[i] = __float2half(f[i]);
+}
+
+// ---------------------------------------------------------------------------------------------
+static const int kLayerIn[5] = {32, 64, 64, 64, 64}, kLayerOut[5] = {64, 64, 64, 64, 3};
+static const int kFPad[5] = {64, 64, 64, 64, 16}, kBPad[5] = {64, 64, 64, 64, 32};
+
+size_t cache_num_weights() {
+    size_t s = 0;
+    for (int l = 0; l < 5; ++l) s += (size_t)kLayerIn[l] * kLayerOut[l];
+    return s;
+}
+
+static CacheDev make_dev(const CacheState &cs) {
+    CacheDev d;
+    std::memset(&d, 0, sizeof(d));
+    d.table = (const __half2 *)cs.table_h.ptr;
+    size_t fo = 0, bo = 0;
+    for (int l = 0; l < 5; ++l) {
+        d.wf[l] = (const __bf16 *)cs.wf.ptr + fo;
+        d.wb[l] = (const __bf16 *)cs.wb.ptr + bo;
+        fo += (size_t)kFPad[l] * kLayerIn[l];
+        bo += (size_t)kLayerIn[l] * kBPad[l];
+    }
+    for (int l = 0; l < 16; ++l) {
+        d.res[l] = cs.res[l];
+        d.dense[l] = cs.dense[l];
+    }
+    d.log2_table = cs.cfg.log2_table;
+    for (int a = 0; a < 3; ++a) {
+        d.lo[a] = cs.lo[a];
+        d.inv_ext[a] = cs.inv_ext[a];
+    }
+    return d;
+}
+
+hipError_t cache_pack(CacheState &cs, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_weights, dim3(16), dim3(256), 0, s, (const float *)cs.w.ptr, (__bf16 *)cs.wf.ptr,
+                       (__bf16 *)cs.wb.ptr);
+    const uint64_t n = cs.table_params();
+    hipLaunchKernelGGL(k_f32_to_half, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
+                       (const float *)cs.table_f.ptr, (__half *)cs.table_h.ptr, n);
+    return hipGetLastError();
+}
+
+hipError_t cache_infer(CacheState &cs, const float *points, uint32_t n, int world, float *out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    TrainDev td;
+    std::memset(&td, 0, sizeof(td));
+    hipLaunchKernelGGL(k_cache_mlp<false>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, n,
+                       (const uint32_t *)nullptr, world, out, td);
+    return hipGetLastError();
+}
+
+hipError_t cache_infer_queue(CacheState &cs, const float *points, const uint32_t *count_dev, uint32_t capacity,
+                             float *out, int grid, hipStream_t s) {
+    TrainDev td;
+    std::memset(&td, 0, sizeof(td));
+    hipLaunchKernelGGL(k_cache_mlp<false>, dim3(grid), dim3(64), 0, s, make_dev(cs), points, capacity, count_dev, 1,
+                       out, td);
+    return hipGetLastError();
+}
+
+hipError_t cache_gradients(CacheState &cs, const float *points, const float *targets, uint32_t n, int world,
+                           float *out, hipStream_t s) {
+    hipError_t e;
+    const uint32_t n_pad = ((n + 255u) / 256u) * 256u;
+    if ((e = cs.act_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
+    if ((e = cs.grad_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(cs.act_t.ptr, 0, (size_t)5 * 64 * n_pad * 2, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(cs.grad_t.ptr, 0, (size_t)5 * 64 * n_pad * 2, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(cs.gw.ptr, 0, cache_num_weights() * 4, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(cs.gtable.ptr, 0, cs.table_params() * 4, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(cs.loss.ptr, 0, 4, s)) != hipSuccess) return e;
+    if (n == 0) return hipSuccess;
+    TrainDev td;
+    td.targets = targets;
+    td.loss = (float *)cs.loss.ptr;
+    td.gtable = (float *)cs.gtable.ptr;
+    td.act_t = (__bf16 *)cs.act_t.ptr;
+    td.grad_t = (__bf16 *)cs.grad_t.ptr;
+    td.n_pad = n_pad;
+    hipLaunchKernelGGL(k_cache_mlp<true>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, n,
+                       (const uint32_t *)nullptr, world, out, td);
+    hipLaunchKernelGGL(k_cache_dw, dim3(n_pad / 256, 5), dim3(256), 0, s, (const __bf16 *)cs.act_t.ptr,
+                       (const __bf16 *)cs.grad_t.ptr, n_pad, n, (float *)cs.gw.ptr);
+    return hipGetLastError();
+}
+
+hipError_t cache_adam(CacheState &cs, hipStream_t s) {
+    cs.step += 1;
+    const float b1 = cs.cfg_beta1, b2 = cs.cfg_beta2, eps = cs.cfg_eps, lr = cs.cfg.learning_rate;
+    const float c1 = 1.f - std::pow(b1, (float)cs.step), c2 = 1.f - std::pow(b2, (float)cs.step);
+    const uint64_t nw = cache_num_weights();
+    hipLaunchKernelGGL(k_adam, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, (float *)cs.w.ptr,
+                       (const float *)cs.gw.ptr, (float *)cs.mw.ptr, (float *)cs.vw.ptr, nw, lr, b1, b2, eps, c1, c2,
+                       (__half *)nullptr);
+    const uint64_t nt = cs.table_params();
+    hipLaunchKernelGGL(k_adam, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, (float *)cs.table_f.ptr,
+                       (const float *)cs.gtable.ptr, (float *)cs.mt.ptr, (float *)cs.vt.ptr, nt, lr, b1, b2, eps, c1,
+                       c2, (__half *)cs.table_h.ptr);
+    hipLaunchKernelGGL(k_pack_weights, dim3(16), dim3(256), 0, s, (const float *)cs.w.ptr, (__bf16 *)cs.wf.ptr,
+                       (__bf16 *)cs.wb.ptr);
+    return hipGetLastError();
+}
+
+}  // namespace spk
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------
+using namespace spk;
+
+namespace {
+struct HostRng {
+    uint64_t s;
+    explicit HostRng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float uniform(float lo, float hi) { return lo + (hi - lo) * (float)((next() >> 40) * (1.0 / 16777216.0)); }
+};
+
+int cache_check(spk_context *ctx) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->cache) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "no radiance cache (spk_cache_create)");
+    return SPK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
+    if (!ctx || !config) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "the cache is sized from the uploaded scene box");
+    if (config->levels != 16 || config->features != 2)
+        return set_error(ctx, SPK_ERR_UNSUPPORTED, "the cache kernels are built for 16 levels x 2 features");
+    if (config->log2_table < 10 || config->log2_table > 24)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "log2_table must be in [10, 24]");
+    if (config->base_resolution < 2 || !(config->per_level_scale >= 1.f) || !(config->learning_rate > 0.f))
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "invalid cache resolution or learning rate");
+    delete ctx->cache;
+    ctx->cache = new CacheState();
+    CacheState &cs = *ctx->cache;
+    cs.cfg = *config;
+    if (cs.cfg.batch == 0) cs.cfg.batch = 1u << 16;
+    const uint64_t T = 1ull << cs.cfg.log2_table;
+    for (uint32_t l = 0; l < 16; ++l) {
+        const double r = std::floor((double)cs.cfg.base_resolution * std::pow((double)cs.cfg.per_level_scale, (double)l));
+        cs.res[l] = (uint32_t)std::max(1.0, std::min(r, 65535.0));
+        const uint64_t side = (uint64_t)cs.res[l] + 1;
+        cs.dense[l] = side * side * side <= T ? 1u : 0u;
+    }
+    // scene box with a small margin
+    const std::vector<float> &v = ctx->host_scene.verts;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < v.size(); i += 3)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], v[i + a]);
+            hi[a] = std::max(hi[a], v[i + a]);
+        }
+    float ext = 0.f;
+    for (int a = 0; a < 3; ++a) ext = std::max(ext, hi[a] - lo[a]);
+    for (int a = 0; a < 3; ++a) {
+        // a cube around the scene keeps the grid cells isotropic
+        const float c = 0.5f * (lo[a] + hi[a]);
+        cs.lo[a] = c - 0.51f * ext;
+        cs.inv_ext[a] = 1.f / (1.02f * ext);
+    }
+    const uint64_t ntab = cs.table_params();
+    const size_t nw = cache_num_weights();
+    hipError_t e;
+    if ((e = cs.table_h.ensure(ntab * 2)) != hipSuccess || (e = cs.table_f.ensure(ntab * 4)) != hipSuccess ||
+        (e = cs.mt.ensure(ntab * 4)) != hipSuccess || (e = cs.vt.ensure(ntab * 4)) != hipSuccess ||
+        (e = cs.gtable.ensure(ntab * 4)) != hipSuccess || (e = cs.w.ensure(nw * 4)) != hipSuccess ||
+        (e = cs.mw.ensure(nw * 4)) != hipSuccess || (e = cs.vw.ensure(nw * 4)) != hipSuccess ||
+        (e = cs.gw.ensure(nw * 4)) != hipSuccess || (e = cs.wf.ensure(16384 * 2)) != hipSuccess ||
+        (e = cs.wb.ensure(16384 * 2)) != hipSuccess || (e = cs.loss.ensure(16)) != hipSuccess)
+        return hip_fail(ctx, e, "cache allocation");
+    // initialisation: table U(-1e-4, 1e-4) (Instant NGP), weights Xavier-uniform
+    HostRng rng(config->seed);
+    std::vector<float> tab(ntab), w(nw);
+    for (auto &x : tab) x = rng.uniform(-1e-4f, 1e-4f);
+    size_t off = 0;
+    for (int l = 0; l < 5; ++l) {
+        const int in = l == 0 ? 32 : 64, out = l == 4 ? 3 : 64;
+        const float lim = std::sqrt(6.f / (float)(in + out));
+        for (int k = 0; k < in * out; ++k) w[off + k] = rng.uniform(-lim, lim);
+        off += (size_t)in * out;
+    }
+    if ((e = hipMemcpy(cs.table_f.ptr, tab.data(), ntab * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(cs.w.ptr, w.data(), nw * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(cs.mt.ptr, 0, ntab * 4)) != hipSuccess || (e = hipMemset(cs.vt.ptr, 0, ntab * 4)) != hipSuccess ||
+        (e = hipMemset(cs.mw.ptr, 0, nw * 4)) != hipSuccess || (e = hipMemset(cs.vw.ptr, 0, nw * 4)) != hipSuccess ||
+        (e = hipMemset(cs.wf.ptr, 0, 16384 * 2)) != hipSuccess || (e = hipMemset(cs.wb.ptr, 0, 16384 * 2)) != hipSuccess)
+        return hip_fail(ctx, e, "cache initialisation");
+    if ((e = cache_pack(cs, ctx->stream)) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return hip_fail(ctx, e, "cache pack");
+    ctx->on_scene_changed();
+    return SPK_OK;
+}
+
+int spk_cache_destroy(spk_context *ctx) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ctx->cache;
+    ctx->cache = nullptr;
+    ctx->on_scene_changed();
+    return SPK_OK;
+}
+
+int spk_cache_infer(spk_context *ctx, const float *points_dev, uint32_t n, int world, float *out_dev) {
+    int rc = cache_check(ctx);
+    if (rc != SPK_OK) return rc;
+    if (n && (!points_dev || !out_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null array");
+    hipError_t e = cache_infer(*ctx->cache, points_dev, n, world, out_dev, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cache inference");
+    return SPK_OK;
+}
+
+int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n, int world,
+                        float *loss, float *dweights, float *dtable) {
+    int rc = cache_check(ctx);
+    if (rc != SPK_OK) return rc;
+    if (n && (!points_dev || !targets_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null array");
+    CacheState &cs = *ctx->cache;
+    hipError_t e = cache_gradients(cs, points_dev, targets_dev, n, world, nullptr, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && loss) e = hipMemcpy(loss, cs.loss.ptr, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && dweights) e = hipMemcpy(dweights, cs.gw.ptr, cache_num_weights() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && dtable) e = hipMemcpy(dtable, cs.gtable.ptr, cs.table_params() * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cache gradients");
+    return SPK_OK;
+}
+
+int spk_cache_train(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n, int world,
+                    float *loss) {
+    int rc = cache_check(ctx);
+    if (rc != SPK_OK) return rc;
+    if (n == 0) return SPK_OK;
+    if (!points_dev || !targets_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null array");
+    CacheState &cs = *ctx->cache;
+    hipError_t e = cache_gradients(cs, points_dev, targets_dev, n, world, nullptr, ctx->stream);
+    if (e == hipSuccess) e = cache_adam(cs, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    float l = 0.f;
+    if (e == hipSuccess) e = hipMemcpy(&l, cs.loss.ptr, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cache training step");
+    ctx->cache_last_loss = l;
+    if (loss) *loss = l;
+    return SPK_OK;
+}
+
+int spk_cache_get_params(spk_context *ctx, float *weights, float *table) {
+    int rc = cache_check(ctx);
+    if (rc != SPK_OK) return rc;
+    CacheState &cs = *ctx->cache;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && weights) e = hipMemcpy(weights, cs.w.ptr, cache_num_weights() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && table) e = hipMemcpy(table, cs.table_f.ptr, cs.table_params() * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cache parameters");
+    return SPK_OK;
+}
+
+int spk_cache_set_params(spk_context *ctx, const float *weights, const float *table) {
+    int rc = cache_check(ctx);
+    if (rc != SPK_OK) return rc;
+    CacheState &cs = *ctx->cache;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && weights) e = hipMemcpy(cs.w.ptr, weights, cache_num_weights() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && table) e = hipMemcpy(cs.table_f.ptr, table, cs.table_params() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = cache_pack(cs, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cache parameters");
+    return SPK_OK;
+}
+
+int spk_cache_info(spk_context *ctx, uint64_t *num_weights, uint64_t *num_table_params, uint32_t *steps,
+                   float *last_loss) {
+    int rc = cache_check(ctx);
+    if (rc != SPK_OK) return rc;
+    if (num_weights) *num_weights = cache_num_weights();
+    if (num_table_params) *num_table_params = ctx->cache->table_params();
+    if (steps) *steps = ctx->cache->step;
+    if (last_loss) *last_loss = ctx->cache_last_loss;
+    return SPK_OK;
+}
+
+}  // extern "C"

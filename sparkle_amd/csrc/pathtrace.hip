@@ -6,9 +6,13 @@
 //   shade   - SamplePixel's loop body (CPURenderer.cpp:259-368) for each hit, in material order;
 //             survivors are appended to the next bounce's queue
 //   shadow  - occlusion rays of next-event estimation (opt-in, not in the reference)
+// With the learned radiance cache enabled, non-training paths stop at the cache bounce and take
+// the cache's prediction of the reflected radiance; a fraction of paths is traced in full and
+// records, at each vertex from the cache bounce on, the radiance gathered after that vertex as a
+// training target.
 // All kernels are persistent: the grid is sized to the device, waves fetch work from counters in
-// device memory, so the whole frame is captured once as a hipGraph and replayed; per-frame
-// values (the frame seed) travel through a small device buffer written before each replay.
+// device memory, so a frame is captured once as a hipGraph and replayed; per-frame values (the
+// frame seed) travel through a small device buffer written before each replay.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -17,13 +21,16 @@
 
 #include "../../include/spk_pt.h"
 #include "shade.h"
+#include "spk_cache.h"
 #include "spk_context.h"
 #include "traverse.h"
 
 namespace spk {
 
 constexpr int kMaxBounce = 64;
-constexpr int kCtrStride = 8;     // counters per bounce
+constexpr int kCtrStride = 8;  // counters per bounce
+constexpr int kCtrGlobal = kMaxBounce * kCtrStride;
+constexpr int kCtrTotal = kCtrGlobal + 16;
 constexpr int kMaxSortKeys = 1024;
 constexpr int kPtBlock = 128;
 
@@ -34,6 +41,16 @@ enum : uint32_t {
     CTR_SHADOWS = 3,  // shadow rays emitted at this bounce
     CTR_SHADOWW = 4,  // shadow work fetch
     CTR_SCATTER = 5,  // scatter work fetch
+};
+enum : uint32_t {
+    GCTR_CACHE_Q = 0,   // cache queries of the frame
+    GCTR_TRAIN = 1,     // training samples recorded (may exceed the capacity)
+    GCTR_APPLY = 2,     // cache apply work fetch
+    GCTR_TARGETS = 3,   // target work fetch
+};
+enum : uint32_t {
+    PF_SKIP_EMISSION = 1u,  // the previous vertex sampled the emitters directly
+    PF_TRAIN = 2u,          // a training path of the radiance cache
 };
 
 struct FrameDev {
@@ -51,7 +68,7 @@ struct PtArgs {
     uint32_t num_emitters;
     float emitter_total;
     float4 *rays;      // 2 per path: (org, tmin), (dir, tmax)
-    float4 *thr;       // throughput rgb, w = 1 when the next emitter hit must be ignored (NEE)
+    float4 *thr;       // throughput rgb, w = path flags (PF_* bits)
     float4 *col;       // colour rgb, w = valid flag
     uint4 *rng;        // xoshiro128+ state per path
     uint4 *nee_rng;    // second stream for light sampling
@@ -59,8 +76,7 @@ struct PtArgs {
     uint32_t *queue1;
     uint4 *hits;       // per queue slot: leaf triangle (or miss), t, u, v
     uint32_t *sorted;  // queue slots in material order
-    float4 *shadow;    // 3 per path: (org, tmax), (dir, path), (contribution)
-    uint32_t *shadow_q;
+    float4 *shadow;    // 3 per shadow ray: (org, tmax), (dir, path), (contribution)
     uint32_t *counters;
     uint32_t *hist;    // per bounce: counts[nkeys], offsets[nkeys]
     uint32_t nkeys;
@@ -70,7 +86,17 @@ struct PtArgs {
     spk_camera_desc cam;
     uint32_t width, height, npix, max_bounce, debug_mode, sky_enabled, flags, sort;
     float sky[3];
-    uint32_t grid;
+    float output_limit;
+    // learned radiance cache
+    uint32_t cache_on, cache_bounce, train_cap;
+    float train_prob;
+    float *cq_points;  // 3 per query
+    float4 *cq_meta;   // (throughput, path)
+    float *cq_out;     // 3 per query
+    float *tr_points;  // 3 per sample
+    float4 *tr_meta0;  // (colour before the vertex, path)
+    float4 *tr_meta1;  // (throughput at the vertex, 0)
+    float *tr_targets; // 3 per sample
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -93,12 +119,21 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *ctr, bool want) {
     return base + (uint32_t)__popcll(below);
 }
 
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
 // ---- generate: reseed per pixel, jitter, lens, camera ray (CPURenderer.cpp:220-238, 398-445) ----
 __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
     const uint32_t tid = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t stride = gridDim.x * kPtBlock;
     // clear this frame's counters and histograms (no memset nodes in the graph)
-    for (uint32_t i = tid; i < (uint32_t)(kMaxBounce * kCtrStride); i += stride) a.counters[i] = 0u;
+    for (uint32_t i = tid; i < (uint32_t)kCtrTotal; i += stride) a.counters[i] = 0u;
     for (uint32_t i = tid; i < a.max_bounce * 2u * a.nkeys; i += stride) a.hist[i] = 0u;
     const uint32_t frame_seed = a.frame->frame_seed;
     const float pw = 1.f / (float)(a.width - 1);
@@ -121,9 +156,15 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
         const F3 o = ld3(a.cam.position) + lens;
         const F3 focus = ld3(a.cam.lower_left) + ld3(a.cam.max_u) * u + ld3(a.cam.max_v) * v;
         const F3 d = normalized(focus - o);
+        uint32_t pf = 0u;
+        if (a.cache_on) {
+            // training paths: an independent hash of pixel and frame, so the path itself is unchanged
+            const float h = (float)(hash32(p * 0x9E3779B9u ^ hash32(frame_seed + 0x632BE5ABu)) >> 8) * (1.f / 16777216.f);
+            if (h < a.train_prob) pf |= PF_TRAIN;
+        }
         a.rays[2 * p] = make_float4(o.x, o.y, o.z, 0.f);
         a.rays[2 * p + 1] = make_float4(d.x, d.y, d.z, INFINITY);
-        a.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+        a.thr[p] = make_float4(1.f, 1.f, 1.f, __uint_as_float(pf));
         a.col[p] = make_float4(0.f, 0.f, 0.f, 1.f);
         a.rng[p] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
         if (a.flags & SPK_FRAME_NEE) {
@@ -245,9 +286,10 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
         const uint32_t base = wave_fetch(ctr + CTR_SHADE);
         if (base >= count) break;
         const uint32_t idx = base + lane_id();
-        bool cont = false, shadow = false;
+        bool cont = false, shadow = false, cache_q = false, train = false;
         uint32_t path = 0;
         float4 sh0 = make_float4(0.f, 0.f, 0.f, 0.f), sh1 = sh0, sh2 = sh0;
+        F3 loc = f3(0.f, 0.f, 0.f), Tv = loc, Cv = loc;
         if (idx < count) {
             const uint32_t slot = a.sort ? a.sorted[idx] : idx;
             path = queue[slot];
@@ -256,7 +298,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
             const F3 d = f3(rd4.x, rd4.y, rd4.z);
             const float4 T4 = a.thr[path];
             F3 T = f3(T4.x, T4.y, T4.z);
-            const bool skip_emission = T4.w != 0.f;
+            const uint32_t pf = __float_as_uint(T4.w);
             float4 C4 = a.col[path];
             F3 C = f3(C4.x, C4.y, C4.z);
             if (h.x == 0xFFFFFFFFu) {
@@ -272,7 +314,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                 const float4 t4 = lerp_attr(s.t0, s.t1, s.t2, u, v);
                 const F3 normal = normalized(f3(n4.x, n4.y, n4.z));
                 const F3 tangent = f3(t4.x, t4.y, t4.z);
-                const F3 loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
+                loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
                 const MaterialGpu mat = a.mats[s.material];
                 if (bounce == 0u && a.normal_out) {
                     a.normal_out[3 * (size_t)path] = normal.x;
@@ -280,8 +322,20 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                     a.normal_out[3 * (size_t)path + 2] = normal.z;
                 }
                 if (mat.emissive_flag) {
-                    if (!skip_emission) C = C + cmul(f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]), T);
+                    if (!(pf & PF_SKIP_EMISSION))
+                        C = C + cmul(f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]), T);
+                } else if (a.cache_on && bounce >= a.cache_bounce && !(pf & PF_TRAIN)) {
+                    // terminate into the cache: radiance leaving this vertex comes from the network
+                    if (bounce == a.cache_bounce) {
+                        cache_q = true;
+                        Tv = T;
+                    }
                 } else {
+                    if (a.cache_on && bounce >= a.cache_bounce && (pf & PF_TRAIN)) {
+                        train = true;
+                        Tv = T;
+                        Cv = C;
+                    }
                     uint4 rs = a.rng[path];
                     Rng rng{rs.x, rs.y, rs.z, rs.w};
                     F3 wi;
@@ -365,9 +419,10 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                             if (alive && !last) {
                                 cont = true;
                                 const F3 o2 = loc + wi * SPK_TOLERANCE;
+                                const uint32_t npf = (pf & PF_TRAIN) | ((nee && diffuse) ? PF_SKIP_EMISSION : 0u);
                                 a.rays[2 * path] = make_float4(o2.x, o2.y, o2.z, SPK_EPS);
                                 a.rays[2 * path + 1] = make_float4(wi.x, wi.y, wi.z, INFINITY);
-                                a.thr[path] = make_float4(T.x, T.y, T.z, (nee && diffuse) ? 1.f : 0.f);
+                                a.thr[path] = make_float4(T.x, T.y, T.z, __uint_as_float(npf));
                                 a.rng[path] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
                             }
                         }
@@ -383,6 +438,23 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
             a.shadow[3 * (size_t)ss] = sh0;
             a.shadow[3 * (size_t)ss + 1] = sh1;
             a.shadow[3 * (size_t)ss + 2] = sh2;
+        }
+        if (a.cache_on) {
+            const uint32_t cs = wave_append(a.counters + kCtrGlobal + GCTR_CACHE_Q, cache_q);
+            if (cache_q) {
+                a.cq_points[3 * (size_t)cs] = loc.x;
+                a.cq_points[3 * (size_t)cs + 1] = loc.y;
+                a.cq_points[3 * (size_t)cs + 2] = loc.z;
+                a.cq_meta[cs] = make_float4(Tv.x, Tv.y, Tv.z, __uint_as_float(path));
+            }
+            const uint32_t ts = wave_append(a.counters + kCtrGlobal + GCTR_TRAIN, train);
+            if (train && ts < a.train_cap) {
+                a.tr_points[3 * (size_t)ts] = loc.x;
+                a.tr_points[3 * (size_t)ts + 1] = loc.y;
+                a.tr_points[3 * (size_t)ts + 2] = loc.z;
+                a.tr_meta0[ts] = make_float4(Cv.x, Cv.y, Cv.z, __uint_as_float(path));
+                a.tr_meta1[ts] = make_float4(Tv.x, Tv.y, Tv.z, 0.f);
+            }
         }
     }
 }
@@ -419,15 +491,50 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) 
     }
 }
 
-// ---- finalize: per-sample clamp to 6 and the valid flag into the G-buffer (CPURenderer.cpp:408-411) ----
+// ---- cache: add throughput x predicted radiance; build training targets ----
+__global__ __launch_bounds__(kPtBlock) void k_cache_apply(PtArgs a) {
+    const uint32_t count = min(a.counters[kCtrGlobal + GCTR_CACHE_Q], a.npix);
+    for (;;) {
+        const uint32_t base = wave_fetch(a.counters + kCtrGlobal + GCTR_APPLY);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        if (i >= count) continue;
+        const float4 m = a.cq_meta[i];
+        const uint32_t path = __float_as_uint(m.w);
+        float4 C = a.col[path];
+        C.x += m.x * fmaxf(a.cq_out[3 * (size_t)i], 0.f);
+        C.y += m.y * fmaxf(a.cq_out[3 * (size_t)i + 1], 0.f);
+        C.z += m.z * fmaxf(a.cq_out[3 * (size_t)i + 2], 0.f);
+        a.col[path] = C;
+    }
+}
+
+// target = radiance gathered after the vertex / throughput at the vertex (clamped to [0, 64])
+__global__ __launch_bounds__(kPtBlock) void k_cache_targets(PtArgs a) {
+    const uint32_t count = min(a.counters[kCtrGlobal + GCTR_TRAIN], a.train_cap);
+    for (;;) {
+        const uint32_t base = wave_fetch(a.counters + kCtrGlobal + GCTR_TARGETS);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        if (i >= count) continue;
+        const float4 m0 = a.tr_meta0[i], m1 = a.tr_meta1[i];
+        const float4 C = a.col[__float_as_uint(m0.w)];
+        const float g[3] = {C.x - m0.x, C.y - m0.y, C.z - m0.z};
+        const float tt[3] = {m1.x, m1.y, m1.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.tr_targets[3 * (size_t)i + k] = fminf(fmaxf(g[k] / fmaxf(tt[k], 1e-4f), 0.f), 64.f);
+    }
+}
+
+// ---- finalize: per-sample clamp and the valid flag into the G-buffer (CPURenderer.cpp:408-411) ----
 __global__ __launch_bounds__(kPtBlock) void k_finalize(PtArgs a) {
     const uint32_t stride = gridDim.x * kPtBlock;
     for (uint32_t p = blockIdx.x * kPtBlock + threadIdx.x; p < a.npix; p += stride) {
         const float4 C = a.col[p];
         float4 o;
-        o.x = fminf(C.x, SPK_OUTPUT_LIMIT);
-        o.y = fminf(C.y, SPK_OUTPUT_LIMIT);
-        o.z = fminf(C.z, SPK_OUTPUT_LIMIT);
+        o.x = fminf(C.x, a.output_limit);
+        o.y = fminf(C.y, a.output_limit);
+        o.z = fminf(C.z, a.output_limit);
         o.w = C.w;
         reinterpret_cast<float4 *>(a.rgba_out)[p] = o;
     }
@@ -476,7 +583,7 @@ const PtEntry kPtEntries[] = {
 };
 
 struct PathState {
-    DeviceBuffer rays, thr, col, rng, nee_rng, queue0, queue1, hits, sorted, shadow, shadow_q, counters, hist;
+    DeviceBuffer rays, thr, col, rng, nee_rng, queue0, queue1, hits, sorted, shadow, counters, hist;
     DeviceBuffer emitters, emitter_cdf, frame;
     FrameDev *frame_host = nullptr;  // pinned
     uint32_t capacity = 0;
@@ -487,7 +594,6 @@ struct PathState {
     hipGraph_t graph_def = nullptr;
     PtArgs graph_args{};
     hipStream_t graph_stream = nullptr;
-    int graph_grid = 0;
     ~PathState() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_def) (void)hipGraphDestroy(graph_def);
@@ -501,11 +607,11 @@ struct PathState {
     }
 };
 
-static int device_grid(spk_context *ctx) {
+static int device_cus(spk_context *ctx) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
         cus = 256;
-    return cus * 8;  // 8 blocks of 128 threads per CU
+    return cus;
 }
 
 static int ensure_paths(spk_context *ctx, uint32_t npix) {
@@ -516,8 +622,7 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
         if ((e = hipHostMalloc((void **)&ps.frame_host, sizeof(FrameDev), hipHostMallocDefault)) != hipSuccess)
             return hip_fail(ctx, e, "pinned frame buffer");
         if ((e = ps.frame.ensure(sizeof(FrameDev))) != hipSuccess) return hip_fail(ctx, e, "frame buffer");
-        if ((e = ps.counters.ensure(kMaxBounce * kCtrStride * sizeof(uint32_t))) != hipSuccess)
-            return hip_fail(ctx, e, "counters");
+        if ((e = ps.counters.ensure(kCtrTotal * sizeof(uint32_t))) != hipSuccess) return hip_fail(ctx, e, "counters");
         if ((e = ps.hist.ensure((size_t)kMaxBounce * 2 * kMaxSortKeys * sizeof(uint32_t))) != hipSuccess)
             return hip_fail(ctx, e, "histograms");
     }
@@ -528,8 +633,7 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
             (e = ps.col.ensure(n * 16)) != hipSuccess || (e = ps.rng.ensure(n * 16)) != hipSuccess ||
             (e = ps.nee_rng.ensure(n * 16)) != hipSuccess || (e = ps.queue0.ensure(n * 4)) != hipSuccess ||
             (e = ps.queue1.ensure(n * 4)) != hipSuccess || (e = ps.hits.ensure(n * 16)) != hipSuccess ||
-            (e = ps.sorted.ensure(n * 4)) != hipSuccess || (e = ps.shadow.ensure(n * 48)) != hipSuccess ||
-            (e = ps.shadow_q.ensure(n * 4)) != hipSuccess)
+            (e = ps.sorted.ensure(n * 4)) != hipSuccess || (e = ps.shadow.ensure(n * 48)) != hipSuccess)
             return hip_fail(ctx, e, "path buffers");
         ps.capacity = npix;
     }
@@ -564,10 +668,24 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
         }
         ps.emitters_ready = true;
     }
+    if (ctx->cache) {
+        CacheState &cs = *ctx->cache;
+        if (cs.capacity < npix) {
+            const size_t n = npix, tcap = cs.cfg.batch;
+            if ((e = cs.cq_points.ensure(n * 12)) != hipSuccess || (e = cs.cq_meta.ensure(n * 16)) != hipSuccess ||
+                (e = cs.cq_out.ensure(n * 12)) != hipSuccess || (e = cs.tr_points.ensure(tcap * 12)) != hipSuccess ||
+                (e = cs.tr_meta0.ensure(tcap * 16)) != hipSuccess || (e = cs.tr_meta1.ensure(tcap * 16)) != hipSuccess ||
+                (e = cs.tr_targets.ensure(tcap * 12)) != hipSuccess)
+                return hip_fail(ctx, e, "cache frame buffers");
+            cs.capacity = npix;
+            ps.drop_graph();
+        }
+    }
     return SPK_OK;
 }
 
-static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t s, int grid) {
+static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t s, int grid, CacheState *cache,
+                                int cus) {
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kPtBlock), 0, s, a);
     for (uint32_t b = 0; b < a.max_bounce; ++b) {
         hipLaunchKernelGGL(pe.extend, dim3(grid), dim3(kPtBlock), 0, s, a, b);
@@ -576,7 +694,15 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
             hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(kPtBlock), 0, s, a, b);
         }
         hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kPtBlock), 0, s, a, b);
-        if ((a.flags & SPK_FRAME_NEE) && a.num_emitters) hipLaunchKernelGGL(pe.shadow, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+        if ((a.flags & SPK_FRAME_NEE) && a.num_emitters)
+            hipLaunchKernelGGL(pe.shadow, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+    }
+    if (a.cache_on) {
+        hipError_t e = cache_infer_queue(*cache, a.cq_points, a.counters + kCtrGlobal + GCTR_CACHE_Q, a.npix, a.cq_out,
+                                         cus * 8, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(kPtBlock), 0, s, a);
+        hipLaunchKernelGGL(k_cache_targets, dim3(grid), dim3(kPtBlock), 0, s, a);
     }
     hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(kPtBlock), 0, s, a);
     return hipGetLastError();
@@ -596,6 +722,8 @@ void spk_context::on_scene_changed() {
 void spk_context::release_all() {
     delete paths;
     paths = nullptr;
+    delete cache;
+    cache = nullptr;
     scene.nodes.release();
     scene.tris.release();
     scene.shade.release();
@@ -631,6 +759,9 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     // part of the device path
     if (frame->debug_mode == 2u || frame->debug_mode == 5u || frame->debug_mode == 6u || frame->debug_mode > 11u)
         return set_error(ctx, SPK_ERR_UNSUPPORTED, "debug mode %u is not supported on the device", frame->debug_mode);
+    const bool cache_on = (frame->flags & SPK_FRAME_CACHE) != 0;
+    if (cache_on && !ctx->cache)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "SPK_FRAME_CACHE needs spk_cache_create first");
     const uint32_t npix = frame->width * frame->height;
     int rc = ensure_paths(ctx, npix);
     if (rc != SPK_OK) return rc;
@@ -661,11 +792,10 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     a.hits = (uint4 *)ps.hits.ptr;
     a.sorted = (uint32_t *)ps.sorted.ptr;
     a.shadow = (float4 *)ps.shadow.ptr;
-    a.shadow_q = (uint32_t *)ps.shadow_q.ptr;
     a.counters = (uint32_t *)ps.counters.ptr;
     a.hist = (uint32_t *)ps.hist.ptr;
     a.nkeys = ds.num_materials + 1u;
-    a.sort = a.nkeys <= (uint32_t)kMaxSortKeys && !(frame->flags & 0x80000000u) ? 1u : 0u;
+    a.sort = a.nkeys <= (uint32_t)kMaxSortKeys && !(frame->flags & SPK_FRAME_NO_SORT) ? 1u : 0u;
     if (!a.sort) a.nkeys = 2u;
     a.rgba_out = rgba_dev;
     a.normal_out = normal_dev;
@@ -677,10 +807,26 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     a.max_bounce = frame->max_bounce;
     a.debug_mode = frame->debug_mode;
     a.sky_enabled = frame->sky_enabled;
-    a.flags = frame->flags;
+    a.flags = frame->flags & ~SPK_FRAME_NO_SORT;
     for (int c = 0; c < 3; ++c) a.sky[c] = frame->sky_color[c];
-    const int grid = device_grid(ctx);
-    a.grid = (uint32_t)grid;
+    a.output_limit = frame->output_limit > 0.f ? frame->output_limit : SPK_OUTPUT_LIMIT;
+    if (cache_on) {
+        CacheState &cs = *ctx->cache;
+        a.cache_on = 1u;
+        a.cache_bounce = cs.cfg.terminate_bounce;
+        a.train_cap = cs.cfg.batch;
+        a.train_prob = cs.cfg.train_fraction > 0.f ? cs.cfg.train_fraction
+                                                   : std::min(1.f, (float)cs.cfg.batch / (float)npix / 2.f);
+        a.cq_points = (float *)cs.cq_points.ptr;
+        a.cq_meta = (float4 *)cs.cq_meta.ptr;
+        a.cq_out = (float *)cs.cq_out.ptr;
+        a.tr_points = (float *)cs.tr_points.ptr;
+        a.tr_meta0 = (float4 *)cs.tr_meta0.ptr;
+        a.tr_meta1 = (float4 *)cs.tr_meta1.ptr;
+        a.tr_targets = (float *)cs.tr_targets.ptr;
+    }
+    const int cus = device_cus(ctx);
+    const int grid = cus * 8;
 
     hipError_t e;
     ps.frame_host->frame_seed = frame->frame_seed;
@@ -692,7 +838,7 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
         ps.drop_graph();
         if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return hip_fail(ctx, e, "begin capture");
-        hipError_t le = enqueue_frame(a, *pe, ctx->stream, grid);
+        hipError_t le = enqueue_frame(a, *pe, ctx->stream, grid, ctx->cache, cus);
         e = hipStreamEndCapture(ctx->stream, &ps.graph_def);
         if (le != hipSuccess) return hip_fail(ctx, le, "frame launch during capture");
         if (e != hipSuccess) return hip_fail(ctx, e, "end capture");
@@ -716,14 +862,35 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
         e = hipGraphLaunch(ps.graph, ctx->stream);
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
+    // online training of the radiance cache on this frame's samples
+    uint32_t trained = 0;
+    if (cache_on && frame->max_bounce > 0) {
+        uint32_t cnt = 0;
+        if ((e = hipMemcpyAsync(&cnt, (uint32_t *)ps.counters.ptr + kCtrGlobal + GCTR_TRAIN, 4, hipMemcpyDeviceToHost,
+                                ctx->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+            return hip_fail(ctx, e, "training sample count");
+        CacheState &cs = *ctx->cache;
+        trained = std::min(cnt, cs.cfg.batch);
+        if (trained >= 64) {
+            if ((e = cache_gradients(cs, (const float *)cs.tr_points.ptr, (const float *)cs.tr_targets.ptr, trained, 1,
+                                     nullptr, ctx->stream)) != hipSuccess ||
+                (e = cache_adam(cs, ctx->stream)) != hipSuccess)
+                return hip_fail(ctx, e, "cache training");
+        }
+    }
     if (stats) (void)hipEventRecord(t1, ctx->stream);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "frame");
+    if (cache_on && trained >= 64) {
+        float l = 0.f;
+        if ((e = hipMemcpy(&l, ctx->cache->loss.ptr, 4, hipMemcpyDeviceToHost)) == hipSuccess) ctx->cache_last_loss = l;
+    }
     if (stats) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, t0, t1);
         (void)hipEventDestroy(t0);
         (void)hipEventDestroy(t1);
-        std::vector<uint32_t> c(kMaxBounce * kCtrStride);
+        std::vector<uint32_t> c(kCtrTotal);
         if ((e = hipMemcpy(c.data(), ps.counters.ptr, c.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
             return hip_fail(ctx, e, "frame counters");
         uint64_t rays = 0, shadows = 0;

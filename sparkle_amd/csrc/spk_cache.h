@@ -1,0 +1,41 @@
+// Host-side state of the learned radiance cache (kernels in cache.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/spk_pt.h"
+#include "spk_context.h"
+
+namespace spk {
+
+struct CacheState {
+    spk_cache_config cfg{};
+    float cfg_beta1 = 0.9f, cfg_beta2 = 0.99f, cfg_eps = 1e-8f;
+    uint32_t res[16] = {0};
+    uint32_t dense[16] = {0};
+    float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};
+    DeviceBuffer table_h, table_f, mt, vt, gtable;  // hash table (FP16 copy, FP32 master, Adam, gradient)
+    DeviceBuffer w, mw, vw, gw, wf, wb;             // MLP weights (FP32 master, Adam, gradient, BF16 copies)
+    DeviceBuffer act_t, grad_t, loss;               // training scratch
+    // per-frame records of the path tracer
+    DeviceBuffer cq_points, cq_meta, cq_out;        // cache queries: point, (throughput, path), radiance
+    DeviceBuffer tr_points, tr_meta0, tr_meta1, tr_targets;  // training samples
+    uint32_t capacity = 0;
+    uint32_t step = 0;
+    uint64_t table_params() const { return (uint64_t)cfg.levels * (1ull << cfg.log2_table) * cfg.features; }
+};
+
+size_t cache_num_weights();
+hipError_t cache_pack(CacheState &cs, hipStream_t s);
+hipError_t cache_infer(CacheState &cs, const float *points, uint32_t n, int world, float *out, hipStream_t s);
+hipError_t cache_gradients(CacheState &cs, const float *points, const float *targets, uint32_t n, int world,
+                           float *out, hipStream_t s);
+hipError_t cache_adam(CacheState &cs, hipStream_t s);
+// Inference over a queue whose length lives in device memory (capturable in a graph): `grid`
+// persistent single-wave blocks walk the 64-query tiles.
+hipError_t cache_infer_queue(CacheState &cs, const float *points, const uint32_t *count_dev, uint32_t capacity,
+                             float *out, int grid, hipStream_t s);
+
+}  // namespace spk

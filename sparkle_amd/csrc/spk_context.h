@@ -50,7 +50,8 @@ struct DeviceScene {
     std::vector<uint32_t> host_emitters;
 };
 
-struct PathState;  // path-tracer buffers, defined with the path-tracer kernels
+struct PathState;   // path-tracer buffers, defined with the path-tracer kernels
+struct CacheState;  // learned radiance cache (spk_cache.h)
 
 }  // namespace spk
 
@@ -68,6 +69,8 @@ struct spk_context {
     spk_camera_desc camera{};
     bool has_camera = false;
     spk::PathState *paths = nullptr;
+    spk::CacheState *cache = nullptr;
+    float cache_last_loss = 0.f;
 
     uint32_t *invalid_counter() { return static_cast<uint32_t *>(counters.ptr); }
     int check_invalid_rays();

@@ -58,6 +58,16 @@ struct TriTest {
     float t, U, V, W, det;
 };
 
+__device__ TriTest tri_wt_precise(WtSetup r, float ox, float oy, float oz, const float4 a, const float4 b,
+                                  const float4 c);
+
+#define SPK_WT_ERR (8.f * 5.9604644775390625e-08f)     /* 8u, u = 2^-24 */
+#define SPK_WT_ERR_ABS (2.f * 5.9604644775390625e-08f) /* 2u */
+
+// The FP32 edge functions are trusted only when a forward error bound certifies their signs; an
+// uncertified triangle (ray within rounding distance of an edge, or the triangle seen edge-on) is
+// re-evaluated in double-float arithmetic. This takes the place of the paper's FP64 fallback and
+// removes the false hits that exact-zero edge values would otherwise produce.
 __device__ __forceinline__ TriTest tri_wt(const WtSetup &r, float ox, float oy, float oz, const float4 a,
                                           const float4 b, const float4 c) {
     TriTest res;
@@ -66,15 +76,35 @@ __device__ __forceinline__ TriTest tri_wt(const WtSetup &r, float ox, float oy, 
     const float Bx0 = b.x - ox, By0 = b.y - oy, Bz0 = b.z - oz;
     const float Cx0 = c.x - ox, Cy0 = c.y - oy, Cz0 = c.z - oz;
     const float Akz = vsel(r.kz, Ax0, Ay0, Az0), Bkz = vsel(r.kz, Bx0, By0, Bz0), Ckz = vsel(r.kz, Cx0, Cy0, Cz0);
-    const float Ax = vsel(r.kx, Ax0, Ay0, Az0) - r.Sx * Akz;
-    const float Ay = vsel(r.ky, Ax0, Ay0, Az0) - r.Sy * Akz;
-    const float Bx = vsel(r.kx, Bx0, By0, Bz0) - r.Sx * Bkz;
-    const float By = vsel(r.ky, Bx0, By0, Bz0) - r.Sy * Bkz;
-    const float Cx = vsel(r.kx, Cx0, Cy0, Cz0) - r.Sx * Ckz;
-    const float Cy = vsel(r.ky, Cx0, Cy0, Cz0) - r.Sy * Ckz;
+    const float Akx = vsel(r.kx, Ax0, Ay0, Az0), Aky = vsel(r.ky, Ax0, Ay0, Az0);
+    const float Bkx = vsel(r.kx, Bx0, By0, Bz0), Bky = vsel(r.ky, Bx0, By0, Bz0);
+    const float Ckx = vsel(r.kx, Cx0, Cy0, Cz0), Cky = vsel(r.ky, Cx0, Cy0, Cz0);
+    const float sAx = r.Sx * Akz, sAy = r.Sy * Akz;
+    const float sBx = r.Sx * Bkz, sBy = r.Sy * Bkz;
+    const float sCx = r.Sx * Ckz, sCy = r.Sy * Ckz;
+    const float Ax = Akx - sAx;
+    const float Ay = Aky - sAy;
+    const float Bx = Bkx - sBx;
+    const float By = Bky - sBy;
+    const float Cx = Ckx - sCx;
+    const float Cy = Cky - sCy;
     const float U = Cx * By - Cy * Bx;
     const float V = Ax * Cy - Ay * Cx;
     const float W = Bx * Ay - By * Ax;
+    const float mAx = fabsf(Akx) + fabsf(sAx), mAy = fabsf(Aky) + fabsf(sAy);
+    const float mBx = fabsf(Bkx) + fabsf(sBx), mBy = fabsf(Bky) + fabsf(sBy);
+    const float mCx = fabsf(Ckx) + fabsf(sCx), mCy = fabsf(Cky) + fabsf(sCy);
+    const float eU = SPK_WT_ERR * (mCx * mBy + mCy * mBx) + SPK_WT_ERR_ABS * fabsf(U);
+    const float eV = SPK_WT_ERR * (mAx * mCy + mAy * mCx) + SPK_WT_ERR_ABS * fabsf(V);
+    const float eW = SPK_WT_ERR * (mBx * mAy + mBy * mAx) + SPK_WT_ERR_ABS * fabsf(W);
+    // reject quickly when two certified values already disagree in sign
+    const bool cU = fabsf(U) > eU, cV = fabsf(V) > eV, cW = fabsf(W) > eW;
+    if (!(cU && cV && cW)) {
+        const bool mixed = (cU && cV && ((U < 0.f) != (V < 0.f))) || (cU && cW && ((U < 0.f) != (W < 0.f))) ||
+                           (cV && cW && ((V < 0.f) != (W < 0.f)));
+        if (mixed) return res;
+        return tri_wt_precise(r, ox, oy, oz, a, b, c);
+    }
     if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return res;
     const float det = U + V + W;
     if (det == 0.f) return res;
@@ -130,6 +160,43 @@ __device__ __forceinline__ DD dd_add(DD a, DD b) {
     return r;
 }
 __device__ __forceinline__ DD dd_neg(DD a) { return DD{-a.h, -a.l}; }
+__device__ __forceinline__ DD dd_sel(int k, DD x, DD y, DD z) { return k == 0 ? x : (k == 1 ? y : z); }
+
+// tri_wt evaluated in double-float arithmetic: exact translation, shear and edge functions carried
+// to ~44 bits, signs decided on the rounded high parts. Taken only for uncertified triangles.
+__device__ __noinline__ TriTest tri_wt_precise(WtSetup r, float ox, float oy, float oz, const float4 a,
+                                               const float4 b, const float4 c) {
+    TriTest res;
+    res.hit = false;
+    const DD A0 = dd_diff(a.x, ox), A1 = dd_diff(a.y, oy), A2 = dd_diff(a.z, oz);
+    const DD B0 = dd_diff(b.x, ox), B1 = dd_diff(b.y, oy), B2 = dd_diff(b.z, oz);
+    const DD C0 = dd_diff(c.x, ox), C1 = dd_diff(c.y, oy), C2 = dd_diff(c.z, oz);
+    const DD Akz = dd_sel(r.kz, A0, A1, A2), Bkz = dd_sel(r.kz, B0, B1, B2), Ckz = dd_sel(r.kz, C0, C1, C2);
+    const DD Sx{r.Sx, 0.f}, Sy{r.Sy, 0.f}, Sz{r.Sz, 0.f};
+    const DD Ax = dd_add(dd_sel(r.kx, A0, A1, A2), dd_neg(dd_mul(Sx, Akz)));
+    const DD Ay = dd_add(dd_sel(r.ky, A0, A1, A2), dd_neg(dd_mul(Sy, Akz)));
+    const DD Bx = dd_add(dd_sel(r.kx, B0, B1, B2), dd_neg(dd_mul(Sx, Bkz)));
+    const DD By = dd_add(dd_sel(r.ky, B0, B1, B2), dd_neg(dd_mul(Sy, Bkz)));
+    const DD Cx = dd_add(dd_sel(r.kx, C0, C1, C2), dd_neg(dd_mul(Sx, Ckz)));
+    const DD Cy = dd_add(dd_sel(r.ky, C0, C1, C2), dd_neg(dd_mul(Sy, Ckz)));
+    const DD U = dd_add(dd_mul(Cx, By), dd_neg(dd_mul(Cy, Bx)));
+    const DD V = dd_add(dd_mul(Ax, Cy), dd_neg(dd_mul(Ay, Cx)));
+    const DD W = dd_add(dd_mul(Bx, Ay), dd_neg(dd_mul(By, Ax)));
+    if ((U.h < 0.f || V.h < 0.f || W.h < 0.f) && (U.h > 0.f || V.h > 0.f || W.h > 0.f)) return res;
+    const DD det = dd_add(dd_add(U, V), W);
+    if (det.h == 0.f) return res;
+    const DD T =
+        dd_add(dd_add(dd_mul(U, dd_mul(Sz, Akz)), dd_mul(V, dd_mul(Sz, Bkz))), dd_mul(W, dd_mul(Sz, Ckz)));
+    if ((det.h < 0.f && T.h > 0.f) || (det.h > 0.f && T.h < 0.f)) return res;
+    const float detf = det.h + det.l;
+    res.hit = true;
+    res.t = (T.h + T.l) / detf;
+    res.U = U.h;
+    res.V = V.h;
+    res.W = W.h;
+    res.det = detf;
+    return res;
+}
 __device__ __forceinline__ float refine_t(float ox, float oy, float oz, float dx, float dy, float dz, const float4 v0,
                                           const float4 v1, const float4 v2, float t_fallback) {
     const DD e1x = dd_diff(v1.x, v0.x), e1y = dd_diff(v1.y, v0.y), e1z = dd_diff(v1.z, v0.z);

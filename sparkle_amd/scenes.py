@@ -280,3 +280,104 @@ def cornell_camera(width=256, height=256):
     # CameraComponent defaults: 35 mm lens on a 24 mm sensor at f/22 -> 37.85 deg, radius 7.95e-4
     return camera_desc(position=(0.0, -3.4, 1.0), front=(0.0, 1.0, 0.0), vertical_fov=None, focus_distance=3.4,
                        aperture_radius=0.035 / 22.0 * 0.5, aspect=width / height)
+
+
+# ---------------------------------------------------------------------------------------------
+# many-instance scene: seed-placed icospheres, tori and noise-displaced boxes on a 20 x 20 ground
+# ---------------------------------------------------------------------------------------------
+_ICO_CACHE = {}
+
+
+def _unit_icosphere(subdiv):
+    if subdiv not in _ICO_CACHE:
+        _ICO_CACHE[subdiv] = icosphere(subdiv, 1.0, (0, 0, 0))
+    return _ICO_CACHE[subdiv]
+
+
+def displaced_box(n, amp, rng):
+    """Box [-1,1]^3 with each face split into n x n cells and displaced along its normal by value
+    noise of amplitude `amp`; normals are the face normals, tangents the face u axis."""
+    pos_all, idx_all, nrm_all, tan_all, uv_all = [], [], [], [], []
+    faces = [((1, 0, 0), (0, 1, 0)), ((-1, 0, 0), (0, -1, 0)), ((0, 1, 0), (-1, 0, 0)), ((0, -1, 0), (1, 0, 0)),
+             ((0, 0, 1), (1, 0, 0)), ((0, 0, -1), (-1, 0, 0))]
+    g = np.linspace(-1.0, 1.0, n + 1)
+    base = 0
+    for nv, tv in faces:
+        nv = np.array(nv, float)
+        tv = np.array(tv, float)
+        bv = np.cross(nv, tv)
+        uu, vv = np.meshgrid(g, g, indexing="xy")
+        disp = amp * (value_noise(n + 1, octaves=3, base_freq=2, seed=int(rng.integers(1 << 30))) - 0.5)
+        # keep face borders closed: no displacement on the rim
+        rim = (np.abs(uu) > 1 - 1e-9) | (np.abs(vv) > 1 - 1e-9)
+        disp[rim] = 0.0
+        p = nv[None, None, :] * (1.0 + disp)[..., None] + uu[..., None] * tv + vv[..., None] * bv
+        pos_all.append(p.reshape(-1, 3))
+        nrm_all.append(np.tile(nv, ((n + 1) ** 2, 1)))
+        tan_all.append(np.tile(np.concatenate([tv, [1.0]]), ((n + 1) ** 2, 1)))
+        uv_all.append(np.stack([(uu + 1) / 2, (vv + 1) / 2], -1).reshape(-1, 2))
+        ci, cj = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+        a = (cj * (n + 1) + ci).reshape(-1) + base
+        b, c, d = a + 1, a + n + 2, a + n + 1
+        idx_all.append(np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)]))
+        base += (n + 1) ** 2
+    return (np.concatenate(pos_all), np.concatenate(idx_all).astype(np.uint32), np.concatenate(nrm_all),
+            np.concatenate(tan_all), np.concatenate(uv_all))
+
+
+def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance=25.0):
+    """`count` seed-placed instances flattened to world space (about `target_tris` triangles):
+    icospheres (subdivision 2-5), tori (R/r = 3) and noise-displaced boxes on a 20 x 20 ground plane;
+    materials 60% Lambertian, 30% GGX metal (roughness U[0.05, 0.6]), 10% dielectric (IOR 1.5);
+    three emissive area lights. Returns (scene, camera-at-1920x1080)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    ground = sc.add_material(base_color=(0.5, 0.5, 0.5), metallic=0.0, roughness=1.0)
+    _add(sc, _quad((-10, -10, 0), (10, -10, 0), (10, 10, 0), (-10, 10, 0), (0, 0, 1), (1, 0, 0)), ground)
+    # per-instance triangle budget steering the mesh resolutions toward target_tris
+    per = target_tris / count
+    side = int(math.ceil(math.sqrt(count)))
+    cell = 19.0 / side
+    for k in range(count):
+        gx, gy = k % side, k // side
+        cx = -9.5 + (gx + 0.5) * cell + rng.uniform(-0.2, 0.2) * cell
+        cy = -9.5 + (gy + 0.5) * cell + rng.uniform(-0.2, 0.2) * cell
+        size = rng.uniform(0.18, 0.38) * cell
+        kind = rng.integers(3)
+        if kind == 0:
+            sub = int(rng.integers(2, 6))
+            p, i, n, t, uv = _unit_icosphere(sub)
+            p = p * size + np.array([cx, cy, size])
+        elif kind == 1:
+            r = size / 4.0
+            nmaj = int(max(16, math.sqrt(per * 2)))
+            nmin = max(8, nmaj // 2)
+            p, i, n, t, uv = torus(3 * r, r, nmaj, nmin, center=(cx, cy, r * 1.05))
+        else:
+            nb = int(max(4, math.sqrt(per / 12)))
+            p, i, n, t, uv = displaced_box(nb, 0.25, rng)
+            ang = rng.uniform(0, math.pi)
+            c, s = math.cos(ang), math.sin(ang)
+            R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
+            p = (p * size) @ R.T + np.array([cx, cy, size * 1.3])
+            n = n @ R.T
+            t = t.copy()
+            t[:, :3] = t[:, :3] @ R.T
+        u = rng.random()
+        if u < 0.6:
+            mat = sc.add_material(base_color=tuple(rng.uniform(0.2, 0.8, 3)), metallic=0.0, roughness=1.0)
+        elif u < 0.9:
+            mat = sc.add_material(base_color=tuple(rng.uniform(0.5, 0.95, 3)), metallic=1.0,
+                                  roughness=float(rng.uniform(0.05, 0.6)))
+        else:
+            mat = sc.add_material(base_color=(1.0, 1.0, 1.0), metallic=0.0, roughness=0.0, eta=1.5,
+                                  type=MATERIAL_DIELECTRIC)
+        _add(sc, (p, i, n, t, uv), mat)
+    for (lx, ly) in ((-5.0, -3.0), (4.0, -4.0), (0.0, 5.0)):
+        light = sc.add_material(base_color=(1, 1, 1), emissive=(light_radiance,) * 3, metallic=0.0)
+        h = 6.0
+        _add(sc, _quad((lx - 1, ly - 1, h), (lx - 1, ly + 1, h), (lx + 1, ly + 1, h), (lx + 1, ly - 1, h),
+                       (0, 0, -1), (1, 0, 0)), light)
+    cam = camera_desc(position=(0.0, -16.0, 7.0), front=(0.0, 1.0, -0.42), vertical_fov=math.radians(50.0),
+                      focus_distance=16.0, aperture_radius=0.0, aspect=1920 / 1080)
+    return sc, cam

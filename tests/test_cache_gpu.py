@@ -1,0 +1,178 @@
+"""Learned radiance cache on the MI355X against the FP64 oracle: BF16-MFMA forward and backward,
+the hash-grid gradient scatter, Adam, and online fitting.
+
+Tolerances: the device computes the MLP on BF16 operands (8-bit mantissa) with FP32 accumulation,
+so outputs are compared to the FP64 oracle (fed the same FP16 table and BF16-rounded weights) within
+3% relative + 2e-3 absolute, and gradients by relative vector error below 5%.
+"""
+import numpy as np
+import pytest
+
+import sparkle_amd as spk
+from sparkle_amd import scenes
+from oracle import refcpu
+from tests.conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+LOG2 = 14  # a small table keeps the FP64 oracle fast
+
+
+def to_dev(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def setup_cache(ctx, seed=0, log2=LOG2, lr=1e-2):
+    sc = scenes.cornell_scene()
+    ctx.upload_scene(sc)
+    ctx.cache_create(spk.cache_config(log2_table=log2, learning_rate=lr, seed=seed))
+    cfg = refcpu.cache_cfg(log2_table=log2)
+    rng = np.random.Generator(np.random.PCG64(seed + 100))
+    info = ctx.cache_info()
+    # larger weights and table values than the initialisation, so every layer is exercised
+    w = rng.normal(0, 0.15, info["num_weights"]).astype(np.float32)
+    t = rng.normal(0, 0.5, info["num_table_params"]).astype(np.float32)
+    ctx.cache_set_params(w, t)
+    return cfg, w, t
+
+
+def oracle_forward(cfg, w, t, pts, targets):
+    t16 = t.astype(np.float16).astype(np.float32)
+    feats = refcpu.hashgrid_encode(cfg, t16, pts)
+    feats_bf = refcpu.to_bf16(feats.astype(np.float32)).astype(np.float64)
+    wb = refcpu.to_bf16(w).astype(np.float64)
+    return refcpu.mlp_loss_grad(cfg, wb, feats_bf, targets) + (feats,)
+
+
+def test_forward_matches_fp64(gpu_ctx):
+    import torch
+
+    cfg, w, t = setup_cache(gpu_ctx)
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = 777  # not a multiple of the 64-query tile
+    pts = rng.random((n, 3)).astype(np.float32)
+    out = torch.zeros(n * 3, device="cuda")
+    print_case("cache forward", n)
+    gpu_ctx.cache_infer(to_dev(pts.reshape(-1)), out, n)
+    g = out.cpu().numpy().reshape(n, 3)
+    _, ref, _, _, _ = oracle_forward(cfg, w, t, pts, np.zeros((n, 3)))
+    err = np.abs(g - ref)
+    assert (err <= 0.03 * np.abs(ref) + 2e-3).mean() > 0.995, err.max()
+    assert np.abs(ref).mean() > 0.05  # the test is not comparing zeros
+
+
+def test_gradients_match_fp64(gpu_ctx):
+    cfg, w, t = setup_cache(gpu_ctx, seed=1)
+    rng = np.random.Generator(np.random.PCG64(6))
+    n = 300
+    pts = rng.random((n, 3)).astype(np.float32)
+    targets = rng.random((n, 3)).astype(np.float32)
+    print_case("cache gradients", n)
+    loss, dw, dtab = gpu_ctx.cache_gradients(to_dev(pts.reshape(-1)), to_dev(targets.reshape(-1)), n, table=True)
+    ref_loss, out, ref_dw, ref_dx, feats = oracle_forward(cfg, w, t, pts, targets)
+    assert abs(loss - ref_loss) <= 0.03 * abs(ref_loss)
+    rel = np.linalg.norm(dw - ref_dw) / np.linalg.norm(ref_dw)
+    assert rel < 0.05, rel
+    # per layer as well (the output layer has only 192 weights)
+    offs = [0, 2048, 6144, 10240, 14336, 14528]
+    for a, b in zip(offs[:-1], offs[1:]):
+        r = np.linalg.norm(dw[a:b] - ref_dw[a:b]) / max(np.linalg.norm(ref_dw[a:b]), 1e-30)
+        assert r < 0.08, (a, r)
+    ref_tab = refcpu.table_backward(cfg, pts, ref_dx)
+    rel_t = np.linalg.norm(dtab - ref_tab) / np.linalg.norm(ref_tab)
+    assert rel_t < 0.05, rel_t
+
+
+def test_adam_first_step_is_sign_of_gradient(gpu_ctx):
+    lr = 1e-3
+    cfg, w, t = setup_cache(gpu_ctx, seed=2, lr=lr)
+    rng = np.random.Generator(np.random.PCG64(7))
+    n = 512
+    pts = to_dev(rng.random((n, 3)).reshape(-1))
+    targets = to_dev(rng.random((n, 3)).reshape(-1))
+    _, dw, dtab = gpu_ctx.cache_gradients(pts, targets, n, table=True)
+    w0, t0 = gpu_ctx.cache_get_params()
+    gpu_ctx.cache_train(pts, targets, n)
+    w1, t1 = gpu_ctx.cache_get_params()
+    # first Adam step: m_hat = g, v_hat = g^2 -> delta = -lr * g / (|g| + eps)
+    big = np.abs(dw) > 1e-5
+    np.testing.assert_allclose(w1[big] - w0[big], -lr * np.sign(dw[big]), rtol=1e-3, atol=1e-7)
+    bigt = np.abs(dtab) > 1e-5
+    np.testing.assert_allclose(t1[bigt] - t0[bigt], -lr * np.sign(dtab[bigt]), rtol=1e-3, atol=1e-7)
+    assert np.array_equal(t1[dtab == 0], t0[dtab == 0])
+
+
+def test_online_training_fits_an_analytic_field(gpu_ctx):
+    """Fitting f(x) = a smooth positive RGB field: the relative L2 loss falls tenfold and the
+    trained network beats the untrained one on held-out points."""
+    import torch
+
+    sc = scenes.cornell_scene()
+    gpu_ctx.upload_scene(sc)
+    gpu_ctx.cache_create(spk.cache_config(log2_table=16, learning_rate=1e-2, seed=3))
+
+    def field(p):
+        return np.stack([1.0 + np.sin(6 * p[:, 0]) * np.cos(4 * p[:, 1]),
+                         0.5 + 0.4 * p[:, 2] ** 2,
+                         0.8 + 0.5 * np.sin(9 * p[:, 0] * p[:, 1])], 1).astype(np.float32)
+
+    rng = np.random.Generator(np.random.PCG64(8))
+    held = rng.random((4096, 3)).astype(np.float32)
+    held_t = field(held)
+
+    def held_error():
+        out = torch.zeros(held.shape[0] * 3, device="cuda")
+        gpu_ctx.cache_infer(to_dev(held.reshape(-1)), out, held.shape[0])
+        y = out.cpu().numpy().reshape(-1, 3)
+        return float(np.mean((y - held_t) ** 2 / (held_t ** 2 + 1e-2)))
+
+    e0 = held_error()
+    losses = []
+    for step in range(300):
+        p = rng.random((1 << 14, 3)).astype(np.float32)
+        losses.append(gpu_ctx.cache_train(to_dev(p.reshape(-1)), to_dev(field(p).reshape(-1)), p.shape[0]))
+    e1 = held_error()
+    print("loss first/last", losses[0], losses[-1], "held-out relL2", e0, e1)
+    assert losses[-1] < 0.1 * losses[0]
+    assert e1 < 0.1 * e0 and e1 < 0.02
+    assert gpu_ctx.cache_info()["steps"] == 300
+    gpu_ctx.cache_destroy()
+
+
+def test_cache_render_trains_online_and_stays_close_to_plain(gpu_ctx):
+    """The path tracer terminates paths into the cache at bounce 2 and trains it on its own
+    samples: the loss falls and the converged image stays close to the plain tracer's."""
+    import torch
+
+    w = h = 64
+    sc = scenes.cornell_scene()
+    gpu_ctx.upload_scene(sc)
+    gpu_ctx.set_camera(scenes.cornell_camera(w, h))
+    gpu_ctx.cache_create(spk.cache_config(log2_table=16, batch=1 << 12, seed=4))
+    rgba = torch.zeros(w * h * 4, device="cuda")
+
+    def frame(seed, flags):
+        f = spk.FrameDesc()
+        f.width, f.height, f.frame_seed, f.max_bounce, f.flags = w, h, seed, 6, flags
+        return f
+
+    losses = []
+    cache_imgs = []
+    for k in range(200):
+        st = gpu_ctx.render_frame(frame(k, spk.FRAME_CACHE), rgba)
+        losses.append(gpu_ctx.cache_info()["last_loss"])
+        if k >= 100:
+            cache_imgs.append(rgba.cpu().numpy().reshape(h, w, 4)[..., :3].copy())
+    plain = []
+    for k in range(200):
+        gpu_ctx.render_frame(frame(1000 + k, 0), rgba)
+        plain.append(rgba.cpu().numpy().reshape(h, w, 4)[..., :3].copy())
+    mc, mp = np.mean(cache_imgs, 0), np.mean(plain, 0)
+    print("loss early/late", np.mean(losses[5:15]), np.mean(losses[-10:]))
+    print("mean flux cache/plain", mc.mean(), mp.mean())
+    assert np.mean(losses[-10:]) < np.mean(losses[5:15])
+    assert abs(mc.mean() - mp.mean()) / mp.mean() < 0.1
+    assert gpu_ctx.cache_info()["steps"] >= 190
+    gpu_ctx.cache_destroy()

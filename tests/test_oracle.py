@@ -213,3 +213,22 @@ def test_instance_transforms_in_the_reference_traversal():
     assert same > 0.999
     m = (ha["prim"] == hb["prim"]) & (ha["prim"] != spk.SPK_MISS)
     assert np.abs(ha["t"][m] - hb["t"][m]).max() < 1e-5
+
+
+def test_edge_on_false_hits_are_rejected():
+    """Regression: for these rays the FP32 edge functions of distant, edge-on heightfield cells
+    round to exact zeros; the certified test re-evaluates them and finds no hit (as FP64 does)."""
+    sc = scenes.heightfield_scene(cells=1024)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=8 * 1024 * 1024, seed=0)
+    ids = np.array([3764800, 3784576, 4979136, 6322624, 6998848])
+    o = refcpu.OracleScene(sc)
+    br = o.trace(rays[ids], refcpu.TRACE_WT_BRUTE)
+    bv = o.trace(rays[ids], refcpu.TRACE_WT)
+    h64, flags, _ = o.trace(rays[ids], refcpu.TRACE_FP64)
+    np.testing.assert_array_equal(br["prim"], h64["prim"])
+    np.testing.assert_array_equal(bv["prim"], h64["prim"])
+    # the first one: both plain FP32 edge values are exactly zero for a cell 0.87 away
+    P = sc.meshes[0]["positions"]
+    I = sc.meshes[0]["indices"]
+    assert not refcpu.tri_wt(rays["org"][ids[0]], rays["dir"][ids[0]], *P[I[466407]])[0]
