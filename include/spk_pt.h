@@ -236,6 +236,36 @@ int spk_accumulate(spk_context *ctx, const float *rgba_dev, float *accum_dev, ui
 int spk_tonemap(spk_context *ctx, const float *accum_dev, float *out_dev, uint32_t width, uint32_t height,
                 float exposure);
 
+/* ---- batches of independent scenes in one launch ------------------------------------------ */
+/* Each scene keeps its own BVH (rebased into one device array), materials, camera and image
+   size; one frame renders every scene's pixels with the same persistent kernels, each path finding
+   its scene's BVH root. Pixel seeds are per scene (as if the scene were rendered alone), so every
+   scene's image is bit-identical to spk_render_frame on that scene alone. NEE and the cache are
+   single-scene features. */
+typedef struct spk_batch_scene {
+    const spk_mesh_desc *meshes;
+    uint32_t num_meshes;
+    const spk_instance_desc *instances;
+    uint32_t num_instances;
+    const spk_material_desc *materials;
+    uint32_t num_materials;
+    spk_camera_desc camera;
+    uint32_t width;
+    uint32_t height;
+} spk_batch_scene;
+
+int spk_batch_upload(spk_context *ctx, const spk_batch_scene *scenes, uint32_t num_scenes,
+                     const spk_build_options *options);
+/* frame->width/height are ignored; rgba_dev holds the scenes' images back to back, scene s
+   starting at pixel offsets[s] (spk_batch_offsets). */
+int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_frame_stats *stats);
+/* num_scenes + 1 pixel offsets (the last is the total pixel count). */
+int spk_batch_offsets(spk_context *ctx, uint64_t *offsets, uint32_t max_count);
+/* Schedule autotuner for batches: times blocks-per-CU choices of the persistent kernels on one
+   frame and keeps the fastest. */
+int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, uint32_t *blocks_per_cu,
+                       double *best_ms);
+
 /* ---- learned radiance cache (no counterpart in the reference) ---------------------------- */
 /* A multiresolution hash-grid encoding (levels x 2 features, 2^log2_table FP16 entries per level)
    feeding a bias-free ReLU MLP 32-64-64-64-64-3 on BF16 matrix cores with FP32 accumulation,

@@ -84,9 +84,27 @@ def build(force=True, verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_host(verbose)
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
     return LIB
+
+
+DEMO = os.path.join(LIBDIR, "render_demo")
+
+
+def build_host(verbose=True):
+    """The C++ host adapter (GpuPathRenderer) and its demo, linked against libspk_pt.so."""
+    host = os.path.join(HERE, "host")
+    tmp = DEMO + ".tmp"
+    cmd = (["g++"] + CXX_FLAGS + ["-I", os.path.join(HERE, "..", "include"),
+                                  os.path.join(host, "gpu_renderer.cpp"), os.path.join(host, "render_demo.cpp"),
+                                  "-L", LIBDIR, "-lspk_pt", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host adapter build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, DEMO)
+    return DEMO
 
 
 if __name__ == "__main__":

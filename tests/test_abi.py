@@ -56,3 +56,16 @@ def test_null_arguments_are_rejected():
     assert lib.spk_destroy(None) == spk.SPK_ERR_INVALID_ARGUMENT
     assert lib.spk_set_camera(None, None) == spk.SPK_ERR_INVALID_ARGUMENT
     assert lib.spk_create(0, None) == spk.SPK_ERR_INVALID_ARGUMENT
+
+
+def test_host_adapter_demo_is_built_and_fails_cleanly_without_a_device():
+    import subprocess
+
+    import torch
+
+    demo = os.path.join(ROOT, "sparkle_amd", "lib", "render_demo")
+    assert os.path.exists(demo), "run __graft_entry__.build()"
+    if torch.cuda.is_available():
+        return
+    r = subprocess.run([demo, "/tmp/spk_demo_nodev.pfm", "8", "8", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "init failed" in r.stderr
