@@ -68,6 +68,116 @@ def load_traffic(config):
     return int(float(fields["hbm_bytes_per_launch"])), "profiles/trace_incoherent_2m_pmc.md"
 
 
+def relmse(img, ref, eps=1e-2):
+    """Relative MSE over pixels and channels: mean((x - r)^2 / (r^2 + eps))."""
+    return float(((img - ref) ** 2 / (ref ** 2 + eps)).mean())
+
+
+def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_bounce=8, warm_frac=0.1):
+    """Equal-time comparison of the plain tracer and the learned-cache tracer on the many-instance
+    scene (about 5.5M triangles, 3 area lights) against a ref_spp plain reference.
+
+    plain: spp frames (seeds 0..spp-1), accumulated by spk_accumulate, wall time T.
+    cache: frames with SPK_FRAME_CACHE (online training every frame) until T has passed; the
+           frames of the first warm_frac of T only train and are not accumulated.
+    reference: ref_spp plain frames with seeds from 1,000,000 on (independent of both), summed in
+           float64 on the device.
+    """
+    import torch
+
+    log(f"relMSE experiment: generating the many-instance scene, {width}x{height}")
+    sc, cam = scenes.many_instance_scene(seed=0, aspect=width / height)
+    ctx = spk.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    t_build = time.perf_counter()
+    ctx.upload_scene(sc)
+    t_build = time.perf_counter() - t_build
+    st = ctx.stats()
+    ctx.set_camera(cam)
+    npx = width * height
+    rgba = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+    acc = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+
+    def frame(seed, flags=0):
+        f = spk.FrameDesc()
+        f.width, f.height, f.frame_seed, f.max_bounce, f.debug_mode = width, height, seed, max_bounce, 0
+        f.sky_enabled, f.flags, f.output_limit = 0, flags, 0.0
+        return f
+
+    # warm up (graph capture) outside every timed region
+    ctx.render_frame(frame(999_999), rgba)
+    torch.cuda.synchronize()
+
+    log(f"relMSE experiment: reference at {ref_spp} spp")
+    ref = torch.zeros(npx * 4, dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    rays_ref = 0
+    for k in range(ref_spp):
+        rays_ref += ctx.render_frame(frame(1_000_000 + k), rgba).rays_traced
+        ref += rgba.double()
+        if (k + 1) % 512 == 0:
+            log(f"  reference {k + 1}/{ref_spp} spp, {time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    ref = (ref / ref_spp).view(height, width, 4)[..., :3]
+    samples_per_s = npx * ref_spp / t_ref
+    mrays_path = rays_ref / t_ref / 1e6
+
+    log(f"relMSE experiment: plain at {spp} spp")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(spp):
+        ctx.render_frame(frame(k), rgba)
+        ctx.accumulate(rgba, acc, npx, k, 1)
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    plain = acc.view(height, width, 4)[..., :3].double()
+    e_plain = relmse(plain, ref)
+
+    log(f"relMSE experiment: learned cache for {T:.2f} s")
+    ctx.cache_create(spk.cache_config())
+    ctx.render_frame(frame(999_998, spk.FRAME_CACHE), rgba)  # graph capture; the cache is then reset
+    ctx.cache_destroy()
+    ctx.cache_create(spk.cache_config())
+    torch.cuda.synchronize()
+    acc.zero_()
+    frames = used = 0
+    losses = []
+    t0 = time.perf_counter()
+    while True:
+        el = time.perf_counter() - t0
+        if el >= T:
+            break
+        ctx.render_frame(frame(frames, spk.FRAME_CACHE), rgba)
+        frames += 1
+        if el >= warm_frac * T:
+            ctx.accumulate(rgba, acc, npx, used, 1)
+            used += 1
+        if frames % 8 == 0:
+            losses.append(ctx.cache_info()["last_loss"])
+    torch.cuda.synchronize()
+    T_cache = time.perf_counter() - t0
+    cache_img = acc.view(height, width, 4)[..., :3].double()
+    e_cache = relmse(cache_img, ref) if used else float("inf")
+    mean_ref = float(ref.mean())
+    out = {
+        "scene": f"many-instance generator seed 0: {st.num_triangles} triangles, 3 area lights, {width}x{height}, "
+                 f"max {max_bounce} bounces, Russian roulette from bounce 3",
+        "reference_spp": ref_spp,
+        "reference_s": round(t_ref, 2),
+        "plain": {"spp": spp, "seconds": round(T, 3), "relmse": e_plain},
+        "cache": {"frames": frames, "accumulated_spp": used, "seconds": round(T_cache, 3), "relmse": e_cache,
+                  "last_loss": losses[-1] if losses else None},
+        "relmse_ratio_plain_over_cache": round(e_plain / e_cache, 3) if e_cache > 0 else None,
+        "reference_mean": mean_ref,
+        "full_path_samples_per_s": round(samples_per_s, 1),
+        "full_path_mrays_per_s": round(mrays_path, 1),
+        "bvh_build_s": round(t_build, 2),
+    }
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +189,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-coherent", action="store_true")
     ap.add_argument("--config", type=str, default=None, help="traversal variant width,stack,block")
+    ap.add_argument("--relmse", type=str, default="1920x1080,64,4096",
+                    help="equal-time relMSE experiment WxH,spp,reference-spp ('off' to skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -230,6 +342,10 @@ def main():
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     ctx.close()
+    if args.relmse != "off":
+        res, spp, ref_spp = args.relmse.split(",")
+        w_, h_ = (int(x) for x in res.split("x"))
+        out["relmse"] = relmse_experiment(spk, scenes, stream, w_, h_, int(spp), int(ref_spp))
     print(json.dumps(out), flush=True)
 
 

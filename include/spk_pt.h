@@ -261,10 +261,10 @@ int spk_batch_upload(spk_context *ctx, const spk_batch_scene *scenes, uint32_t n
 int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_frame_stats *stats);
 /* num_scenes + 1 pixel offsets (the last is the total pixel count). */
 int spk_batch_offsets(spk_context *ctx, uint64_t *offsets, uint32_t max_count);
-/* Schedule autotuner for batches: times blocks-per-CU choices of the persistent kernels on one
-   frame and keeps the fastest. */
+/* Schedule autotuner for batches: times the persistent grid (blocks per CU) and the material sort
+   of the bounce queues (on/off) on one frame and keeps the fastest; results never depend on either. */
 int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, uint32_t *blocks_per_cu,
-                       double *best_ms);
+                       uint32_t *sorted, double *best_ms);
 
 /* ---- learned radiance cache (no counterpart in the reference) ---------------------------- */
 /* A multiresolution hash-grid encoding (levels x 2 features, 2^log2_table FP16 entries per level)

@@ -122,6 +122,12 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("rays_traced", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
 
 
+class BatchScene(ctypes.Structure):
+    _fields_ = [("meshes", ctypes.c_void_p), ("num_meshes", ctypes.c_uint32), ("instances", ctypes.c_void_p),
+                ("num_instances", ctypes.c_uint32), ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32),
+                ("camera", CameraDesc), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -160,6 +166,11 @@ def load_library():
         "spk_render_frame": (I, [P, ctypes.POINTER(FrameDesc), P, P, ctypes.POINTER(FrameStats)]),
         "spk_accumulate": (I, [P, P, P, U64, U32, U32]),
         "spk_tonemap": (I, [P, P, P, U32, U32, F]),
+        "spk_batch_upload": (I, [P, P, U32, ctypes.POINTER(BuildOptions)]),
+        "spk_batch_render": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(FrameStats)]),
+        "spk_batch_offsets": (I, [P, ctypes.POINTER(U64), U32]),
+        "spk_batch_autotune": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(U32), ctypes.POINTER(U32),
+                                   ctypes.POINTER(ctypes.c_double)]),
         "spk_cache_create": (I, [P, ctypes.POINTER(CacheConfig)]),
         "spk_cache_destroy": (I, [P]),
         "spk_cache_infer": (I, [P, P, U32, I, P]),
@@ -361,6 +372,39 @@ class Context:
         self._check(self.lib.spk_render_frame(self.h, ctypes.byref(frame), _ptr(rgba_dev), _ptr(normal_dev),
                                               ctypes.byref(st)))
         return st
+
+    def upload_batch(self, items, bvh_width=8, max_leaf_tris=3, num_threads=0):
+        """spk_batch_upload: items are (Scene, CameraDesc, width, height). Returns the pixel offsets."""
+        arr = (BatchScene * len(items))()
+        keep = []
+        for k, (scene, cam, w, h) in enumerate(items):
+            ms, ins, mats = scene.c_arrays()
+            keep.append((ms, ins, mats))
+            arr[k].meshes = ctypes.cast(ms, ctypes.c_void_p)
+            arr[k].num_meshes = len(scene.meshes)
+            arr[k].instances = ctypes.cast(ins, ctypes.c_void_p)
+            arr[k].num_instances = len(scene.instances)
+            arr[k].materials = ctypes.cast(mats, ctypes.c_void_p)
+            arr[k].num_materials = len(scene.materials)
+            arr[k].camera = cam
+            arr[k].width = w
+            arr[k].height = h
+        opts = BuildOptions(bvh_width, max_leaf_tris, num_threads)
+        self._check(self.lib.spk_batch_upload(self.h, arr, len(items), ctypes.byref(opts)))
+        offs = (ctypes.c_uint64 * (len(items) + 1))()
+        self._check(self.lib.spk_batch_offsets(self.h, offs, len(items) + 1))
+        return [int(x) for x in offs]
+
+    def render_batch(self, frame, rgba_dev):
+        st = FrameStats()
+        self._check(self.lib.spk_batch_render(self.h, ctypes.byref(frame), _ptr(rgba_dev), ctypes.byref(st)))
+        return st
+
+    def autotune_batch(self, frame, rgba_dev):
+        b, srt, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        self._check(self.lib.spk_batch_autotune(self.h, ctypes.byref(frame), _ptr(rgba_dev), ctypes.byref(b),
+                                                ctypes.byref(srt), ctypes.byref(ms)))
+        return b.value, srt.value, ms.value
 
     def accumulate(self, rgba_dev, accum_dev, num_pixels, cumulated, spp=1):
         return self._check(self.lib.spk_accumulate(self.h, _ptr(rgba_dev), _ptr(accum_dev), num_pixels,

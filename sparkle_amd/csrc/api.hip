@@ -156,8 +156,7 @@ static int upload_built(spk_context *ctx, int stack) {
     HostScene &hs = ctx->host_scene;
     const std::vector<MaterialGpu> &mats = ctx->host_mats;
     const uint32_t n = (uint32_t)hs.bvh.tris.size();
-    std::vector<TriShade> shade(n);
-    for (uint32_t k = 0; k < n; ++k) shade[k] = hs.shade_in[hs.bvh.tri_input[k]];
+    const std::vector<TriShade> shade = leaf_order_shade(hs);
     hipError_t e;
     DeviceScene &ds = ctx->scene;
     ds.valid = false;
@@ -188,6 +187,101 @@ static int upload_built(spk_context *ctx, int stack) {
     ds.valid = true;
     fill_stats(hs, &ctx->stats, ds.nodes.bytes + ds.tris.bytes + ds.shade.bytes + ds.materials.bytes);
     return SPK_OK;
+}
+
+// Validates and flattens one scene, builds its BVH and converts its shading attributes and
+// materials (everything spk_scene_upload and spk_batch_upload do on the host).
+int prepare_scene(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_instance_desc *instances,
+                  uint32_t num_instances, const spk_material_desc *materials, uint32_t num_materials,
+                  const spk_build_options *options, HostScene &hs, std::vector<MaterialGpu> &mats, std::string &err) {
+    if (!materials || num_materials == 0) {
+        err = "scene needs at least one material";
+        return SPK_ERR_INVALID_ARGUMENT;
+    }
+    for (uint32_t i = 0; instances && i < num_instances; ++i)
+        if (instances[i].material >= num_materials) {
+            err = "instance " + std::to_string(i) + " names material " + std::to_string(instances[i].material) +
+                  " of " + std::to_string(num_materials);
+            return SPK_ERR_INVALID_SCENE;
+        }
+    mats.assign(num_materials, MaterialGpu());
+    for (uint32_t m = 0; m < num_materials; ++m) {
+        const spk_material_desc &d = materials[m];
+        MaterialGpu &g = mats[m];
+        std::memset(&g, 0, sizeof(g));
+        g.type = d.type;
+        for (int c = 0; c < 3; ++c) g.base[c] = d.base_color[c], g.emissive[c] = d.emissive[c];
+        g.metallic = d.metallic;
+        g.roughness = d.roughness;
+        g.eta = d.eta;
+        const float en = std::sqrt(d.emissive[0] * d.emissive[0] + d.emissive[1] * d.emissive[1] +
+                                   d.emissive[2] * d.emissive[2]);
+        g.emissive_flag = en > 1e-6f ? 1u : 0u;
+        if (d.type != SPK_MATERIAL_PBR && d.type != SPK_MATERIAL_DIELECTRIC)
+            {
+            err = "material " + std::to_string(m) + " has unknown type " + std::to_string(d.type);
+            return SPK_ERR_INVALID_SCENE;
+        }
+    }
+
+    int rc = flatten_and_build(meshes, num_meshes, instances, num_instances, options, hs, err);
+    if (rc != SPK_OK) return rc;
+    if (pick_trace_stack((int)hs.bvh.depth) < 0) {
+        err = "BVH depth " + std::to_string(hs.bvh.depth) + " exceeds the largest traversal stack (" +
+              std::to_string(max_trace_stack()) + ")";
+        return SPK_ERR_STACK_DEPTH;
+    }
+
+    // shading attributes in input order (permuted to leaf order at upload)
+    const uint32_t n = (uint32_t)hs.bvh.tris.size();
+    hs.shade_in.assign(n, TriShade());
+    for (uint32_t src = 0; src < n; ++src) {
+        const uint32_t inst = hs.shade_src[src].inst, f = hs.shade_src[src].face;
+        const spk_instance_desc &ins = instances[inst];
+        const spk_mesh_desc &md = meshes[ins.mesh];
+        const float *M = ins.world_from_local;
+        TriShade &s = hs.shade_in[src];
+        std::memset(&s, 0, sizeof(s));
+        float *ns[3] = {s.n0, s.n1, s.n2}, *ts[3] = {s.t0, s.t1, s.t2}, *uvs[3] = {s.uv0, s.uv1, s.uv2};
+        const float *v = &hs.verts[9 * (size_t)src];
+        // geometric normal when the mesh has none
+        const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        float gn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float gl = std::sqrt(gn[0] * gn[0] + gn[1] * gn[1] + gn[2] * gn[2]);
+        if (gl > 0.f) gn[0] /= gl, gn[1] /= gl, gn[2] /= gl;
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t vi = md.indices[3 * (size_t)f + c];
+            if (md.normals) {
+                const float *nn = md.normals + 3 * (size_t)vi;
+                for (int r = 0; r < 3; ++r) ns[c][r] = M[4 * r] * nn[0] + M[4 * r + 1] * nn[1] + M[4 * r + 2] * nn[2];
+            } else {
+                for (int r = 0; r < 3; ++r) ns[c][r] = gn[r];
+            }
+            if (md.tangents) {
+                const float *tt = md.tangents + 4 * (size_t)vi;
+                for (int r = 0; r < 3; ++r) ts[c][r] = M[4 * r] * tt[0] + M[4 * r + 1] * tt[1] + M[4 * r + 2] * tt[2];
+                ts[c][3] = tt[3];
+            } else {
+                // default tangent (1, 0, 0) through the 3x3 part
+                ts[c][0] = M[0], ts[c][1] = M[4], ts[c][2] = M[8];
+                ts[c][3] = 1.f;
+            }
+            if (md.uvs) {
+                uvs[c][0] = md.uvs[2 * (size_t)vi];
+                uvs[c][1] = md.uvs[2 * (size_t)vi + 1];
+            }
+        }
+        s.material = ins.material;
+    }
+    return SPK_OK;
+}
+
+// Shading attributes permuted to the BVH's leaf order.
+std::vector<TriShade> leaf_order_shade(const HostScene &hs) {
+    const uint32_t n = (uint32_t)hs.bvh.tris.size();
+    std::vector<TriShade> shade(n);
+    for (uint32_t k = 0; k < n; ++k) shade[k] = hs.shade_in[hs.bvh.tri_input[k]];
+    return shade;
 }
 
 }  // namespace spk
@@ -301,79 +395,13 @@ int spk_scene_upload(spk_context *ctx, const spk_mesh_desc *meshes, uint32_t num
                      const spk_build_options *options) {
     if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
-    for (uint32_t i = 0; instances && i < num_instances; ++i)
-        if (instances[i].material >= num_materials)
-            return set_error(ctx, SPK_ERR_INVALID_SCENE, "instance %u names material %u of %u", i,
-                             instances[i].material, num_materials);
-    if (!materials || num_materials == 0)
-        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "scene needs at least one material");
     HostScene hs;
+    std::vector<MaterialGpu> mats;
     std::string err;
-    int rc = flatten_and_build(meshes, num_meshes, instances, num_instances, options, hs, err);
+    int rc = prepare_scene(meshes, num_meshes, instances, num_instances, materials, num_materials, options, hs, mats,
+                           err);
     if (rc != SPK_OK) return set_error(ctx, rc, "%s", err.c_str());
     const int stack = pick_trace_stack((int)hs.bvh.depth);
-    if (stack < 0)
-        return set_error(ctx, SPK_ERR_STACK_DEPTH, "BVH depth %u exceeds the largest traversal stack (%d)",
-                         hs.bvh.depth, max_trace_stack());
-
-    // shading attributes in input order (permuted to leaf order at upload)
-    const uint32_t n = (uint32_t)hs.bvh.tris.size();
-    hs.shade_in.assign(n, TriShade());
-    for (uint32_t src = 0; src < n; ++src) {
-        const uint32_t inst = hs.shade_src[src].inst, f = hs.shade_src[src].face;
-        const spk_instance_desc &ins = instances[inst];
-        const spk_mesh_desc &md = meshes[ins.mesh];
-        const float *M = ins.world_from_local;
-        TriShade &s = hs.shade_in[src];
-        std::memset(&s, 0, sizeof(s));
-        float *ns[3] = {s.n0, s.n1, s.n2}, *ts[3] = {s.t0, s.t1, s.t2}, *uvs[3] = {s.uv0, s.uv1, s.uv2};
-        const float *v = &hs.verts[9 * (size_t)src];
-        // geometric normal when the mesh has none
-        const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
-        float gn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-        const float gl = std::sqrt(gn[0] * gn[0] + gn[1] * gn[1] + gn[2] * gn[2]);
-        if (gl > 0.f) gn[0] /= gl, gn[1] /= gl, gn[2] /= gl;
-        for (int c = 0; c < 3; ++c) {
-            const uint32_t vi = md.indices[3 * (size_t)f + c];
-            if (md.normals) {
-                const float *nn = md.normals + 3 * (size_t)vi;
-                for (int r = 0; r < 3; ++r) ns[c][r] = M[4 * r] * nn[0] + M[4 * r + 1] * nn[1] + M[4 * r + 2] * nn[2];
-            } else {
-                for (int r = 0; r < 3; ++r) ns[c][r] = gn[r];
-            }
-            if (md.tangents) {
-                const float *tt = md.tangents + 4 * (size_t)vi;
-                for (int r = 0; r < 3; ++r) ts[c][r] = M[4 * r] * tt[0] + M[4 * r + 1] * tt[1] + M[4 * r + 2] * tt[2];
-                ts[c][3] = tt[3];
-            } else {
-                // default tangent (1, 0, 0) through the 3x3 part
-                ts[c][0] = M[0], ts[c][1] = M[4], ts[c][2] = M[8];
-                ts[c][3] = 1.f;
-            }
-            if (md.uvs) {
-                uvs[c][0] = md.uvs[2 * (size_t)vi];
-                uvs[c][1] = md.uvs[2 * (size_t)vi + 1];
-            }
-        }
-        s.material = ins.material;
-    }
-    std::vector<MaterialGpu> mats(num_materials);
-    for (uint32_t m = 0; m < num_materials; ++m) {
-        const spk_material_desc &d = materials[m];
-        MaterialGpu &g = mats[m];
-        std::memset(&g, 0, sizeof(g));
-        g.type = d.type;
-        for (int c = 0; c < 3; ++c) g.base[c] = d.base_color[c], g.emissive[c] = d.emissive[c];
-        g.metallic = d.metallic;
-        g.roughness = d.roughness;
-        g.eta = d.eta;
-        const float en = std::sqrt(d.emissive[0] * d.emissive[0] + d.emissive[1] * d.emissive[1] +
-                                   d.emissive[2] * d.emissive[2]);
-        g.emissive_flag = en > 1e-6f ? 1u : 0u;
-        if (d.type != SPK_MATERIAL_PBR && d.type != SPK_MATERIAL_DIELECTRIC)
-            return set_error(ctx, SPK_ERR_INVALID_SCENE, "material %u has unknown type %u", m, d.type);
-    }
-
     ctx->host_mats = mats;
     ctx->host_scene = std::move(hs);
     int rc2 = upload_built(ctx, stack);

@@ -16,7 +16,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/spk_pt.h"
@@ -53,9 +55,20 @@ enum : uint32_t {
     PF_TRAIN = 2u,          // a training path of the radiance cache
 };
 
+// Per-scene record of a frame: BVH root, image size, first pixel of the scene's image, camera.
+// spk_render_frame uses one record, rewritten with the frame seed before each replay (a camera move
+// needs no new graph); spk_batch_render one per scene, uploaded with the batch.
+struct SceneRec {
+    uint32_t root, width, height, pix_base;
+    spk_camera_desc cam;
+    uint32_t pad;
+};
+static_assert(sizeof(SceneRec) == 112, "SceneRec layout");
+
 struct FrameDev {
     uint32_t frame_seed;
     uint32_t pad[3];
+    SceneRec scene0;
 };
 
 struct PtArgs {
@@ -83,8 +96,10 @@ struct PtArgs {
     float *rgba_out;
     float *normal_out;
     const FrameDev *frame;
-    spk_camera_desc cam;
-    uint32_t width, height, npix, max_bounce, debug_mode, sky_enabled, flags, sort;
+    const SceneRec *scenes;  // num_scenes records, sorted by pix_base
+    uint32_t *path_scene;    // scene of each path (written only when num_scenes > 1)
+    uint32_t num_scenes, grid;
+    uint32_t npix, max_bounce, debug_mode, sky_enabled, flags, sort;
     float sky[3];
     float output_limit;
     // learned radiance cache
@@ -128,7 +143,23 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
     return x;
 }
 
+// last scene whose first pixel is <= p
+__device__ __forceinline__ uint32_t scene_of_pixel(const PtArgs &a, uint32_t p) {
+    uint32_t lo = 0u, hi = a.num_scenes - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (a.scenes[mid].pix_base <= p) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ const SceneRec &scene_of_path(const PtArgs &a, uint32_t path) {
+    return a.scenes[a.num_scenes > 1u ? a.path_scene[path] : 0u];
+}
+
 // ---- generate: reseed per pixel, jitter, lens, camera ray (CPURenderer.cpp:220-238, 398-445) ----
+// Pixel p of the frame is pixel p - pix_base of its scene's image; seeds, jitter and camera are the
+// scene's own, so a scene renders the same alone or in a batch.
 __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
     const uint32_t tid = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t stride = gridDim.x * kPtBlock;
@@ -136,25 +167,33 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
     for (uint32_t i = tid; i < (uint32_t)kCtrTotal; i += stride) a.counters[i] = 0u;
     for (uint32_t i = tid; i < a.max_bounce * 2u * a.nkeys; i += stride) a.hist[i] = 0u;
     const uint32_t frame_seed = a.frame->frame_seed;
-    const float pw = 1.f / (float)(a.width - 1);
-    const float ph = 1.f / (float)(a.height - 1);
     for (uint32_t p = tid; p < a.npix; p += stride) {
-        const uint32_t j = p / a.width, i = p - j * a.width;
+        uint32_t sid = 0u;
+        if (a.num_scenes > 1u) {
+            sid = scene_of_pixel(a, p);
+            a.path_scene[p] = sid;
+        }
+        const SceneRec &S = a.scenes[sid];
+        const uint32_t W = S.width, H = S.height, q = p - S.pix_base;
+        const float pw = 1.f / (float)(W - 1);
+        const float ph = 1.f / (float)(H - 1);
+        const uint32_t j = q / W, i = q - j * W;
         uint64_t seed;
         if (a.flags & SPK_FRAME_SEED64)
-            seed = (uint64_t)j * a.width + i + (uint64_t)frame_seed * a.width * a.height;
+            seed = (uint64_t)j * W + i + (uint64_t)frame_seed * W * H;
         else
-            seed = (uint32_t)(j * a.width + i + frame_seed * a.width * a.height);
+            seed = (uint32_t)(j * W + i + frame_seed * W * H);
         Rng rng = rng_seed(seed);
         const float u = ((float)i + rng_unit(rng)) * pw;
         const float v = ((float)j + rng_unit(rng)) * ph;
         float nx, ny;
         rng_disk(rng, nx, ny);
-        nx = nx * a.cam.lens_radius;
-        ny = ny * a.cam.lens_radius;
-        const F3 lens = ld3(a.cam.right) * nx + ld3(a.cam.up) * ny;
-        const F3 o = ld3(a.cam.position) + lens;
-        const F3 focus = ld3(a.cam.lower_left) + ld3(a.cam.max_u) * u + ld3(a.cam.max_v) * v;
+        const spk_camera_desc &cam = S.cam;
+        nx = nx * cam.lens_radius;
+        ny = ny * cam.lens_radius;
+        const F3 lens = ld3(cam.right) * nx + ld3(cam.up) * ny;
+        const F3 o = ld3(cam.position) + lens;
+        const F3 focus = ld3(cam.lower_left) + ld3(cam.max_u) * u + ld3(cam.max_v) * v;
         const F3 d = normalized(focus - o);
         uint32_t pf = 0u;
         if (a.cache_on) {
@@ -207,7 +246,8 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
         best.U = best.V = best.W = 0.f;
         best.det = 1.f;
         TraverseCounters c{0u, 0u};
-        traverse<WIDTH, STACK, false>(a.nodes, a.tris, 0u, r, stack + threadIdx.x, kPtBlock, best, c);
+        const uint32_t root = a.num_scenes > 1u ? a.scenes[a.path_scene[path]].root : 0u;
+        traverse<WIDTH, STACK, false>(a.nodes, a.tris, root, r, stack + threadIdx.x, kPtBlock, best, c);
         uint4 h;
         uint32_t key = a.nkeys - 1u;  // misses sort last
         if (best.prim != 0xFFFFFFFFu) {
@@ -350,10 +390,11 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                             case 8: C = f3(1.f, 1.f, 1.f) * mat.roughness; break;
                             case 9: C = f3(mat.base[0], mat.base[1], mat.base[2]); break;
                             case 10: C = f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]); break;
-                            case 11:
-                                C = f3(1.f, 1.f, 1.f) * (dot(loc - ld3(a.cam.position), ld3(a.cam.front)) /
-                                                         a.cam.far_plane);
+                            case 11: {
+                                const spk_camera_desc &cam = scene_of_path(a, path).cam;
+                                C = f3(1.f, 1.f, 1.f) * (dot(loc - ld3(cam.position), ld3(cam.front)) / cam.far_plane);
                                 break;
+                            }
                             default: break;
                         }
                         done = true;
@@ -583,7 +624,7 @@ const PtEntry kPtEntries[] = {
 };
 
 struct PathState {
-    DeviceBuffer rays, thr, col, rng, nee_rng, queue0, queue1, hits, sorted, shadow, counters, hist;
+    DeviceBuffer rays, thr, col, rng, nee_rng, queue0, queue1, hits, sorted, shadow, counters, hist, path_scene;
     DeviceBuffer emitters, emitter_cdf, frame;
     FrameDev *frame_host = nullptr;  // pinned
     uint32_t capacity = 0;
@@ -605,6 +646,21 @@ struct PathState {
         graph = nullptr;
         graph_def = nullptr;
     }
+};
+
+// A batch of independent scenes: every scene's BVH rebased into one node array and one triangle
+// array (child and triangle offsets shifted, material ids offset by the scene's first material),
+// plus one SceneRec per scene.
+struct BatchState {
+    DeviceScene scene;
+    DeviceBuffer table;
+    std::vector<SceneRec> host_table;
+    std::vector<uint64_t> offsets;  // num_scenes + 1 pixel offsets
+    uint32_t num_scenes = 0;
+    uint32_t npix = 0;
+    uint32_t blocks_per_cu = 8;
+    uint32_t sort = 1;
+    uint64_t num_tris = 0;
 };
 
 static int device_cus(spk_context *ctx) {
@@ -633,7 +689,8 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
             (e = ps.col.ensure(n * 16)) != hipSuccess || (e = ps.rng.ensure(n * 16)) != hipSuccess ||
             (e = ps.nee_rng.ensure(n * 16)) != hipSuccess || (e = ps.queue0.ensure(n * 4)) != hipSuccess ||
             (e = ps.queue1.ensure(n * 4)) != hipSuccess || (e = ps.hits.ensure(n * 16)) != hipSuccess ||
-            (e = ps.sorted.ensure(n * 4)) != hipSuccess || (e = ps.shadow.ensure(n * 48)) != hipSuccess)
+            (e = ps.sorted.ensure(n * 4)) != hipSuccess || (e = ps.shadow.ensure(n * 48)) != hipSuccess ||
+            (e = ps.path_scene.ensure(n * 4)) != hipSuccess)
             return hip_fail(ctx, e, "path buffers");
         ps.capacity = npix;
     }
@@ -724,6 +781,8 @@ void spk_context::release_all() {
     paths = nullptr;
     delete cache;
     cache = nullptr;
+    delete batch;
+    batch = nullptr;
     scene.nodes.release();
     scene.tris.release();
     scene.shade.release();
@@ -743,16 +802,23 @@ int spk_set_camera(spk_context *ctx, const spk_camera_desc *camera) {
     return SPK_OK;
 }
 
-int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, float *normal_dev,
-                     spk_frame_stats *stats) {
-    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
-    if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
-    if (!ctx->has_camera) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "no camera set");
-    if (!rgba_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null output");
-    if (frame->width < 2 || frame->height < 2)
-        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "frame must be at least 2x2 (pixel size 1/(W-1))");
-    if ((uint64_t)frame->width * frame->height >= (1ull << 31))
-        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "frame too large");
+}  // extern "C"
+
+namespace spk {
+
+// What one frame traces: a device scene and its scene records.
+struct FrameSource {
+    const DeviceScene *ds;
+    const SceneRec *scenes_dev;  // null: the single record in the frame buffer (filled from single)
+    SceneRec single;
+    uint32_t num_scenes;
+    uint32_t npix;
+    uint32_t blocks_per_cu;
+    uint32_t sort;
+};
+
+static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameSource &src, float *rgba_dev,
+                     float *normal_dev, spk_frame_stats *stats) {
     if (frame->max_bounce > (uint32_t)kMaxBounce)
         return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "max_bounce above %d", kMaxBounce);
     // modes that rewrite the colour after the loop (RayDepth, Indirect/DirectionalLighting) are not
@@ -762,11 +828,11 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     const bool cache_on = (frame->flags & SPK_FRAME_CACHE) != 0;
     if (cache_on && !ctx->cache)
         return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "SPK_FRAME_CACHE needs spk_cache_create first");
-    const uint32_t npix = frame->width * frame->height;
+    const uint32_t npix = src.npix;
     int rc = ensure_paths(ctx, npix);
     if (rc != SPK_OK) return rc;
     PathState &ps = *ctx->paths;
-    const DeviceScene &ds = ctx->scene;
+    const DeviceScene &ds = *src.ds;
     const PtEntry *pe = nullptr;
     for (const PtEntry &e : kPtEntries)
         if (e.width == ds.width && e.stack == pick_trace_stack(ds.depth)) pe = &e;
@@ -795,14 +861,18 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     a.counters = (uint32_t *)ps.counters.ptr;
     a.hist = (uint32_t *)ps.hist.ptr;
     a.nkeys = ds.num_materials + 1u;
-    a.sort = a.nkeys <= (uint32_t)kMaxSortKeys && !(frame->flags & SPK_FRAME_NO_SORT) ? 1u : 0u;
+    a.sort = src.sort && a.nkeys <= (uint32_t)kMaxSortKeys && !(frame->flags & SPK_FRAME_NO_SORT) ? 1u : 0u;
     if (!a.sort) a.nkeys = 2u;
     a.rgba_out = rgba_dev;
     a.normal_out = normal_dev;
     a.frame = (const FrameDev *)ps.frame.ptr;
-    a.cam = ctx->camera;
-    a.width = frame->width;
-    a.height = frame->height;
+    a.scenes = src.scenes_dev ? src.scenes_dev
+                              : (const SceneRec *)((const char *)ps.frame.ptr + offsetof(FrameDev, scene0));
+    a.path_scene = (uint32_t *)ps.path_scene.ptr;
+    a.num_scenes = src.num_scenes;
+    const int cus = device_cus(ctx);
+    const int grid = cus * (int)src.blocks_per_cu;
+    a.grid = (uint32_t)grid;
     a.npix = npix;
     a.max_bounce = frame->max_bounce;
     a.debug_mode = frame->debug_mode;
@@ -825,11 +895,10 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
         a.tr_meta1 = (float4 *)cs.tr_meta1.ptr;
         a.tr_targets = (float *)cs.tr_targets.ptr;
     }
-    const int cus = device_cus(ctx);
-    const int grid = cus * 8;
 
     hipError_t e;
     ps.frame_host->frame_seed = frame->frame_seed;
+    ps.frame_host->scene0 = src.single;
     if ((e = hipMemcpyAsync(ps.frame.ptr, ps.frame_host, sizeof(FrameDev), hipMemcpyHostToDevice, ctx->stream)) !=
         hipSuccess)
         return hip_fail(ctx, e, "frame parameters");
@@ -902,6 +971,211 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
         stats->shadow_rays = shadows;
         stats->milliseconds = ms;
     }
+    return SPK_OK;
+}
+
+// Rebases one scene's BVH into the batch arrays: child indices by node_off, triangle indices by
+// tri_off, material ids by mat_off.
+template <class Node>
+static void append_nodes(const std::vector<uint8_t> &src, uint32_t count, uint32_t node_off, uint32_t tri_off,
+                         std::vector<uint8_t> &dst) {
+    const size_t at = dst.size();
+    dst.resize(at + (size_t)count * sizeof(Node));
+    for (uint32_t k = 0; k < count; ++k) {
+        Node nd;
+        std::memcpy(&nd, src.data() + (size_t)k * sizeof(Node), sizeof(Node));
+        nd.child_base += node_off;
+        nd.tri_base += tri_off;
+        std::memcpy(dst.data() + at + (size_t)k * sizeof(Node), &nd, sizeof(Node));
+    }
+}
+
+}  // namespace spk
+
+extern "C" {
+
+int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, float *normal_dev,
+                     spk_frame_stats *stats) {
+    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
+    if (!ctx->has_camera) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "no camera set");
+    if (!rgba_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null output");
+    if (frame->width < 2 || frame->height < 2)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "frame must be at least 2x2 (pixel size 1/(W-1))");
+    if ((uint64_t)frame->width * frame->height >= (1ull << 31))
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "frame too large");
+    FrameSource src;
+    std::memset(&src, 0, sizeof(src));
+    src.ds = &ctx->scene;
+    src.scenes_dev = nullptr;
+    src.single.root = 0u;
+    src.single.width = frame->width;
+    src.single.height = frame->height;
+    src.single.pix_base = 0u;
+    src.single.cam = ctx->camera;
+    src.num_scenes = 1u;
+    src.npix = frame->width * frame->height;
+    src.blocks_per_cu = 8u;
+    src.sort = 1u;
+    return run_frame(ctx, frame, src, rgba_dev, normal_dev, stats);
+}
+
+int spk_batch_upload(spk_context *ctx, const spk_batch_scene *scenes, uint32_t num_scenes,
+                     const spk_build_options *options) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (!scenes || num_scenes == 0) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "batch needs at least one scene");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ctx->batch;
+    ctx->batch = nullptr;
+    if (ctx->paths) ctx->paths->drop_graph();
+    auto bs = std::make_unique<BatchState>();
+    std::vector<uint8_t> nodes;
+    std::vector<TriRecord> tris;
+    std::vector<TriShade> shade;
+    std::vector<MaterialGpu> mats;
+    uint64_t pix = 0;
+    int width = 0, depth = 0;
+    uint32_t num_nodes = 0;
+    for (uint32_t s = 0; s < num_scenes; ++s) {
+        const spk_batch_scene &sc = scenes[s];
+        if (sc.width < 2 || sc.height < 2)
+            return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "scene %u: image must be at least 2x2", s);
+        for (float v : {sc.camera.position[0], sc.camera.position[1], sc.camera.position[2], sc.camera.lens_radius})
+            if (!std::isfinite(v)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "scene %u: non-finite camera", s);
+        HostScene hs;
+        std::vector<MaterialGpu> sm;
+        std::string err;
+        int rc = prepare_scene(sc.meshes, sc.num_meshes, sc.instances, sc.num_instances, sc.materials,
+                               sc.num_materials, options, hs, sm, err);
+        if (rc != SPK_OK) return set_error(ctx, rc, "scene %u: %s", s, err.c_str());
+        if (s == 0) width = (int)hs.input.width;
+        depth = std::max(depth, (int)hs.bvh.depth);
+        const uint64_t node_off = num_nodes, tri_off = tris.size(), mat_off = mats.size();
+        if (node_off + hs.bvh.num_nodes >= (1ull << 32) || tri_off + hs.bvh.tris.size() >= (1ull << 32) ||
+            mat_off + sm.size() >= (1ull << 31))
+            return set_error(ctx, SPK_ERR_INVALID_SCENE, "batch exceeds 2^32 nodes or triangles");
+        if (width == 8)
+            append_nodes<NodeW8>(hs.bvh.nodes, hs.bvh.num_nodes, (uint32_t)node_off, (uint32_t)tri_off, nodes);
+        else
+            append_nodes<NodeW4>(hs.bvh.nodes, hs.bvh.num_nodes, (uint32_t)node_off, (uint32_t)tri_off, nodes);
+        num_nodes += hs.bvh.num_nodes;
+        for (const TriRecord &t : hs.bvh.tris) {
+            TriRecord r = t;
+            uint32_t m;
+            std::memcpy(&m, &r.v1[3], 4);
+            m += (uint32_t)mat_off;
+            std::memcpy(&r.v1[3], &m, 4);
+            tris.push_back(r);
+        }
+        for (TriShade sh : leaf_order_shade(hs)) {
+            sh.material += (uint32_t)mat_off;
+            shade.push_back(sh);
+        }
+        mats.insert(mats.end(), sm.begin(), sm.end());
+        SceneRec rec;
+        std::memset(&rec, 0, sizeof(rec));
+        rec.root = (uint32_t)node_off;
+        rec.width = sc.width;
+        rec.height = sc.height;
+        rec.pix_base = (uint32_t)pix;
+        rec.cam = sc.camera;
+        bs->host_table.push_back(rec);
+        bs->offsets.push_back(pix);
+        pix += (uint64_t)sc.width * sc.height;
+        if (pix >= (1ull << 31)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "batch has 2^31 or more pixels");
+    }
+    bs->offsets.push_back(pix);
+    if (pick_trace_stack(depth) < 0)
+        return set_error(ctx, SPK_ERR_STACK_DEPTH, "BVH depth %d exceeds the largest traversal stack", depth);
+    DeviceScene &ds = bs->scene;
+    hipError_t e;
+    if ((e = ds.nodes.ensure(nodes.size())) != hipSuccess ||
+        (e = ds.tris.ensure(tris.size() * sizeof(TriRecord))) != hipSuccess ||
+        (e = ds.shade.ensure(shade.size() * sizeof(TriShade))) != hipSuccess ||
+        (e = ds.materials.ensure(mats.size() * sizeof(MaterialGpu))) != hipSuccess ||
+        (e = bs->table.ensure(bs->host_table.size() * sizeof(SceneRec))) != hipSuccess)
+        return hip_fail(ctx, e, "batch allocation");
+    if ((e = hipMemcpy(ds.nodes.ptr, nodes.data(), nodes.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(ds.tris.ptr, tris.data(), tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(ds.shade.ptr, shade.data(), shade.size() * sizeof(TriShade), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(ds.materials.ptr, mats.data(), mats.size() * sizeof(MaterialGpu), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(bs->table.ptr, bs->host_table.data(), bs->host_table.size() * sizeof(SceneRec),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(ctx, e, "batch upload");
+    ds.num_tris = (uint32_t)tris.size();
+    ds.num_nodes = num_nodes;
+    ds.num_materials = (uint32_t)mats.size();
+    ds.width = width;
+    ds.depth = depth;
+    ds.variant = TraceVariant{width, pick_trace_stack(depth), 128};
+    ds.valid = true;
+    bs->num_scenes = num_scenes;
+    bs->npix = (uint32_t)pix;
+    bs->num_tris = tris.size();
+    ctx->batch = bs.release();
+    return SPK_OK;
+}
+
+int spk_batch_offsets(spk_context *ctx, uint64_t *offsets, uint32_t max_count) {
+    if (!ctx || !offsets) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->batch) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
+    if (max_count < ctx->batch->num_scenes + 1u)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "offsets need %u entries", ctx->batch->num_scenes + 1u);
+    std::copy(ctx->batch->offsets.begin(), ctx->batch->offsets.end(), offsets);
+    return SPK_OK;
+}
+
+int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_frame_stats *stats) {
+    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->batch) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
+    if (!rgba_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null output");
+    if (frame->flags & (SPK_FRAME_NEE | SPK_FRAME_CACHE))
+        return set_error(ctx, SPK_ERR_UNSUPPORTED, "NEE and the radiance cache are single-scene features");
+    const BatchState &bs = *ctx->batch;
+    FrameSource src;
+    std::memset(&src, 0, sizeof(src));
+    src.ds = &bs.scene;
+    src.scenes_dev = (const SceneRec *)bs.table.ptr;
+    src.num_scenes = bs.num_scenes;
+    src.npix = bs.npix;
+    src.blocks_per_cu = bs.blocks_per_cu;
+    src.sort = bs.sort;
+    return run_frame(ctx, frame, src, rgba_dev, nullptr, stats);
+}
+
+int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, uint32_t *blocks_per_cu,
+                       uint32_t *sorted, double *best_ms) {
+    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->batch) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
+    BatchState &bs = *ctx->batch;
+    static const uint32_t kBlocks[] = {2u, 4u, 8u, 16u};
+    double best = 1e30;
+    uint32_t best_b = 8u, best_s = 1u;
+    for (uint32_t srt = 0; srt < 2; ++srt) {
+        for (uint32_t b : kBlocks) {
+            bs.blocks_per_cu = b;
+            bs.sort = srt;
+            spk_frame_stats st{};
+            // first frame captures the graph, the second is timed
+            int rc = spk_batch_render(ctx, frame, rgba_dev, nullptr);
+            if (rc == SPK_OK) rc = spk_batch_render(ctx, frame, rgba_dev, &st);
+            if (rc != SPK_OK) return rc;
+            if (st.milliseconds < best) {
+                best = st.milliseconds;
+                best_b = b;
+                best_s = srt;
+            }
+        }
+    }
+    bs.blocks_per_cu = best_b;
+    bs.sort = best_s;
+    if (blocks_per_cu) *blocks_per_cu = best_b;
+    if (sorted) *sorted = best_s;
+    if (best_ms) *best_ms = best;
     return SPK_OK;
 }
 

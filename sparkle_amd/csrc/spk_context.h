@@ -52,6 +52,15 @@ struct DeviceScene {
 
 struct PathState;   // path-tracer buffers, defined with the path-tracer kernels
 struct CacheState;  // learned radiance cache (spk_cache.h)
+struct BatchState;  // batch of independent scenes, defined with the path-tracer kernels
+
+// Validates and flattens one scene, builds its BVH, converts shading attributes (input order) and
+// materials. Returns an SPK_* code and a message in err.
+int prepare_scene(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_instance_desc *instances,
+                  uint32_t num_instances, const spk_material_desc *materials, uint32_t num_materials,
+                  const spk_build_options *options, HostScene &hs, std::vector<MaterialGpu> &mats, std::string &err);
+// Shading attributes permuted to the BVH's leaf order.
+std::vector<TriShade> leaf_order_shade(const HostScene &hs);
 
 }  // namespace spk
 
@@ -70,6 +79,7 @@ struct spk_context {
     bool has_camera = false;
     spk::PathState *paths = nullptr;
     spk::CacheState *cache = nullptr;
+    spk::BatchState *batch = nullptr;
     float cache_last_loss = 0.f;
 
     uint32_t *invalid_counter() { return static_cast<uint32_t *>(counters.ptr); }

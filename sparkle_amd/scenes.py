@@ -325,17 +325,21 @@ def displaced_box(n, amp, rng):
             np.concatenate(tan_all), np.concatenate(uv_all))
 
 
-def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance=25.0):
+def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance=25.0, aspect=1920 / 1080):
     """`count` seed-placed instances flattened to world space (about `target_tris` triangles):
-    icospheres (subdivision 2-5), tori (R/r = 3) and noise-displaced boxes on a 20 x 20 ground plane;
-    materials 60% Lambertian, 30% GGX metal (roughness U[0.05, 0.6]), 10% dielectric (IOR 1.5);
-    three emissive area lights. Returns (scene, camera-at-1920x1080)."""
+    icospheres (subdivision 2-5, capped by the per-instance budget), tori (R/r = 3) and noise-displaced
+    boxes on a 20 x 20 ground plane; materials 60% Lambertian, 30% GGX metal (roughness U[0.05, 0.6]),
+    10% dielectric (IOR 1.5); three emissive area lights. Returns (scene, camera with the given
+    aspect ratio, 1920x1080 by default)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sc = Scene()
     ground = sc.add_material(base_color=(0.5, 0.5, 0.5), metallic=0.0, roughness=1.0)
     _add(sc, _quad((-10, -10, 0), (10, -10, 0), (10, 10, 0), (-10, 10, 0), (0, 0, 1), (1, 0, 0)), ground)
     # per-instance triangle budget steering the mesh resolutions toward target_tris
     per = target_tris / count
+    max_sub = 2
+    while max_sub < 5 and 20 * 4 ** (max_sub + 1) <= 2 * per:
+        max_sub += 1
     side = int(math.ceil(math.sqrt(count)))
     cell = 19.0 / side
     for k in range(count):
@@ -345,7 +349,7 @@ def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance
         size = rng.uniform(0.18, 0.38) * cell
         kind = rng.integers(3)
         if kind == 0:
-            sub = int(rng.integers(2, 6))
+            sub = min(int(rng.integers(2, 6)), max_sub)
             p, i, n, t, uv = _unit_icosphere(sub)
             p = p * size + np.array([cx, cy, size])
         elif kind == 1:
@@ -379,5 +383,20 @@ def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance
         _add(sc, _quad((lx - 1, ly - 1, h), (lx - 1, ly + 1, h), (lx + 1, ly + 1, h), (lx + 1, ly - 1, h),
                        (0, 0, -1), (1, 0, 0)), light)
     cam = camera_desc(position=(0.0, -16.0, 7.0), front=(0.0, 1.0, -0.42), vertical_fov=math.radians(50.0),
-                      focus_distance=16.0, aperture_radius=0.0, aspect=1920 / 1080)
+                      focus_distance=16.0, aperture_radius=0.0, aspect=aspect)
     return sc, cam
+
+
+def batch_scenes(seed=0, count=64, min_tris=1_000, max_tris=500_000, sizes=(64, 128, 256, 512)):
+    """`count` independent scenes of the many-instance generator: triangle budgets log-uniform in
+    [min_tris, max_tris], square images with sides drawn from `sizes`. Returns a list of
+    (scene, camera, width, height)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    out = []
+    for k in range(count):
+        target = float(np.exp(rng.uniform(math.log(min_tris), math.log(max_tris))))
+        side = int(sizes[int(rng.integers(len(sizes)))])
+        inst = int(min(400, max(4, target // 1500)))
+        sc, cam = many_instance_scene(seed=seed * 1000 + k, count=inst, target_tris=target, aspect=1.0)
+        out.append((sc, cam, side, side))
+    return out

@@ -77,3 +77,19 @@ def test_full_size_heightfield_builds():
     assert st.num_triangles == 2 * 1024 * 1024
     assert st.depth <= 24
     assert st.build_seconds < 30
+
+
+def test_batch_scene_generator_spans_the_requested_sizes():
+    from sparkle_amd import scenes
+
+    items = scenes.batch_scenes(seed=0, count=64)
+    tris = np.array([sc.num_triangles for sc, _, _, _ in items])
+    sides = {w for _, _, w, h in items}
+    assert all(w == h for _, _, w, h in items)
+    assert sides <= {64, 128, 256, 512} and len(sides) == 4
+    # log-uniform budgets in [1k, 500k]: small and large scenes both present, within a factor of the range
+    assert tris.min() < 5_000 and tris.max() > 200_000
+    assert tris.min() >= 1_000 and tris.max() < 1_500_000
+    # the many-instance scene itself is unchanged by the per-instance budget cap
+    sc, _ = scenes.many_instance_scene(seed=0)
+    assert sc.num_triangles == 5_533_044
