@@ -101,7 +101,8 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     def frame(seed, flags=0):
         f = spk.FrameDesc()
         f.width, f.height, f.frame_seed, f.max_bounce, f.debug_mode = width, height, seed, max_bounce, 0
-        f.sky_enabled, f.flags, f.output_limit = 0, flags, 0.0
+        # 64-bit seed composition: at 1080p the reference's uint32 j*W + i + f*W*H wraps from frame 2072 on
+        f.sky_enabled, f.flags, f.output_limit = 0, flags | spk.FRAME_SEED64, 0.0
         return f
 
     # warm up (graph capture) outside every timed region
@@ -178,6 +179,59 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     return out
 
 
+def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
+    """Many scenes of uneven size in one launch: `count` many-instance scenes (triangle budgets
+    log-uniform in [1k, 500k], images 64^2..512^2), `spp` frames of `max_bounce` bounces each, all
+    scenes' paths in the same persistent kernels; schedule picked by spk_batch_autotune."""
+    import torch
+
+    log(f"batch experiment: generating {count} scenes")
+    items = scenes.batch_scenes(seed=0, count=count)
+    ctx = spk.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    t0 = time.perf_counter()
+    offs = ctx.upload_batch(items)
+    t_build = time.perf_counter() - t0
+    npx = offs[-1]
+    rgba = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+    acc = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+
+    def frame(seed):
+        f = spk.FrameDesc()
+        f.width, f.height, f.frame_seed, f.max_bounce = 0, 0, seed, max_bounce
+        f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, 0, 0.0
+        return f
+
+    blocks, srt, tune_ms = ctx.autotune_batch(frame(0), rgba)
+    torch.cuda.synchronize()
+    rays = 0
+    t0 = time.perf_counter()
+    for k in range(spp):
+        rays += ctx.render_batch(frame(k), rgba).rays_traced
+        ctx.accumulate(rgba, acc, npx, k, 1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    img = acc.view(-1, 4)
+    finite = bool(torch.isfinite(img).all().item())
+    tris = [sc.num_triangles for sc, _, _, _ in items]
+    out = {
+        "scenes": count,
+        "triangles_total": int(sum(tris)),
+        "triangles_min_max": [int(min(tris)), int(max(tris))],
+        "pixels_per_frame": npx,
+        "spp": spp,
+        "max_bounce": max_bounce,
+        "seconds": round(dt, 3),
+        "samples_per_s": round(npx * spp / dt, 1),
+        "mrays_per_s": round(rays / dt / 1e6, 1),
+        "schedule": {"blocks_per_cu": blocks, "material_sort": bool(srt), "tuned_frame_ms": round(tune_ms, 3)},
+        "build_s": round(t_build, 2),
+        "finite": finite,
+    }
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +245,7 @@ def main():
     ap.add_argument("--config", type=str, default=None, help="traversal variant width,stack,block")
     ap.add_argument("--relmse", type=str, default="1920x1080,64,4096",
                     help="equal-time relMSE experiment WxH,spp,reference-spp ('off' to skip)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -342,6 +397,8 @@ def main():
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     ctx.close()
+    if not args.no_batch:
+        out["batch"] = batch_experiment(spk, scenes, stream)
     if args.relmse != "off":
         res, spp, ref_spp = args.relmse.split(",")
         w_, h_ = (int(x) for x in res.split("x"))

@@ -399,9 +399,10 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                         }
                         done = true;
                     }
-                    if (!done && nee && diffuse) {
+                    if (!done && nee && diffuse && !last) {
                         // next-event estimation for the Lambertian lobe: one area-sampled point on an
-                        // emissive triangle, Lambert BRDF base/pi, two-sided emitters
+                        // emissive triangle, Lambert BRDF base/pi, two-sided emitters. Not at the last
+                        // vertex: the BSDF-only estimator never reaches an emitter from there.
                         uint4 ns = a.nee_rng[path];
                         Rng nr{ns.x, ns.y, ns.z, ns.w};
                         const float r0 = rng_unit(nr) * a.emitter_total;
