@@ -242,7 +242,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-coherent", action="store_true")
-    ap.add_argument("--config", type=str, default=None, help="traversal variant width,stack,block")
+    ap.add_argument("--config", type=str, default=None, help="traversal variant width,stack,block[,refill]")
     ap.add_argument("--relmse", type=str, default="1920x1080,64,4096",
                     help="equal-time relMSE experiment WxH,spp,reference-spp ('off' to skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
@@ -289,15 +289,17 @@ def main():
 
     tune = None
     if args.config:
-        w_, s_, b_ = (int(x) for x in args.config.split(","))
-        ctx.set_trace_config(w_, s_, b_)
+        parts = [int(x) for x in args.config.split(",")]
+        w_, s_, b_ = parts[:3]
+        r_ = parts[3] if len(parts) > 3 else 0
+        ctx.set_trace_config(w_, s_, b_, r_)
         st = ctx.stats()
-        tune = {"bvh_width": w_, "stack": s_, "block": b_, "fixed": True}
+        tune = {"bvh_width": w_, "stack": s_, "block": b_, "refill": r_, "fixed": True}
     elif not args.no_autotune:
         log("autotuning traversal variants")
         sub = min(n, 1 << 21)
         cfg, ms = ctx.autotune(d_rays, sub, tune_width=True)
-        tune = {"bvh_width": cfg[0], "stack": cfg[1], "block": cfg[2]}
+        tune = {"bvh_width": cfg[0], "stack": cfg[1], "block": cfg[2], "refill": cfg[3]}
         st = ctx.stats()
 
     # traversal counts for the algorithmic-bytes model
@@ -353,7 +355,7 @@ def main():
 
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     cur = ctx.trace_config()
-    traffic, traffic_src = load_traffic(f"{cur[0]},{cur[1]},{cur[2]}")
+    traffic, traffic_src = load_traffic(",".join(str(x) for x in cur))
     out = {
         "metric": METRIC,
         "value": round(value, 2),

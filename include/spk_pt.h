@@ -209,6 +209,8 @@ typedef struct spk_trace_config {
     uint32_t bvh_width;
     uint32_t stack;
     uint32_t block;
+    uint32_t refill; /* 0: one ray per thread; k > 0: persistent lanes, a wave takes new rays once k of
+                        its 64 lanes are idle */
 } spk_trace_config;
 
 #define SPK_TUNE_WIDTH 1u /* also try the other BVH width (rebuilds the tree on the host) */

@@ -79,10 +79,11 @@ class SceneStats(ctypes.Structure):
 
 
 class TraceConfig(ctypes.Structure):
-    _fields_ = [("bvh_width", ctypes.c_uint32), ("stack", ctypes.c_uint32), ("block", ctypes.c_uint32)]
+    _fields_ = [("bvh_width", ctypes.c_uint32), ("stack", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("refill", ctypes.c_uint32)]
 
     def as_tuple(self):
-        return (self.bvh_width, self.stack, self.block)
+        return (self.bvh_width, self.stack, self.block, self.refill)
 
 
 TUNE_WIDTH = 1
@@ -345,15 +346,16 @@ class Context:
         self._check(self.lib.spk_get_trace_config(self.h, ctypes.byref(c)))
         return c.as_tuple()
 
-    def set_trace_config(self, bvh_width, stack, block):
-        c = TraceConfig(bvh_width, stack, block)
+    def set_trace_config(self, bvh_width, stack, block, refill=0):
+        """refill 0: one ray per thread; k > 0: persistent lanes refilled once k lanes of a wave are idle."""
+        c = TraceConfig(bvh_width, stack, block, refill)
         return self._check(self.lib.spk_set_trace_config(self.h, ctypes.byref(c)))
 
     def list_trace_configs(self):
-        arr = (TraceConfig * 64)()
+        arr = (TraceConfig * 256)()
         cnt = ctypes.c_uint32()
-        self._check(self.lib.spk_list_trace_configs(self.h, arr, 64, ctypes.byref(cnt)))
-        return [arr[i].as_tuple() for i in range(min(cnt.value, 64))]
+        self._check(self.lib.spk_list_trace_configs(self.h, arr, 256, ctypes.byref(cnt)))
+        return [arr[i].as_tuple() for i in range(min(cnt.value, 256))]
 
     def autotune(self, rays_dev, n=None, tune_width=False):
         if n is None:

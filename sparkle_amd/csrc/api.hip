@@ -18,6 +18,8 @@ using namespace spk;
 
 namespace spk {
 
+constexpr int kMaxVariants = 256;
+
 int set_error(spk_context *ctx, int code, const char *fmt, ...) {
     if (ctx) {
         char buf[1024];
@@ -422,7 +424,7 @@ int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t 
     if (n && (!rays_dev || !hits_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null ray or hit array");
     if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
     hipError_t e = launch_trace(ctx->scene.variant, rays_dev, n, hits_dev, ctx->scene.nodes.ptr, ctx->scene.tris.ptr,
-                                0u, ctx->invalid_counter(), nullptr, nullptr, ctx->stream);
+                                0u, ctx->invalid_counter(), nullptr, nullptr, ctx->work_counter(), ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
     return SPK_OK;
 }
@@ -460,7 +462,7 @@ int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint3
     hipError_t e;
     if ((e = ctx->tmp_hits.ensure(n * sizeof(spk_hit))) != hipSuccess) return hip_fail(ctx, e, "hit buffer");
     e = launch_trace(ctx->scene.variant, rays_dev, n, ctx->tmp_hits.ptr, ctx->scene.nodes.ptr, ctx->scene.tris.ptr,
-                     0u, ctx->invalid_counter(), nodes_dev, tris_dev, ctx->stream);
+                     0u, ctx->invalid_counter(), nodes_dev, tris_dev, ctx->work_counter(), ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "count launch");
     return spk_synchronize(ctx);
 }
@@ -485,18 +487,21 @@ int spk_get_trace_config(spk_context *ctx, spk_trace_config *out) {
     out->bvh_width = (uint32_t)ctx->scene.variant.width;
     out->stack = (uint32_t)ctx->scene.variant.stack;
     out->block = (uint32_t)ctx->scene.variant.block;
+    out->refill = (uint32_t)ctx->scene.variant.refill;
     return SPK_OK;
 }
 
 int spk_list_trace_configs(spk_context *ctx, spk_trace_config *out, uint32_t max_out, uint32_t *count) {
     if (!ctx || !count) return SPK_ERR_INVALID_ARGUMENT;
     if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
-    TraceVariant vs[64];
-    const int nv = std::min(64, trace_variants(vs, 64, ctx->scene.width));
+    TraceVariant vs[kMaxVariants];
+    const int nv = std::min(kMaxVariants, trace_variants(vs, kMaxVariants, ctx->scene.width));
     uint32_t k = 0;
     for (int i = 0; i < nv; ++i) {
         if (vs[i].stack < ctx->scene.depth) continue;
-        if (out && k < max_out) out[k] = spk_trace_config{(uint32_t)vs[i].width, (uint32_t)vs[i].stack, (uint32_t)vs[i].block};
+        if (out && k < max_out)
+            out[k] = spk_trace_config{(uint32_t)vs[i].width, (uint32_t)vs[i].stack, (uint32_t)vs[i].block,
+                                      (uint32_t)vs[i].refill};
         ++k;
     }
     *count = k;
@@ -522,17 +527,18 @@ int spk_set_trace_config(spk_context *ctx, const spk_trace_config *config) {
     if ((int)config->stack < ctx->scene.depth)
         return set_error(ctx, SPK_ERR_STACK_DEPTH, "stack %u is smaller than the tree depth %d", config->stack,
                          ctx->scene.depth);
-    TraceVariant vs[64];
-    const int nv = std::min(64, trace_variants(vs, 64, ctx->scene.width));
+    TraceVariant vs[kMaxVariants];
+    const int nv = std::min(kMaxVariants, trace_variants(vs, kMaxVariants, ctx->scene.width));
     for (int i = 0; i < nv; ++i) {
-        if (vs[i].stack == (int)config->stack && vs[i].block == (int)config->block) {
+        if (vs[i].stack == (int)config->stack && vs[i].block == (int)config->block &&
+            vs[i].refill == (int)config->refill) {
             ctx->scene.variant = vs[i];
             ctx->on_scene_changed();
             return SPK_OK;
         }
     }
-    return set_error(ctx, SPK_ERR_UNSUPPORTED, "no compiled variant with stack %u and block %u", config->stack,
-                     config->block);
+    return set_error(ctx, SPK_ERR_UNSUPPORTED, "no compiled variant with stack %u, block %u and refill %u",
+                     config->stack, config->block, config->refill);
 }
 
 static int time_variant(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, double &ms) {
@@ -548,7 +554,8 @@ static int time_variant(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, d
         e = hipEventRecord(a, ctx->stream);
         if (e == hipSuccess)
             e = launch_trace(ctx->scene.variant, rays_dev, n, ctx->tmp_hits.ptr, ctx->scene.nodes.ptr,
-                             ctx->scene.tris.ptr, 0u, ctx->invalid_counter(), nullptr, nullptr, ctx->stream);
+                             ctx->scene.tris.ptr, 0u, ctx->invalid_counter(), nullptr, nullptr, ctx->work_counter(),
+                             ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float t = 0.f;
@@ -569,19 +576,19 @@ int spk_autotune(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t
     if (!rays_dev || n == 0) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "autotune needs rays");
     const int widths[2] = {ctx->scene.width, ctx->scene.width == 8 ? 4 : 8};
     const int nw = (flags & SPK_TUNE_WIDTH) ? 2 : 1;
-    spk_trace_config best{0, 0, 0};
+    spk_trace_config best{0, 0, 0, 0};
     double best_t = 1e300;
     for (int wi = 0; wi < nw; ++wi) {
-        spk_trace_config cfg{(uint32_t)widths[wi], 64u, 64u};
+        spk_trace_config cfg{(uint32_t)widths[wi], 64u, 64u, 0u};
         if ((int)cfg.bvh_width != ctx->scene.width) {
             int rc = spk_set_trace_config(ctx, &cfg);
             if (rc != SPK_OK) return rc;
         }
-        spk_trace_config list[64];
+        spk_trace_config list[kMaxVariants];
         uint32_t cnt = 0;
-        int rc = spk_list_trace_configs(ctx, list, 64, &cnt);
+        int rc = spk_list_trace_configs(ctx, list, kMaxVariants, &cnt);
         if (rc != SPK_OK) return rc;
-        for (uint32_t i = 0; i < std::min<uint32_t>(cnt, 64); ++i) {
+        for (uint32_t i = 0; i < std::min<uint32_t>(cnt, kMaxVariants); ++i) {
             rc = spk_set_trace_config(ctx, &list[i]);
             if (rc != SPK_OK) return rc;
             double ms = 0;

@@ -99,6 +99,7 @@ struct PtArgs {
     const SceneRec *scenes;  // num_scenes records, sorted by pix_base
     uint32_t *path_scene;    // scene of each path (written only when num_scenes > 1)
     uint32_t num_scenes, grid;
+    uint32_t refill;  // extend: a wave takes new paths once this many of its lanes are idle (1..64)
     uint32_t npix, max_bounce, debug_mode, sky_enabled, flags, sort;
     float sky[3];
     float output_limit;
@@ -221,6 +222,8 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
 }
 
 // ---- extend: closest hit for every queued path + material histogram ----
+// Lanes trace one path each and a wave takes new queue slots (one atomic) once `refill` of its
+// lanes are idle, so a wave does not wait for its slowest ray before fetching more work.
 template <int WIDTH, int STACK>
 __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) {
     __shared__ uint2 stack[STACK * kPtBlock];
@@ -231,36 +234,56 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
     if (a.sort)
         for (uint32_t k = threadIdx.x; k < a.nkeys; k += kPtBlock) lhist[k] = 0u;
     __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+    Traversal<WIDTH, STACK, false> tr;
+    uint32_t slot = 0u, path = 0u;
+    bool active = false, drained = false;
     for (;;) {
-        const uint32_t base = wave_fetch(ctr + CTR_EXTEND);
-        if (base >= count) break;
-        const uint32_t slot = base + lane_id();
-        if (slot >= count) continue;
-        const uint32_t path = queue[slot];
-        const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
-        RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
-        BestHit best;
-        best.t = r.tmax;
-        best.prim = 0xFFFFFFFFu;
-        best.tri = 0xFFFFFFFFu;
-        best.U = best.V = best.W = 0.f;
-        best.det = 1.f;
-        TraverseCounters c{0u, 0u};
-        const uint32_t root = a.num_scenes > 1u ? a.scenes[a.path_scene[path]].root : 0u;
-        traverse<WIDTH, STACK, false>(a.nodes, a.tris, root, r, stack + threadIdx.x, kPtBlock, best, c);
-        uint4 h;
-        uint32_t key = a.nkeys - 1u;  // misses sort last
-        if (best.prim != 0xFFFFFFFFu) {
-            const float t = final_t(a.tris, r, best);
-            h = make_uint4(best.tri, __float_as_uint(t), __float_as_uint(best.V / best.det),
-                           __float_as_uint(best.W / best.det));
-            key = a.shade[best.tri].material;
-            if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
-        } else {
-            h = make_uint4(0xFFFFFFFFu, __float_as_uint(INFINITY), 0u, 0u);
+        if (!drained) {
+            const uint64_t idle = __ballot(!active);
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            if (nidle >= a.refill) {
+                uint32_t base = 0u;
+                if (lane == 0u) base = atomicAdd(ctr + CTR_EXTEND, nidle);
+                base = __shfl(base, 0);
+                drained = base + nidle >= count;
+                if (!active) {
+                    const uint32_t my = base + (uint32_t)__popcll(idle & below);
+                    if (my < count) {
+                        path = queue[my];
+                        const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
+                        const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
+                        const uint32_t root = a.num_scenes > 1u ? a.scenes[a.path_scene[path]].root : 0u;
+                        tr.init(r, root);
+                        slot = my;
+                        active = true;
+                    }
+                }
+            }
         }
-        a.hits[slot] = h;
-        if (a.sort) atomicAdd(&lhist[key], 1u);
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        if (active && !tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
+            uint4 h;
+            uint32_t key = a.nkeys - 1u;  // misses sort last
+            if (tr.best.prim != 0xFFFFFFFFu) {
+                const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
+                const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
+                const float t = final_t(a.tris, r, tr.best);
+                h = make_uint4(tr.best.tri, __float_as_uint(t), __float_as_uint(tr.best.V / tr.best.det),
+                               __float_as_uint(tr.best.W / tr.best.det));
+                key = a.shade[tr.best.tri].material;
+                if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
+            } else {
+                h = make_uint4(0xFFFFFFFFu, __float_as_uint(INFINITY), 0u, 0u);
+            }
+            a.hits[slot] = h;
+            if (a.sort) atomicAdd(&lhist[key], 1u);
+            active = false;
+        }
     }
     if (a.sort) {
         __syncthreads();
@@ -618,6 +641,7 @@ struct PtEntry {
     ExtendFn extend, shadow;
 };
 const PtEntry kPtEntries[] = {
+    {8, 12, k_extend<8, 12>, k_shadow<8, 12>}, {4, 12, k_extend<4, 12>, k_shadow<4, 12>},
     {8, 16, k_extend<8, 16>, k_shadow<8, 16>}, {8, 24, k_extend<8, 24>, k_shadow<8, 24>},
     {8, 32, k_extend<8, 32>, k_shadow<8, 32>}, {8, 64, k_extend<8, 64>, k_shadow<8, 64>},
     {4, 16, k_extend<4, 16>, k_shadow<4, 16>}, {4, 24, k_extend<4, 24>, k_shadow<4, 24>},
@@ -816,6 +840,7 @@ struct FrameSource {
     uint32_t npix;
     uint32_t blocks_per_cu;
     uint32_t sort;
+    uint32_t refill;
 };
 
 static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameSource &src, float *rgba_dev,
@@ -874,6 +899,7 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     const int cus = device_cus(ctx);
     const int grid = cus * (int)src.blocks_per_cu;
     a.grid = (uint32_t)grid;
+    a.refill = std::min(std::max(src.refill, 1u), 64u);
     a.npix = npix;
     a.max_bounce = frame->max_bounce;
     a.debug_mode = frame->debug_mode;
@@ -1018,6 +1044,7 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     src.npix = frame->width * frame->height;
     src.blocks_per_cu = 8u;
     src.sort = 1u;
+    src.refill = 16u;
     return run_frame(ctx, frame, src, rgba_dev, normal_dev, stats);
 }
 
@@ -1145,6 +1172,7 @@ int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     src.npix = bs.npix;
     src.blocks_per_cu = bs.blocks_per_cu;
     src.sort = bs.sort;
+    src.refill = 16u;
     return run_frame(ctx, frame, src, rgba_dev, nullptr, stats);
 }
 

@@ -69,7 +69,7 @@ struct spk_context {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string error;
-    spk::DeviceBuffer counters;  // [0] invalid rays
+    spk::DeviceBuffer counters;  // [0] invalid rays, [2..3] ray counter of the persistent kernels
     spk::DeviceBuffer tmp_rays, tmp_hits;
     spk::DeviceScene scene;
     spk::HostScene host_scene;
@@ -83,6 +83,9 @@ struct spk_context {
     float cache_last_loss = 0.f;
 
     uint32_t *invalid_counter() { return static_cast<uint32_t *>(counters.ptr); }
+    unsigned long long *work_counter() {
+        return reinterpret_cast<unsigned long long *>(static_cast<uint32_t *>(counters.ptr) + 2);
+    }
     int check_invalid_rays();
     void on_scene_changed();
     void release_all();

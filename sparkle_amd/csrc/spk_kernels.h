@@ -10,20 +10,22 @@ struct TraceVariant {
     int width;  // 4 or 8
     int stack;  // LDS stack entries per lane
     int block;  // threads per block
+    int refill; // 0: one ray per thread; k > 0: persistent lanes, a wave refills when k lanes are idle
 };
 
 // Closest hit of n rays (32-byte records) into n 8-byte hits. invalid_count (device) is
 // incremented once per NaN/inf/zero-direction ray. cnt_nodes/cnt_tris (device, may be null)
-// receive per-ray traversal counts when non-null.
+// receive per-ray traversal counts when non-null. work (device, 8 bytes) is the ray counter of
+// the persistent variants; it is cleared by a kernel launched just before them.
 hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, void *hits, const void *nodes,
                         const void *tris, uint32_t root, uint32_t *invalid_count, uint32_t *cnt_nodes,
-                        uint32_t *cnt_tris, hipStream_t stream);
+                        uint32_t *cnt_tris, unsigned long long *work, hipStream_t stream);
 
 // Largest stack depth any compiled variant supports.
 int max_trace_stack();
 // Smallest compiled stack size >= depth, or -1.
 int pick_trace_stack(int depth);
-// Compiled (stack, block) pairs, for the autotuner.
+// Compiled (stack, block, refill) configurations of one width, for the autotuner.
 int trace_variants(TraceVariant *out, int max_out, int width);
 
 }  // namespace spk

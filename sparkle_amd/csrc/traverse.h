@@ -250,39 +250,53 @@ __device__ __forceinline__ float safe_inv(float d) {
     return __builtin_amdgcn_rcpf(dd);
 }
 
-// One closest-hit query over a wide BVH. STACK entries per lane live in LDS at
+// Closest-hit traversal state of one ray over a wide BVH. STACK entries per lane live in LDS at
 // stack[k * stride]; each entry is a node group {child_base, imask << 8 | pending-hit bits}.
+// step() pops a group when the current one is exhausted, visits one node (all child boxes) and
+// tests that node's leaf triangles; traverse() runs steps until the ray is done, and the
+// persistent closest-hit kernel interleaves steps with refilling finished lanes.
 template <int WIDTH, int STACK, bool COUNT, bool ANYHIT = false>
-__device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const uint4 *__restrict__ tris,
-                                         uint32_t root, const RayIn &ray, uint2 *stack, int stride, BestHit &best,
-                                         TraverseCounters &cnt) {
-    const float ox = ray.ox, oy = ray.oy, oz = ray.oz;
-    const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
-    const WtSetup wt = wt_setup(dx, dy, dz);
-    const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
-    const uint32_t oct = (dx < 0.f ? 1u : 0u) | (dy < 0.f ? 2u : 0u) | (dz < 0.f ? 4u : 0u);
-    const float box_tmin = fmaxf(ray.tmin, 0.f);
+struct Traversal {
+    float ox, oy, oz, ix, iy, iz, box_tmin;
+    WtSetup wt;
+    uint32_t oct;
+    // current node group: slot s of the group's node was hit iff bit (s ^ g_oct) of g_hits is set
+    uint32_t g_base, g_imask, g_hits, g_oct;
+    int sp;
+    BestHit best;
+    TraverseCounters cnt;
 
-    // current node group: slot s of the group's node was hit iff bit (s ^ oct) of `hits` is set
-    uint32_t g_base = root, g_imask = 1u, g_hits;
-    if (WIDTH == 8) {
-        g_hits = 1u << oct;
-    } else {
-        g_hits = 1u;  // 4-wide: rank order is resolved per node below
+    __device__ __forceinline__ void init(const RayIn &ray, uint32_t root) {
+        ox = ray.ox, oy = ray.oy, oz = ray.oz;
+        wt = wt_setup(ray.dx, ray.dy, ray.dz);
+        ix = safe_inv(ray.dx), iy = safe_inv(ray.dy), iz = safe_inv(ray.dz);
+        oct = (ray.dx < 0.f ? 1u : 0u) | (ray.dy < 0.f ? 2u : 0u) | (ray.dz < 0.f ? 4u : 0u);
+        box_tmin = fmaxf(ray.tmin, 0.f);
+        g_base = root;
+        g_imask = 1u;
+        g_hits = WIDTH == 8 ? 1u << oct : 1u;  // 4-wide: rank order is resolved per node below
+        g_oct = WIDTH == 8 ? oct : 0u;
+        sp = 0;
+        best.t = ray.tmax;
+        best.prim = 0xFFFFFFFFu;
+        best.tri = 0xFFFFFFFFu;
+        best.U = best.V = best.W = 0.f;
+        best.det = 1.f;
+        cnt.nodes = 0u;
+        cnt.tris = 0u;
     }
-    uint32_t g_oct = WIDTH == 8 ? oct : 0u;
-    int sp = 0;
 
-    for (;;) {
+    // Returns false once the ray is done (stack empty, or any hit for an occlusion query).
+    __device__ __forceinline__ bool step(const uint4 *__restrict__ nodes, const uint4 *__restrict__ tris,
+                                         uint2 *stack, int stride) {
         if (g_hits == 0u) {
-            if (sp == 0) break;
+            if (sp == 0) return false;
             --sp;
             const uint2 e = stack[sp * stride];
             g_base = e.x;
             g_imask = (e.y >> 8) & 0xffu;
-            g_hits = (e.y >> 16) & 0xffu;
+            g_hits = (e.y >> 16) & 0xffu;  // pushed entries always hold a pending hit
             g_oct = e.y & 0xffu;
-            continue;
         }
         const uint32_t r = __builtin_ctz(g_hits);
         g_hits &= g_hits - 1u;
@@ -294,6 +308,7 @@ __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const 
         }
         if (COUNT) cnt.nodes++;
 
+        const bool nxs = (oct & 1u) != 0u, nys = (oct & 2u) != 0u, nzs = (oct & 4u) != 0u;
         const float tcull = best.t * SPK_BOX_SLACK;
         uint32_t new_hits = 0u, tri_bits = 0u;
         uint32_t child_base, tri_base, imask, node_oct;
@@ -309,12 +324,12 @@ __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const 
             const float sx = exp2i(ex) * ix, sy = exp2i(ey) * iy, sz = exp2i(ez) * iz;
             const float ax = (px - ox) * ix, ay = (py - oy) * iy, az = (pz - oz) * iz;
             // near planes are qlo for a positive direction, qhi for a negative one
-            const uint32_t nx0 = dx < 0.f ? n3.z : n2.x, nx1 = dx < 0.f ? n3.w : n2.y;
-            const uint32_t fx0 = dx < 0.f ? n2.x : n3.z, fx1 = dx < 0.f ? n2.y : n3.w;
-            const uint32_t ny0 = dy < 0.f ? n4.x : n2.z, ny1 = dy < 0.f ? n4.y : n2.w;
-            const uint32_t fy0 = dy < 0.f ? n2.z : n4.x, fy1 = dy < 0.f ? n2.w : n4.y;
-            const uint32_t nz0 = dz < 0.f ? n4.z : n3.x, nz1 = dz < 0.f ? n4.w : n3.y;
-            const uint32_t fz0 = dz < 0.f ? n3.x : n4.z, fz1 = dz < 0.f ? n3.y : n4.w;
+            const uint32_t nx0 = nxs ? n3.z : n2.x, nx1 = nxs ? n3.w : n2.y;
+            const uint32_t fx0 = nxs ? n2.x : n3.z, fx1 = nxs ? n2.y : n3.w;
+            const uint32_t ny0 = nys ? n4.x : n2.z, ny1 = nys ? n4.y : n2.w;
+            const uint32_t fy0 = nys ? n2.z : n4.x, fy1 = nys ? n2.w : n4.y;
+            const uint32_t nz0 = nzs ? n4.z : n3.x, nz1 = nzs ? n4.w : n3.y;
+            const uint32_t fz0 = nzs ? n3.x : n4.z, fz1 = nzs ? n3.y : n4.w;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const int k = s & 3;
@@ -354,12 +369,11 @@ __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const 
             // n1.w: qlo_x, n2.x: qlo_y, n2.y: qlo_z, n2.z: qhi_x, n2.w: qhi_y, n3.x: qhi_z, n3.y: axis
             const float sx = exp2i(ex) * ix, sy = exp2i(ey) * iy, sz = exp2i(ez) * iz;
             const float ax = (px - ox) * ix, ay = (py - oy) * iy, az = (pz - oz) * iz;
-            const uint32_t wnx = dx < 0.f ? n2.z : n1.w, wfx = dx < 0.f ? n1.w : n2.z;
-            const uint32_t wny = dy < 0.f ? n2.w : n2.x, wfy = dy < 0.f ? n2.x : n2.w;
-            const uint32_t wnz = dz < 0.f ? n3.x : n2.y, wfz = dz < 0.f ? n2.y : n3.x;
+            const uint32_t wnx = nxs ? n2.z : n1.w, wfx = nxs ? n1.w : n2.z;
+            const uint32_t wny = nys ? n2.w : n2.x, wfy = nys ? n2.x : n2.w;
+            const uint32_t wnz = nzs ? n3.x : n2.y, wfz = nzs ? n2.y : n3.x;
             const uint32_t axis = n3.y;
-            const float dax = axis == 0u ? dx : (axis == 1u ? dy : dz);
-            node_oct = dax < 0.f ? 3u : 0u;
+            node_oct = ((oct >> (axis < 3u ? axis : 2u)) & 1u) ? 3u : 0u;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const float tnx = __builtin_fmaf(ubyte(wnx, s), sx, ax);
@@ -409,11 +423,31 @@ __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const 
                     best.V = tt.V;
                     best.W = tt.W;
                     best.det = tt.det;
-                    if (ANYHIT) return;  // occlusion query: any hit in (tmin, tmax] ends it
+                    if (ANYHIT) {  // occlusion query: any hit in (tmin, tmax] ends it
+                        g_hits = 0u;
+                        sp = 0;
+                        return false;
+                    }
                 }
             }
         }
+        return true;
     }
+};
+
+// One closest-hit (or, with ANYHIT, occlusion) query. best.t must hold tmax on entry.
+template <int WIDTH, int STACK, bool COUNT, bool ANYHIT = false>
+__device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const uint4 *__restrict__ tris,
+                                         uint32_t root, const RayIn &ray, uint2 *stack, int stride, BestHit &best,
+                                         TraverseCounters &cnt) {
+    Traversal<WIDTH, STACK, COUNT, ANYHIT> tr;
+    tr.init(ray, root);
+    tr.best.t = best.t;
+    while (tr.step(nodes, tris, stack, stride)) {
+    }
+    best = tr.best;
+    cnt.nodes += tr.cnt.nodes;
+    cnt.tris += tr.cnt.tris;
 }
 
 // refined distance of the final hit (bit-identical to the oracle's)

@@ -162,6 +162,46 @@ def test_all_kernel_variants_agree(gpu_ctx):
     assert seen >= 10
 
 
+def test_persistent_kernels_match_counts_and_invalid_rays(gpu_ctx):
+    """Persistent lanes with refill give the same hits and traversal counts as one ray per thread,
+    report invalid rays the same way, and handle a ray count that is not a multiple of the wave."""
+    import torch
+
+    sc = scenes.heightfield_scene(cells=128)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=200_003, seed=21)
+    rays["org"][[5, 70_000]] = np.nan
+    rays["dir"][199_999] = 0.0
+    n = rays.shape[0]
+    for width in (4, 8):
+        gpu_ctx.upload_scene(sc, bvh_width=width)
+        gpu_ctx.set_trace_config(width, 16, 64, 0)
+        rc, ref = gpu_trace(gpu_ctx, rays)
+        assert rc == spk.SPK_ERR_INVALID_RAY
+        d = to_dev(rays)
+        cn = torch.zeros(n, dtype=torch.int32, device="cuda")
+        ct = torch.zeros(n, dtype=torch.int32, device="cuda")
+        rc0 = gpu_ctx.lib.spk_trace_count(gpu_ctx.h, d.data_ptr(), n, cn.data_ptr(), ct.data_ptr())
+        assert rc0 == spk.SPK_ERR_INVALID_RAY
+        ref_counts = (cn.cpu().numpy().copy(), ct.cpu().numpy().copy())
+        for refill in (1, 16, 32):
+            print_case("persistent", width, refill)
+            gpu_ctx.set_trace_config(width, 16, 64, refill)
+            assert gpu_ctx.trace_config() == (width, 16, 64, refill)
+            rc, h = gpu_trace(gpu_ctx, rays)
+            assert rc == spk.SPK_ERR_INVALID_RAY
+            np.testing.assert_array_equal(h["prim"], ref["prim"])
+            np.testing.assert_array_equal(h["t"].view(np.uint32), ref["t"].view(np.uint32))
+            cn.fill_(-1)
+            ct.fill_(-1)
+            rc1 = gpu_ctx.lib.spk_trace_count(gpu_ctx.h, d.data_ptr(), n, cn.data_ptr(), ct.data_ptr())
+            assert rc1 == spk.SPK_ERR_INVALID_RAY
+            np.testing.assert_array_equal(cn.cpu().numpy(), ref_counts[0])
+            np.testing.assert_array_equal(ct.cpu().numpy(), ref_counts[1])
+        rc, _ = gpu_trace(gpu_ctx, rays[:1000])
+        assert rc == spk.SPK_OK
+
+
 def test_autotune_keeps_results(gpu_ctx):
     sc = scenes.heightfield_scene(cells=256)
     lo, hi = scenes.scene_aabb(sc)
