@@ -23,33 +23,51 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, rays, seconds_target=12.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, rays, seconds_target=2.0, runs=5):
     """The oracle's restatement of the reference CPU path (two-level BVH without tmax culling,
-    Moller-Trumbore, one task per chunk over all host threads), timed on a bounded sample."""
+    Moller-Trumbore, one task per chunk over the host threads), timed on a bounded sample: median of
+    `runs` timed runs after a warm-up, each about `seconds_target` long, on all threads of the
+    process's share (at most 16), and on one thread. BVH build and upload are excluded."""
     from oracle import refcpu
 
     threads = min(os.cpu_count() or 1, 16)
     o = refcpu.OracleScene(scene)
-    sample = rays[: 1 << 14]
-    t0 = time.perf_counter()
-    o.trace(sample, refcpu.TRACE_REF_MT, threads)
-    dt = time.perf_counter() - t0
-    # grow the sample to roughly seconds_target of CPU work
-    n = int(min(rays.shape[0], max(1 << 14, (1 << 14) * seconds_target / max(dt, 1e-3) / 2)))
-    sample = rays[:n]
-    runs = []
-    for _ in range(2):
+
+    def rate(nthreads, mode, budget):
+        sample = rays[: 1 << 12]
         t0 = time.perf_counter()
-        o.trace(sample, refcpu.TRACE_REF_MT, threads)
-        runs.append(time.perf_counter() - t0)
-    mt_rate = n / min(runs) / 1e6
-    t0 = time.perf_counter()
-    o.trace(sample[: n // 2], refcpu.TRACE_WT, threads)
-    wt_rate = (n // 2) / (time.perf_counter() - t0) / 1e6
+        o.trace(sample, mode, nthreads)  # warm-up and size probe
+        dt = max(time.perf_counter() - t0, 1e-4)
+        n = int(min(rays.shape[0], max(1 << 12, (1 << 12) * budget / dt)))
+        sample = rays[:n]
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            o.trace(sample, mode, nthreads)
+            times.append(time.perf_counter() - t0)
+        return n / float(np.median(times)) / 1e6, n
+
+    mt_rate, n = rate(threads, refcpu.TRACE_REF_MT, seconds_target)
+    mt1_rate, n1 = rate(1, refcpu.TRACE_REF_MT, seconds_target / 2)
+    wt_rate, nw = rate(threads, refcpu.TRACE_WT, seconds_target / 2)
     return {"value": round(mt_rate, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the 8,388,608 incoherent rays, reference Moller-Trumbore two-level BVH "
-                      f"without tmax culling, best of 2; watertight FP32 variant {wt_rate:.3f} Mrays/s",
-            "watertight_value": round(wt_rate, 3)}
+                      f"without tmax culling, median of {runs} runs on {threads} threads ({cpu_model()}); "
+                      f"1 thread: {mt1_rate:.3f} Mrays/s on {n1} rays; watertight FP32 variant "
+                      f"{wt_rate:.3f} Mrays/s on {nw} rays",
+            "one_thread_value": round(mt1_rate, 3),
+            "watertight_value": round(wt_rate, 3),
+            "cpu_model": cpu_model()}
 
 
 def load_traffic(config):
@@ -74,14 +92,20 @@ def relmse(img, ref, eps=1e-2):
 
 
 def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_bounce=8, warm_frac=0.1):
-    """Equal-time comparison of the plain tracer and the learned-cache tracer on the many-instance
-    scene (about 5.5M triangles, 3 area lights) against a ref_spp plain reference.
+    """Equal-time comparison of estimators on the many-instance scene (about 5.5M triangles, 3 area
+    lights) against a ref_spp plain reference.
 
-    plain: spp frames (seeds 0..spp-1), accumulated by spk_accumulate, wall time T.
-    cache: frames with SPK_FRAME_CACHE (online training every frame) until T has passed; the
-           frames of the first warm_frac of T only train and are not accumulated.
-    reference: ref_spp plain frames with seeds from 1,000,000 on (independent of both), summed in
-           float64 on the device.
+    plain        the reference's estimator: spp frames (seeds 0..spp-1) accumulated by spk_accumulate;
+                 its wall time T is the budget of every other estimator
+    plain_nee    plain + next-event estimation
+    cache        learned radiance cache (terminate at bounce 2, online training every frame)
+    cache_nee    cache + next-event estimation
+    cache1_nee   cache from bounce 1 + next-event estimation
+    Each estimator renders frames until T has passed; the cache estimators start untrained (reset
+    after their graph-capture frame) and accumulate only the frames after warm_frac * T. The reference
+    uses seeds from 1,000,000 on (independent of every estimator), summed in float64 on the device.
+    All frames use the 64-bit seed composition: at 1080p the reference's uint32 j*W + i + f*W*H wraps
+    from frame 2072 on.
     """
     import torch
 
@@ -101,12 +125,10 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     def frame(seed, flags=0):
         f = spk.FrameDesc()
         f.width, f.height, f.frame_seed, f.max_bounce, f.debug_mode = width, height, seed, max_bounce, 0
-        # 64-bit seed composition: at 1080p the reference's uint32 j*W + i + f*W*H wraps from frame 2072 on
         f.sky_enabled, f.flags, f.output_limit = 0, flags | spk.FRAME_SEED64, 0.0
         return f
 
-    # warm up (graph capture) outside every timed region
-    ctx.render_frame(frame(999_999), rgba)
+    ctx.render_frame(frame(999_999), rgba)  # graph capture outside every timed region
     torch.cuda.synchronize()
 
     log(f"relMSE experiment: reference at {ref_spp} spp")
@@ -121,8 +143,6 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     torch.cuda.synchronize()
     t_ref = time.perf_counter() - t0
     ref = (ref / ref_spp).view(height, width, 4)[..., :3]
-    samples_per_s = npx * ref_spp / t_ref
-    mrays_path = rays_ref / t_ref / 1e6
 
     log(f"relMSE experiment: plain at {spp} spp")
     torch.cuda.synchronize()
@@ -132,47 +152,55 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
         ctx.accumulate(rgba, acc, npx, k, 1)
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
-    plain = acc.view(height, width, 4)[..., :3].double()
-    e_plain = relmse(plain, ref)
+    results = {"plain": {"frames": spp, "accumulated_spp": spp, "seconds": round(T, 3),
+                         "relmse": relmse(acc.view(height, width, 4)[..., :3].double(), ref)}}
 
-    log(f"relMSE experiment: learned cache for {T:.2f} s")
-    ctx.cache_create(spk.cache_config())
-    ctx.render_frame(frame(999_998, spk.FRAME_CACHE), rgba)  # graph capture; the cache is then reset
-    ctx.cache_destroy()
-    ctx.cache_create(spk.cache_config())
-    torch.cuda.synchronize()
-    acc.zero_()
-    frames = used = 0
-    losses = []
-    t0 = time.perf_counter()
-    while True:
-        el = time.perf_counter() - t0
-        if el >= T:
-            break
-        ctx.render_frame(frame(frames, spk.FRAME_CACHE), rgba)
-        frames += 1
-        if el >= warm_frac * T:
-            ctx.accumulate(rgba, acc, npx, used, 1)
-            used += 1
-        if frames % 8 == 0:
-            losses.append(ctx.cache_info()["last_loss"])
-    torch.cuda.synchronize()
-    T_cache = time.perf_counter() - t0
-    cache_img = acc.view(height, width, 4)[..., :3].double()
-    e_cache = relmse(cache_img, ref) if used else float("inf")
-    mean_ref = float(ref.mean())
+    variants = [("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
+                ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
+    for name, flags, bounce in variants:
+        log(f"relMSE experiment: {name} for {T:.2f} s")
+        if bounce is not None:
+            ctx.cache_create(spk.cache_config(terminate_bounce=bounce))
+        ctx.render_frame(frame(999_998, flags), rgba)  # graph capture
+        if bounce is not None:
+            ctx.cache_reset()
+        torch.cuda.synchronize()
+        acc.zero_()
+        frames = used = 0
+        warm = warm_frac * T if bounce is not None else 0.0
+        t0 = time.perf_counter()
+        while True:
+            el = time.perf_counter() - t0
+            if el >= T:
+                break
+            ctx.render_frame(frame(frames, flags), rgba)
+            frames += 1
+            if el >= warm:
+                ctx.accumulate(rgba, acc, npx, used, 1)
+                used += 1
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        r = {"frames": frames, "accumulated_spp": used, "seconds": round(dt, 3),
+             "relmse": relmse(acc.view(height, width, 4)[..., :3].double(), ref) if used else None}
+        if bounce is not None:
+            r["terminate_bounce"] = bounce
+            r["last_loss"] = ctx.cache_info()["last_loss"]
+            ctx.cache_destroy()
+        results[name] = r
+
+    e_plain = results["plain"]["relmse"]
     out = {
         "scene": f"many-instance generator seed 0: {st.num_triangles} triangles, 3 area lights, {width}x{height}, "
                  f"max {max_bounce} bounces, Russian roulette from bounce 3",
+        "relmse_definition": "mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)",
         "reference_spp": ref_spp,
         "reference_s": round(t_ref, 2),
-        "plain": {"spp": spp, "seconds": round(T, 3), "relmse": e_plain},
-        "cache": {"frames": frames, "accumulated_spp": used, "seconds": round(T_cache, 3), "relmse": e_cache,
-                  "last_loss": losses[-1] if losses else None},
-        "relmse_ratio_plain_over_cache": round(e_plain / e_cache, 3) if e_cache > 0 else None,
-        "reference_mean": mean_ref,
-        "full_path_samples_per_s": round(samples_per_s, 1),
-        "full_path_mrays_per_s": round(mrays_path, 1),
+        "equal_time_s": round(T, 3),
+        "estimators": results,
+        "ratio_plain_over": {k: round(e_plain / v["relmse"], 3) for k, v in results.items()
+                             if k != "plain" and v["relmse"]},
+        "full_path_samples_per_s": round(npx * ref_spp / t_ref, 1),
+        "full_path_mrays_per_s": round(rays_ref / t_ref / 1e6, 1),
         "bvh_build_s": round(t_build, 2),
     }
     ctx.close()
@@ -391,6 +419,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "algorithmic_bytes_per_ray": round(bytes_per_ray, 1),
+            "compulsory_bytes_per_ray": 40,
             "kernel_avg_ms": round(avg_ms, 4),
             "traffic_source": traffic_src,
         },

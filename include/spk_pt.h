@@ -286,6 +286,9 @@ typedef struct spk_cache_config {
 } spk_cache_config;
 
 int spk_cache_create(spk_context *ctx, const spk_cache_config *config);
+/* Back to the initial parameters of the configured seed (Adam state and step count cleared),
+   keeping every buffer, so a captured frame graph stays valid. */
+int spk_cache_reset(spk_context *ctx);
 int spk_cache_destroy(spk_context *ctx);
 /* n points (device, 3 floats each; world space when world != 0, else in [0,1]^3) -> n x 3 radiance. */
 int spk_cache_infer(spk_context *ctx, const float *points_dev, uint32_t n, int world, float *out_dev);

@@ -174,6 +174,7 @@ def load_library():
                                    ctypes.POINTER(ctypes.c_double)]),
         "spk_cache_create": (I, [P, ctypes.POINTER(CacheConfig)]),
         "spk_cache_destroy": (I, [P]),
+        "spk_cache_reset": (I, [P]),
         "spk_cache_infer": (I, [P, P, U32, I, P]),
         "spk_cache_train": (I, [P, P, P, U32, I, ctypes.POINTER(F)]),
         "spk_cache_gradients": (I, [P, P, P, U32, I, ctypes.POINTER(F), P, P]),
@@ -419,6 +420,9 @@ class Context:
     def cache_create(self, cfg=None):
         cfg = cfg or cache_config()
         return self._check(self.lib.spk_cache_create(self.h, ctypes.byref(cfg)))
+
+    def cache_reset(self):
+        return self._check(self.lib.spk_cache_reset(self.h))
 
     def cache_destroy(self):
         return self._check(self.lib.spk_cache_destroy(self.h))

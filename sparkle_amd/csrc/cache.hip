@@ -494,6 +494,34 @@ int cache_check(spk_context *ctx) {
 
 extern "C" {
 
+// Initial parameters from the configured seed: table U(-1e-4, 1e-4) (Instant NGP), weights
+// Xavier-uniform, Adam moments and step count zero.
+static int cache_init_params(spk_context *ctx, CacheState &cs) {
+    const uint64_t ntab = cs.table_params();
+    const size_t nw = cache_num_weights();
+    hipError_t e;
+    HostRng rng(cs.cfg.seed);
+    std::vector<float> tab(ntab), w(nw);
+    for (auto &x : tab) x = rng.uniform(-1e-4f, 1e-4f);
+    size_t off = 0;
+    for (int l = 0; l < 5; ++l) {
+        const int in = l == 0 ? 32 : 64, out = l == 4 ? 3 : 64;
+        const float lim = std::sqrt(6.f / (float)(in + out));
+        for (int k = 0; k < in * out; ++k) w[off + k] = rng.uniform(-lim, lim);
+        off += (size_t)in * out;
+    }
+    if ((e = hipMemcpy(cs.table_f.ptr, tab.data(), ntab * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(cs.w.ptr, w.data(), nw * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(cs.mt.ptr, 0, ntab * 4)) != hipSuccess || (e = hipMemset(cs.vt.ptr, 0, ntab * 4)) != hipSuccess ||
+        (e = hipMemset(cs.mw.ptr, 0, nw * 4)) != hipSuccess || (e = hipMemset(cs.vw.ptr, 0, nw * 4)) != hipSuccess ||
+        (e = hipMemset(cs.wf.ptr, 0, 16384 * 2)) != hipSuccess || (e = hipMemset(cs.wb.ptr, 0, 16384 * 2)) != hipSuccess)
+        return hip_fail(ctx, e, "cache initialisation");
+    if ((e = cache_pack(cs, ctx->stream)) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return hip_fail(ctx, e, "cache pack");
+    cs.step = 0;
+    return SPK_OK;
+}
+
 int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
     if (!ctx || !config) return SPK_ERR_INVALID_ARGUMENT;
     if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "the cache is sized from the uploaded scene box");
@@ -541,27 +569,19 @@ int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
         (e = cs.gw.ensure(nw * 4)) != hipSuccess || (e = cs.wf.ensure(16384 * 2)) != hipSuccess ||
         (e = cs.wb.ensure(16384 * 2)) != hipSuccess || (e = cs.loss.ensure(16)) != hipSuccess)
         return hip_fail(ctx, e, "cache allocation");
-    // initialisation: table U(-1e-4, 1e-4) (Instant NGP), weights Xavier-uniform
-    HostRng rng(config->seed);
-    std::vector<float> tab(ntab), w(nw);
-    for (auto &x : tab) x = rng.uniform(-1e-4f, 1e-4f);
-    size_t off = 0;
-    for (int l = 0; l < 5; ++l) {
-        const int in = l == 0 ? 32 : 64, out = l == 4 ? 3 : 64;
-        const float lim = std::sqrt(6.f / (float)(in + out));
-        for (int k = 0; k < in * out; ++k) w[off + k] = rng.uniform(-lim, lim);
-        off += (size_t)in * out;
-    }
-    if ((e = hipMemcpy(cs.table_f.ptr, tab.data(), ntab * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(cs.w.ptr, w.data(), nw * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemset(cs.mt.ptr, 0, ntab * 4)) != hipSuccess || (e = hipMemset(cs.vt.ptr, 0, ntab * 4)) != hipSuccess ||
-        (e = hipMemset(cs.mw.ptr, 0, nw * 4)) != hipSuccess || (e = hipMemset(cs.vw.ptr, 0, nw * 4)) != hipSuccess ||
-        (e = hipMemset(cs.wf.ptr, 0, 16384 * 2)) != hipSuccess || (e = hipMemset(cs.wb.ptr, 0, 16384 * 2)) != hipSuccess)
-        return hip_fail(ctx, e, "cache initialisation");
-    if ((e = cache_pack(cs, ctx->stream)) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-        return hip_fail(ctx, e, "cache pack");
+    int rc = cache_init_params(ctx, cs);
+    if (rc != SPK_OK) return rc;
     ctx->on_scene_changed();
     return SPK_OK;
+}
+
+int spk_cache_reset(spk_context *ctx) {
+    int rc = cache_check(ctx);
+    if (rc != SPK_OK) return rc;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "cache reset");
+    ctx->cache_last_loss = 0.f;
+    return cache_init_params(ctx, *ctx->cache);
 }
 
 int spk_cache_destroy(spk_context *ctx) {

@@ -6,10 +6,10 @@
 //   shade   - SamplePixel's loop body (CPURenderer.cpp:259-368) for each hit, in material order;
 //             survivors are appended to the next bounce's queue
 //   shadow  - occlusion rays of next-event estimation (opt-in, not in the reference)
-// With the learned radiance cache enabled, non-training paths stop at the cache bounce and take
-// the cache's prediction of the reflected radiance; a fraction of paths is traced in full and
-// records, at each vertex from the cache bounce on, the radiance gathered after that vertex as a
-// training target.
+// With the learned radiance cache enabled, non-training paths stop at the first non-metal vertex
+// from the cache bounce on and take the cache's prediction of the reflected radiance; a fraction
+// of paths is traced in full and records, at each such vertex, the radiance gathered after that
+// vertex as a training target.
 // All kernels are persistent: the grid is sized to the device, waves fetch work from counters in
 // device memory, so a frame is captured once as a hipGraph and replayed; per-frame values (the
 // frame seed) travel through a small device buffer written before each replay.
@@ -310,23 +310,40 @@ __global__ __launch_bounds__(1024) void k_sort_scan(PtArgs a, uint32_t bounce) {
     if (k < a.nkeys) gh[a.nkeys + k] = part[k] - v;  // exclusive offsets
 }
 
+// One atomic per distinct key of a wave (not per slot): most slots of a wave share a few materials,
+// and per-slot atomics on one hot key serialise in L2.
 __global__ __launch_bounds__(kPtBlock) void k_sort_scatter(PtArgs a, uint32_t bounce) {
     uint32_t *ctr = a.counters + bounce * kCtrStride;
     const uint32_t count = ctr[CTR_QUEUE];
     uint32_t *offs = a.hist + (size_t)bounce * 2u * a.nkeys + a.nkeys;
+    const uint32_t lane = lane_id();
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
     for (;;) {
         const uint32_t base = wave_fetch(ctr + CTR_SCATTER);
         if (base >= count) break;
-        const uint32_t slot = base + lane_id();
-        if (slot >= count) continue;
-        const uint4 h = a.hits[slot];
+        const uint32_t slot = base + lane;
+        const bool valid = slot < count;
         uint32_t key = a.nkeys - 1u;
-        if (h.x != 0xFFFFFFFFu) {
-            key = a.shade[h.x].material;
-            if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
+        if (valid) {
+            const uint4 h = a.hits[slot];
+            if (h.x != 0xFFFFFFFFu) {
+                key = a.shade[h.x].material;
+                if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
+            }
         }
-        const uint32_t pos = atomicAdd(&offs[key], 1u);
-        a.sorted[pos] = slot;
+        uint64_t pending = __ballot(valid);
+        uint32_t pos = 0u;
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t k = __shfl(key, leader);
+            const uint64_t peers = __ballot(valid && key == k);
+            uint32_t first = 0u;
+            if ((int)lane == leader) first = atomicAdd(&offs[k], (uint32_t)__popcll(peers));
+            first = __shfl(first, leader);
+            if (valid && key == k) pos = first + (uint32_t)__popcll(peers & below);
+            pending &= ~peers;
+        }
+        if (valid) a.sorted[pos] = slot;
     }
 }
 
@@ -384,17 +401,20 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                     a.normal_out[3 * (size_t)path + 1] = normal.y;
                     a.normal_out[3 * (size_t)path + 2] = normal.z;
                 }
+                // the cache is a function of position only, so it stands in for the path tail only at
+                // vertices whose reflected radiance barely depends on direction (non-metal PBR); paths
+                // continue through metal and glass vertices to the next such vertex
+                const bool cache_vertex = a.cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
+                                          mat.metallic < 0.5f;
                 if (mat.emissive_flag) {
                     if (!(pf & PF_SKIP_EMISSION))
                         C = C + cmul(f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]), T);
-                } else if (a.cache_on && bounce >= a.cache_bounce && !(pf & PF_TRAIN)) {
+                } else if (cache_vertex && !(pf & PF_TRAIN)) {
                     // terminate into the cache: radiance leaving this vertex comes from the network
-                    if (bounce == a.cache_bounce) {
-                        cache_q = true;
-                        Tv = T;
-                    }
+                    cache_q = true;
+                    Tv = T;
                 } else {
-                    if (a.cache_on && bounce >= a.cache_bounce && (pf & PF_TRAIN)) {
+                    if (cache_vertex) {  // a training path: record the vertex, its target follows
                         train = true;
                         Tv = T;
                         Cv = C;

@@ -161,23 +161,33 @@ def test_accumulate_and_tonemap(gpu_ctx):
     np.testing.assert_allclose(o[::-1, :, :3], aces, rtol=1e-5, atol=1e-6)
 
 
+def tile_means(frames, t=4):
+    """Per-frame means over t x t pixel tiles: (frames, H/t, W/t, 3)."""
+    f, h, w, c = frames.shape
+    return frames.reshape(f, h // t, t, w // t, t, c).mean((2, 4))
+
+
 def test_nee_is_unbiased_against_bsdf_sampling(gpu_ctx):
     """Next-event estimation (not in the reference) must converge to the same image as the
-    reference's BSDF-only estimator: per-pixel means within 3 sigma, total flux within 2%. The
-    per-sample clamp is lifted here: clamping at 6 cuts the two estimators' samples differently."""
+    reference's BSDF-only estimator. Compared on 4x4-pixel tile means over 512 frames: single pixels
+    of the BSDF-only estimator are too heavy-tailed (rare light hits) for a 3-sigma test at this
+    sample count, tile means are not. The per-sample clamp is lifted: clamping at 6 cuts the two
+    estimators' samples differently."""
     w = h = 32
-    frames = 256
+    frames = 512
     sc = scenes.cornell_scene()
     r = Renderer(gpu_ctx, sc, scenes.cornell_camera(w, h), w, h)
-    plain = np.stack([r.render(k, bounce=6, limit=1e30)[0][..., :3] for k in range(frames)])
-    nee = np.stack([r.render(k, bounce=6, flags=spk.FRAME_NEE, limit=1e30)[0][..., :3] for k in range(frames)])
+    plain = tile_means(np.stack([r.render(k, bounce=6, limit=1e30)[0][..., :3] for k in range(frames)]))
+    nee = tile_means(np.stack([r.render(k, bounce=6, flags=spk.FRAME_NEE, limit=1e30)[0][..., :3]
+                               for k in range(frames)]))
     mp, mn = plain.mean(0), nee.mean(0)
     se = np.sqrt(plain.var(0, ddof=1) / frames + nee.var(0, ddof=1) / frames)
-    ok = np.abs(mp - mn) <= 3 * se + 1e-3
-    assert ok.mean() > 0.98, ok.mean()
-    assert abs(mp.mean() - mn.mean()) / mp.mean() < 0.02
+    z = np.abs(mp - mn) / (se + 1e-9)
+    print("tiles within 3 sigma", (z <= 3).mean(), "max z", z.max(), "flux", mp.mean(), mn.mean())
+    assert (z <= 3).mean() >= 0.98 and z.max() < 5, (z <= 3).mean()
+    assert abs(mp.mean() - mn.mean()) / mp.mean() < 0.015
     # and it has lower variance
-    assert nee.var(0).mean() < plain.var(0).mean()
+    assert nee.var(0).mean() < 0.5 * plain.var(0).mean()
 
 
 def test_invalid_frame_arguments(gpu_ctx):
