@@ -234,6 +234,13 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
 /* accum = rgba + (accum - rgba) * cumulated / (cumulated + spp), on the device. */
 int spk_accumulate(spk_context *ctx, const float *rgba_dev, float *accum_dev, uint64_t num_pixels,
                    uint32_t cumulated, uint32_t spp);
+/* CPURenderer::DenoisePass spatial part (CPURenderer.cpp:452-525): every invalid pixel (w <= 0) of
+   rgba_dev takes the colour of a valid, non-sky neighbour with a normal within dot >= 0.99 (the
+   reference's direction order; where several neighbours qualify, the last in row-major order, as
+   the reference's loop on one thread). rgba_dev and normal_dev (3 floats per pixel) are
+   spk_render_frame's outputs; out_dev receives the filled frame (not in place). */
+int spk_spatial_fill(spk_context *ctx, const float *rgba_dev, const float *normal_dev, float *out_dev, uint32_t width,
+                     uint32_t height);
 /* ACES film curve with exposure, vertically flipped, to RGBA float (CPURenderer.cpp:554-576). */
 int spk_tonemap(spk_context *ctx, const float *accum_dev, float *out_dev, uint32_t width, uint32_t height,
                 float exposure);

@@ -1336,6 +1336,40 @@ void orcl_accumulate(const float *rgba, float *accum, uint64_t num_pixels, uint3
     for (uint64_t i = 0; i < 4 * num_pixels; ++i) accum[i] = rgba[i] + (accum[i] - rgba[i]) * ma;
 }
 
+// SpatialDenoisePixel over the whole frame (CPURenderer.cpp:452-504, 511-525), in the order a single
+// thread runs it: rows j = 0..H-1, columns i = 0..W-1. Every valid, non-sky pixel copies its colour
+// into the first of its 8 neighbours (fixed direction order) that is in the image, invalid
+// (w <= 0) and has a normal within dot >= 0.99 of its own; later writers overwrite earlier ones.
+// (The reference runs rows in parallel, so where two rows target the same pixel its winner is not
+// fixed; this order is the single-threaded one.)
+void orcl_spatial_fill(const float *rgba, const float *normal, float *out, uint32_t w, uint32_t h) {
+    static const int dirs[8][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}, {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+    std::memcpy(out, rgba, sizeof(float) * 4 * (size_t)w * h);
+    for (uint32_t j = 0; j < h; ++j) {
+        for (uint32_t i = 0; i < w; ++i) {
+            const size_t p = (size_t)j * w + i;
+            const float *n = normal + 3 * p;
+            const bool valid = rgba[4 * p + 3] > 0.f;
+            const bool sky = valid && n[0] == 0.f && n[1] == 0.f && n[2] == 0.f;
+            if (!valid || sky) continue;
+            for (int k = 0; k < 8; ++k) {
+                const int si = (int)i + dirs[k][0], sj = (int)j + dirs[k][1];
+                if (si < 0 || si >= (int)w || sj < 0 || sj >= (int)h) continue;
+                const size_t q = (size_t)sj * w + si;
+                if (rgba[4 * q + 3] > 0.f) continue;
+                const float *m = normal + 3 * q;
+                const float sim = (m[0] * n[0] + m[1] * n[1]) + m[2] * n[2];
+                if (sim < 0.99f) continue;
+                out[4 * q] = rgba[4 * p];
+                out[4 * q + 1] = rgba[4 * p + 1];
+                out[4 * q + 2] = rgba[4 * p + 2];
+                out[4 * q + 3] = 1.f;
+                break;
+            }
+        }
+    }
+}
+
 void orcl_camera_setup(const float position[3], const float right[3], const float front[3], const float up[3],
                        float vertical_fov, float focus_distance, float aperture_radius, float aspect,
                        orcl_camera *c) {

@@ -149,6 +149,7 @@ int orcl_render_frame(const orcl_scene *scene, const orcl_camera *camera, const 
                       float *rgba, float *normal);
 /* Temporal accumulation as in CPURenderer::DenoisePass (spatial fill off). */
 void orcl_accumulate(const float *rgba, float *accum, uint64_t num_pixels, uint32_t cumulated, uint32_t spp);
+void orcl_spatial_fill(const float *rgba, const float *normal, float *out, uint32_t w, uint32_t h);
 /* Ray-segment statistics of the last orcl_render_frame call (extend rays traced). */
 uint64_t orcl_last_render_rays(void);
 

@@ -83,6 +83,7 @@ def lib():
             "orcl_random_unit_stream": (None, [U64, P, U32]),
             "orcl_render_frame": (I, [P, P, P, I, P, P]),
             "orcl_accumulate": (None, [P, P, U64, U32, U32]),
+            "orcl_spatial_fill": (None, [P, P, P, U32, U32]),
             "orcl_last_render_rays": (U64, []),
             "orcl_camera_setup": (None, [P, P, P, P, F, F, F, F, P]),
             "orcl_hashgrid_encode": (None, [P, P, P, U32, P]),
@@ -192,6 +193,17 @@ def accumulate(rgba, accum, cumulated, spp=1):
     rgba = np.ascontiguousarray(rgba, dtype=np.float32)
     assert accum.flags["C_CONTIGUOUS"] and accum.dtype == np.float32
     lib().orcl_accumulate(rgba.ctypes.data, accum.ctypes.data, rgba.size // 4, cumulated, spp)
+
+
+def spatial_fill(rgba, normal):
+    """CPURenderer's 8-neighbour fill of invalid pixels (single-threaded order). rgba (H, W, 4),
+    normal (H, W, 3), float32; returns the filled copy."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    normal = np.ascontiguousarray(normal, dtype=np.float32)
+    h, w = rgba.shape[:2]
+    out = np.empty_like(rgba)
+    lib().orcl_spatial_fill(rgba.ctypes.data, normal.ctypes.data, out.ctypes.data, w, h)
+    return out
 
 
 def tri_mt(o, d, v0, v1, v2):

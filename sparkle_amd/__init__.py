@@ -167,6 +167,7 @@ def load_library():
         "spk_render_frame": (I, [P, ctypes.POINTER(FrameDesc), P, P, ctypes.POINTER(FrameStats)]),
         "spk_accumulate": (I, [P, P, P, U64, U32, U32]),
         "spk_tonemap": (I, [P, P, P, U32, U32, F]),
+        "spk_spatial_fill": (I, [P, P, P, P, U32, U32]),
         "spk_batch_upload": (I, [P, P, U32, ctypes.POINTER(BuildOptions)]),
         "spk_batch_render": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(FrameStats)]),
         "spk_batch_offsets": (I, [P, ctypes.POINTER(U64), U32]),
@@ -412,6 +413,10 @@ class Context:
     def accumulate(self, rgba_dev, accum_dev, num_pixels, cumulated, spp=1):
         return self._check(self.lib.spk_accumulate(self.h, _ptr(rgba_dev), _ptr(accum_dev), num_pixels,
                                                    cumulated, spp))
+
+    def spatial_fill(self, rgba_dev, normal_dev, out_dev, width, height):
+        return self._check(self.lib.spk_spatial_fill(self.h, _ptr(rgba_dev), _ptr(normal_dev), _ptr(out_dev),
+                                                     width, height))
 
     def tonemap(self, accum_dev, out_dev, width, height, exposure=1.0):
         return self._check(self.lib.spk_tonemap(self.h, _ptr(accum_dev), _ptr(out_dev), width, height, exposure))

@@ -654,6 +654,53 @@ __global__ void k_tonemap(const float4 *__restrict__ acc, float4 *__restrict__ o
                                                     1.f);
 }
 
+// ---- spatial fill of invalid pixels (CPURenderer.cpp:452-504) as a gather ----
+// The reference scatters: each valid, non-sky source copies its colour into the first of its 8
+// neighbours (direction order below) that is invalid with a normal within dot >= 0.99. Here each
+// invalid pixel looks at its 8 possible sources and takes the last one in row-major order whose
+// first such neighbour it is, which is the single-threaded order of the reference's loop.
+__constant__ int kFillDir[8][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}, {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+// source offsets (source = pixel - direction) in decreasing row-major order of the source
+__constant__ int kFillSrc[8][2] = {{1, 1}, {0, 1}, {-1, 1}, {1, 0}, {-1, 0}, {1, -1}, {0, -1}, {-1, -1}};
+
+__device__ __forceinline__ float fill_dot(const float *a, const float *b) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+__global__ void k_spatial_fill(const float4 *__restrict__ rgba, const float *__restrict__ normal,
+                               float4 *__restrict__ out, uint32_t w, uint32_t h) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint64_t)w * h) return;
+    const float4 c = rgba[p];
+    if (c.w > 0.f) {
+        out[p] = c;
+        return;
+    }
+    const int j = (int)(p / w), i = (int)(p - (uint64_t)j * w);
+    for (int k = 0; k < 8; ++k) {
+        const int si = i + kFillSrc[k][0], sj = j + kFillSrc[k][1];
+        if (si < 0 || si >= (int)w || sj < 0 || sj >= (int)h) continue;
+        const size_t s = (size_t)sj * w + si;
+        const float4 sc = rgba[s];
+        const float *n = normal + 3 * s;
+        if (!(sc.w > 0.f) || (n[0] == 0.f && n[1] == 0.f && n[2] == 0.f)) continue;
+        // the first qualifying neighbour of this source
+        for (int d = 0; d < 8; ++d) {
+            const int ti = si + kFillDir[d][0], tj = sj + kFillDir[d][1];
+            if (ti < 0 || ti >= (int)w || tj < 0 || tj >= (int)h) continue;
+            const size_t t = (size_t)tj * w + ti;
+            if (rgba[t].w > 0.f) continue;
+            if (fill_dot(normal + 3 * t, n) < 0.99f) continue;
+            if (t == p) {
+                out[p] = make_float4(sc.x, sc.y, sc.z, 1.f);
+                return;
+            }
+            break;
+        }
+    }
+    out[p] = c;
+}
+
 // ---------------------------------------------------------------------------------------------
 using ExtendFn = void (*)(PtArgs, uint32_t);
 struct PtEntry {
@@ -1241,6 +1288,22 @@ int spk_accumulate(spk_context *ctx, const float *rgba_dev, float *accum_dev, ui
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "accumulate");
+    return SPK_OK;
+}
+
+int spk_spatial_fill(spk_context *ctx, const float *rgba_dev, const float *normal_dev, float *out_dev, uint32_t width,
+                     uint32_t height) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (!rgba_dev || !normal_dev || !out_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null buffer");
+    if (out_dev == rgba_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "the fill cannot run in place");
+    const uint64_t n = (uint64_t)width * height;
+    if (n == 0) return SPK_OK;
+    if (n >= (1ull << 40)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "image too large");
+    hipLaunchKernelGGL(k_spatial_fill, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float4 *)rgba_dev, normal_dev, (float4 *)out_dev, width, height);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "spatial fill");
     return SPK_OK;
 }
 

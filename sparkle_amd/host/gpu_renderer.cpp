@@ -18,7 +18,9 @@ void GpuPathRenderer::Release() {
     spk_device_free(ctx_, gbuffer_);
     spk_device_free(ctx_, frame_buffer_);
     spk_device_free(ctx_, image_);
-    gbuffer_ = frame_buffer_ = image_ = nullptr;
+    spk_device_free(ctx_, normals_);
+    spk_device_free(ctx_, ping_pong_);
+    gbuffer_ = frame_buffer_ = image_ = normals_ = ping_pong_ = nullptr;
 }
 
 std::string GpuPathRenderer::LastError() const { return ctx_ ? spk_last_error(ctx_) : "no context"; }
@@ -39,7 +41,9 @@ int GpuPathRenderer::InitRenderResources(const SceneDesc &scene, const RenderCon
     const size_t px = (size_t)config.width * config.height;
     if ((rc = spk_device_alloc(ctx_, px * 16, (void **)&gbuffer_)) != SPK_OK ||
         (rc = spk_device_alloc(ctx_, px * 16, (void **)&frame_buffer_)) != SPK_OK ||
-        (rc = spk_device_alloc(ctx_, px * 16, (void **)&image_)) != SPK_OK)
+        (rc = spk_device_alloc(ctx_, px * 16, (void **)&image_)) != SPK_OK ||
+        (config.spatial_denoise && ((rc = spk_device_alloc(ctx_, px * 12, (void **)&normals_)) != SPK_OK ||
+                                    (rc = spk_device_alloc(ctx_, px * 16, (void **)&ping_pong_)) != SPK_OK)))
         return rc;
     dispatched_sample_count_ = 0;
     cumulated_sample_count_ = 0;
@@ -90,7 +94,7 @@ int GpuPathRenderer::BasePass() {
     for (int c = 0; c < 3; ++c) f.sky_color[c] = config_.sky_color[c];
     f.flags = config_.frame_flags;
     spk_frame_stats st;
-    int rc = spk_render_frame(ctx_, &f, gbuffer_, nullptr, &st);
+    int rc = spk_render_frame(ctx_, &f, gbuffer_, normals_, &st);
     if (rc == SPK_OK) {
         last_frame_ms_ = st.milliseconds;
         last_frame_rays_ = st.rays_traced + st.shadow_rays;
@@ -99,7 +103,13 @@ int GpuPathRenderer::BasePass() {
 }
 
 int GpuPathRenderer::DenoisePass() {
-    return spk_accumulate(ctx_, gbuffer_, frame_buffer_, (uint64_t)config_.width * config_.height,
+    const float *pass_input = gbuffer_;
+    if (config_.spatial_denoise) {
+        int rc = spk_spatial_fill(ctx_, gbuffer_, normals_, ping_pong_, config_.width, config_.height);
+        if (rc != SPK_OK) return rc;
+        pass_input = ping_pong_;
+    }
+    return spk_accumulate(ctx_, pass_input, frame_buffer_, (uint64_t)config_.width * config_.height,
                           cumulated_sample_count_, actual_sample_per_pixel_);
 }
 

@@ -7,7 +7,8 @@
 //   Render               CPURenderer::Render (CPURenderer.cpp:119-208): clear on dirty, BasePass,
 //                        DenoisePass (temporal), ToneMappingPass, sample bookkeeping
 //   BasePass             CPURenderer::BasePass (CPURenderer.cpp:419-450) -> spk_render_frame
-//   DenoisePass          CPURenderer::DenoisePass, temporal part (CPURenderer.cpp:506-552)
+//   DenoisePass          CPURenderer::DenoisePass (CPURenderer.cpp:506-552): optional spatial fill of
+//                        invalid pixels, then the temporal running mean
 //   ToneMappingPass      CPURenderer::ToneMappingPass (CPURenderer.cpp:554-576)
 #pragma once
 
@@ -30,6 +31,7 @@ struct RenderConfig {
     float sky_color[3] = {0.5f, 0.7f, 1.0f};
     float exposure = 1.f;
     uint32_t frame_flags = 0;  // SPK_FRAME_* (NEE, cache) on top of the reference estimator
+    bool spatial_denoise = false;  // RenderConfig::spatial_denoise (packaged config: off)
 };
 
 struct SceneDesc {
@@ -67,6 +69,8 @@ private:
     int device_ = 0;
     RenderConfig config_;
     float *gbuffer_ = nullptr;       // one frame's samples, RGBA (valid flag in alpha)
+    float *normals_ = nullptr;       // first-hit world normals (spatial fill only)
+    float *ping_pong_ = nullptr;     // spatially filled frame (spatial fill only)
     float *frame_buffer_ = nullptr;  // accumulation across frames
     float *image_ = nullptr;         // tone-mapped output
     uint32_t actual_sample_per_pixel_ = 1;

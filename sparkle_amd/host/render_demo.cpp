@@ -1,6 +1,6 @@
 // Minimal C++ host of the drop-in path: builds a small Cornell-style room, renders it through
 // GpuPathRenderer (CPURenderer's role over the C ABI) and writes the tone-mapped image as PFM.
-//   render_demo <out.pfm> [width height frames]
+//   render_demo <out.pfm> [width height frames [spatial_denoise 0/1]]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -96,6 +96,7 @@ int main(int argc, char **argv) {
     cfg.width = W;
     cfg.height = H;
     cfg.max_sample_per_pixel = (uint32_t)frames;
+    cfg.spatial_denoise = argc > 5 && atoi(argv[5]) != 0;
     GpuPathRenderer r(0);
     int rc = r.InitRenderResources(scene, cfg, cam);
     if (rc != SPK_OK) {

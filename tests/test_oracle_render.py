@@ -108,3 +108,78 @@ def test_samples_are_clamped_and_valid_flags_are_signed():
     assert set(np.unique(rgba[..., 3])).issubset({-1.0, 1.0})
     # the light is visible: some samples reach the clamp
     assert (rgba[..., :3] == 6.0).any()
+
+
+def _fill_case(valid, normals, colors):
+    h, w = valid.shape
+    rgba = np.zeros((h, w, 4), np.float32)
+    rgba[..., :3] = colors
+    rgba[..., 3] = np.where(valid, 1.0, -1.0)
+    return rgba, np.asarray(normals, np.float32)
+
+
+def test_spatial_fill_known_answers():
+    from oracle import refcpu
+
+    up = np.zeros((3, 3, 3), np.float32)
+    up[..., 2] = 1.0
+    colors = np.arange(27, dtype=np.float32).reshape(3, 3, 3)
+    # (a) only the centre is invalid: every valid pixel's first qualifying direction is checked in
+    # the order (1,0),(0,1),(-1,0),(0,-1),(-1,-1),(1,-1),(-1,1),(1,1); sources whose first hit is the
+    # centre: (0,1)->(1,0) dir, (1,0)->(0,1) dir, (2,1)->(-1,0), (1,2)->(0,-1), (2,2)->(-1,-1),
+    # (0,2)->(1,-1), (2,0)->(-1,1), (0,0)->(1,1). The last in row-major order is (row 2, col 2).
+    valid = np.ones((3, 3), bool)
+    valid[1, 1] = False
+    rgba, nrm = _fill_case(valid, up, colors)
+    out = refcpu.spatial_fill(rgba, nrm)
+    np.testing.assert_array_equal(out[1, 1], np.r_[colors[2, 2], 1.0])
+    np.testing.assert_array_equal(np.delete(out.reshape(9, 4), 4, 0), np.delete(rgba.reshape(9, 4), 4, 0))
+    # (b) the bottom-right source is sky (zero normal): the next source in reverse row-major order
+    # whose first qualifying neighbour is the centre is (row 2, col 1)
+    nrm_b = up.copy()
+    nrm_b[2, 2] = 0.0
+    out = refcpu.spatial_fill(rgba, nrm_b)
+    np.testing.assert_array_equal(out[1, 1], np.r_[colors[2, 1], 1.0])
+    # (c) the centre's normal is turned away (dot < 0.99): nothing is filled
+    nrm_c = up.copy()
+    nrm_c[1, 1] = [0.0, 0.2, 0.98]
+    out = refcpu.spatial_fill(rgba, nrm_c)
+    np.testing.assert_array_equal(out, rgba)
+    # (d) two invalid pixels in a row: (0,0) fills its first neighbour (1,0) only, once
+    valid = np.zeros((1, 3), bool)
+    valid[0, 0] = True
+    rgba, nrm = _fill_case(valid, up[:1], colors[:1])
+    out = refcpu.spatial_fill(rgba, nrm)
+    np.testing.assert_array_equal(out[0, 1], np.r_[colors[0, 0], 1.0])
+    np.testing.assert_array_equal(out[0, 2], rgba[0, 2])
+
+
+def test_spatial_fill_properties_on_random_gbuffers():
+    """Valid pixels never change; a filled pixel holds a qualifying neighbour's colour with w = 1;
+    an invalid pixel with no qualifying source neighbour is untouched."""
+    from oracle import refcpu
+
+    rng = np.random.Generator(np.random.PCG64(5))
+    dirs = [(1, 0), (0, 1), (-1, 0), (0, -1), (-1, -1), (1, -1), (-1, 1), (1, 1)]
+    for h, w in ((17, 23), (1, 40), (40, 1)):
+        valid = rng.random((h, w)) > 0.35
+        pal = np.array([[0, 0, 1], [0, 0.1, 0.995], [1, 0, 0], [0, 0, 0]], np.float32)
+        nrm = pal[rng.integers(0, 4, (h, w))]
+        rgba, nrm = _fill_case(valid, nrm, rng.random((h, w, 3)).astype(np.float32))
+        out = refcpu.spatial_fill(rgba, nrm)
+        np.testing.assert_array_equal(out[valid], rgba[valid])
+        for j in range(h):
+            for i in range(w):
+                if valid[j, i]:
+                    continue
+                srcs = []
+                for dx, dy in dirs:
+                    sj, si = j - dy, i - dx
+                    if 0 <= sj < h and 0 <= si < w and valid[sj, si] and nrm[sj, si].any():
+                        d = np.float32(nrm[j, i] @ nrm[sj, si])
+                        if d >= np.float32(0.99):
+                            srcs.append(rgba[sj, si, :3])
+                if not srcs:
+                    np.testing.assert_array_equal(out[j, i], rgba[j, i])
+                elif out[j, i, 3] == 1.0:
+                    assert any(np.array_equal(out[j, i, :3], c) for c in srcs)

@@ -204,3 +204,33 @@ def test_invalid_frame_arguments(gpu_ctx):
     st = gpu_ctx.render_frame(frame(8, 8, 0, bounce=0), buf)
     out = buf.cpu().numpy().reshape(8, 8, 4)
     assert (out[..., :3] == 0).all() and (out[..., 3] == 1).all()
+
+
+def test_spatial_fill_matches_oracle(gpu_ctx):
+    """The gather kernel reproduces the oracle's single-threaded scatter bit for bit, on random
+    G-buffers (invalid pixels, sky pixels, normals on both sides of the 0.99 threshold) and on a
+    rendered frame."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(8))
+    pal = np.array([[0, 0, 1], [0, 0.1, 0.995], [1, 0, 0], [0, 0, 0]], np.float32)
+    cases = []
+    for h, w in ((64, 48), (1, 300), (300, 1), (2, 2)):
+        rgba = np.zeros((h, w, 4), np.float32)
+        rgba[..., :3] = rng.random((h, w, 3))
+        rgba[..., 3] = np.where(rng.random((h, w)) > 0.4, 1.0, -1.0)
+        cases.append((rgba, pal[rng.integers(0, 4, (h, w))]))
+    w = h = 48
+    r = Renderer(gpu_ctx, scenes.cornell_scene(), scenes.cornell_camera(w, h), w, h)
+    g, gn, _ = r.render(1)
+    cases.append((g, gn))
+    for rgba, nrm in cases:
+        h, w = rgba.shape[:2]
+        print_case("spatial fill", h, w)
+        d_in = torch.from_numpy(np.ascontiguousarray(rgba)).cuda()
+        d_n = torch.from_numpy(np.ascontiguousarray(nrm, dtype=np.float32)).cuda()
+        d_out = torch.empty_like(d_in)
+        gpu_ctx.spatial_fill(d_in, d_n, d_out, w, h)
+        np.testing.assert_array_equal(d_out.cpu().numpy(), refcpu.spatial_fill(rgba, nrm))
+    with pytest.raises(spk.SpkError):
+        gpu_ctx.spatial_fill(d_in, d_n, d_in, w, h)
