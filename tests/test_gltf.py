@@ -1,0 +1,81 @@
+"""glTF writer/reader with the reference loader's mapping rules (no GPU)."""
+import json
+import math
+
+import numpy as np
+
+import sparkle_amd as spk
+from sparkle_amd import gltf, scenes
+
+
+def world_triangles(sc):
+    out = []
+    for inst in sc.instances:
+        m = sc.meshes[inst["mesh"]]
+        M = inst["M"].astype(np.float64)
+        p = m["positions"].astype(np.float64) @ M[:, :3].T + M[:, 3]
+        out.append(p[m["indices"].reshape(-1)])
+    return np.concatenate(out)
+
+
+def test_cornell_round_trip(tmp_path):
+    sc = scenes.cornell_scene(0)
+    path = str(tmp_path / "cornell.gltf")
+    gltf.write_gltf(sc, path)
+    doc = json.load(open(path))
+    assert doc["asset"]["version"] == "2.0" and len(doc["nodes"]) == len(sc.instances)
+    back = gltf.read_gltf(path)
+    assert back.num_triangles == sc.num_triangles
+    # the reader's +90 degrees about X uses cos(float(pi/2)) = -4.4e-8, as the reference does
+    np.testing.assert_allclose(world_triangles(back), world_triangles(sc), rtol=0, atol=1e-6)
+    for a, b in zip(back.materials, sc.materials):
+        np.testing.assert_array_equal(np.float32(a["base_color"]), np.float32(b["base_color"]))
+        np.testing.assert_array_equal(np.float32(a["emissive"]), np.float32(b["emissive"]))
+        assert np.float32(a["metallic"]) == np.float32(b["metallic"])
+        assert np.float32(a["roughness"]) == np.float32(b["roughness"])
+        assert a["type"] == spk.MATERIAL_PBR
+    rc, st, msg = spk.build_check(back)
+    assert rc == spk.SPK_OK, msg
+
+
+def _doc_with(nodes, material):
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    idx = np.array([0, 1, 2], np.uint32)
+    blob = pos.tobytes() + idx.tobytes()
+    import base64
+
+    return {
+        "asset": {"version": "2.0"},
+        "scenes": [{"nodes": [0]}],
+        "nodes": nodes,
+        "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "indices": 1, "material": 0}]}],
+        "materials": [material],
+        "buffers": [{"byteLength": len(blob),
+                     "uri": "data:application/octet-stream;base64," + base64.b64encode(blob).decode()}],
+        "bufferViews": [{"buffer": 0, "byteOffset": 0, "byteLength": 36},
+                        {"buffer": 0, "byteOffset": 36, "byteLength": 12}],
+        "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3",
+                       "min": [0, 0, 0], "max": [1, 1, 0]},
+                      {"bufferView": 1, "componentType": 5125, "count": 3, "type": "SCALAR"}],
+    }
+
+
+def test_untextured_factors_are_ignored_and_nodes_compose(tmp_path):
+    # factors without textures are dropped (white, metallic 1, roughness 1, no emission)
+    mat = {"pbrMetallicRoughness": {"baseColorFactor": [0.2, 0.3, 0.4, 1], "metallicFactor": 0.0,
+                                    "roughnessFactor": 0.5}, "emissiveFactor": [5, 5, 5]}
+    # parent translated by (0, 0, 1); child scaled by 2 and translated by (1, 0, 0): every node also
+    # gets +90 degrees about X (y-up to z-up), so the child's frame is rotated twice
+    nodes = [{"translation": [0, 0, 1], "children": [1]}, {"mesh": 0, "scale": [2, 2, 2], "translation": [1, 0, 0]}]
+    path = str(tmp_path / "t.gltf")
+    json.dump(_doc_with(nodes, mat), open(path, "w"))
+    sc = gltf.read_gltf(path)
+    m = sc.materials[0]
+    assert tuple(m["base_color"]) == (1.0, 1.0, 1.0) and m["metallic"] == 1.0 and m["roughness"] == 1.0
+    assert tuple(m["emissive"]) == (0.0, 0.0, 0.0)
+    p = world_triangles(sc)
+    # closed form with exact 90-degree rotations R(v) = (x, -z, y); a node maps v to R(v) + T
+    R = lambda v: np.array([v[0], -v[2], v[1]])
+    want = [R(R(2 * np.array(v, float)) + np.array([1.0, 0, 0])) + np.array([0, 0, 1.0])
+            for v in ([0, 0, 0], [1, 0, 0], [0, 1, 0])]
+    np.testing.assert_allclose(p, np.array(want), atol=1e-6)
