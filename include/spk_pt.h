@@ -90,7 +90,12 @@ typedef struct spk_instance_desc {
 #define SPK_MATERIAL_PBR 0u
 #define SPK_MATERIAL_DIELECTRIC 1u
 
-/* Field set of MaterialRenderData; textures are not part of the hot path. */
+/* Field set of MaterialRenderData (MaterialRenderProxy.h:39-58) and MaterialResource's textures
+   (io/Material.h:9-23). A texture field holds 0 for none (so a zeroed struct has no textures) or
+   k + 1 for entry k of the context's texture table (spk_textures_upload); then the getters of
+   MaterialRenderProxy.h:117-162 apply: base = tex(uv).rgb * base_color, metallic = tex(uv).b *
+   metallic, roughness = tex(uv).g * roughness, emissive = tex(uv).rgb * emissive, and the normal
+   map perturbs the shading normal in the tangent frame (MeshRenderProxy.cpp:350-356). */
 typedef struct spk_material_desc {
     uint32_t type; /* SPK_MATERIAL_* */
     float base_color[3];
@@ -98,7 +103,24 @@ typedef struct spk_material_desc {
     float metallic;
     float roughness;
     float eta;
+    uint32_t base_color_texture;
+    uint32_t metallic_roughness_texture;
+    uint32_t emissive_texture;
+    uint32_t normal_texture;
 } spk_material_desc;
+
+#define SPK_TEXTURE_RGBA8_UNORM 0u /* 4 bytes per texel, value / 255 */
+#define SPK_TEXTURE_RGBA8_SRGB 1u  /* as UNORM, then the sRGB decode of Image::Sample */
+#define SPK_TEXTURE_RGBA32F 2u     /* 16 bytes per texel */
+
+/* One 2D image (io/Image.h): row-major texels, sampled bilinearly with wrapping (Image.h:112-152). */
+typedef struct spk_texture_desc {
+    uint32_t width;
+    uint32_t height;
+    uint32_t format; /* SPK_TEXTURE_* */
+    uint32_t reserved;
+    const void *pixels;
+} spk_texture_desc;
 
 typedef struct spk_build_options {
     uint32_t bvh_width;     /* 8 (default) or 4 */
@@ -181,6 +203,10 @@ int spk_scene_upload(spk_context *ctx, const spk_mesh_desc *meshes, uint32_t num
                      const spk_material_desc *materials, uint32_t num_materials,
                      const spk_build_options *options);
 int spk_scene_stats_get(spk_context *ctx, spk_scene_stats *out);
+/* Replaces the context's texture table (shared by the scene and batch uploads that follow; a
+   material naming a texture index beyond the table is rejected at upload). Call before
+   spk_scene_upload / spk_batch_upload; changing it invalidates an uploaded scene. */
+int spk_textures_upload(spk_context *ctx, const spk_texture_desc *textures, uint32_t num_textures);
 
 /* Host-only check of the BVH builder (no device needed): flattens and builds exactly as
    spk_scene_upload does, verifies the traversal invariants (every triangle referenced once and

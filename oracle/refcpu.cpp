@@ -524,10 +524,19 @@ struct Instance {
 
 }  // namespace orcl
 
+namespace orcl {
+struct Texture {
+    uint32_t w = 0, h = 0;
+    bool srgb = false;
+    std::vector<float> texels;  // 4 per texel
+};
+}  // namespace orcl
+
 struct orcl_scene {
     std::vector<orcl::Mesh> meshes;
     std::vector<orcl::Instance> instances;  // in TLAS leaf order
     std::vector<orcl_material> materials;
+    std::vector<orcl::Texture> textures;
     orcl::Bvh tlas;
     uint64_t total_tris = 0;
 };
@@ -884,6 +893,26 @@ orcl_scene *orcl_scene_create(const orcl_mesh *meshes, uint32_t num_meshes, cons
 }
 
 void orcl_scene_destroy(orcl_scene *scene) { delete scene; }
+
+void orcl_scene_set_textures(orcl_scene *scene, const orcl_texture *textures, uint32_t num_textures) {
+    scene->textures.assign(num_textures, orcl::Texture());
+    for (uint32_t k = 0; k < num_textures; ++k) {
+        const orcl_texture &t = textures[k];
+        orcl::Texture &d = scene->textures[k];
+        d.w = t.width;
+        d.h = t.height;
+        d.srgb = t.format == 1u;
+        const size_t n = 4 * (size_t)t.width * t.height;
+        d.texels.resize(n);
+        if (t.format == 2u) {
+            std::memcpy(d.texels.data(), t.pixels, n * sizeof(float));
+        } else {
+            // ColorToVec (Utilities.h:377-380)
+            const uint8_t *src = static_cast<const uint8_t *>(t.pixels);
+            for (size_t i = 0; i < n; ++i) d.texels[i] = static_cast<float>(src[i]) / 255.f;
+        }
+    }
+}
 uint64_t orcl_scene_num_triangles(const orcl_scene *scene) { return scene->total_tris; }
 
 void orcl_set_fp64_eps(double eps_edge, double eps_tie, double eps_graze) {
@@ -1022,6 +1051,60 @@ static inline T lerp3(const T &v0, const T &v1, const T &v2, float u, float v) {
     return v0 + (v1 - v0) * u + (v2 - v0) * v;  // Utilities.h:114-117
 }
 
+// WrapMod (Utilities.h:526-529)
+static inline unsigned wrap_mod(int a, unsigned b) { return (static_cast<unsigned>(a % static_cast<int>(b)) + b) % b; }
+
+// SRGBtoLinear (Utilities.h:416-425), one channel
+static inline float srgb_to_linear(float c) {
+    float cutoff = c < 0.04045f ? 1.f : 0.f;
+    float higher = std::pow((c + 0.055f) / 1.055f, 2.4f);
+    float lower = c / 12.92f;
+    return higher * (1.f - cutoff) + lower * cutoff;
+}
+
+// Image::Sample (io/Image.h:112-152): uv scaled by (width - 1, height - 1), Decompose truncates
+// (Utilities.h:124-128), bilinear with wrapping, pseudo-sRGB decode after filtering
+static V3 texture_sample(const Texture &t, float u, float v) {
+    const float px = u * static_cast<float>(t.w - 1), py = v * static_cast<float>(t.h - 1);
+    const int iu = static_cast<int>(px), iv = static_cast<int>(py);
+    const float fu = px - static_cast<float>(iu), fv = py - static_cast<float>(iv);
+    auto at = [&](unsigned x, unsigned y) { return &t.texels[4 * ((size_t)y * t.w + x)]; };
+    const float *s00 = at(wrap_mod(iu, t.w), wrap_mod(iv, t.h));
+    const float *s10 = at(wrap_mod(iu + 1, t.w), wrap_mod(iv, t.h));
+    const float *s01 = at(wrap_mod(iu, t.w), wrap_mod(iv + 1, t.h));
+    const float *s11 = at(wrap_mod(iu + 1, t.w), wrap_mod(iv + 1, t.h));
+    float r[3];
+    for (int c = 0; c < 3; ++c) {
+        const float l0 = s00[c] + (s10[c] - s00[c]) * fu;
+        const float l1 = s01[c] + (s11[c] - s01[c]) * fu;
+        r[c] = l0 + (l1 - l0) * fv;
+        if (t.srgb) r[c] = srgb_to_linear(r[c]);
+    }
+    return V3(r[0], r[1], r[2]);
+}
+
+// MaterialRenderProxy getters at a texture coordinate (MaterialRenderProxy.h:117-162)
+struct MatEval {
+    V3 base, emissive;
+    float metallic, roughness;
+};
+static MatEval eval_material(const orcl_scene &sc, const orcl_material &m, float tu, float tv) {
+    MatEval e;
+    e.base = V3(m.base_color);
+    e.emissive = V3(m.emissive);
+    e.metallic = m.metallic;
+    e.roughness = m.roughness;
+    if (m.base_color_texture >= 0) e.base = cmul(texture_sample(sc.textures[m.base_color_texture], tu, tv), e.base);
+    if (m.metallic_roughness_texture >= 0) {
+        const V3 t = texture_sample(sc.textures[m.metallic_roughness_texture], tu, tv);
+        e.metallic = t.z * m.metallic;
+        e.roughness = t.y * m.roughness;
+    }
+    if (m.emissive_texture >= 0)
+        e.emissive = cmul(texture_sample(sc.textures[m.emissive_texture], tu, tv), e.emissive);
+    return e;
+}
+
 // SceneRenderProxy::Intersect<false> + MeshRenderProxy::GetIntersection (MeshRenderProxy.cpp:340-361)
 static Hit scene_hit(const orcl_scene &sc, V3 o, V3 d) {
     Candidate c;
@@ -1040,8 +1123,19 @@ static Hit scene_hit(const orcl_scene &sc, V3 o, V3 d) {
     V3 n(0, 0, 0);
     if (!m.nrm.empty()) n = lerp3(V3(&m.nrm[3 * i0]), V3(&m.nrm[3 * i1]), V3(&m.nrm[3 * i2]), c.u, c.v);
     V3 tg = kRight;
-    if (!m.tan.empty())
+    float handedness = 1.f;
+    if (!m.tan.empty()) {
         tg = lerp3(V3(&m.tan[4 * i0]), V3(&m.tan[4 * i1]), V3(&m.tan[4 * i2]), c.u, c.v);
+        handedness = m.tan[4 * i0 + 3] + (m.tan[4 * i1 + 3] - m.tan[4 * i0 + 3]) * c.u +
+                     (m.tan[4 * i2 + 3] - m.tan[4 * i0 + 3]) * c.v;
+    }
+    const orcl_material &mat = sc.materials[ins.material];
+    if (mat.normal_texture >= 0) {
+        // TangentSpaceToWorldSpace (Utilities.h:398-404) with GetNormal = sample * 2 - 1
+        const V3 tn = texture_sample(sc.textures[mat.normal_texture], h.uvx, h.uvy) * 2.f - kOnes;
+        const V3 bit = cross(n, tg) * handedness;
+        n = normalized(tg * tn.x + bit * tn.y + n * tn.z);
+    }
     // TransformDirectionTangentSpace on the inverse = the forward 3x3 (Transform.h:105-108)
     h.normal = normalized(ins.to_world.dir(n));
     h.tangent = ins.to_world.dir(tg);
@@ -1181,8 +1275,9 @@ static Sample dielectric_sample(Rng &rng, V3 wo, V3 base, float eta) {
 }
 
 // MaterialRenderProxy::SampleSurface for Pbr / Dielectric proxies
-static V3 sample_surface(Rng &rng, const orcl_material &mat, V3 ray_dir, V3 normal, V3 tangent, V3 &w_i) {
-    V3 base(mat.base_color);
+static V3 sample_surface(Rng &rng, const orcl_material &mat, const MatEval &me, V3 ray_dir, V3 normal, V3 tangent,
+                         V3 &w_i) {
+    V3 base = me.base;
     V3 wo = -ray_dir;
     V3 u, v, w;
     local_axes(normal, tangent, u, v, w);
@@ -1191,7 +1286,7 @@ static V3 sample_surface(Rng &rng, const orcl_material &mat, V3 ray_dir, V3 norm
     if (mat.type == ORCL_MAT_DIELECTRIC) {
         s = dielectric_sample(rng, lwo, base, mat.eta);
     } else {
-        s = specular_sample(rng, lwo, base, mat.roughness, mat.metallic);
+        s = specular_sample(rng, lwo, base, me.roughness, me.metallic);
         if (!s.valid) s = lambert_sample(rng, base);
     }
     V3 u2, v2, w2;
@@ -1235,14 +1330,14 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
             break;
         }
         const orcl_material &mat = sc.materials[h.inst->material];
+        const MatEval me = eval_material(sc, mat, h.uvx, h.uvy);
         if (bounce == 0) res.normal = h.normal;
-        V3 E(mat.emissive);
-        if (norm(E) > kEps) {
-            res.color = res.color + cmul(E, T);
+        if (norm(me.emissive) > kEps) {
+            res.color = res.color + cmul(me.emissive, T);
             break;
         }
         V3 wi;
-        V3 f = sample_surface(rng, mat, d, h.normal, h.tangent, wi);
+        V3 f = sample_surface(rng, mat, me, d, h.normal, h.tangent, wi);
         switch (fr.debug_mode) {
             case 3:  // Normal
                 res.color = (h.normal + kOnes) * 0.5f;
@@ -1251,16 +1346,16 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
                 res.color = (wi + kOnes) * 0.5f;
                 return res;
             case 7:
-                res.color = kOnes * mat.metallic;
+                res.color = kOnes * me.metallic;
                 return res;
             case 8:
-                res.color = kOnes * mat.roughness;
+                res.color = kOnes * me.roughness;
                 return res;
             case 9:
-                res.color = V3(mat.base_color);
+                res.color = me.base;
                 return res;
             case 10:
-                res.color = V3(mat.emissive);
+                res.color = me.emissive;
                 return res;
             case 11:
                 res.color = kOnes * (dot(h.location - V3(cam.position), V3(cam.front)) / cam.far_plane);
@@ -1515,3 +1610,10 @@ void orcl_adam(double *param, const double *grad, double *m, double *v, uint64_t
 }
 
 }  // extern "C"
+
+extern "C" void orcl_texture_sample(const orcl_scene *scene, int index, float u, float v, float *out) {
+    const orcl::V3 r = orcl::texture_sample(scene->textures[index], u, v);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}

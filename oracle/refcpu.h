@@ -82,7 +82,20 @@ typedef struct orcl_material {
     float metallic;
     float roughness;
     float eta;
+    int32_t base_color_texture; /* texture index, -1 = none */
+    int32_t metallic_roughness_texture;
+    int32_t emissive_texture;
+    int32_t normal_texture;
 } orcl_material;
+
+/* RGBA texels, row-major: 8-bit channels (format 0 UNORM, 1 sRGB) or float (format 2). */
+typedef struct orcl_texture {
+    uint32_t width;
+    uint32_t height;
+    uint32_t format;
+    uint32_t reserved;
+    const void *pixels;
+} orcl_texture;
 
 typedef struct orcl_camera {
     float position[3];
@@ -126,6 +139,10 @@ typedef struct orcl_scene orcl_scene;
 orcl_scene *orcl_scene_create(const orcl_mesh *meshes, uint32_t num_meshes, const orcl_instance *instances,
                               uint32_t num_instances, const orcl_material *materials, uint32_t num_materials);
 void orcl_scene_destroy(orcl_scene *scene);
+/* Texture table the materials' texture indices refer to (copied). */
+void orcl_scene_set_textures(orcl_scene *scene, const orcl_texture *textures, uint32_t num_textures);
+/* Image::Sample of texture `index` at (u, v): rgb into out[3]. */
+void orcl_texture_sample(const orcl_scene *scene, int index, float u, float v, float *out);
 uint64_t orcl_scene_num_triangles(const orcl_scene *scene);
 
 /* Closest hit for each ray. flags may be NULL (only written in FP64 mode). */

@@ -34,7 +34,14 @@ class OInstance(ctypes.Structure):
 
 class OMaterial(ctypes.Structure):
     _fields_ = [("type", ctypes.c_uint32), ("base_color", ctypes.c_float * 3), ("emissive", ctypes.c_float * 3),
-                ("metallic", ctypes.c_float), ("roughness", ctypes.c_float), ("eta", ctypes.c_float)]
+                ("metallic", ctypes.c_float), ("roughness", ctypes.c_float), ("eta", ctypes.c_float),
+                ("base_color_texture", ctypes.c_int32), ("metallic_roughness_texture", ctypes.c_int32),
+                ("emissive_texture", ctypes.c_int32), ("normal_texture", ctypes.c_int32)]
+
+
+class OTexture(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("pixels", ctypes.c_void_p)]
 
 
 class OCamera(ctypes.Structure):
@@ -72,6 +79,8 @@ def lib():
                                 ctypes.c_double)
         sig = {
             "orcl_scene_create": (P, [P, U32, P, U32, P, U32]),
+            "orcl_scene_set_textures": (None, [P, P, U32]),
+            "orcl_texture_sample": (None, [P, ctypes.c_int, ctypes.c_float, ctypes.c_float, P]),
             "orcl_scene_destroy": (None, [P]),
             "orcl_scene_num_triangles": (U64, [P]),
             "orcl_trace": (I, [P, P, U64, I, I, P, P, P]),
@@ -140,9 +149,28 @@ class OracleScene:
             mats[k].metallic = m["metallic"]
             mats[k].roughness = m["roughness"]
             mats[k].eta = m["eta"]
+            mats[k].base_color_texture = m.get("base_color_texture", -1)
+            mats[k].metallic_roughness_texture = m.get("metallic_roughness_texture", -1)
+            mats[k].emissive_texture = m.get("emissive_texture", -1)
+            mats[k].normal_texture = m.get("normal_texture", -1)
         self.h = L.orcl_scene_create(ms, len(scene.meshes), ins, len(scene.instances), mats, len(scene.materials))
         if not self.h:
             raise RuntimeError("orcl_scene_create failed")
+        texs = getattr(scene, "textures", [])
+        if texs:
+            arr = (OTexture * len(texs))()
+            keep = []
+            for k, (px, fmt) in enumerate(texs):
+                px = np.ascontiguousarray(px, dtype=np.float32 if fmt == 2 else np.uint8)
+                keep.append(px)
+                arr[k].width, arr[k].height, arr[k].format = px.shape[1], px.shape[0], fmt
+                arr[k].pixels = px.ctypes.data
+            L.orcl_scene_set_textures(self.h, arr, len(texs))
+
+    def texture_sample(self, index, u, v):
+        out = np.zeros(3, np.float32)
+        lib().orcl_texture_sample(self.h, index, float(u), float(v), out.ctypes.data)
+        return out
 
     def __del__(self):
         try:

@@ -64,7 +64,19 @@ class InstanceDesc(ctypes.Structure):
 
 class MaterialDesc(ctypes.Structure):
     _fields_ = [("type", ctypes.c_uint32), ("base_color", ctypes.c_float * 3), ("emissive", ctypes.c_float * 3),
-                ("metallic", ctypes.c_float), ("roughness", ctypes.c_float), ("eta", ctypes.c_float)]
+                ("metallic", ctypes.c_float), ("roughness", ctypes.c_float), ("eta", ctypes.c_float),
+                ("base_color_texture", ctypes.c_uint32), ("metallic_roughness_texture", ctypes.c_uint32),
+                ("emissive_texture", ctypes.c_uint32), ("normal_texture", ctypes.c_uint32)]  # 0 none, k+1
+
+
+TEXTURE_RGBA8_UNORM = 0
+TEXTURE_RGBA8_SRGB = 1
+TEXTURE_RGBA32F = 2
+
+
+class TextureDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("pixels", ctypes.c_void_p)]
 
 
 class BuildOptions(ctypes.Structure):
@@ -154,6 +166,7 @@ def load_library():
         "spk_copy_to_host": (I, [P, P, P, ctypes.c_size_t]),
         "spk_scene_upload": (I, [P, P, U32, P, U32, P, U32, P]),
         "spk_scene_stats_get": (I, [P, ctypes.POINTER(SceneStats)]),
+        "spk_textures_upload": (I, [P, P, U32]),
         "spk_build_check": (I, [P, U32, P, U32, P, ctypes.POINTER(SceneStats), ctypes.c_char_p, ctypes.c_size_t]),
         "spk_trace_closest": (I, [P, P, U64, P]),
         "spk_trace_closest_async": (I, [P, P, U64, P]),
@@ -207,6 +220,7 @@ class Scene:
         self.meshes = []
         self.instances = []
         self.materials = []
+        self.textures = []  # (pixels (H, W, 4), TEXTURE_* format)
 
     def add_mesh(self, positions, indices, normals=None, tangents=None, uvs=None):
         m = dict(positions=np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3),
@@ -218,10 +232,24 @@ class Scene:
         return len(self.meshes) - 1
 
     def add_material(self, base_color=(1, 1, 1), emissive=(0, 0, 0), metallic=1.0, roughness=1.0, eta=0.0,
-                     type=MATERIAL_PBR):
-        self.materials.append(dict(type=type, base_color=tuple(base_color), emissive=tuple(emissive),
-                                   metallic=float(metallic), roughness=float(roughness), eta=float(eta)))
+                     type=MATERIAL_PBR, base_color_texture=-1, metallic_roughness_texture=-1, emissive_texture=-1,
+                     normal_texture=-1):
+        m = dict(type=type, base_color=tuple(base_color), emissive=tuple(emissive), metallic=float(metallic),
+                 roughness=float(roughness), eta=float(eta))
+        tex = dict(base_color_texture=int(base_color_texture),
+                   metallic_roughness_texture=int(metallic_roughness_texture),
+                   emissive_texture=int(emissive_texture), normal_texture=int(normal_texture))
+        if any(v >= 0 for v in tex.values()):
+            m.update(tex)
+        self.materials.append(m)
         return len(self.materials) - 1
+
+    def add_texture(self, pixels, fmt=TEXTURE_RGBA8_UNORM):
+        """pixels (H, W, 4): uint8 for the RGBA8 formats, float32 for TEXTURE_RGBA32F."""
+        px = np.ascontiguousarray(pixels, dtype=np.float32 if fmt == TEXTURE_RGBA32F else np.uint8)
+        assert px.ndim == 3 and px.shape[2] == 4
+        self.textures.append((px, fmt))
+        return len(self.textures) - 1
 
     def add_instance(self, mesh, material, world_from_local=None):
         M = np.eye(3, 4, dtype=np.float32) if world_from_local is None else np.asarray(world_from_local,
@@ -256,6 +284,10 @@ class Scene:
             mats[k].metallic = m["metallic"]
             mats[k].roughness = m["roughness"]
             mats[k].eta = m["eta"]
+            mats[k].base_color_texture = m.get("base_color_texture", -1) + 1
+            mats[k].metallic_roughness_texture = m.get("metallic_roughness_texture", -1) + 1
+            mats[k].emissive_texture = m.get("emissive_texture", -1) + 1
+            mats[k].normal_texture = m.get("normal_texture", -1) + 1
         return ms, ins, mats
 
 
@@ -309,7 +341,25 @@ class Context:
     def synchronize(self):
         return self._check(self.lib.spk_synchronize(self.h))
 
+    def upload_textures(self, textures):
+        """spk_textures_upload: textures is a list of (pixels, format) with pixels (H, W, 4) uint8 or
+        float32."""
+        arr = (TextureDesc * max(1, len(textures)))()
+        keep = []
+        for k, (px, fmt) in enumerate(textures):
+            px = np.ascontiguousarray(px, dtype=np.float32 if fmt == TEXTURE_RGBA32F else np.uint8)
+            assert px.ndim == 3 and px.shape[2] == 4
+            keep.append(px)
+            arr[k].width, arr[k].height, arr[k].format = px.shape[1], px.shape[0], fmt
+            arr[k].pixels = px.ctypes.data
+        return self._check(self.lib.spk_textures_upload(self.h, arr, len(textures)))
+
     def upload_scene(self, scene, bvh_width=8, max_leaf_tris=3, num_threads=0):
+        """spk_textures_upload (the scene's textures, when it has any or the table is not empty)
+        followed by spk_scene_upload."""
+        if scene.textures or getattr(self, "_has_textures", False):
+            self.upload_textures(scene.textures)
+            self._has_textures = bool(scene.textures)
         ms, ins, mats = scene.c_arrays()
         opts = BuildOptions(bvh_width, max_leaf_tris, num_threads)
         self._check(self.lib.spk_scene_upload(self.h, ms, len(scene.meshes), ins, len(scene.instances), mats,
@@ -381,8 +431,15 @@ class Context:
         """spk_batch_upload: items are (Scene, CameraDesc, width, height). Returns the pixel offsets."""
         arr = (BatchScene * len(items))()
         keep = []
+        textures = []
         for k, (scene, cam, w, h) in enumerate(items):
             ms, ins, mats = scene.c_arrays()
+            # one texture table for the batch: this scene's indices shift by the textures before it
+            for j in range(len(scene.materials)):
+                for f in ("base_color_texture", "metallic_roughness_texture", "emissive_texture", "normal_texture"):
+                    if getattr(mats[j], f):
+                        setattr(mats[j], f, getattr(mats[j], f) + len(textures))
+            textures.extend(scene.textures)
             keep.append((ms, ins, mats))
             arr[k].meshes = ctypes.cast(ms, ctypes.c_void_p)
             arr[k].num_meshes = len(scene.meshes)
@@ -393,6 +450,9 @@ class Context:
             arr[k].camera = cam
             arr[k].width = w
             arr[k].height = h
+        if textures or getattr(self, "_has_textures", False):
+            self.upload_textures(textures)
+            self._has_textures = bool(textures)
         opts = BuildOptions(bvh_width, max_leaf_tris, num_threads)
         self._check(self.lib.spk_batch_upload(self.h, arr, len(items), ctypes.byref(opts)))
         offs = (ctypes.c_uint64 * (len(items) + 1))()

@@ -195,7 +195,8 @@ static int upload_built(spk_context *ctx, int stack) {
 // materials (everything spk_scene_upload and spk_batch_upload do on the host).
 int prepare_scene(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_instance_desc *instances,
                   uint32_t num_instances, const spk_material_desc *materials, uint32_t num_materials,
-                  const spk_build_options *options, HostScene &hs, std::vector<MaterialGpu> &mats, std::string &err) {
+                  uint32_t num_textures, const spk_build_options *options, HostScene &hs,
+                  std::vector<MaterialGpu> &mats, std::string &err) {
     if (!materials || num_materials == 0) {
         err = "scene needs at least one material";
         return SPK_ERR_INVALID_ARGUMENT;
@@ -219,6 +220,22 @@ int prepare_scene(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_in
         const float en = std::sqrt(d.emissive[0] * d.emissive[0] + d.emissive[1] * d.emissive[1] +
                                    d.emissive[2] * d.emissive[2]);
         g.emissive_flag = en > 1e-6f ? 1u : 0u;
+        // texture fields: 0 = none, k + 1 = table entry k
+        const uint32_t tex[4] = {d.base_color_texture, d.metallic_roughness_texture, d.emissive_texture,
+                                 d.normal_texture};
+        int32_t idx[4];
+        for (int k = 0; k < 4; ++k) {
+            if (tex[k] > num_textures) {
+                err = "material " + std::to_string(m) + " names texture " + std::to_string(tex[k] - 1) + " of " +
+                      std::to_string(num_textures);
+                return SPK_ERR_INVALID_SCENE;
+            }
+            idx[k] = (int32_t)tex[k] - 1;
+        }
+        g.tex_base = idx[0];
+        g.tex_mr = idx[1];
+        g.tex_emissive = idx[2];
+        g.tex_normal = idx[3];
         if (d.type != SPK_MATERIAL_PBR && d.type != SPK_MATERIAL_DIELECTRIC)
             {
             err = "material " + std::to_string(m) + " has unknown type " + std::to_string(d.type);
@@ -400,8 +417,8 @@ int spk_scene_upload(spk_context *ctx, const spk_mesh_desc *meshes, uint32_t num
     HostScene hs;
     std::vector<MaterialGpu> mats;
     std::string err;
-    int rc = prepare_scene(meshes, num_meshes, instances, num_instances, materials, num_materials, options, hs, mats,
-                           err);
+    int rc = prepare_scene(meshes, num_meshes, instances, num_instances, materials, num_materials,
+                           ctx->num_textures, options, hs, mats, err);
     if (rc != SPK_OK) return set_error(ctx, rc, "%s", err.c_str());
     const int stack = pick_trace_stack((int)hs.bvh.depth);
     ctx->host_mats = mats;
@@ -409,6 +426,54 @@ int spk_scene_upload(spk_context *ctx, const spk_mesh_desc *meshes, uint32_t num
     int rc2 = upload_built(ctx, stack);
     if (rc2 != SPK_OK) return rc2;
     ctx->on_scene_changed();
+    return SPK_OK;
+}
+
+int spk_textures_upload(spk_context *ctx, const spk_texture_desc *textures, uint32_t num_textures) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (num_textures && !textures) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null texture array");
+    (void)hipSetDevice(ctx->device);
+    std::vector<TexRecord> table(num_textures);
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < num_textures; ++k) {
+        const spk_texture_desc &t = textures[k];
+        if (t.width == 0 || t.height == 0 || t.width > 65536 || t.height > 65536 || !t.pixels)
+            return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "texture %u: empty or larger than 65536^2", k);
+        if (t.format > SPK_TEXTURE_RGBA32F)
+            return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "texture %u: unknown format %u", k, t.format);
+        std::memset(&table[k], 0, sizeof(TexRecord));
+        table[k].offset = total;
+        table[k].width = t.width;
+        table[k].height = t.height;
+        table[k].srgb = t.format == SPK_TEXTURE_RGBA8_SRGB ? 1u : 0u;
+        total += (uint64_t)t.width * t.height;
+    }
+    // texels as float4: 8-bit channels become value / 255 (ColorToVec, Utilities.h:377-380)
+    std::vector<float> texels(4 * std::max<uint64_t>(total, 1));
+    for (uint32_t k = 0; k < num_textures; ++k) {
+        const spk_texture_desc &t = textures[k];
+        const uint64_t n = (uint64_t)t.width * t.height;
+        float *dst = texels.data() + 4 * table[k].offset;
+        if (t.format == SPK_TEXTURE_RGBA32F) {
+            std::memcpy(dst, t.pixels, n * 16);
+        } else {
+            const uint8_t *src = static_cast<const uint8_t *>(t.pixels);
+            for (uint64_t i = 0; i < 4 * n; ++i) dst[i] = static_cast<float>(src[i]) / 255.f;
+        }
+    }
+    hipError_t e;
+    if ((e = ctx->texels.ensure(texels.size() * 4)) != hipSuccess ||
+        (e = ctx->tex_table.ensure(std::max<size_t>(1, table.size()) * sizeof(TexRecord))) != hipSuccess)
+        return hip_fail(ctx, e, "texture allocation");
+    if ((e = hipMemcpy(ctx->texels.ptr, texels.data(), texels.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (num_textures && (e = hipMemcpy(ctx->tex_table.ptr, table.data(), table.size() * sizeof(TexRecord),
+                                        hipMemcpyHostToDevice)) != hipSuccess))
+        return hip_fail(ctx, e, "texture upload");
+    ctx->num_textures = num_textures;
+    // materials of an uploaded scene or batch may name textures that no longer exist
+    ctx->scene.valid = false;
+    ctx->on_scene_changed();
+    ctx->drop_batch();
     return SPK_OK;
 }
 

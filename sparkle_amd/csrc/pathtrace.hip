@@ -78,6 +78,8 @@ struct PtArgs {
     const MaterialGpu *mats;
     const uint32_t *emitters;
     const float *emitter_cdf;
+    const TexRecord *tex_table;  // texture table and float4 texels
+    const float4 *texels;
     uint32_t num_emitters;
     float emitter_total;
     float4 *rays;      // 2 per path: (org, tmin), (dir, tmax)
@@ -351,7 +353,35 @@ __device__ __forceinline__ float4 lerp_attr(const float *p0, const float *p1, co
     // v0 + (v1 - v0) * u + (v2 - v0) * v (Utilities.h:114-117)
     return make_float4(p0[0] + (p1[0] - p0[0]) * u + (p2[0] - p0[0]) * v,
                        p0[1] + (p1[1] - p0[1]) * u + (p2[1] - p0[1]) * v,
-                       p0[2] + (p1[2] - p0[2]) * u + (p2[2] - p0[2]) * v, 0.f);
+                       p0[2] + (p1[2] - p0[2]) * u + (p2[2] - p0[2]) * v,
+                       p0[3] + (p1[3] - p0[3]) * u + (p2[3] - p0[3]) * v);
+}
+
+// Material getters at a uv (MaterialRenderProxy.h:117-162)
+struct MatEval {
+    F3 base, emissive;
+    float metallic, roughness;
+    bool emits;
+};
+
+__device__ __forceinline__ MatEval eval_material(const PtArgs &a, const MaterialGpu &m, float tu, float tv) {
+    MatEval e;
+    e.base = f3(m.base[0], m.base[1], m.base[2]);
+    e.emissive = f3(m.emissive[0], m.emissive[1], m.emissive[2]);
+    e.metallic = m.metallic;
+    e.roughness = m.roughness;
+    e.emits = m.emissive_flag != 0u;
+    if (m.tex_base >= 0) e.base = cmul(tex_sample(a.tex_table, a.texels, m.tex_base, tu, tv), e.base);
+    if (m.tex_mr >= 0) {
+        const F3 t = tex_sample(a.tex_table, a.texels, m.tex_mr, tu, tv);
+        e.metallic = t.z * m.metallic;
+        e.roughness = t.y * m.roughness;
+    }
+    if (m.tex_emissive >= 0) {
+        e.emissive = cmul(tex_sample(a.tex_table, a.texels, m.tex_emissive, tu, tv), e.emissive);
+        e.emits = sqrtf(sqnorm(e.emissive)) > SPK_EPS;
+    }
+    return e;
 }
 
 // ---- shade: SamplePixel's loop body (CPURenderer.cpp:261-368) ----
@@ -392,10 +422,21 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                 const float u = __uint_as_float(h.z), v = __uint_as_float(h.w), t = __uint_as_float(h.y);
                 const float4 n4 = lerp_attr(s.n0, s.n1, s.n2, u, v);
                 const float4 t4 = lerp_attr(s.t0, s.t1, s.t2, u, v);
-                const F3 normal = normalized(f3(n4.x, n4.y, n4.z));
-                const F3 tangent = f3(t4.x, t4.y, t4.z);
-                loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
+                // texture coordinate (Mesh.h:119-132)
+                const float tu = s.uv0[0] + (s.uv1[0] - s.uv0[0]) * u + (s.uv2[0] - s.uv0[0]) * v;
+                const float tv = s.uv0[1] + (s.uv1[1] - s.uv0[1]) * u + (s.uv2[1] - s.uv0[1]) * v;
                 const MaterialGpu mat = a.mats[s.material];
+                F3 nrm = f3(n4.x, n4.y, n4.z);
+                const F3 tangent = f3(t4.x, t4.y, t4.z);
+                if (mat.tex_normal >= 0) {
+                    // normal map in the tangent frame (MeshRenderProxy.cpp:350-356, Utilities.h:398-404)
+                    const F3 tn = tex_sample(a.tex_table, a.texels, mat.tex_normal, tu, tv) * 2.f - f3(1.f, 1.f, 1.f);
+                    const F3 bit = cross(nrm, tangent) * t4.w;
+                    nrm = normalized(tangent * tn.x + bit * tn.y + nrm * tn.z);
+                }
+                const F3 normal = normalized(nrm);
+                loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
+                const MatEval me = eval_material(a, mat, tu, tv);
                 if (bounce == 0u && a.normal_out) {
                     a.normal_out[3 * (size_t)path] = normal.x;
                     a.normal_out[3 * (size_t)path + 1] = normal.y;
@@ -405,10 +446,9 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                 // vertices whose reflected radiance barely depends on direction (non-metal PBR); paths
                 // continue through metal and glass vertices to the next such vertex
                 const bool cache_vertex = a.cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
-                                          mat.metallic < 0.5f;
-                if (mat.emissive_flag) {
-                    if (!(pf & PF_SKIP_EMISSION))
-                        C = C + cmul(f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]), T);
+                                          me.metallic < 0.5f;
+                if (me.emits) {
+                    if (!(pf & PF_SKIP_EMISSION)) C = C + cmul(me.emissive, T);
                 } else if (cache_vertex && !(pf & PF_TRAIN)) {
                     // terminate into the cache: radiance leaving this vertex comes from the network
                     cache_q = true;
@@ -423,16 +463,17 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                     Rng rng{rs.x, rs.y, rs.z, rs.w};
                     F3 wi;
                     bool diffuse = false;
-                    const F3 f = sample_surface(rng, mat, d, normal, tangent, wi, diffuse);
+                    const F3 f = sample_surface(rng, mat, me.base, me.metallic, me.roughness, d, normal, tangent, wi,
+                                                diffuse);
                     bool done = false;
                     if (a.debug_mode != SPK_DEBUG_COLOR && bounce == 0u) {
                         switch (a.debug_mode) {
                             case 3: C = (normal + f3(1.f, 1.f, 1.f)) * 0.5f; break;
                             case 4: C = (wi + f3(1.f, 1.f, 1.f)) * 0.5f; break;
-                            case 7: C = f3(1.f, 1.f, 1.f) * mat.metallic; break;
-                            case 8: C = f3(1.f, 1.f, 1.f) * mat.roughness; break;
-                            case 9: C = f3(mat.base[0], mat.base[1], mat.base[2]); break;
-                            case 10: C = f3(mat.emissive[0], mat.emissive[1], mat.emissive[2]); break;
+                            case 7: C = f3(1.f, 1.f, 1.f) * me.metallic; break;
+                            case 8: C = f3(1.f, 1.f, 1.f) * me.roughness; break;
+                            case 9: C = me.base; break;
+                            case 10: C = me.emissive; break;
                             case 11: {
                                 const spk_camera_desc &cam = scene_of_path(a, path).cam;
                                 C = f3(1.f, 1.f, 1.f) * (dot(loc - ld3(cam.position), ld3(cam.front)) / cam.far_plane);
@@ -476,12 +517,19 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                         const float cos_x = dot(normal, wl);
                         const float cos_y = fabsf(dot(ng, wl)) / area2;
                         const MaterialGpu &em = a.mats[vb.w];  // v1.w carries the material id
-                        if (cos_x > 0.f && cos_y > 0.f && dist > 2e-4f && w_tri > 0.f) {
+                        // emitted radiance at the sampled point (uv from its barycentrics)
+                        const TriShade &es = a.shade[et];
+                        const float bu = b1, bv = 1.f - b0 - b1;
+                        const MatEval lme = eval_material(a, em, es.uv0[0] + (es.uv1[0] - es.uv0[0]) * bu +
+                                                                     (es.uv2[0] - es.uv0[0]) * bv,
+                                                          es.uv0[1] + (es.uv1[1] - es.uv0[1]) * bu +
+                                                              (es.uv2[1] - es.uv0[1]) * bv);
+                        if (lme.emits && cos_x > 0.f && cos_y > 0.f && dist > 2e-4f && w_tri > 0.f) {
                             // pdf over area: (w_tri / total) / area; contribution f*cos_x*Le*cos_y/(d^2 pdf)
                             const float area = 0.5f * area2;
                             const float g = cos_x * cos_y / dist2 * area * a.emitter_total / w_tri / SPK_PI;
-                            const F3 base3 = f3(mat.base[0], mat.base[1], mat.base[2]);
-                            const F3 Le = f3(em.emissive[0], em.emissive[1], em.emissive[2]);
+                            const F3 base3 = me.base;
+                            const F3 Le = lme.emissive;
                             const F3 contrib = cmul(cmul(T, base3), Le) * g;
                             shadow = true;
                             sh0 = make_float4(loc.x + wl.x * SPK_TOLERANCE, loc.y + wl.y * SPK_TOLERANCE,
@@ -868,6 +916,12 @@ void spk_context::on_scene_changed() {
     }
 }
 
+void spk_context::drop_batch() {
+    if (paths) paths->drop_graph();
+    delete batch;
+    batch = nullptr;
+}
+
 void spk_context::release_all() {
     delete paths;
     paths = nullptr;
@@ -881,6 +935,9 @@ void spk_context::release_all() {
     scene.materials.release();
     scene.emitters.release();
     scene.valid = false;
+    texels.release();
+    tex_table.release();
+    num_textures = 0;
 }
 
 extern "C" {
@@ -937,6 +994,8 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     a.tris = (const uint4 *)ds.tris.ptr;
     a.shade = (const TriShade *)ds.shade.ptr;
     a.mats = (const MaterialGpu *)ds.materials.ptr;
+    a.tex_table = (const TexRecord *)ctx->tex_table.ptr;
+    a.texels = (const float4 *)ctx->texels.ptr;
     a.emitters = (const uint32_t *)ps.emitters.ptr;
     a.emitter_cdf = (const float *)ps.emitter_cdf.ptr;
     a.num_emitters = ps.num_emitters;
@@ -1142,7 +1201,7 @@ int spk_batch_upload(spk_context *ctx, const spk_batch_scene *scenes, uint32_t n
         std::vector<MaterialGpu> sm;
         std::string err;
         int rc = prepare_scene(sc.meshes, sc.num_meshes, sc.instances, sc.num_instances, sc.materials,
-                               sc.num_materials, options, hs, sm, err);
+                               sc.num_materials, ctx->num_textures, options, hs, sm, err);
         if (rc != SPK_OK) return set_error(ctx, rc, "scene %u: %s", s, err.c_str());
         if (s == 0) width = (int)hs.input.width;
         depth = std::max(depth, (int)hs.bvh.depth);

@@ -212,10 +212,44 @@ __device__ __forceinline__ Sampled dielectric_sample(Rng &rng, F3 wo, F3 base, f
     return s;
 }
 
-// MaterialRenderProxy::SampleSurface for the PBR and dielectric proxies; w_i in world space
-__device__ __forceinline__ F3 sample_surface(Rng &rng, const MaterialGpu &mat, F3 ray_dir, F3 normal, F3 tangent,
-                                             F3 &w_i, bool &diffuse) {
-    const F3 base = f3(mat.base[0], mat.base[1], mat.base[2]);
+// ---- textures: Image::Sample (io/Image.h:112-152) ----
+__device__ __forceinline__ uint32_t wrap_mod(int a, uint32_t b) {
+    return ((uint32_t)(a % (int)b) + b) % b;  // Utilities.h:526-529
+}
+
+__device__ __forceinline__ float srgb_to_linear(float c) {
+    // Utilities.h:416-425, per channel
+    const float cutoff = c < 0.04045f ? 1.f : 0.f;
+    const float higher = powf((c + 0.055f) / 1.055f, 2.4f);
+    const float lower = c / 12.92f;
+    return higher * (1.f - cutoff) + lower * cutoff;
+}
+
+// bilinear with wrapping; uv scales by (width - 1, height - 1) (Image.h:21, 123) and the integer
+// part is a truncation (Utilities.h:124-128)
+__device__ __forceinline__ F3 tex_sample(const TexRecord *__restrict__ table, const float4 *__restrict__ texels,
+                                         int index, float u, float v) {
+    const TexRecord t = table[index];
+    const float px = u * (float)(t.width - 1u), py = v * (float)(t.height - 1u);
+    const int iu = (int)px, iv = (int)py;
+    const float fu = px - (float)iu, fv = py - (float)iv;
+    const uint32_t x0 = wrap_mod(iu, t.width), x1 = wrap_mod(iu + 1, t.width);
+    const uint32_t y0 = wrap_mod(iv, t.height), y1 = wrap_mod(iv + 1, t.height);
+    const float4 *T = texels + t.offset;
+    const float4 s00 = T[(size_t)y0 * t.width + x0], s10 = T[(size_t)y0 * t.width + x1];
+    const float4 s01 = T[(size_t)y1 * t.width + x0], s11 = T[(size_t)y1 * t.width + x1];
+    const F3 l0 = f3(s00.x + (s10.x - s00.x) * fu, s00.y + (s10.y - s00.y) * fu, s00.z + (s10.z - s00.z) * fu);
+    const F3 l1 = f3(s01.x + (s11.x - s01.x) * fu, s01.y + (s11.y - s01.y) * fu, s01.z + (s11.z - s01.z) * fu);
+    F3 r = f3(l0.x + (l1.x - l0.x) * fv, l0.y + (l1.y - l0.y) * fv, l0.z + (l1.z - l0.z) * fv);
+    if (t.srgb) r = f3(srgb_to_linear(r.x), srgb_to_linear(r.y), srgb_to_linear(r.z));
+    return r;
+}
+
+// MaterialRenderProxy::SampleSurface for the PBR and dielectric proxies; w_i in world space. base,
+// metallic and roughness are the material getters' values at the hit's uv (MaterialRenderProxy.h:117-144).
+__device__ __forceinline__ F3 sample_surface(Rng &rng, const MaterialGpu &mat, F3 base, float metallic,
+                                             float roughness, F3 ray_dir, F3 normal, F3 tangent, F3 &w_i,
+                                             bool &diffuse) {
     const F3 wo = -ray_dir;
     F3 u, v, w;
     local_axes(normal, tangent, u, v, w);
@@ -224,7 +258,7 @@ __device__ __forceinline__ F3 sample_surface(Rng &rng, const MaterialGpu &mat, F
     if (mat.type == 1u) {
         s = dielectric_sample(rng, lwo, base, mat.eta);
     } else {
-        s = specular_sample(rng, lwo, base, mat.roughness, mat.metallic);
+        s = specular_sample(rng, lwo, base, roughness, metallic);
         if (!s.valid) s = lambert_sample(rng, base);
     }
     F3 u2, v2, w2;

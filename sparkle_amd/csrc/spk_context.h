@@ -58,7 +58,8 @@ struct BatchState;  // batch of independent scenes, defined with the path-tracer
 // materials. Returns an SPK_* code and a message in err.
 int prepare_scene(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_instance_desc *instances,
                   uint32_t num_instances, const spk_material_desc *materials, uint32_t num_materials,
-                  const spk_build_options *options, HostScene &hs, std::vector<MaterialGpu> &mats, std::string &err);
+                  uint32_t num_textures, const spk_build_options *options, HostScene &hs,
+                  std::vector<MaterialGpu> &mats, std::string &err);
 // Shading attributes permuted to the BVH's leaf order.
 std::vector<TriShade> leaf_order_shade(const HostScene &hs);
 
@@ -72,6 +73,8 @@ struct spk_context {
     spk::DeviceBuffer counters;  // [0] invalid rays, [2..3] ray counter of the persistent kernels
     spk::DeviceBuffer tmp_rays, tmp_hits;
     spk::DeviceScene scene;
+    spk::DeviceBuffer texels, tex_table;  // texture table (spk_textures_upload)
+    uint32_t num_textures = 0;
     spk::HostScene host_scene;
     std::vector<spk::MaterialGpu> host_mats;
     spk_scene_stats stats{};
@@ -88,6 +91,7 @@ struct spk_context {
     }
     int check_invalid_rays();
     void on_scene_changed();
+    void drop_batch();
     void release_all();
 };
 

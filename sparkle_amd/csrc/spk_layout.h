@@ -66,9 +66,20 @@ struct MaterialGpu {
     float metallic;
     float roughness;
     float eta;
-    uint32_t emissive_flag;  // |emissive| > 1e-6
+    uint32_t emissive_flag;  // |emissive factor| > 1e-6: may emit (with a texture: decided per hit)
     uint32_t pad;
+    int32_t tex_base, tex_mr, tex_emissive, tex_normal;  // texture table indices, -1 = none
 };
-static_assert(sizeof(MaterialGpu) == 48, "MaterialGpu must be 48 bytes");
+static_assert(sizeof(MaterialGpu) == 64, "MaterialGpu must be 64 bytes");
+
+// One texture of the texture table: texels (float4, converted at upload) start at `offset` of the
+// texel array, row-major.
+struct TexRecord {
+    uint64_t offset;
+    uint32_t width, height;
+    uint32_t srgb;  // apply the sRGB decode after filtering
+    uint32_t pad[3];
+};
+static_assert(sizeof(TexRecord) == 32, "TexRecord must be 32 bytes");
 
 }  // namespace spk

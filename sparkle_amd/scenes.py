@@ -6,7 +6,8 @@ import math
 
 import numpy as np
 
-from . import RAY_DTYPE, Scene, camera_desc, MATERIAL_PBR, MATERIAL_DIELECTRIC
+from . import (MATERIAL_DIELECTRIC, MATERIAL_PBR, RAY_DTYPE, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA8_UNORM, TEXTURE_RGBA32F,
+               Scene, camera_desc)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -272,6 +273,48 @@ def cornell_scene(seed=0):
     _add(sc, box_mesh((-0.4, 0.3, 0.6), (0.3, 0.3, 0.6), a1), m_box1)
     _add(sc, box_mesh((0.45, -0.2, 0.3), (0.3, 0.3, 0.3), a2), m_box2)
     _add(sc, icosphere(3, radius=0.3, center=(0.4, -0.25, 0.9)), m_metal)
+    return sc
+
+
+def textured_cornell_scene(seed=0):
+    """The Cornell room with textures on every material channel the reference samples: an sRGB
+    checker base colour on the floor (uv repeated 3.5 times, so wrapping is exercised), a
+    metallic-roughness texture on the back wall, an emissive texture (factor x texture) on the
+    light and a bumpy normal map on the left wall."""
+    sc = cornell_scene(seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 101))
+    n = 16
+    jj, ii = np.mgrid[0:n, 0:n]
+    checker = np.where((ii // 4 + jj // 4) % 2 == 0, 230, 40).astype(np.uint8)
+    base = np.stack([checker, checker // 2 + 60, 255 - checker, np.full_like(checker, 255)], -1)
+    t_base = sc.add_texture(base, TEXTURE_RGBA8_SRGB)
+    mr = np.zeros((8, 8, 4), np.uint8)
+    mr[..., 1] = rng.integers(40, 255, (8, 8))   # roughness in g
+    mr[..., 2] = rng.integers(0, 255, (8, 8))    # metallic in b
+    mr[..., 3] = 255
+    t_mr = sc.add_texture(mr, TEXTURE_RGBA8_UNORM)
+    em = np.ones((4, 4, 4), np.float32)
+    em[..., :3] = rng.uniform(0.3, 1.0, (4, 4, 3))
+    t_em = sc.add_texture(em, TEXTURE_RGBA32F)
+    x = np.linspace(0, 4 * np.pi, 32)
+    bump = np.stack([0.5 + 0.35 * np.sin(x)[None, :].repeat(32, 0), 0.5 + 0.35 * np.cos(x)[:, None].repeat(32, 1),
+                     np.full((32, 32), 0.9), np.ones((32, 32))], -1)
+    t_nrm = sc.add_texture((bump * 255).astype(np.uint8), TEXTURE_RGBA8_UNORM)
+    # instance order of cornell_scene: floor, ceiling, back, left, right, light, boxes, sphere
+    floor, back, left, light = (sc.instances[k]["material"] for k in (0, 2, 3, 5))
+    sc.materials[floor]["base_color_texture"] = t_base
+    sc.materials[back]["metallic_roughness_texture"] = t_mr
+    sc.materials[back]["metallic"] = 0.8
+    sc.materials[back]["roughness"] = 0.9
+    sc.materials[light]["emissive_texture"] = t_em
+    sc.materials[left]["normal_texture"] = t_nrm
+    for k in (0, 2, 3, 5):
+        fill = {"base_color_texture": -1, "metallic_roughness_texture": -1, "emissive_texture": -1,
+                "normal_texture": -1}
+        fill.update({f: v for f, v in sc.materials[sc.instances[k]["material"]].items() if f.endswith("_texture")})
+        sc.materials[sc.instances[k]["material"]].update(fill)
+    sc.meshes[sc.instances[0]["mesh"]]["uvs"] = (sc.meshes[sc.instances[0]["mesh"]]["uvs"] * 3.5 - 0.25).astype(
+        np.float32)
     return sc
 
 

@@ -32,7 +32,8 @@ def test_header_has_no_cpp_or_torch_types():
 def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(spk.MeshDesc) == 56
     assert ctypes.sizeof(spk.InstanceDesc) == 56
-    assert ctypes.sizeof(spk.MaterialDesc) == 40
+    assert ctypes.sizeof(spk.MaterialDesc) == 56
+    assert ctypes.sizeof(spk.TextureDesc) == 24
     assert ctypes.sizeof(spk.CameraDesc) == 92
     assert ctypes.sizeof(spk.FrameDesc) == 44
     assert ctypes.sizeof(spk.CacheConfig) == 40

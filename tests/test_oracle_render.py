@@ -183,3 +183,93 @@ def test_spatial_fill_properties_on_random_gbuffers():
                     np.testing.assert_array_equal(out[j, i], rgba[j, i])
                 elif out[j, i, 3] == 1.0:
                     assert any(np.array_equal(out[j, i, :3], c) for c in srcs)
+
+
+def _tex_scene(textures):
+    sc = spk.Scene()
+    sc.textures = list(textures)
+    m = sc.add_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32),
+                    np.array([[0, 0, 1]] * 3, np.float32), np.array([[1, 0, 0, 1]] * 3, np.float32),
+                    np.array([[0, 0], [1, 0], [0, 1]], np.float32))
+    sc.add_instance(m, sc.add_material())
+    return refcpu.OracleScene(sc)
+
+
+def _floor_camera():
+    # looks straight down at a patch of the Cornell floor that no object covers
+    return scenes.camera_desc(position=(-0.6, -0.8, 0.5), front=(0.0, 0.01, -1.0), vertical_fov=0.3,
+                              focus_distance=0.5, aspect=1.0)
+
+
+def test_texture_sampling_known_answers():
+    """Image::Sample: uv scaled by (w-1, h-1), truncation toward zero, wrap, bilinear, then the sRGB
+    decode; closed forms on a 3x2 texture."""
+    px = np.zeros((2, 3, 4), np.uint8)
+    px[..., 0] = [[0, 51, 102], [153, 204, 255]]
+    px[..., 1] = 255
+    px[..., 3] = 255
+    o = _tex_scene([(px, spk.TEXTURE_RGBA8_UNORM), (px, spk.TEXTURE_RGBA8_SRGB)])
+    r = lambda i, j: np.float32(px[j, i, 0]) / np.float32(255.0)
+    # texel centres: u = i / (w - 1), v = j / (h - 1)
+    for i in range(3):
+        for j in range(2):
+            assert o.texture_sample(0, i / 2, j)[0] == r(i, j)
+    # halfway in u between texels 0 and 1 of row 0, a quarter in v towards row 1
+    a = r(0, 0) + (r(1, 0) - r(0, 0)) * np.float32(0.5)
+    b = r(0, 1) + (r(1, 1) - r(0, 1)) * np.float32(0.5)
+    np.testing.assert_allclose(o.texture_sample(0, 0.25, 0.25)[0], a + (b - a) * 0.25, rtol=1e-6)
+    # wrap: u = 1.5 -> px = 3 -> texel 3 mod 3 = 0
+    assert o.texture_sample(0, 1.5, 0.0)[0] == r(0, 0)
+    # negative uv: truncation toward zero leaves a negative fraction (extrapolation towards texel 1)
+    np.testing.assert_allclose(o.texture_sample(0, -0.25, 0.0)[0], r(0, 0) + (r(1, 0) - r(0, 0)) * -0.5, rtol=1e-6)
+    # sRGB decode (Utilities.h:416-425): 0 -> 0, 1 -> 1, 0.2 -> ((0.2 + 0.055) / 1.055)^2.4
+    assert o.texture_sample(1, 0.0, 0.0)[0] == 0.0
+    assert o.texture_sample(1, 1.0, 1.0)[0] == 1.0
+    np.testing.assert_allclose(o.texture_sample(1, 0.5, 0.0)[0], ((0.2 + 0.055) / 1.055) ** 2.4, rtol=1e-6)
+    small = np.zeros((1, 1, 4), np.uint8)
+    small[..., 0] = 10  # 10/255 = 0.0392 < 0.04045: linear segment
+    o2 = _tex_scene([(small, spk.TEXTURE_RGBA8_SRGB)])
+    np.testing.assert_allclose(o2.texture_sample(0, 0.3, 0.7)[0], 10 / 255 / 12.92, rtol=1e-6)
+    f = np.random.Generator(np.random.PCG64(1)).random((3, 5, 4)).astype(np.float32)
+    o3 = _tex_scene([(f, spk.TEXTURE_RGBA32F)])
+    np.testing.assert_array_equal(o3.texture_sample(0, 0.75, 0.5), f[1, 3, :3])
+
+
+def test_textured_material_getters_in_debug_modes():
+    """Debug albedo/metallic/roughness/emissive of a textured material equal the getters'
+    closed forms: a 1x1 texture makes every channel constant."""
+    sc = scenes.cornell_scene(0)
+    c = np.array([[[128, 64, 32, 255]]], np.uint8)
+    t = sc.add_texture(c, spk.TEXTURE_RGBA8_UNORM)
+    floor = sc.instances[0]["material"]
+    sc.materials[floor].update(base_color_texture=t, metallic_roughness_texture=t, emissive_texture=-1,
+                               normal_texture=-1, metallic=0.5, roughness=0.75)
+    o = refcpu.OracleScene(sc)
+    w = h = 32
+    cam = _floor_camera()
+    tex = c[0, 0, :3].astype(np.float32) / np.float32(255.0)
+    base = np.float32(sc.materials[floor]["base_color"]) * tex
+    for mode, want in ((9, base), (7, np.full(3, tex[2] * np.float32(0.5))), (8, np.full(3, tex[1] * np.float32(0.75)))):
+        img, _ = o.render_frame(cam, w, h, frame_seed=0, debug_mode=mode)
+        np.testing.assert_allclose(img[..., :3].reshape(-1, 3), np.tile(want, (w * h, 1)), rtol=1e-6)
+
+
+def test_normal_map_tilts_the_shading_normal():
+    """A normal map texel (255, 128, 128) maps to tangent-space (1, 1/255, 1/255): the shading normal
+    of a floor with tangent +x turns to about +x (debug normal mode, closed form)."""
+    sc = scenes.cornell_scene(0)
+    nm = np.array([[[255, 128, 128, 255]]], np.uint8)
+    t = sc.add_texture(nm, spk.TEXTURE_RGBA8_UNORM)
+    floor = sc.instances[0]["material"]
+    sc.materials[floor].update(normal_texture=t, base_color_texture=-1, metallic_roughness_texture=-1,
+                               emissive_texture=-1)
+    o = refcpu.OracleScene(sc)
+    w = h = 16
+    cam = _floor_camera()
+    img, nrm = o.render_frame(cam, w, h, frame_seed=0, debug_mode=3)
+    tn = np.array([255, 128, 128], np.float32) / np.float32(255.0) * 2 - 1
+    T, N = np.array([1.0, 0, 0]), np.array([0, 0, 1.0])
+    B = np.cross(N, T)
+    want = T * tn[0] + B * tn[1] + N * tn[2]
+    want /= np.linalg.norm(want)
+    np.testing.assert_allclose(nrm.reshape(-1, 3), np.tile(want, (w * h, 1)), atol=1e-6)

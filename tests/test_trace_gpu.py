@@ -198,7 +198,7 @@ def test_persistent_kernels_match_counts_and_invalid_rays(gpu_ctx):
             assert rc1 == spk.SPK_ERR_INVALID_RAY
             np.testing.assert_array_equal(cn.cpu().numpy(), ref_counts[0])
             np.testing.assert_array_equal(ct.cpu().numpy(), ref_counts[1])
-        rc, _ = gpu_trace(gpu_ctx, rays[:1000])
+        rc, _ = gpu_trace(gpu_ctx, rays[100:1100])  # no invalid ray in this slice
         assert rc == spk.SPK_OK
 
 
