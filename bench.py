@@ -71,19 +71,20 @@ def cpu_baseline(scene, rays, seconds_target=2.0, runs=5):
 
 
 def load_traffic(config):
-    """HBM-side bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, when it
+    """HBM-side bytes per launch and L2 hit rate from the committed rocprofv3 counter summary, when it
     was taken with the kernel configuration this run uses."""
     path = os.path.join(ROOT, "profiles", "trace_incoherent_2m_pmc.md")
     if not os.path.exists(path):
-        return None, None
+        return None, None, None
     fields = {}
     for line in open(path):
-        if ":" in line and line.split(":")[0] in ("config", "hbm_bytes_per_launch"):
+        if ":" in line and line.split(":")[0] in ("config", "hbm_bytes_per_launch", "l2_hit_rate"):
             k, v = line.split(":", 1)
             fields[k.strip()] = v.strip()
     if fields.get("config") != config:
-        return None, f"profiles/trace_incoherent_2m_pmc.md was taken with config {fields.get('config')}"
-    return int(float(fields["hbm_bytes_per_launch"])), "profiles/trace_incoherent_2m_pmc.md"
+        return None, None, f"profiles/trace_incoherent_2m_pmc.md was taken with config {fields.get('config')}"
+    l2 = float(fields["l2_hit_rate"]) if "l2_hit_rate" in fields else None
+    return int(float(fields["hbm_bytes_per_launch"])), l2, "profiles/trace_incoherent_2m_pmc.md"
 
 
 def relmse(img, ref, eps=1e-2):
@@ -207,6 +208,56 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     return out
 
 
+MLP_FLOP_FORWARD = 2 * (32 * 64 + 3 * 64 * 64 + 64 * 3)  # 29,056 per query
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+
+
+def cache_kernel_bench(spk, scenes, stream, n_infer=1920 * 1080, n_train=1 << 16, reps=20):
+    """The learned radiance cache's kernels alone: inference on n_infer points (one 1080p frame of
+    queries) and training steps (forward, backward, Adam) on n_train samples, uniform points in the
+    unit cube, timed with HIP events on the context's stream. Algorithmic FLOPs: 29,056 per query
+    forward, 3x that per training sample (forward, input gradients, weight gradients)."""
+    import torch
+
+    ctx = spk.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.upload_scene(scenes.cornell_scene(0))  # the cache's grid is sized from a scene box
+    ctx.cache_create(spk.cache_config())
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    pts = torch.rand(n_infer * 3, device="cuda", generator=g)
+    out = torch.empty(n_infer * 3, device="cuda")
+    tp = torch.rand(n_train * 3, device="cuda", generator=g)
+    tt = torch.rand(n_train * 3, device="cuda", generator=g)
+    for _ in range(3):
+        ctx.cache_infer(pts, out, n_infer)
+        ctx.cache_train(tp, tt, n_train)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        ctx.cache_infer(pts, out, n_infer)
+    b.record(stream)
+    torch.cuda.synchronize()
+    infer_ms = a.elapsed_time(b) / reps
+    a.record(stream)
+    for _ in range(reps):
+        ctx.cache_train(tp, tt, n_train)
+    b.record(stream)
+    torch.cuda.synchronize()
+    train_ms = a.elapsed_time(b) / reps
+    ctx.close()
+    infer_tf = MLP_FLOP_FORWARD * n_infer / (infer_ms / 1e3) / 1e12
+    train_tf = 3 * MLP_FLOP_FORWARD * n_train / (train_ms / 1e3) / 1e12
+    return {"infer_queries": n_infer, "infer_ms": round(infer_ms, 4),
+            "infer_mqueries_per_s": round(n_infer / (infer_ms / 1e3) / 1e6, 1),
+            "infer_tflops_algorithmic": round(infer_tf, 2), "infer_frac_bf16_peak": round(infer_tf / BF16_PEAK_TFLOPS, 4),
+            "train_samples": n_train, "train_step_ms": round(train_ms, 4),
+            "train_tflops_algorithmic": round(train_tf, 2),
+            "train_frac_bf16_peak": round(train_tf / BF16_PEAK_TFLOPS, 4),
+            "bound": "hash-grid gathers (16 levels x 8 corners x 4 B per query) and table-gradient atomics"}
+
+
 def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
     """Many scenes of uneven size in one launch: `count` many-instance scenes (triangle budgets
     log-uniform in [1k, 500k], images 64^2..512^2), `spp` frames of `max_bounce` bounces each, all
@@ -274,6 +325,8 @@ def main():
     ap.add_argument("--relmse", type=str, default="1920x1080,64,4096",
                     help="equal-time relMSE experiment WxH,spp,reference-spp ('off' to skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
+    ap.add_argument("--cache-kernels", action="store_true",
+                    help="only time the learned cache's inference and training kernels (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -287,6 +340,15 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel(), "copy_bytes_written": b.numel(),
                           "copies": args.copy_calibration}))
+        return
+    if args.cache_kernels:
+        import torch
+        import sparkle_amd as spk
+        from sparkle_amd import scenes
+
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        print(json.dumps({"cache_kernels": cache_kernel_bench(spk, scenes, stream)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by image tiles or scenes, "
@@ -383,7 +445,7 @@ def main():
 
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     cur = ctx.trace_config()
-    traffic, traffic_src = load_traffic(",".join(str(x) for x in cur))
+    traffic, l2_hit, traffic_src = load_traffic(",".join(str(x) for x in cur))
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -422,12 +484,14 @@ def main():
             "compulsory_bytes_per_ray": 40,
             "kernel_avg_ms": round(avg_ms, 4),
             "traffic_source": traffic_src,
+            "l2_hit_rate": l2_hit,
         },
     }
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     ctx.close()
+    out["cache_kernels"] = cache_kernel_bench(spk, scenes, stream)
     if not args.no_batch:
         out["batch"] = batch_experiment(spk, scenes, stream)
     if args.relmse != "off":
