@@ -9,12 +9,15 @@ The reader applies the mapping rules of Sparkle's loader (libraries/source/io/sc
     material is white, fully metallic, roughness 1, not emissive (the `MaterialResource` defaults,
     io/Material.h:9-23); base = texture x factor, metallic = texture.b x factor, roughness =
     texture.g x factor, emissive = texture x factor (MaterialRenderProxy.h:117-162);
-  * every material is a PBR material (the loader creates no dielectric).
-Textures other than 1x1 constants are not supported on the device path and raise.
+  * every material is a PBR material (the loader creates no dielectric);
+  * base-colour and emissive textures are colour data (sRGB decode after filtering), metallic-
+    roughness and normal textures are linear (TextureCompression profiles Color / Data / Normal,
+    `:485-517`).
+Images are embedded PNGs (8-bit RGB or RGBA, no interlace) decoded by this module.
 
 The writer stores each instance as its own mesh in world space, rotated by -90 degrees about X so
-the reader's +90 degrees restores it, and gives every material 1x1 white base-colour,
-metallic-roughness and emissive textures so that the loader applies the factors.
+the reader's +90 degrees restores it. A material channel without a texture gets a 1x1 white
+texture so that the loader applies its factor; RGBA8 textures of the scene are written as PNG.
 """
 import base64
 import json
@@ -24,7 +27,7 @@ import zlib
 
 import numpy as np
 
-from . import MATERIAL_PBR, Scene
+from . import MATERIAL_PBR, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA8_UNORM, Scene
 
 _FLOAT, _UINT = 5126, 5125
 _ARRAY_BUFFER, _ELEMENT_ARRAY_BUFFER = 34962, 34963
@@ -91,11 +94,6 @@ def _png_decode(data):
     return out
 
 
-def _srgb_decode(c):
-    # Image.h:112-150 decodes colour textures with a gamma-2.2 approximation
-    return np.power(c, 2.2)
-
-
 _RX_NEG90 = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 1.0], [0.0, -1.0, 0.0]])
 
 
@@ -141,20 +139,36 @@ def write_gltf(scene, path):
                                        "mode": 4}]})
         nodes.append({"mesh": len(meshes) - 1})
 
-    white = "data:image/png;base64," + base64.b64encode(_png_rgba(np.full((1, 1, 4), 255, np.uint8))).decode()
+    images = ["data:image/png;base64," + base64.b64encode(_png_rgba(np.full((1, 1, 4), 255, np.uint8))).decode()]
+    image_of = {}
+
+    def tex_ref(k):
+        if k is None or k < 0:
+            return {"index": 0}
+        if k not in image_of:
+            px, fmt = scene.textures[k]
+            if px.dtype != np.uint8:
+                raise NotImplementedError("only 8-bit textures are written")
+            images.append("data:image/png;base64," + base64.b64encode(_png_rgba(px)).decode())
+            image_of[k] = len(images) - 1
+        return {"index": image_of[k]}
+
     materials = []
     for mat in scene.materials:
-        materials.append({
+        m = {
             "pbrMetallicRoughness": {
                 "baseColorFactor": [float(x) for x in mat["base_color"]] + [1.0],
-                "baseColorTexture": {"index": 0},
+                "baseColorTexture": tex_ref(mat.get("base_color_texture")),
                 "metallicFactor": float(mat["metallic"]),
                 "roughnessFactor": float(mat["roughness"]),
-                "metallicRoughnessTexture": {"index": 0},
+                "metallicRoughnessTexture": tex_ref(mat.get("metallic_roughness_texture")),
             },
             "emissiveFactor": [float(x) for x in mat["emissive"]],
-            "emissiveTexture": {"index": 0},
-        })
+            "emissiveTexture": tex_ref(mat.get("emissive_texture")),
+        }
+        if mat.get("normal_texture", -1) >= 0:
+            m["normalTexture"] = tex_ref(mat["normal_texture"])
+        materials.append(m)
     doc = {
         "asset": {"version": "2.0", "generator": "sparkle_amd.gltf"},
         "scene": 0,
@@ -162,8 +176,8 @@ def write_gltf(scene, path):
         "nodes": nodes,
         "meshes": meshes,
         "materials": materials,
-        "textures": [{"source": 0}],
-        "images": [{"uri": white}],
+        "textures": [{"source": i} for i in range(len(images))],
+        "images": [{"uri": u} for u in images],
         "buffers": [{"byteLength": len(blob),
                      "uri": "data:application/octet-stream;base64," + base64.b64encode(bytes(blob)).decode()}],
         "bufferViews": views,
@@ -215,35 +229,33 @@ def read_gltf(path):
             out = np.stack([np.frombuffer(buf, dt, nc, base + k * stride) for k in range(a["count"])])
         return out.copy()
 
-    images = []
-    for im in doc.get("images", []):
-        px = _png_decode(_data_uri(im["uri"])).astype(np.float64) / 255.0
-        images.append(px)
-
-    def tex_const(ref):
-        tex = doc["textures"][ref["index"]]
-        px = images[tex["source"]]
-        if px.shape[0] != 1 or px.shape[1] != 1:
-            raise NotImplementedError("textures larger than 1x1 are not on the device path")
-        return px[0, 0]
-
+    images = [_png_decode(_data_uri(im["uri"])) for im in doc.get("images", [])]
     sc = Scene()
+    made = {}
+
+    def texture(ref, fmt):
+        key = (doc["textures"][ref["index"]]["source"], fmt)
+        if key not in made:
+            made[key] = sc.add_texture(images[key[0]], fmt)
+        return made[key]
+
     for m in doc.get("materials", []):
-        base, metallic, roughness, emissive = np.ones(3), 1.0, 1.0, np.zeros(3)
+        # MaterialResource defaults (io/Material.h:9-23); factors only with their texture
+        kw = dict(base_color=(1.0, 1.0, 1.0), emissive=(0.0, 0.0, 0.0), metallic=1.0, roughness=1.0)
         pbr = m.get("pbrMetallicRoughness", {})
         if "baseColorTexture" in pbr:
-            t = _srgb_decode(tex_const(pbr["baseColorTexture"])[:3])
-            base = t * np.asarray(pbr.get("baseColorFactor", [1, 1, 1, 1])[:3], np.float64)
+            kw["base_color_texture"] = texture(pbr["baseColorTexture"], TEXTURE_RGBA8_SRGB)
+            kw["base_color"] = tuple(float(np.float32(x)) for x in pbr.get("baseColorFactor", [1, 1, 1, 1])[:3])
+        if "normalTexture" in m:
+            kw["normal_texture"] = texture(m["normalTexture"], TEXTURE_RGBA8_UNORM)
         if "metallicRoughnessTexture" in pbr:
-            t = tex_const(pbr["metallicRoughnessTexture"])
-            metallic = float(t[2] * pbr.get("metallicFactor", 1.0))
-            roughness = float(t[1] * pbr.get("roughnessFactor", 1.0))
+            kw["metallic_roughness_texture"] = texture(pbr["metallicRoughnessTexture"], TEXTURE_RGBA8_UNORM)
+            kw["metallic"] = float(np.float32(pbr.get("metallicFactor", 1.0)))
+            kw["roughness"] = float(np.float32(pbr.get("roughnessFactor", 1.0)))
         if "emissiveTexture" in m:
-            t = _srgb_decode(tex_const(m["emissiveTexture"])[:3])
-            emissive = t * np.asarray(m.get("emissiveFactor", [0, 0, 0]), np.float64)
-        sc.add_material(base_color=tuple(np.float32(base)), emissive=tuple(np.float32(emissive)),
-                        metallic=float(np.float32(metallic)), roughness=float(np.float32(roughness)),
-                        type=MATERIAL_PBR)
+            kw["emissive_texture"] = texture(m["emissiveTexture"], TEXTURE_RGBA8_SRGB)
+            kw["emissive"] = tuple(float(np.float32(x)) for x in m.get("emissiveFactor", [0, 0, 0]))
+        sc.add_material(type=MATERIAL_PBR, **kw)
     if not sc.materials:
         sc.add_material(metallic=1.0, roughness=1.0)
 

@@ -79,3 +79,29 @@ def test_untextured_factors_are_ignored_and_nodes_compose(tmp_path):
     want = [R(R(2 * np.array(v, float)) + np.array([1.0, 0, 0])) + np.array([0, 0, 1.0])
             for v in ([0, 0, 0], [1, 0, 0], [0, 1, 0])]
     np.testing.assert_allclose(p, np.array(want), atol=1e-6)
+
+
+def test_textured_round_trip(tmp_path):
+    """Non-constant textures travel as PNG: base colour and emissive come back as sRGB colour data,
+    metallic-roughness and normal maps as linear data, pixels unchanged."""
+    sc = scenes.cornell_scene(0)
+    rng = np.random.Generator(np.random.PCG64(3))
+    px = rng.integers(0, 256, (5, 7, 4), dtype=np.uint8)
+    mr = rng.integers(0, 256, (3, 4, 4), dtype=np.uint8)
+    tb = sc.add_texture(px, spk.TEXTURE_RGBA8_SRGB)
+    tm = sc.add_texture(mr, spk.TEXTURE_RGBA8_UNORM)
+    floor = sc.instances[0]["material"]
+    sc.materials[floor].update(base_color_texture=tb, metallic_roughness_texture=tm, normal_texture=tm,
+                               emissive_texture=-1)
+    path = str(tmp_path / "tex.gltf")
+    gltf.write_gltf(sc, path)
+    back = gltf.read_gltf(path)
+    m = back.materials[floor]
+    bpx, bfmt = back.textures[m["base_color_texture"]]
+    mpx, mfmt = back.textures[m["metallic_roughness_texture"]]
+    npx, nfmt = back.textures[m["normal_texture"]]
+    np.testing.assert_array_equal(bpx, px)
+    np.testing.assert_array_equal(mpx, mr)
+    np.testing.assert_array_equal(npx, mr)
+    assert bfmt == spk.TEXTURE_RGBA8_SRGB and mfmt == spk.TEXTURE_RGBA8_UNORM and nfmt == spk.TEXTURE_RGBA8_UNORM
+    assert m["metallic"] == np.float32(sc.materials[floor]["metallic"])
