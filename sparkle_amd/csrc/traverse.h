@@ -346,14 +346,10 @@ struct Traversal {
                 const float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * SPK_BOX_SLACK, tcull);
                 const uint32_t meta = ((s < 4 ? n1.z : n1.w) >> (8 * k)) & 0xffu;
                 const bool inner = (imask >> s) & 1u;
-                const bool hit = tnear <= tfar && (inner || meta != 0u);
-                if (hit) {
-                    if (inner) {
-                        new_hits |= 1u << (s ^ oct);
-                    } else {
-                        tri_bits |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
-                    }
-                }
+                const bool hit = tnear <= tfar;
+                // selects, not branches: every lane evaluates both outcomes of every slot
+                new_hits |= (hit && inner) ? (1u << (s ^ oct)) : 0u;
+                tri_bits |= (hit && !inner) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
             }
             node_oct = oct;
         } else {
@@ -386,14 +382,10 @@ struct Traversal {
                 const float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * SPK_BOX_SLACK, tcull);
                 const uint32_t meta = (meta_w >> (8 * s)) & 0xffu;
                 const bool inner = (imask >> s) & 1u;
-                const bool hit = tnear <= tfar && (inner || meta != 0u);
-                if (hit) {
-                    if (inner) {
-                        new_hits |= 1u << (s ^ node_oct);
-                    } else {
-                        tri_bits |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
-                    }
-                }
+                const bool hit = tnear <= tfar;
+                // selects, not branches: every lane evaluates both outcomes of every slot
+                new_hits |= (hit && inner) ? (1u << (s ^ node_oct)) : 0u;
+                tri_bits |= (hit && !inner) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
             }
         }
         g_base = child_base;
