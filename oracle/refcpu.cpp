@@ -532,11 +532,21 @@ struct Texture {
 };
 }  // namespace orcl
 
+namespace orcl {
+struct Emitter {  // world-space emissive triangle for next-event estimation
+    float p[9];
+    float uv[6];
+    uint32_t material;
+};
+}  // namespace orcl
+
 struct orcl_scene {
     std::vector<orcl::Mesh> meshes;
     std::vector<orcl::Instance> instances;  // in TLAS leaf order
     std::vector<orcl_material> materials;
     std::vector<orcl::Texture> textures;
+    std::vector<orcl::Emitter> emitters;  // instance order, face order
+    std::vector<float> emitter_cdf;       // running sum of area x mean emitted radiance
     orcl::Bvh tlas;
     uint64_t total_tris = 0;
 };
@@ -889,6 +899,36 @@ orcl_scene *orcl_scene_create(const orcl_mesh *meshes, uint32_t num_meshes, cons
     sc->instances.resize(num_instances);
     for (uint32_t i = 0; i < num_instances; ++i) sc->instances[i] = inst[sc->tlas.prim_ids[i]];
     sc->materials.assign(materials, materials + num_materials);
+    // emitters for next-event estimation: world-space emissive triangles in the caller's instance
+    // order, weighted by area x mean emitted radiance (factor)
+    double acc = 0.0;
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        const orcl_material &mt = materials[inst[i].material];
+        const float en = std::sqrt(mt.emissive[0] * mt.emissive[0] + mt.emissive[1] * mt.emissive[1] +
+                                   mt.emissive[2] * mt.emissive[2]);
+        if (!(en > kEps)) continue;
+        const Mesh &m = sc->meshes[inst[i].mesh];
+        for (uint32_t f = 0; f < m.ntri; ++f) {
+            Emitter e;
+            V3 v[3];
+            m.tri(f, v[0], v[1], v[2]);
+            for (int k = 0; k < 3; ++k) {
+                const V3 w = inst[i].to_world.point(v[k]);
+                e.p[3 * k] = w.x, e.p[3 * k + 1] = w.y, e.p[3 * k + 2] = w.z;
+                const uint32_t vi = m.idx[3 * (size_t)f + k];
+                e.uv[2 * k] = m.uv.empty() ? 0.f : m.uv[2 * vi];
+                e.uv[2 * k + 1] = m.uv.empty() ? 0.f : m.uv[2 * vi + 1];
+            }
+            e.material = inst[i].material;
+            const double e1[3] = {(double)e.p[3] - e.p[0], (double)e.p[4] - e.p[1], (double)e.p[5] - e.p[2]};
+            const double e2[3] = {(double)e.p[6] - e.p[0], (double)e.p[7] - e.p[1], (double)e.p[8] - e.p[2]};
+            const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                         cz = e1[0] * e2[1] - e1[1] * e2[0];
+            acc += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz) * (mt.emissive[0] + mt.emissive[1] + mt.emissive[2]) / 3.0;
+            sc->emitters.push_back(e);
+            sc->emitter_cdf.push_back((float)acc);
+        }
+    }
     return sc;
 }
 
@@ -1042,6 +1082,7 @@ struct Hit {
     bool hit = false;
     float t = 0;
     V3 location, normal, tangent;
+    V3 gnormal;  // unnormalised geometric normal of the world-space triangle (e1 x e2)
     float uvx = 0, uvy = 0;
     const Instance *inst = nullptr;
 };
@@ -1139,6 +1180,11 @@ static Hit scene_hit(const orcl_scene &sc, V3 o, V3 d) {
     // TransformDirectionTangentSpace on the inverse = the forward 3x3 (Transform.h:105-108)
     h.normal = normalized(ins.to_world.dir(n));
     h.tangent = ins.to_world.dir(tg);
+    {
+        const V3 w0 = ins.to_world.point(m.vtx(i0)), w1 = ins.to_world.point(m.vtx(i1)),
+                 w2 = ins.to_world.point(m.vtx(i2));
+        h.gnormal = cross(w1 - w0, w2 - w0);
+    }
     h.t = c.t;
     h.location = o + d * c.t;  // Intersection::Update -> ray.At(t)
     h.inst = &ins;
@@ -1276,7 +1322,7 @@ static Sample dielectric_sample(Rng &rng, V3 wo, V3 base, float eta) {
 
 // MaterialRenderProxy::SampleSurface for Pbr / Dielectric proxies
 static V3 sample_surface(Rng &rng, const orcl_material &mat, const MatEval &me, V3 ray_dir, V3 normal, V3 tangent,
-                         V3 &w_i) {
+                         V3 &w_i, bool *diffuse = nullptr) {
     V3 base = me.base;
     V3 wo = -ray_dir;
     V3 u, v, w;
@@ -1287,7 +1333,10 @@ static V3 sample_surface(Rng &rng, const orcl_material &mat, const MatEval &me, 
         s = dielectric_sample(rng, lwo, base, mat.eta);
     } else {
         s = specular_sample(rng, lwo, base, me.roughness, me.metallic);
-        if (!s.valid) s = lambert_sample(rng, base);
+        if (!s.valid) {
+            s = lambert_sample(rng, base);
+            if (diffuse) *diffuse = true;
+        }
     }
     V3 u2, v2, w2;
     local_axes(normal, tangent, u2, v2, w2);
@@ -1303,8 +1352,57 @@ struct PixelResult {
 
 static std::atomic<uint64_t> g_rays{0};
 
+// area pdf of a point on an emitter of this material under the emitter sampling below
+static float light_pdf_area(const orcl_scene &sc, const orcl_material &m) {
+    return (m.emissive[0] + m.emissive[1] + m.emissive[2]) / 3.f / sc.emitter_cdf.back();
+}
+
+// Next-event estimation as the device does it (no reference counterpart): one emissive triangle
+// picked by area x mean radiance, one uniform point on it, Lambert BRDF base/pi, two-sided
+// emitters, occlusion over (1e-4, dist - 1e-4) of the segment. Returns the contribution before
+// visibility and sets `visible`.
+static V3 nee_sample(const orcl_scene &sc, Rng &nr, V3 loc, V3 normal, V3 base, V3 T, bool &visible) {
+    visible = false;
+    const float total = sc.emitter_cdf.back();
+    const float r0 = nr.unit() * total;
+    size_t lo = 0, hi = sc.emitters.size() - 1;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if (sc.emitter_cdf[mid] < r0) lo = mid + 1; else hi = mid;
+    }
+    const Emitter &e = sc.emitters[lo];
+    const float w_tri = sc.emitter_cdf[lo] - (lo ? sc.emitter_cdf[lo - 1] : 0.f);
+    const float r1 = nr.unit(), r2 = nr.unit();
+    const V3 p0(&e.p[0]), p1(&e.p[3]), p2(&e.p[6]);
+    const float su = std::sqrt(r1);
+    const float b0 = 1.f - su, b1 = r2 * su;
+    const V3 y = p0 * b0 + p1 * b1 + p2 * (1.f - b0 - b1);
+    const V3 ng = cross(p1 - p0, p2 - p0);
+    const float area2 = std::sqrt(sqnorm(ng));
+    const V3 dl = y - loc;
+    const float dist2 = sqnorm(dl);
+    const float dist = std::sqrt(dist2);
+    const V3 wl = dl / dist;
+    const float cos_x = dot(normal, wl);
+    const float cos_y = std::fabs(dot(ng, wl)) / area2;
+    const float bu = b1, bv = 1.f - b0 - b1;
+    const MatEval lme = eval_material(sc, sc.materials[e.material], e.uv[0] + (e.uv[2] - e.uv[0]) * bu + (e.uv[4] - e.uv[0]) * bv,
+                                      e.uv[1] + (e.uv[3] - e.uv[1]) * bu + (e.uv[5] - e.uv[1]) * bv);
+    if (!(norm(lme.emissive) > kEps) || !(cos_x > 0.f) || !(cos_y > 0.f) || !(dist > 2e-4f) || !(w_tri > 0.f))
+        return V3(0, 0, 0);
+    const float area = 0.5f * area2;
+    // balance heuristic against the cosine-sampled direction
+    const float p_light = light_pdf_area(sc, sc.materials[e.material]) * dist2 / cos_y;
+    const float p_bsdf = cos_x / kPi;
+    const float g = cos_x * cos_y / dist2 * area * total / w_tri / kPi * (p_light / (p_light + p_bsdf));
+    Candidate c;
+    scene_intersect_ref(sc, loc + wl * kTolerance, wl, c);
+    visible = !(c.inst >= 0 && c.t < dist - 2.f * kTolerance);
+    return cmul(cmul(T, base), lme.emissive) * g;
+}
+
 static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, const orcl_frame &fr, Rng &rng,
-                                float u, float v, uint64_t &nrays) {
+                                float u, float v, uint64_t &nrays, Rng *nee_rng = nullptr) {
     // SetupViewRay (CPURenderer.cpp:220-238)
     float nx, ny;
     rng.disk(nx, ny);
@@ -1318,6 +1416,9 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
     V3 T = kOnes;
     PixelResult res;
     unsigned bounce = 0;
+    bool skip_emission = false;  // the previous vertex did next-event estimation
+    float prev_pdf = 0.f;
+    const bool nee = nee_rng && !sc.emitters.empty();
     for (; bounce < fr.max_bounce; bounce++) {
         Hit h = scene_hit(sc, o, d);
         nrays++;
@@ -1333,11 +1434,18 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
         const MatEval me = eval_material(sc, mat, h.uvx, h.uvy);
         if (bounce == 0) res.normal = h.normal;
         if (norm(me.emissive) > kEps) {
-            res.color = res.color + cmul(me.emissive, T);
+            if (!skip_emission) {
+                res.color = res.color + cmul(me.emissive, T);
+            } else {
+                const float cos_y = std::fabs(dot(h.gnormal, d)) / std::sqrt(sqnorm(h.gnormal));
+                const float p_light = light_pdf_area(sc, mat) * h.t * h.t / cos_y;
+                if (cos_y > 0.f) res.color = res.color + cmul(me.emissive, T) * (prev_pdf / (prev_pdf + p_light));
+            }
             break;
         }
         V3 wi;
-        V3 f = sample_surface(rng, mat, me, d, h.normal, h.tangent, wi);
+        bool diffuse = false;
+        V3 f = sample_surface(rng, mat, me, d, h.normal, h.tangent, wi, &diffuse);
         switch (fr.debug_mode) {
             case 3:  // Normal
                 res.color = (h.normal + kOnes) * 0.5f;
@@ -1363,6 +1471,14 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
             default:
                 break;
         }
+        const bool last = bounce + 1 >= fr.max_bounce;
+        if (nee && diffuse && !last) {
+            bool visible = false;
+            const V3 contrib = nee_sample(sc, *nee_rng, h.location, h.normal, me.base, T, visible);
+            if (visible) res.color = res.color + contrib;
+        }
+        skip_emission = nee && diffuse;
+        if (skip_emission) prev_pdf = std::max(dot(wi, h.normal), 0.f) / kPi;
         T = cmul(T, f);
         if (sqnorm(T) < kEps) {
             res.valid = -1.f;
@@ -1403,13 +1519,15 @@ int orcl_render_frame(const orcl_scene *scene, const orcl_camera *camera, const 
                     seed = (uint64_t)j * fr.width + i + (uint64_t)fr.frame_seed * fr.width * fr.height;
                 }
                 Rng rng(seed);
+                Rng nee_rng(seed ^ 0x5bd1e9955bd1e995ull);
                 float u = (static_cast<float>(i) + rng.unit()) * pw;
                 float v = (static_cast<float>(j) + rng.unit()) * ph;
-                PixelResult r = sample_pixel(*scene, cam, fr, rng, u, v, nrays);
+                PixelResult r = sample_pixel(*scene, cam, fr, rng, u, v, nrays, fr.nee ? &nee_rng : nullptr);
                 const size_t p = (size_t)j * fr.width + i;
-                rgba[4 * p + 0] = std::min(r.color.x, kOutputLimit);
-                rgba[4 * p + 1] = std::min(r.color.y, kOutputLimit);
-                rgba[4 * p + 2] = std::min(r.color.z, kOutputLimit);
+                const float limit = fr.output_limit > 0.f ? fr.output_limit : kOutputLimit;
+                rgba[4 * p + 0] = std::min(r.color.x, limit);
+                rgba[4 * p + 1] = std::min(r.color.y, limit);
+                rgba[4 * p + 2] = std::min(r.color.z, limit);
                 rgba[4 * p + 3] = r.valid;
                 if (normal) {
                     normal[3 * p + 0] = r.normal.x;

@@ -118,6 +118,8 @@ typedef struct orcl_frame {
     uint32_t sky_enabled;
     float sky_color[3];
     uint32_t seed_hi; /* high 32 bits of a 64-bit pixel seed (0 = reference uint32 wrap) */
+    uint32_t nee;      /* next-event estimation as the device's SPK_FRAME_NEE (no reference counterpart) */
+    float output_limit; /* per-sample clamp; 0 = the reference's 6 */
 } orcl_frame;
 
 enum {

@@ -53,7 +53,8 @@ class OCamera(ctypes.Structure):
 class OFrame(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32),
                 ("max_bounce", ctypes.c_uint32), ("debug_mode", ctypes.c_uint32), ("sky_enabled", ctypes.c_uint32),
-                ("sky_color", ctypes.c_float * 3), ("seed_hi", ctypes.c_uint32)]
+                ("sky_color", ctypes.c_float * 3), ("seed_hi", ctypes.c_uint32), ("nee", ctypes.c_uint32),
+                ("output_limit", ctypes.c_float)]
 
 
 class OCacheConfig(ctypes.Structure):
@@ -196,9 +197,10 @@ class OracleScene:
         return hits
 
     def render_frame(self, cam, width, height, frame_seed=0, max_bounce=4, debug_mode=0, sky=None, seed64=False,
-                     threads=None):
+                     threads=None, nee=False, output_limit=0.0):
         fr = OFrame(width, height, frame_seed, max_bounce, debug_mode, 1 if sky is not None else 0,
-                    (ctypes.c_float * 3)(*(sky if sky is not None else (0, 0, 0))), 1 if seed64 else 0)
+                    (ctypes.c_float * 3)(*(sky if sky is not None else (0, 0, 0))), 1 if seed64 else 0,
+                    1 if nee else 0, output_limit)
         oc = OCamera()
         for f in ("position", "right", "up", "front", "lower_left", "max_u", "max_v"):
             getattr(oc, f)[:] = list(getattr(cam, f))

@@ -51,7 +51,7 @@ enum : uint32_t {
     GCTR_TARGETS = 3,   // target work fetch
 };
 enum : uint32_t {
-    PF_SKIP_EMISSION = 1u,  // the previous vertex sampled the emitters directly
+    PF_NEE_PREV = 1u,  // the previous vertex sampled the emitters directly (MIS-weight emission hits)
     PF_TRAIN = 2u,          // a training path of the radiance cache
 };
 
@@ -84,6 +84,7 @@ struct PtArgs {
     float emitter_total;
     float4 *rays;      // 2 per path: (org, tmin), (dir, tmax)
     float4 *thr;       // throughput rgb, w = path flags (PF_* bits)
+    float *prev_pdf;   // solid-angle pdf of the direction sampled at the previous (diffuse) vertex
     float4 *col;       // colour rgb, w = valid flag
     uint4 *rng;        // xoshiro128+ state per path
     uint4 *nee_rng;    // second stream for light sampling
@@ -357,6 +358,12 @@ __device__ __forceinline__ float4 lerp_attr(const float *p0, const float *p1, co
                        p0[3] + (p1[3] - p0[3]) * u + (p2[3] - p0[3]) * v);
 }
 
+// Area pdf with which next-event estimation picks a point of an emitter of this material: triangles
+// are drawn by area x mean emitted radiance (factor), points uniformly, so the pdf is lum / total.
+__device__ __forceinline__ float light_pdf_area(const PtArgs &a, const MaterialGpu &m) {
+    return (m.emissive[0] + m.emissive[1] + m.emissive[2]) / 3.f / a.emitter_total;
+}
+
 // Material getters at a uv (MaterialRenderProxy.h:117-162)
 struct MatEval {
     F3 base, emissive;
@@ -448,7 +455,21 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                 const bool cache_vertex = a.cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
                                           me.metallic < 0.5f;
                 if (me.emits) {
-                    if (!(pf & PF_SKIP_EMISSION)) C = C + cmul(me.emissive, T);
+                    if (pf & PF_NEE_PREV) {
+                        // the previous vertex also sampled this emitter: balance-heuristic weight of
+                        // the BSDF-sampled direction against light sampling of this point
+                        const uint4 *tp = a.tris + (size_t)h.x * 3u;
+                        const uint4 va = tp[0], vb = tp[1], vc = tp[2];
+                        const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
+                        const F3 ng = cross(f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z)) - p0,
+                                            f3(__uint_as_float(vc.x), __uint_as_float(vc.y), __uint_as_float(vc.z)) - p0);
+                        const float cos_y = fabsf(dot(ng, d)) / sqrtf(sqnorm(ng));
+                        const float p_light = light_pdf_area(a, mat) * t * t / cos_y;
+                        const float p_bsdf = a.prev_pdf[path];
+                        if (cos_y > 0.f) C = C + cmul(me.emissive, T) * (p_bsdf / (p_bsdf + p_light));
+                    } else {
+                        C = C + cmul(me.emissive, T);
+                    }
                 } else if (cache_vertex && !(pf & PF_TRAIN)) {
                     // terminate into the cache: radiance leaving this vertex comes from the network
                     cache_q = true;
@@ -525,9 +546,13 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                                                           es.uv0[1] + (es.uv1[1] - es.uv0[1]) * bu +
                                                               (es.uv2[1] - es.uv0[1]) * bv);
                         if (lme.emits && cos_x > 0.f && cos_y > 0.f && dist > 2e-4f && w_tri > 0.f) {
-                            // pdf over area: (w_tri / total) / area; contribution f*cos_x*Le*cos_y/(d^2 pdf)
+                            // pdf over area: (w_tri / total) / area; contribution f*cos_x*Le*cos_y/(d^2 pdf),
+                            // weighted by the balance heuristic against the cosine-sampled direction
                             const float area = 0.5f * area2;
-                            const float g = cos_x * cos_y / dist2 * area * a.emitter_total / w_tri / SPK_PI;
+                            const float p_light = light_pdf_area(a, em) * dist2 / cos_y;
+                            const float p_bsdf = cos_x / SPK_PI;
+                            const float w_mis = p_light / (p_light + p_bsdf);
+                            const float g = cos_x * cos_y / dist2 * area * a.emitter_total / w_tri / SPK_PI * w_mis;
                             const F3 base3 = me.base;
                             const F3 Le = lme.emissive;
                             const F3 contrib = cmul(cmul(T, base3), Le) * g;
@@ -552,7 +577,8 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                             if (alive && !last) {
                                 cont = true;
                                 const F3 o2 = loc + wi * SPK_TOLERANCE;
-                                const uint32_t npf = (pf & PF_TRAIN) | ((nee && diffuse) ? PF_SKIP_EMISSION : 0u);
+                                const uint32_t npf = (pf & PF_TRAIN) | ((nee && diffuse) ? PF_NEE_PREV : 0u);
+                                if (nee && diffuse) a.prev_pdf[path] = fmaxf(dot(wi, normal), 0.f) / SPK_PI;
                                 a.rays[2 * path] = make_float4(o2.x, o2.y, o2.z, SPK_EPS);
                                 a.rays[2 * path + 1] = make_float4(wi.x, wi.y, wi.z, INFINITY);
                                 a.thr[path] = make_float4(T.x, T.y, T.z, __uint_as_float(npf));
@@ -765,6 +791,7 @@ const PtEntry kPtEntries[] = {
 
 struct PathState {
     DeviceBuffer rays, thr, col, rng, nee_rng, queue0, queue1, hits, sorted, shadow, counters, hist, path_scene;
+    DeviceBuffer prev_pdf;
     DeviceBuffer emitters, emitter_cdf, frame;
     FrameDev *frame_host = nullptr;  // pinned
     uint32_t capacity = 0;
@@ -830,7 +857,7 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
             (e = ps.nee_rng.ensure(n * 16)) != hipSuccess || (e = ps.queue0.ensure(n * 4)) != hipSuccess ||
             (e = ps.queue1.ensure(n * 4)) != hipSuccess || (e = ps.hits.ensure(n * 16)) != hipSuccess ||
             (e = ps.sorted.ensure(n * 4)) != hipSuccess || (e = ps.shadow.ensure(n * 48)) != hipSuccess ||
-            (e = ps.path_scene.ensure(n * 4)) != hipSuccess)
+            (e = ps.path_scene.ensure(n * 4)) != hipSuccess || (e = ps.prev_pdf.ensure(n * 4)) != hipSuccess)
             return hip_fail(ctx, e, "path buffers");
         ps.capacity = npix;
     }
@@ -1002,6 +1029,7 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     a.emitter_total = ps.emitter_total;
     a.rays = (float4 *)ps.rays.ptr;
     a.thr = (float4 *)ps.thr.ptr;
+    a.prev_pdf = (float *)ps.prev_pdf.ptr;
     a.col = (float4 *)ps.col.ptr;
     a.rng = (uint4 *)ps.rng.ptr;
     a.nee_rng = (uint4 *)ps.nee_rng.ptr;

@@ -273,3 +273,31 @@ def test_normal_map_tilts_the_shading_normal():
     want = T * tn[0] + B * tn[1] + N * tn[2]
     want /= np.linalg.norm(want)
     np.testing.assert_allclose(nrm.reshape(-1, 3), np.tile(want, (w * h, 1)), atol=1e-6)
+
+
+def test_oracle_nee_is_unbiased_and_reduces_variance():
+    """The oracle's restatement of next-event estimation with the balance heuristic converges to the
+    BSDF-only estimator (flux per frame, z-score over 2048 frames each) with lower per-pixel variance;
+    without the MIS weights the light 1 cm under the ceiling made it 12x noisier."""
+    w = h = 32
+    sc = scenes.cornell_scene()
+    cam = scenes.cornell_camera(w, h)
+    o = refcpu.OracleScene(sc)
+    stats = {}
+    for nee in (False, True):
+        flux, var = [], 0.0
+        n = 2048
+        acc = np.zeros((h, w, 3))
+        acc2 = np.zeros((h, w, 3))
+        for k in range(n):
+            img, _ = o.render_frame(cam, w, h, frame_seed=k + (10 ** 6 if nee else 0), max_bounce=6, nee=nee,
+                                    output_limit=1e30, seed64=True)
+            x = img[..., :3].astype(np.float64)
+            flux.append(x.mean())
+            acc += x
+            acc2 += x * x
+        m = acc / n
+        stats[nee] = (np.mean(flux), np.std(flux, ddof=1) / np.sqrt(n), (acc2 / n - m * m).mean())
+    z = (stats[True][0] - stats[False][0]) / np.hypot(stats[True][1], stats[False][1])
+    assert abs(z) < 3.5, (z, stats)
+    assert stats[True][2] < 0.5 * stats[False][2], stats

@@ -168,25 +168,27 @@ def tile_means(frames, t=4):
 
 
 def test_nee_is_unbiased_against_bsdf_sampling(gpu_ctx):
-    """Next-event estimation (not in the reference) must converge to the same image as the
-    reference's BSDF-only estimator. Compared on 4x4-pixel tile means over 512 frames: single pixels
-    of the BSDF-only estimator are too heavy-tailed (rare light hits) for a 3-sigma test at this
-    sample count, tile means are not. The per-sample clamp is lifted: clamping at 6 cuts the two
-    estimators' samples differently."""
+    """Next-event estimation with the balance heuristic (not in the reference) must converge to the
+    same image as the reference's BSDF-only estimator: total flux per frame within 3.5 standard
+    errors over 1024 frames each, 4x4-pixel tile means within 3 sigma on >= 97% of tiles (single
+    pixels of the BSDF-only estimator are too heavy-tailed for a per-pixel test), and lower variance.
+    The per-sample clamp is lifted: clamping at 6 cuts the two estimators' samples differently."""
     w = h = 32
-    frames = 512
+    frames = 1024
     sc = scenes.cornell_scene()
     r = Renderer(gpu_ctx, sc, scenes.cornell_camera(w, h), w, h)
-    plain = tile_means(np.stack([r.render(k, bounce=6, limit=1e30)[0][..., :3] for k in range(frames)]))
-    nee = tile_means(np.stack([r.render(k, bounce=6, flags=spk.FRAME_NEE, limit=1e30)[0][..., :3]
-                               for k in range(frames)]))
-    mp, mn = plain.mean(0), nee.mean(0)
-    se = np.sqrt(plain.var(0, ddof=1) / frames + nee.var(0, ddof=1) / frames)
-    z = np.abs(mp - mn) / (se + 1e-9)
-    print("tiles within 3 sigma", (z <= 3).mean(), "max z", z.max(), "flux", mp.mean(), mn.mean())
-    assert (z <= 3).mean() >= 0.98 and z.max() < 5, (z <= 3).mean()
-    assert abs(mp.mean() - mn.mean()) / mp.mean() < 0.015
-    # and it has lower variance
+    plain = np.stack([r.render(k, bounce=6, limit=1e30)[0][..., :3] for k in range(frames)]).astype(np.float64)
+    nee = np.stack([r.render(k + 10 ** 6, bounce=6, flags=spk.FRAME_NEE | spk.FRAME_SEED64, limit=1e30)[0][..., :3]
+                    for k in range(frames)]).astype(np.float64)
+    fp, fn = plain.mean((1, 2, 3)), nee.mean((1, 2, 3))
+    zf = (fn.mean() - fp.mean()) / np.hypot(fn.std(ddof=1), fp.std(ddof=1)) * math.sqrt(frames)
+    tp, tn = tile_means(plain), tile_means(nee)
+    se = np.sqrt(tp.var(0, ddof=1) / frames + tn.var(0, ddof=1) / frames)
+    z = np.abs(tn.mean(0) - tp.mean(0)) / (se + 1e-9)
+    print("flux", fp.mean(), fn.mean(), "z", zf, "tiles within 3 sigma", (z <= 3).mean(), "variance",
+          plain.var(0).mean(), nee.var(0).mean())
+    assert abs(zf) < 3.5
+    assert (z <= 3).mean() >= 0.97, (z <= 3).mean()
     assert nee.var(0).mean() < 0.5 * plain.var(0).mean()
 
 
