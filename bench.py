@@ -281,7 +281,7 @@ def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
         f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, 0, 0.0
         return f
 
-    blocks, srt, tune_ms = ctx.autotune_batch(frame(0), rgba)
+    (blocks, srt, refill), tune_ms = ctx.autotune_batch(frame(0), rgba)
     torch.cuda.synchronize()
     rays = 0
     t0 = time.perf_counter()
@@ -303,7 +303,8 @@ def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
         "seconds": round(dt, 3),
         "samples_per_s": round(npx * spp / dt, 1),
         "mrays_per_s": round(rays / dt / 1e6, 1),
-        "schedule": {"blocks_per_cu": blocks, "material_sort": bool(srt), "tuned_frame_ms": round(tune_ms, 3)},
+        "schedule": {"blocks_per_cu": blocks, "material_sort": bool(srt), "refill": refill,
+                     "tuned_frame_ms": round(tune_ms, 3)},
         "build_s": round(t_build, 2),
         "finite": finite,
     }

@@ -296,10 +296,24 @@ int spk_batch_upload(spk_context *ctx, const spk_batch_scene *scenes, uint32_t n
 int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_frame_stats *stats);
 /* num_scenes + 1 pixel offsets (the last is the total pixel count). */
 int spk_batch_offsets(spk_context *ctx, uint64_t *offsets, uint32_t max_count);
-/* Schedule autotuner for batches: times the persistent grid (blocks per CU) and the material sort
-   of the bounce queues (on/off) on one frame and keeps the fastest; results never depend on either. */
-int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, uint32_t *blocks_per_cu,
-                       uint32_t *sorted, double *best_ms);
+/* Launch schedule of the path tracer's persistent kernels. Images never depend on it. */
+typedef struct spk_schedule {
+    uint32_t blocks_per_cu; /* persistent grid: 128-thread blocks per CU, 1..32 (default 8) */
+    uint32_t material_sort; /* 1: shade each bounce in material order (counting sort; default 1) */
+    uint32_t refill;        /* extend: a wave takes new paths once this many of its 64 lanes are
+                               idle, 1..64 (default 16; 64 = whole waves) */
+    uint32_t reserved;
+} spk_schedule;
+
+/* batch = 0: the schedule of spk_render_frame; batch != 0: of spk_batch_render. */
+int spk_get_schedule(spk_context *ctx, int batch, spk_schedule *out);
+int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule);
+/* Schedule autotuners: time blocks_per_cu {2, 4, 8, 16} x material_sort {0, 1} x refill {16, 64} on
+   one frame (after a capture frame per candidate), keep and return the fastest. */
+int spk_render_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,
+                        double *best_ms);
+int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,
+                       double *best_ms);
 
 /* ---- learned radiance cache (no counterpart in the reference) ---------------------------- */
 /* A multiresolution hash-grid encoding (levels x 2 features, 2^log2_table FP16 entries per level)

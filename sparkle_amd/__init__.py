@@ -135,6 +135,14 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("rays_traced", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("material_sort", ctypes.c_uint32), ("refill", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+    def as_tuple(self):
+        return (self.blocks_per_cu, self.material_sort, self.refill)
+
+
 class BatchScene(ctypes.Structure):
     _fields_ = [("meshes", ctypes.c_void_p), ("num_meshes", ctypes.c_uint32), ("instances", ctypes.c_void_p),
                 ("num_instances", ctypes.c_uint32), ("materials", ctypes.c_void_p), ("num_materials", ctypes.c_uint32),
@@ -184,8 +192,12 @@ def load_library():
         "spk_batch_upload": (I, [P, P, U32, ctypes.POINTER(BuildOptions)]),
         "spk_batch_render": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(FrameStats)]),
         "spk_batch_offsets": (I, [P, ctypes.POINTER(U64), U32]),
-        "spk_batch_autotune": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(U32), ctypes.POINTER(U32),
+        "spk_batch_autotune": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(Schedule),
                                    ctypes.POINTER(ctypes.c_double)]),
+        "spk_render_autotune": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(Schedule),
+                                    ctypes.POINTER(ctypes.c_double)]),
+        "spk_get_schedule": (I, [P, I, ctypes.POINTER(Schedule)]),
+        "spk_set_schedule": (I, [P, I, ctypes.POINTER(Schedule)]),
         "spk_cache_create": (I, [P, ctypes.POINTER(CacheConfig)]),
         "spk_cache_destroy": (I, [P]),
         "spk_cache_reset": (I, [P]),
@@ -465,10 +477,27 @@ class Context:
         return st
 
     def autotune_batch(self, frame, rgba_dev):
-        b, srt, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
-        self._check(self.lib.spk_batch_autotune(self.h, ctypes.byref(frame), _ptr(rgba_dev), ctypes.byref(b),
-                                                ctypes.byref(srt), ctypes.byref(ms)))
-        return b.value, srt.value, ms.value
+        """spk_batch_autotune: returns ((blocks_per_cu, material_sort, refill), best frame ms)."""
+        sc, ms = Schedule(), ctypes.c_double()
+        self._check(self.lib.spk_batch_autotune(self.h, ctypes.byref(frame), _ptr(rgba_dev), ctypes.byref(sc),
+                                                ctypes.byref(ms)))
+        return sc.as_tuple(), ms.value
+
+    def autotune_render(self, frame, rgba_dev):
+        """spk_render_autotune: returns ((blocks_per_cu, material_sort, refill), best frame ms)."""
+        sc, ms = Schedule(), ctypes.c_double()
+        self._check(self.lib.spk_render_autotune(self.h, ctypes.byref(frame), _ptr(rgba_dev), ctypes.byref(sc),
+                                                 ctypes.byref(ms)))
+        return sc.as_tuple(), ms.value
+
+    def schedule(self, batch=False):
+        sc = Schedule()
+        self._check(self.lib.spk_get_schedule(self.h, 1 if batch else 0, ctypes.byref(sc)))
+        return sc.as_tuple()
+
+    def set_schedule(self, blocks_per_cu, material_sort, refill, batch=False):
+        sc = Schedule(blocks_per_cu, material_sort, refill, 0)
+        return self._check(self.lib.spk_set_schedule(self.h, 1 if batch else 0, ctypes.byref(sc)))
 
     def accumulate(self, rgba_dev, accum_dev, num_pixels, cumulated, spp=1):
         return self._check(self.lib.spk_accumulate(self.h, _ptr(rgba_dev), _ptr(accum_dev), num_pixels,

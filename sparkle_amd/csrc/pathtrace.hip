@@ -825,8 +825,7 @@ struct BatchState {
     std::vector<uint64_t> offsets;  // num_scenes + 1 pixel offsets
     uint32_t num_scenes = 0;
     uint32_t npix = 0;
-    uint32_t blocks_per_cu = 8;
-    uint32_t sort = 1;
+    spk_schedule schedule{8u, 1u, 16u, 0u};
     uint64_t num_tris = 0;
 };
 
@@ -1196,9 +1195,9 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     src.single.cam = ctx->camera;
     src.num_scenes = 1u;
     src.npix = frame->width * frame->height;
-    src.blocks_per_cu = 8u;
-    src.sort = 1u;
-    src.refill = 16u;
+    src.blocks_per_cu = ctx->schedule.blocks_per_cu;
+    src.sort = ctx->schedule.material_sort;
+    src.refill = ctx->schedule.refill;
     return run_frame(ctx, frame, src, rgba_dev, normal_dev, stats);
 }
 
@@ -1324,42 +1323,86 @@ int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     src.scenes_dev = (const SceneRec *)bs.table.ptr;
     src.num_scenes = bs.num_scenes;
     src.npix = bs.npix;
-    src.blocks_per_cu = bs.blocks_per_cu;
-    src.sort = bs.sort;
-    src.refill = 16u;
+    src.blocks_per_cu = bs.schedule.blocks_per_cu;
+    src.sort = bs.schedule.material_sort;
+    src.refill = bs.schedule.refill;
     return run_frame(ctx, frame, src, rgba_dev, nullptr, stats);
 }
 
-int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, uint32_t *blocks_per_cu,
-                       uint32_t *sorted, double *best_ms) {
-    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
-    if (!ctx->batch) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
-    BatchState &bs = *ctx->batch;
+static int check_schedule(spk_context *ctx, const spk_schedule &sc) {
+    if (sc.blocks_per_cu < 1 || sc.blocks_per_cu > 32 || sc.material_sort > 1 || sc.refill < 1 || sc.refill > 64)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "schedule out of range (blocks 1..32, sort 0/1, refill 1..64)");
+    return SPK_OK;
+}
+
+static spk_schedule *schedule_of(spk_context *ctx, int batch) {
+    if (!batch) return &ctx->schedule;
+    return ctx->batch ? &ctx->batch->schedule : nullptr;
+}
+
+int spk_get_schedule(spk_context *ctx, int batch, spk_schedule *out) {
+    if (!ctx || !out) return SPK_ERR_INVALID_ARGUMENT;
+    spk_schedule *sc = schedule_of(ctx, batch);
+    if (!sc) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
+    *out = *sc;
+    return SPK_OK;
+}
+
+int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule) {
+    if (!ctx || !schedule) return SPK_ERR_INVALID_ARGUMENT;
+    spk_schedule *sc = schedule_of(ctx, batch);
+    if (!sc) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
+    int rc = check_schedule(ctx, *schedule);
+    if (rc != SPK_OK) return rc;
+    *sc = *schedule;
+    sc->reserved = 0u;
+    return SPK_OK;
+}
+
+// Times every candidate schedule on one frame (a capture frame first, then a timed one).
+static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *frame, float *rgba_dev,
+                             spk_schedule *chosen, double *best_ms) {
+    spk_schedule *target = schedule_of(ctx, batch);
+    if (!target) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
     static const uint32_t kBlocks[] = {2u, 4u, 8u, 16u};
+    static const uint32_t kRefill[] = {16u, 64u};
     double best = 1e30;
-    uint32_t best_b = 8u, best_s = 1u;
+    spk_schedule best_s = *target;
     for (uint32_t srt = 0; srt < 2; ++srt) {
         for (uint32_t b : kBlocks) {
-            bs.blocks_per_cu = b;
-            bs.sort = srt;
-            spk_frame_stats st{};
-            // first frame captures the graph, the second is timed
-            int rc = spk_batch_render(ctx, frame, rgba_dev, nullptr);
-            if (rc == SPK_OK) rc = spk_batch_render(ctx, frame, rgba_dev, &st);
-            if (rc != SPK_OK) return rc;
-            if (st.milliseconds < best) {
-                best = st.milliseconds;
-                best_b = b;
-                best_s = srt;
+            for (uint32_t r : kRefill) {
+                *target = spk_schedule{b, srt, r, 0u};
+                spk_frame_stats st{};
+                int rc = batch ? spk_batch_render(ctx, frame, rgba_dev, nullptr)
+                               : spk_render_frame(ctx, frame, rgba_dev, nullptr, nullptr);
+                if (rc == SPK_OK)
+                    rc = batch ? spk_batch_render(ctx, frame, rgba_dev, &st)
+                               : spk_render_frame(ctx, frame, rgba_dev, nullptr, &st);
+                if (rc != SPK_OK) return rc;
+                if (st.milliseconds < best) {
+                    best = st.milliseconds;
+                    best_s = *target;
+                }
             }
         }
     }
-    bs.blocks_per_cu = best_b;
-    bs.sort = best_s;
-    if (blocks_per_cu) *blocks_per_cu = best_b;
-    if (sorted) *sorted = best_s;
+    *target = best_s;
+    if (chosen) *chosen = best_s;
     if (best_ms) *best_ms = best;
     return SPK_OK;
+}
+
+int spk_render_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,
+                        double *best_ms) {
+    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
+    return autotune_schedule(ctx, 0, frame, rgba_dev, chosen, best_ms);
+}
+
+int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,
+                       double *best_ms) {
+    if (!ctx || !frame) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->batch) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
+    return autotune_schedule(ctx, 1, frame, rgba_dev, chosen, best_ms);
 }
 
 int spk_accumulate(spk_context *ctx, const float *rgba_dev, float *accum_dev, uint64_t num_pixels,

@@ -80,6 +80,7 @@ struct spk_context {
     spk_scene_stats stats{};
     spk_camera_desc camera{};
     bool has_camera = false;
+    spk_schedule schedule{8u, 1u, 16u, 0u};  // spk_render_frame's launch schedule
     spk::PathState *paths = nullptr;
     spk::CacheState *cache = nullptr;
     spk::BatchState *batch = nullptr;

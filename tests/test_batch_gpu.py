@@ -99,8 +99,9 @@ def test_batch_schedule_does_not_change_results():
         ctx.render_batch(f, rgba)
         ref = rgba.cpu().numpy().copy()
         print_case("batch autotune")
-        blocks, srt, ms = ctx.autotune_batch(f, rgba)
-        assert blocks in (2, 4, 8, 16) and srt in (0, 1) and ms > 0
+        (blocks, srt, refill), ms = ctx.autotune_batch(f, rgba)
+        assert blocks in (2, 4, 8, 16) and srt in (0, 1) and refill in (16, 64) and ms > 0
+        assert ctx.schedule(batch=True) == (blocks, srt, refill)
         ctx.render_batch(f, rgba)
         assert np.array_equal(rgba.cpu().numpy(), ref)
         print_case("batch unsorted")

@@ -120,6 +120,28 @@ def test_material_sort_does_not_change_results(gpu_ctx):
     assert np.array_equal(a, b)
 
 
+def test_schedule_and_autotune_do_not_change_results(gpu_ctx):
+    """Every launch schedule (grid size, material sort, lane refill) gives the same image; the
+    autotuner keeps one of its candidates; out-of-range schedules are rejected."""
+    w, h = 40, 30
+    r = Renderer(gpu_ctx, scenes.cornell_scene(), scenes.cornell_camera(w, h), w, h)
+    ref, _, _ = r.render(9, bounce=6)
+    for sched in ((2, 0, 1), (16, 1, 64), (4, 1, 16), (1, 0, 33)):
+        print_case("schedule", sched)
+        gpu_ctx.set_schedule(*sched)
+        img, _, _ = r.render(9, bounce=6)
+        np.testing.assert_array_equal(img, ref)
+    (b, srt, refill), ms = gpu_ctx.autotune_render(frame(w, h, 9, bounce=6), r.rgba)
+    assert b in (2, 4, 8, 16) and srt in (0, 1) and refill in (16, 64) and ms > 0
+    img, _, _ = r.render(9, bounce=6)
+    np.testing.assert_array_equal(img, ref)
+    with pytest.raises(spk.SpkError):
+        gpu_ctx.set_schedule(0, 1, 16)
+    with pytest.raises(spk.SpkError):
+        gpu_ctx.set_schedule(8, 1, 65)
+    gpu_ctx.set_schedule(8, 1, 16)
+
+
 def test_sky_and_open_scene(gpu_ctx):
     w, h = 48, 32
     sc = scenes.heightfield_scene(cells=64)
