@@ -258,6 +258,51 @@ def cache_kernel_bench(spk, scenes, stream, n_infer=1920 * 1080, n_train=1 << 16
             "bound": "hash-grid gathers (16 levels x 8 corners x 4 B per query) and table-gradient atomics"}
 
 
+def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_bounce=8, autotune=True):
+    """The full plain path trace of the many-instance scene (about 5.5M triangles, 400 instances,
+    3 area lights): spp frames of one path per pixel, max_bounce bounces, Russian roulette from
+    bounce 3, material-sorted queues, the frame graph replayed; schedule from spk_render_autotune."""
+    import torch
+
+    log(f"render experiment: many-instance scene {width}x{height}, {spp} spp")
+    sc, cam = scenes.many_instance_scene(seed=0, aspect=width / height)
+    ctx = spk.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    t0 = time.perf_counter()
+    st = ctx.upload_scene(sc)
+    t_build = time.perf_counter() - t0
+    ctx.set_camera(cam)
+    npx = width * height
+    rgba = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+    acc = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+
+    def frame(seed):
+        f = spk.FrameDesc()
+        f.width, f.height, f.frame_seed, f.max_bounce = width, height, seed, max_bounce
+        f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, 0, 0.0
+        return f
+
+    sched = ctx.schedule()
+    if autotune:
+        sched, _ = ctx.autotune_render(frame(10 ** 6), rgba)
+    ctx.render_frame(frame(10 ** 6 + 1), rgba)
+    torch.cuda.synchronize()
+    rays = 0
+    t0 = time.perf_counter()
+    for k in range(spp):
+        rays += ctx.render_frame(frame(k), rgba).rays_traced
+        ctx.accumulate(rgba, acc, npx, k, 1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return {"scene": f"many-instance generator seed 0: {st.num_triangles} triangles, {width}x{height}, "
+                     f"max {max_bounce} bounces", "spp": spp, "seconds": round(dt, 3),
+            "ms_per_frame": round(dt / spp * 1e3, 3), "samples_per_s": round(npx * spp / dt, 1),
+            "mrays_per_s": round(rays / dt / 1e6, 1), "rays_per_sample": round(rays / (npx * spp), 3),
+            "schedule": {"blocks_per_cu": sched[0], "material_sort": bool(sched[1]), "refill": sched[2]},
+            "bvh": {"width": st.bvh_width, "depth": st.depth, "nodes": st.num_nodes, "build_s": round(t_build, 2)}}
+
+
 def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
     """Many scenes of uneven size in one launch: `count` many-instance scenes (triangle budgets
     log-uniform in [1k, 500k], images 64^2..512^2), `spp` frames of `max_bounce` bounces each, all
@@ -326,6 +371,8 @@ def main():
     ap.add_argument("--relmse", type=str, default="1920x1080,64,4096",
                     help="equal-time relMSE experiment WxH,spp,reference-spp ('off' to skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
+    ap.add_argument("--render-only", type=int, default=0,
+                    help="only run the full path trace of the many-instance scene for N spp (one JSON line)")
     ap.add_argument("--cache-kernels", action="store_true",
                     help="only time the learned cache's inference and training kernels (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -341,6 +388,15 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel(), "copy_bytes_written": b.numel(),
                           "copies": args.copy_calibration}))
+        return
+    if args.render_only:
+        import torch
+        import sparkle_amd as spk
+        from sparkle_amd import scenes
+
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        print(json.dumps({"full_path": render_experiment(spk, scenes, stream, spp=args.render_only)}), flush=True)
         return
     if args.cache_kernels:
         import torch
@@ -493,6 +549,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     ctx.close()
     out["cache_kernels"] = cache_kernel_bench(spk, scenes, stream)
+    out["full_path"] = render_experiment(spk, scenes, stream)
     if not args.no_batch:
         out["batch"] = batch_experiment(spk, scenes, stream)
     if args.relmse != "off":
