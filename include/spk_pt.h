@@ -203,6 +203,21 @@ int spk_scene_upload(spk_context *ctx, const spk_mesh_desc *meshes, uint32_t num
                      const spk_material_desc *materials, uint32_t num_materials,
                      const spk_build_options *options);
 int spk_scene_stats_get(spk_context *ctx, spk_scene_stats *out);
+/* Analytic sphere (SpherePrimitive / SphereRenderProxy.cpp:5-48, SphereRenderProxy.h:22-36):
+   hit t = -half_b -+ sqrt(half_b^2 - c) with half_b = d.(o - center), c = |o - center|^2 - r^2
+   (near root from outside, far root from inside; ray directions are taken as unit length, as in
+   the reference), normal (p - center)/|p - center|, tangent the normal's major axis, uv 0. */
+typedef struct spk_sphere_desc {
+    float center[3];
+    float radius;
+    uint32_t material;
+    uint32_t reserved[3];
+} spk_sphere_desc;
+
+/* Replaces the uploaded scene's spheres (none after spk_scene_upload). Sphere k has the global
+   primitive id num_triangles + k in hit records. */
+int spk_scene_set_spheres(spk_context *ctx, const spk_sphere_desc *spheres, uint32_t num_spheres);
+
 /* Replaces the context's texture table (shared by the scene and batch uploads that follow; a
    material naming a texture index beyond the table is rejected at upload). Call before
    spk_scene_upload / spk_batch_upload; changing it invalidates an uploaded scene. */

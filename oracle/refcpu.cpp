@@ -533,6 +533,11 @@ struct Texture {
 }  // namespace orcl
 
 namespace orcl {
+struct Sphere {
+    float c[3];
+    float r2;  // scaled_radius_2_ (SphereRenderProxy.h:29-32)
+    uint32_t material;
+};
 struct Emitter {  // world-space emissive triangle for next-event estimation
     float p[9];
     float uv[6];
@@ -546,6 +551,7 @@ struct orcl_scene {
     std::vector<orcl_material> materials;
     std::vector<orcl::Texture> textures;
     std::vector<orcl::Emitter> emitters;  // instance order, face order
+    std::vector<orcl::Sphere> spheres;
     std::vector<float> emitter_cdf;       // running sum of area x mean emitted radiance
     orcl::Bvh tlas;
     uint64_t total_tris = 0;
@@ -558,8 +564,21 @@ struct Candidate {
     float u = 0, v = 0;
     uint32_t face = 0;
     int inst = -1;
+    int sphere = -1;
     uint64_t gid = ~0ull;
 };
+
+// SphereRenderProxy::IntersectInternal (SphereRenderProxy.cpp:5-38): a = 1 for unit directions
+static inline bool sphere_t(const Sphere &s, V3 o, V3 d, float &t) {
+    const float cx = o.x - s.c[0], cy = o.y - s.c[1], cz = o.z - s.c[2];
+    const float half_b = (d.x * cx + d.y * cy) + d.z * cz;
+    const float c = ((cx * cx + cy * cy) + cz * cz) - s.r2;
+    const float disc = half_b * half_b - c;
+    if (disc < 0.f) return false;
+    const float sq = std::sqrt(disc);
+    t = -half_b + (c >= 0.f ? -sq : sq);
+    return true;
+}
 
 // BLAS traversal + MT + world_t (MeshRenderProxy.cpp:94-194), shared candidate across meshes
 static void blas_intersect_ref(const orcl_scene &sc, int ii, V3 wo, V3 wd, Candidate &cand) {
@@ -622,10 +641,9 @@ static void blas_intersect_ref(const orcl_scene &sc, int ii, V3 wo, V3 wd, Candi
 
 // TLAS traversal (SceneRenderProxy.cpp:82-126)
 static void scene_intersect_ref(const orcl_scene &sc, V3 o, V3 d, Candidate &cand) {
-    if (sc.tlas.nodes.empty()) return;
     SlabRay sr = slab_setup(o, d);
     std::vector<uint32_t> stack;
-    stack.push_back(0);
+    if (!sc.tlas.nodes.empty()) stack.push_back(0);
     while (!stack.empty()) {
         uint32_t ni = stack.back();
         stack.pop_back();
@@ -655,6 +673,16 @@ static void scene_intersect_ref(const orcl_scene &sc, V3 o, V3 d, Candidate &can
         for (uint32_t k = leaf.first; k < leaf.first + leaf.count; ++k) blas_intersect_ref(sc, (int)k, o, d, cand);
     }
     if (cand.inst >= 0) cand.gid = sc.instances[cand.inst].prim_offset + cand.face;
+    // spheres: IsCloserHit = t > 0 and t < the candidate's t
+    for (size_t k = 0; k < sc.spheres.size(); ++k) {
+        float t;
+        if (sphere_t(sc.spheres[k], o, d, t) && t > 0.f && t < cand.t) {
+            cand.t = t;
+            cand.inst = -1;
+            cand.sphere = (int)k;
+            cand.gid = sc.total_tris + k;
+        }
+    }
 }
 
 // watertight closest hit, order-independent: smaller t wins, exact ties go to the smaller id
@@ -662,6 +690,7 @@ struct WtBest {
     float t;
     uint64_t gid = ~0ull;
     int inst = -1;
+    int sphere = -1;
     uint32_t face = 0;
     float U = 0, V = 0, W = 0, det = 1;
 };
@@ -934,6 +963,16 @@ orcl_scene *orcl_scene_create(const orcl_mesh *meshes, uint32_t num_meshes, cons
 
 void orcl_scene_destroy(orcl_scene *scene) { delete scene; }
 
+void orcl_scene_set_spheres(orcl_scene *scene, const orcl_sphere *spheres, uint32_t num_spheres) {
+    scene->spheres.resize(num_spheres);
+    for (uint32_t k = 0; k < num_spheres; ++k) {
+        orcl::Sphere &d = scene->spheres[k];
+        for (int a = 0; a < 3; ++a) d.c[a] = spheres[k].center[a];
+        d.r2 = spheres[k].radius * spheres[k].radius;
+        d.material = spheres[k].material;
+    }
+}
+
 void orcl_scene_set_textures(orcl_scene *scene, const orcl_texture *textures, uint32_t num_textures) {
     scene->textures.assign(num_textures, orcl::Texture());
     for (uint32_t k = 0; k < num_textures; ++k) {
@@ -976,7 +1015,7 @@ int orcl_trace(const orcl_scene *scene, const orcl_ray *rays, uint64_t num_rays,
             if (mode == ORCL_TRACE_REF_MT) {
                 Candidate c;
                 scene_intersect_ref(sc, o, d, c);
-                if (c.inst >= 0) {
+                if (c.inst >= 0 || c.sphere >= 0) {
                     hits[i].prim = (uint32_t)c.gid;
                     hits[i].t = c.t;
                 }
@@ -988,7 +1027,21 @@ int orcl_trace(const orcl_scene *scene, const orcl_ray *rays, uint64_t num_rays,
                 if (std::isnan(r.tmin) || std::isnan(r.tmax)) continue;
                 // a negative tmin is treated as 0, as on the device
                 scene_intersect_wt(sc, o, d, std::max(r.tmin, 0.f), r.tmax, best, mode == ORCL_TRACE_WT_BRUTE);
-                if (best.gid != ~0ull) {
+                // spheres after the triangles, as on the device: smaller t, then smaller id
+                for (size_t k = 0; k < sc.spheres.size(); ++k) {
+                    float t;
+                    const uint64_t gid = sc.total_tris + k;
+                    if (sphere_t(sc.spheres[k], o, d, t) && t > std::max(r.tmin, 0.f) && t > 0.f &&
+                        (t < best.t || (t == best.t && gid < best.gid))) {
+                        best.t = t;
+                        best.gid = gid;
+                        best.sphere = (int)k;
+                    }
+                }
+                if (best.sphere >= 0) {
+                    hits[i].prim = (uint32_t)best.gid;
+                    hits[i].t = best.t;
+                } else if (best.gid != ~0ull) {
                     const Instance &ins = sc.instances[best.inst];
                     const Mesh &m = sc.meshes[ins.mesh];
                     V3 a, bb, c;
@@ -1083,6 +1136,8 @@ struct Hit {
     float t = 0;
     V3 location, normal, tangent;
     V3 gnormal;  // unnormalised geometric normal of the world-space triangle (e1 x e2)
+    uint32_t material = 0;
+    bool sphere = false;
     float uvx = 0, uvy = 0;
     const Instance *inst = nullptr;
 };
@@ -1146,11 +1201,31 @@ static MatEval eval_material(const orcl_scene &sc, const orcl_material &m, float
     return e;
 }
 
+// GetPossibleMajorAxis (Utilities.h:222-234)
+static inline V3 major_axis(V3 n) {
+    const float inv_sqrt3 = 0.577350269189625764509f;
+    if (std::fabs(n.x) < inv_sqrt3) return V3(1, 0, 0);
+    if (std::fabs(n.y) < inv_sqrt3) return V3(0, 1, 0);
+    return V3(0, 0, 1);
+}
+
 // SceneRenderProxy::Intersect<false> + MeshRenderProxy::GetIntersection (MeshRenderProxy.cpp:340-361)
 static Hit scene_hit(const orcl_scene &sc, V3 o, V3 d) {
     Candidate c;
     scene_intersect_ref(sc, o, d, c);
     Hit h;
+    if (c.sphere >= 0) {
+        // SphereRenderProxy::GetIntersection (SphereRenderProxy.h:22-31): uv 0, major-axis tangent
+        const Sphere &sp = sc.spheres[c.sphere];
+        h.t = c.t;
+        h.location = o + d * c.t;
+        h.normal = normalized(h.location - V3(sp.c[0], sp.c[1], sp.c[2]));
+        h.tangent = major_axis(h.normal);
+        h.material = sp.material;
+        h.sphere = true;
+        h.hit = true;
+        return h;
+    }
     if (c.inst < 0) return h;
     const Instance &ins = sc.instances[c.inst];
     const Mesh &m = sc.meshes[ins.mesh];
@@ -1188,6 +1263,7 @@ static Hit scene_hit(const orcl_scene &sc, V3 o, V3 d) {
     h.t = c.t;
     h.location = o + d * c.t;  // Intersection::Update -> ray.At(t)
     h.inst = &ins;
+    h.material = ins.material;
     h.hit = true;
     return h;
 }
@@ -1430,11 +1506,11 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
             }
             break;
         }
-        const orcl_material &mat = sc.materials[h.inst->material];
+        const orcl_material &mat = sc.materials[h.material];
         const MatEval me = eval_material(sc, mat, h.uvx, h.uvy);
         if (bounce == 0) res.normal = h.normal;
         if (norm(me.emissive) > kEps) {
-            if (!skip_emission) {
+            if (!skip_emission || h.sphere) {  // spheres are not among the sampled emitters
                 res.color = res.color + cmul(me.emissive, T);
             } else {
                 const float cos_y = std::fabs(dot(h.gnormal, d)) / std::sqrt(sqnorm(h.gnormal));

@@ -141,6 +141,15 @@ typedef struct orcl_scene orcl_scene;
 orcl_scene *orcl_scene_create(const orcl_mesh *meshes, uint32_t num_meshes, const orcl_instance *instances,
                               uint32_t num_instances, const orcl_material *materials, uint32_t num_materials);
 void orcl_scene_destroy(orcl_scene *scene);
+/* Analytic spheres (SphereRenderProxy.cpp:5-48): closed-form hit with unit directions assumed;
+   sphere k has the global id num_triangles + k. Ignored by the FP64 trace mode. */
+typedef struct orcl_sphere {
+    float center[3];
+    float radius;
+    uint32_t material;
+    uint32_t reserved[3];
+} orcl_sphere;
+void orcl_scene_set_spheres(orcl_scene *scene, const orcl_sphere *spheres, uint32_t num_spheres);
 /* Texture table the materials' texture indices refer to (copied). */
 void orcl_scene_set_textures(orcl_scene *scene, const orcl_texture *textures, uint32_t num_textures);
 /* Image::Sample of texture `index` at (u, v): rgb into out[3]. */

@@ -39,6 +39,11 @@ class OMaterial(ctypes.Structure):
                 ("emissive_texture", ctypes.c_int32), ("normal_texture", ctypes.c_int32)]
 
 
+class OSphere(ctypes.Structure):
+    _fields_ = [("center", ctypes.c_float * 3), ("radius", ctypes.c_float), ("material", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
 class OTexture(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("pixels", ctypes.c_void_p)]
@@ -81,6 +86,7 @@ def lib():
         sig = {
             "orcl_scene_create": (P, [P, U32, P, U32, P, U32]),
             "orcl_scene_set_textures": (None, [P, P, U32]),
+            "orcl_scene_set_spheres": (None, [P, P, U32]),
             "orcl_texture_sample": (None, [P, ctypes.c_int, ctypes.c_float, ctypes.c_float, P]),
             "orcl_scene_destroy": (None, [P]),
             "orcl_scene_num_triangles": (U64, [P]),
@@ -157,6 +163,14 @@ class OracleScene:
         self.h = L.orcl_scene_create(ms, len(scene.meshes), ins, len(scene.instances), mats, len(scene.materials))
         if not self.h:
             raise RuntimeError("orcl_scene_create failed")
+        sph = getattr(scene, "spheres", [])
+        if sph:
+            arr = (OSphere * len(sph))()
+            for k, (c, r, m) in enumerate(sph):
+                arr[k].center[:] = list(c)
+                arr[k].radius = r
+                arr[k].material = m
+            L.orcl_scene_set_spheres(self.h, arr, len(sph))
         texs = getattr(scene, "textures", [])
         if texs:
             arr = (OTexture * len(texs))()

@@ -74,6 +74,11 @@ TEXTURE_RGBA8_SRGB = 1
 TEXTURE_RGBA32F = 2
 
 
+class SphereDesc(ctypes.Structure):
+    _fields_ = [("center", ctypes.c_float * 3), ("radius", ctypes.c_float), ("material", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
 class TextureDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("pixels", ctypes.c_void_p)]
@@ -175,6 +180,7 @@ def load_library():
         "spk_scene_upload": (I, [P, P, U32, P, U32, P, U32, P]),
         "spk_scene_stats_get": (I, [P, ctypes.POINTER(SceneStats)]),
         "spk_textures_upload": (I, [P, P, U32]),
+        "spk_scene_set_spheres": (I, [P, P, U32]),
         "spk_build_check": (I, [P, U32, P, U32, P, ctypes.POINTER(SceneStats), ctypes.c_char_p, ctypes.c_size_t]),
         "spk_trace_closest": (I, [P, P, U64, P]),
         "spk_trace_closest_async": (I, [P, P, U64, P]),
@@ -233,6 +239,7 @@ class Scene:
         self.instances = []
         self.materials = []
         self.textures = []  # (pixels (H, W, 4), TEXTURE_* format)
+        self.spheres = []   # (center (3,), radius, material): analytic spheres
 
     def add_mesh(self, positions, indices, normals=None, tangents=None, uvs=None):
         m = dict(positions=np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3),
@@ -255,6 +262,10 @@ class Scene:
             m.update(tex)
         self.materials.append(m)
         return len(self.materials) - 1
+
+    def add_sphere(self, center, radius, material):
+        self.spheres.append((tuple(float(c) for c in center), float(radius), int(material)))
+        return len(self.spheres) - 1
 
     def add_texture(self, pixels, fmt=TEXTURE_RGBA8_UNORM):
         """pixels (H, W, 4): uint8 for the RGBA8 formats, float32 for TEXTURE_RGBA32F."""
@@ -376,7 +387,18 @@ class Context:
         opts = BuildOptions(bvh_width, max_leaf_tris, num_threads)
         self._check(self.lib.spk_scene_upload(self.h, ms, len(scene.meshes), ins, len(scene.instances), mats,
                                               len(scene.materials), ctypes.byref(opts)))
+        if getattr(scene, "spheres", None):
+            self.set_spheres(scene.spheres)
         return self.stats()
+
+    def set_spheres(self, spheres):
+        """spk_scene_set_spheres: spheres is a list of (center, radius, material)."""
+        arr = (SphereDesc * max(1, len(spheres)))()
+        for k, (c, r, m) in enumerate(spheres):
+            arr[k].center[:] = list(c)
+            arr[k].radius = r
+            arr[k].material = m
+        return self._check(self.lib.spk_scene_set_spheres(self.h, arr, len(spheres)))
 
     def stats(self):
         st = SceneStats()

@@ -186,6 +186,7 @@ static int upload_built(spk_context *ctx, int stack) {
     ds.host_emitters.clear();
     for (uint32_t k = 0; k < n; ++k)
         if (mats[shade[k].material].emissive_flag) ds.host_emitters.push_back(k);
+    ds.num_spheres = 0;  // spheres are set after each upload (spk_scene_set_spheres)
     ds.valid = true;
     fill_stats(hs, &ctx->stats, ds.nodes.bytes + ds.tris.bytes + ds.shade.bytes + ds.materials.bytes);
     return SPK_OK;
@@ -477,6 +478,36 @@ int spk_textures_upload(spk_context *ctx, const spk_texture_desc *textures, uint
     return SPK_OK;
 }
 
+int spk_scene_set_spheres(spk_context *ctx, const spk_sphere_desc *spheres, uint32_t num_spheres) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
+    if (num_spheres && !spheres) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null sphere array");
+    if ((uint64_t)ctx->scene.num_tris + num_spheres >= 0xFFFFFFFFull || num_spheres >= (1u << 31))
+        return set_error(ctx, SPK_ERR_INVALID_SCENE, "too many primitives");
+    std::vector<float> rec(8 * (size_t)std::max(num_spheres, 1u), 0.f);
+    for (uint32_t k = 0; k < num_spheres; ++k) {
+        const spk_sphere_desc &sd = spheres[k];
+        if (sd.material >= ctx->scene.num_materials)
+            return set_error(ctx, SPK_ERR_INVALID_SCENE, "sphere %u names material %u of %u", k, sd.material,
+                             ctx->scene.num_materials);
+        if (!(sd.radius > 0.f) || !std::isfinite(sd.radius) || !std::isfinite(sd.center[0]) ||
+            !std::isfinite(sd.center[1]) || !std::isfinite(sd.center[2]))
+            return set_error(ctx, SPK_ERR_INVALID_SCENE, "sphere %u has a non-finite centre or radius", k);
+        rec[8 * k] = sd.center[0];
+        rec[8 * k + 1] = sd.center[1];
+        rec[8 * k + 2] = sd.center[2];
+        rec[8 * k + 3] = sd.radius * sd.radius;  // scaled_radius_2_ (SphereRenderProxy.h:29-32)
+        std::memcpy(&rec[8 * k + 4], &sd.material, 4);
+    }
+    hipError_t e;
+    if ((e = ctx->scene.spheres.ensure(rec.size() * 4)) != hipSuccess ||
+        (e = hipMemcpy(ctx->scene.spheres.ptr, rec.data(), rec.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(ctx, e, "sphere upload");
+    ctx->scene.num_spheres = num_spheres;
+    ctx->on_scene_changed();
+    return SPK_OK;
+}
+
 int spk_scene_stats_get(spk_context *ctx, spk_scene_stats *out) {
     if (!ctx || !out) return SPK_ERR_INVALID_ARGUMENT;
     if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
@@ -488,8 +519,10 @@ int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t 
     if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
     if (n && (!rays_dev || !hits_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null ray or hit array");
     if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
-    hipError_t e = launch_trace(ctx->scene.variant, rays_dev, n, hits_dev, ctx->scene.nodes.ptr, ctx->scene.tris.ptr,
-                                0u, ctx->invalid_counter(), nullptr, nullptr, ctx->work_counter(), ctx->stream);
+    const DeviceScene &ds = ctx->scene;
+    hipError_t e = launch_trace(ds.variant, rays_dev, n, hits_dev, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(),
+                                nullptr, nullptr, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
+                                ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
     return SPK_OK;
 }
@@ -526,8 +559,10 @@ int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint3
     if (!nodes_dev && !tris_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "no count array");
     hipError_t e;
     if ((e = ctx->tmp_hits.ensure(n * sizeof(spk_hit))) != hipSuccess) return hip_fail(ctx, e, "hit buffer");
-    e = launch_trace(ctx->scene.variant, rays_dev, n, ctx->tmp_hits.ptr, ctx->scene.nodes.ptr, ctx->scene.tris.ptr,
-                     0u, ctx->invalid_counter(), nodes_dev, tris_dev, ctx->work_counter(), ctx->stream);
+    const DeviceScene &ds = ctx->scene;
+    e = launch_trace(ds.variant, rays_dev, n, ctx->tmp_hits.ptr, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(),
+                     nodes_dev, tris_dev, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
+                     ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "count launch");
     return spk_synchronize(ctx);
 }
@@ -620,7 +655,7 @@ static int time_variant(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, d
         if (e == hipSuccess)
             e = launch_trace(ctx->scene.variant, rays_dev, n, ctx->tmp_hits.ptr, ctx->scene.nodes.ptr,
                              ctx->scene.tris.ptr, 0u, ctx->invalid_counter(), nullptr, nullptr, ctx->work_counter(),
-                             ctx->stream);
+                             ctx->scene.spheres.ptr, ctx->scene.num_spheres, ctx->scene.num_tris, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float t = 0.f;

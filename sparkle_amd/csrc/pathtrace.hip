@@ -80,6 +80,8 @@ struct PtArgs {
     const float *emitter_cdf;
     const TexRecord *tex_table;  // texture table and float4 texels
     const float4 *texels;
+    const float4 *spheres;  // analytic spheres, tested after the BVH
+    uint32_t num_spheres, sphere_base;
     uint32_t num_emitters;
     float emitter_total;
     float4 *rays;      // 2 per path: (org, tmin), (dir, tmax)
@@ -145,6 +147,12 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
     x *= 0x846ca68bU;
     x ^= x >> 16;
     return x;
+}
+
+// material of a hit record's primitive (leaf triangle, or SPK_SPHERE_BIT | sphere)
+__device__ __forceinline__ uint32_t hit_material(const PtArgs &a, uint32_t prim) {
+    if (prim & SPK_SPHERE_BIT) return __float_as_uint(a.spheres[2 * (prim & ~SPK_SPHERE_BIT) + 1].x);
+    return a.shade[prim].material;
 }
 
 // last scene whose first pixel is <= p
@@ -272,13 +280,18 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
         if (active && !tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
             uint4 h;
             uint32_t key = a.nkeys - 1u;  // misses sort last
+            const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
+            const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
+            if (a.num_spheres) spheres_closest(a.spheres, a.num_spheres, a.sphere_base, r, tr.best);
             if (tr.best.prim != 0xFFFFFFFFu) {
-                const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
-                const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
                 const float t = final_t(a.tris, r, tr.best);
-                h = make_uint4(tr.best.tri, __float_as_uint(t), __float_as_uint(tr.best.V / tr.best.det),
-                               __float_as_uint(tr.best.W / tr.best.det));
-                key = a.shade[tr.best.tri].material;
+                if (tr.best.tri & SPK_SPHERE_BIT) {
+                    h = make_uint4(tr.best.tri, __float_as_uint(t), 0u, 0u);
+                } else {
+                    h = make_uint4(tr.best.tri, __float_as_uint(t), __float_as_uint(tr.best.V / tr.best.det),
+                                   __float_as_uint(tr.best.W / tr.best.det));
+                }
+                key = hit_material(a, tr.best.tri);
                 if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
             } else {
                 h = make_uint4(0xFFFFFFFFu, __float_as_uint(INFINITY), 0u, 0u);
@@ -330,7 +343,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sort_scatter(PtArgs a, uint32_t bo
         if (valid) {
             const uint4 h = a.hits[slot];
             if (h.x != 0xFFFFFFFFu) {
-                key = a.shade[h.x].material;
+                key = hit_material(a, h.x);
                 if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
             }
         }
@@ -356,6 +369,14 @@ __device__ __forceinline__ float4 lerp_attr(const float *p0, const float *p1, co
                        p0[1] + (p1[1] - p0[1]) * u + (p2[1] - p0[1]) * v,
                        p0[2] + (p1[2] - p0[2]) * u + (p2[2] - p0[2]) * v,
                        p0[3] + (p1[3] - p0[3]) * u + (p2[3] - p0[3]) * v);
+}
+
+// GetPossibleMajorAxis (Utilities.h:222-234)
+__device__ __forceinline__ F3 major_axis(F3 n) {
+    const float inv_sqrt3 = 0.577350269189625764509f;
+    if (fabsf(n.x) < inv_sqrt3) return f3(1.f, 0.f, 0.f);
+    if (fabsf(n.y) < inv_sqrt3) return f3(0.f, 1.f, 0.f);
+    return f3(0.f, 0.f, 1.f);
 }
 
 // Area pdf with which next-event estimation picks a point of an emitter of this material: triangles
@@ -425,24 +446,38 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                     C = C + cmul(sky, T);
                 }
             } else {
-                const TriShade &s = a.shade[h.x];
                 const float u = __uint_as_float(h.z), v = __uint_as_float(h.w), t = __uint_as_float(h.y);
-                const float4 n4 = lerp_attr(s.n0, s.n1, s.n2, u, v);
-                const float4 t4 = lerp_attr(s.t0, s.t1, s.t2, u, v);
-                // texture coordinate (Mesh.h:119-132)
-                const float tu = s.uv0[0] + (s.uv1[0] - s.uv0[0]) * u + (s.uv2[0] - s.uv0[0]) * v;
-                const float tv = s.uv0[1] + (s.uv1[1] - s.uv0[1]) * u + (s.uv2[1] - s.uv0[1]) * v;
-                const MaterialGpu mat = a.mats[s.material];
-                F3 nrm = f3(n4.x, n4.y, n4.z);
-                const F3 tangent = f3(t4.x, t4.y, t4.z);
-                if (mat.tex_normal >= 0) {
-                    // normal map in the tangent frame (MeshRenderProxy.cpp:350-356, Utilities.h:398-404)
-                    const F3 tn = tex_sample(a.tex_table, a.texels, mat.tex_normal, tu, tv) * 2.f - f3(1.f, 1.f, 1.f);
-                    const F3 bit = cross(nrm, tangent) * t4.w;
-                    nrm = normalized(tangent * tn.x + bit * tn.y + nrm * tn.z);
-                }
-                const F3 normal = normalized(nrm);
                 loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
+                const bool is_sphere = (h.x & SPK_SPHERE_BIT) != 0u;
+                F3 normal, tangent;
+                float tu = 0.f, tv = 0.f;
+                MaterialGpu mat;
+                if (is_sphere) {
+                    // SphereRenderProxy::GetIntersection (SphereRenderProxy.h:22-31): uv 0, no normal map
+                    const uint32_t k = h.x & ~SPK_SPHERE_BIT;
+                    const float4 sc = a.spheres[2 * k];
+                    mat = a.mats[__float_as_uint(a.spheres[2 * k + 1].x)];
+                    normal = normalized(loc - f3(sc.x, sc.y, sc.z));
+                    tangent = major_axis(normal);
+                } else {
+                    const TriShade &s = a.shade[h.x];
+                    const float4 n4 = lerp_attr(s.n0, s.n1, s.n2, u, v);
+                    const float4 t4 = lerp_attr(s.t0, s.t1, s.t2, u, v);
+                    // texture coordinate (Mesh.h:119-132)
+                    tu = s.uv0[0] + (s.uv1[0] - s.uv0[0]) * u + (s.uv2[0] - s.uv0[0]) * v;
+                    tv = s.uv0[1] + (s.uv1[1] - s.uv0[1]) * u + (s.uv2[1] - s.uv0[1]) * v;
+                    mat = a.mats[s.material];
+                    F3 nrm = f3(n4.x, n4.y, n4.z);
+                    tangent = f3(t4.x, t4.y, t4.z);
+                    if (mat.tex_normal >= 0) {
+                        // normal map in the tangent frame (MeshRenderProxy.cpp:350-356, Utilities.h:398-404)
+                        const F3 tn =
+                            tex_sample(a.tex_table, a.texels, mat.tex_normal, tu, tv) * 2.f - f3(1.f, 1.f, 1.f);
+                        const F3 bit = cross(nrm, tangent) * t4.w;
+                        nrm = normalized(tangent * tn.x + bit * tn.y + nrm * tn.z);
+                    }
+                    normal = normalized(nrm);
+                }
                 const MatEval me = eval_material(a, mat, tu, tv);
                 if (bounce == 0u && a.normal_out) {
                     a.normal_out[3 * (size_t)path] = normal.x;
@@ -455,7 +490,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
                 const bool cache_vertex = a.cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
                                           me.metallic < 0.5f;
                 if (me.emits) {
-                    if (pf & PF_NEE_PREV) {
+                    if ((pf & PF_NEE_PREV) && !is_sphere) {  // spheres are not among the sampled emitters
                         // the previous vertex also sampled this emitter: balance-heuristic weight of
                         // the BSDF-sampled direction against light sampling of this point
                         const uint4 *tp = a.tris + (size_t)h.x * 3u;
@@ -639,6 +674,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) 
         best.det = 1.f;
         TraverseCounters c{0u, 0u};
         traverse<WIDTH, STACK, false, true>(a.nodes, a.tris, 0u, r, stack + threadIdx.x, kPtBlock, best, c);
+        if (best.prim == 0xFFFFFFFFu && a.num_spheres) spheres_closest<true>(a.spheres, a.num_spheres, a.sphere_base, r, best);
         if (best.prim == 0xFFFFFFFFu) {
             const uint32_t path = __float_as_uint(s1.w);
             float4 C = a.col[path];
@@ -1021,6 +1057,9 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     a.shade = (const TriShade *)ds.shade.ptr;
     a.mats = (const MaterialGpu *)ds.materials.ptr;
     a.tex_table = (const TexRecord *)ctx->tex_table.ptr;
+    a.spheres = (const float4 *)ds.spheres.ptr;
+    a.num_spheres = ds.spheres.ptr ? ds.num_spheres : 0u;
+    a.sphere_base = ds.num_tris;
     a.texels = (const float4 *)ctx->texels.ptr;
     a.emitters = (const uint32_t *)ps.emitters.ptr;
     a.emitter_cdf = (const float *)ps.emitter_cdf.ptr;

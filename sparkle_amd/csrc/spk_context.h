@@ -43,7 +43,8 @@ struct HostScene {
 
 struct DeviceScene {
     DeviceBuffer nodes, tris, shade, materials, emitters;
-    uint32_t num_tris = 0, num_nodes = 0, num_materials = 0, num_emitters = 0;
+    DeviceBuffer spheres;  // 2 x float4 per sphere: (center, r^2), (material bits, 0, 0, 0)
+    uint32_t num_tris = 0, num_nodes = 0, num_materials = 0, num_emitters = 0, num_spheres = 0;
     int width = 8, depth = 0;
     TraceVariant variant{8, 16, 128};
     bool valid = false;

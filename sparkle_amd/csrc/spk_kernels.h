@@ -17,9 +17,11 @@ struct TraceVariant {
 // incremented once per NaN/inf/zero-direction ray. cnt_nodes/cnt_tris (device, may be null)
 // receive per-ray traversal counts when non-null. work (device, 8 bytes) is the ray counter of
 // the persistent variants; it is cleared by a kernel launched just before them.
+// spheres (device, may be null): 2 float4 per sphere, tested after the BVH with ids id_base + k.
 hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, void *hits, const void *nodes,
                         const void *tris, uint32_t root, uint32_t *invalid_count, uint32_t *cnt_nodes,
-                        uint32_t *cnt_tris, unsigned long long *work, hipStream_t stream);
+                        uint32_t *cnt_tris, unsigned long long *work, const void *spheres, uint32_t num_spheres,
+                        uint32_t sphere_id_base, hipStream_t stream);
 
 // Largest stack depth any compiled variant supports.
 int max_trace_stack();

@@ -6,12 +6,19 @@
 
 namespace spk {
 
+// Analytic spheres of the scene, tested after the BVH (few, debug primitives in the reference).
+struct SphereSet {
+    const float4 *spheres;
+    uint32_t count, id_base;
+};
+
 template <int WIDTH, int STACK, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_trace(const float4 *__restrict__ rays, uint64_t n,
                                                  uint2 *__restrict__ hits, const uint4 *__restrict__ nodes,
                                                  const uint4 *__restrict__ tris, uint32_t root,
                                                  uint32_t *__restrict__ invalid_count,
-                                                 uint32_t *__restrict__ cnt_nodes, uint32_t *__restrict__ cnt_tris) {
+                                                 uint32_t *__restrict__ cnt_nodes, uint32_t *__restrict__ cnt_tris,
+                                                 SphereSet sph) {
     __shared__ uint2 stack[STACK * BLOCK];
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -29,6 +36,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const float4 *__restrict__ rays
         best.U = best.V = best.W = 0.f;
         best.det = 1.f;
         traverse<WIDTH, STACK, COUNT>(nodes, tris, root, r, stack + threadIdx.x, BLOCK, best, c);
+        if (sph.count) spheres_closest(sph.spheres, sph.count, sph.id_base, r, best);
         if (best.prim != 0xFFFFFFFFu) out = make_uint2(best.prim, __float_as_uint(final_t(tris, r, best)));
     }
     hits[i] = out;
@@ -52,7 +60,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_persistent(
     const float4 *__restrict__ rays, uint64_t n, uint2 *__restrict__ hits, const uint4 *__restrict__ nodes,
     const uint4 *__restrict__ tris, uint32_t root, uint32_t *__restrict__ invalid_count,
     uint32_t *__restrict__ cnt_nodes, uint32_t *__restrict__ cnt_tris, unsigned long long *__restrict__ next_ray,
-    uint32_t refill_min) {
+    uint32_t refill_min, SphereSet sph) {
     __shared__ uint2 stack[STACK * BLOCK];
     uint2 *st = stack + threadIdx.x;
     const uint32_t lane = __lane_id();
@@ -97,11 +105,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace_persistent(
         }
         if (active && !tr.step(nodes, tris, st, BLOCK)) {
             uint2 out = make_uint2(0xFFFFFFFFu, 0x7f800000u);
-            if (tr.best.prim != 0xFFFFFFFFu) {
-                const float4 a = rays[2 * id], b = rays[2 * id + 1];
-                const RayIn r{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-                out = make_uint2(tr.best.prim, __float_as_uint(final_t(tris, r, tr.best)));
-            }
+            const float4 a = rays[2 * id], b = rays[2 * id + 1];
+            const RayIn r{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            if (sph.count) spheres_closest(sph.spheres, sph.count, sph.id_base, r, tr.best);
+            if (tr.best.prim != 0xFFFFFFFFu) out = make_uint2(tr.best.prim, __float_as_uint(final_t(tris, r, tr.best)));
             hits[id] = out;
             if (COUNT) {
                 if (cnt_nodes) cnt_nodes[id] = tr.cnt.nodes;
@@ -116,9 +123,9 @@ __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 namespace {
 using TraceFn = void (*)(const float4 *, uint64_t, uint2 *, const uint4 *, const uint4 *, uint32_t, uint32_t *,
-                         uint32_t *, uint32_t *);
+                         uint32_t *, uint32_t *, SphereSet);
 using PersistentFn = void (*)(const float4 *, uint64_t, uint2 *, const uint4 *, const uint4 *, uint32_t, uint32_t *,
-                              uint32_t *, uint32_t *, unsigned long long *, uint32_t);
+                              uint32_t *, uint32_t *, unsigned long long *, uint32_t, SphereSet);
 struct Entry {
     int width, stack, block;
     bool count;
@@ -150,7 +157,9 @@ const int kRefill[] = {1, 16, 32};
 
 hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, void *hits, const void *nodes,
                         const void *tris, uint32_t root, uint32_t *invalid_count, uint32_t *cnt_nodes,
-                        uint32_t *cnt_tris, unsigned long long *work, hipStream_t stream) {
+                        uint32_t *cnt_tris, unsigned long long *work, const void *spheres, uint32_t num_spheres,
+                        uint32_t sphere_id_base, hipStream_t stream) {
+    const SphereSet sph{(const float4 *)spheres, spheres ? num_spheres : 0u, sphere_id_base};
     const bool count = cnt_nodes != nullptr || cnt_tris != nullptr;
     for (int k = 0; k < kNumEntries; ++k) {
         const Entry &e = kEntries[k];
@@ -174,13 +183,13 @@ hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, voi
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, work);
             hipLaunchKernelGGL(e.pfn, dim3((uint32_t)grid), dim3(v.block), 0, stream, (const float4 *)rays, n,
                                (uint2 *)hits, (const uint4 *)nodes, (const uint4 *)tris, root, invalid_count,
-                               cnt_nodes, cnt_tris, work, refill);
+                               cnt_nodes, cnt_tris, work, refill, sph);
             return hipGetLastError();
         }
         if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
         hipLaunchKernelGGL(e.fn, dim3((uint32_t)blocks), dim3(v.block), 0, stream, (const float4 *)rays, n,
                            (uint2 *)hits, (const uint4 *)nodes, (const uint4 *)tris, root, invalid_count, cnt_nodes,
-                           cnt_tris);
+                           cnt_tris, sph);
         return hipGetLastError();
     }
     return hipErrorInvalidValue;

@@ -442,8 +442,41 @@ __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const 
     cnt.tris += tr.cnt.tris;
 }
 
+// ---- analytic spheres (SphereRenderProxy.cpp:5-38), tested after the BVH: spheres[2k] = (center,
+// r^2), spheres[2k+1].x = material bits. A sphere hit stores tri = SPK_SPHERE_BIT | k and prim =
+// id_base + k; ties with triangles keep the smaller id (the triangle).
+#define SPK_SPHERE_BIT 0x80000000u
+
+__device__ __forceinline__ bool sphere_t(const float4 s, const RayIn &r, float &t) {
+    const float cx = r.ox - s.x, cy = r.oy - s.y, cz = r.oz - s.z;
+    const float half_b = (r.dx * cx + r.dy * cy) + r.dz * cz;
+    const float c = ((cx * cx + cy * cy) + cz * cz) - s.w;
+    const float disc = half_b * half_b - c;  // a = 1: unit directions, as the reference assumes
+    if (disc < 0.f) return false;
+    const float sq = sqrtf(disc);
+    t = -half_b + (c >= 0.f ? -sq : sq);
+    return true;
+}
+
+template <bool ANYHIT = false>
+__device__ __forceinline__ void spheres_closest(const float4 *__restrict__ spheres, uint32_t n, uint32_t id_base,
+                                                const RayIn &r, BestHit &best) {
+    const float tmin = fmaxf(r.tmin, 0.f);
+    for (uint32_t k = 0; k < n; ++k) {
+        float t;
+        if (!sphere_t(spheres[2 * k], r, t)) continue;
+        if (t > tmin && t > 0.f && (t < best.t || (t == best.t && id_base + k < best.prim))) {
+            best.t = t;
+            best.prim = id_base + k;
+            best.tri = SPK_SPHERE_BIT | k;
+            if (ANYHIT) return;
+        }
+    }
+}
+
 // refined distance of the final hit (bit-identical to the oracle's)
 __device__ __forceinline__ float final_t(const uint4 *__restrict__ tris, const RayIn &ray, const BestHit &best) {
+    if (best.tri & SPK_SPHERE_BIT) return best.t;  // closed form already
     const uint4 *tp = tris + (size_t)best.tri * 3u;
     const uint4 a = tp[0], b = tp[1], c = tp[2];
     const float4 fa = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), 0.f);

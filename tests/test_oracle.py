@@ -232,3 +232,32 @@ def test_edge_on_false_hits_are_rejected():
     P = sc.meshes[0]["positions"]
     I = sc.meshes[0]["indices"]
     assert not refcpu.tri_wt(rays["org"][ids[0]], rays["dir"][ids[0]], *P[I[466407]])[0]
+
+
+def test_analytic_sphere_closed_forms():
+    """SphereRenderProxy: near root from outside, far root from inside, misses, id num_tris + k,
+    and ordering against triangles (smaller t wins; the triangle wins an exact tie)."""
+    sc = spk.Scene()
+    m = sc.add_mesh(np.array([[-1, -1, 3], [1, -1, 3], [0, 1, 3]], np.float32), np.array([[0, 1, 2]], np.uint32),
+                    np.array([[0, 0, -1]] * 3, np.float32))
+    mat = sc.add_material()
+    sc.add_instance(m, mat)
+    sc.add_sphere((0.0, 0.0, 0.0), 1.0, mat)
+    sc.add_sphere((0.0, 0.0, 6.0), 0.5, mat)
+    o = refcpu.OracleScene(sc)
+    rays = np.zeros(5, dtype=spk.RAY_DTYPE)
+    rays["org"] = [[0, 0, -5], [0, 0, 0], [0, 2, -5], [0, 0, 4], [0.25, 0.25, -5]]
+    rays["dir"] = [[0, 0, 1], [0, 0, 1], [0, 0, 1], [0, 0, 1], [0, 0, 1]]
+    rays["tmax"] = np.inf
+    for mode in (refcpu.TRACE_REF_MT, refcpu.TRACE_WT):
+        h = o.trace(rays, mode)
+        assert h["prim"][0] == 1 and h["t"][0] == 4.0          # outside: near root
+        assert h["prim"][1] == 1 and h["t"][1] == 1.0          # inside: far root
+        assert h["prim"][2] == spk.SPK_MISS                    # passes beside both
+        assert h["prim"][3] == 2 and h["t"][3] == 1.5          # second sphere, id num_tris + 1
+        # through the unit sphere: its far side at z = +sqrt(1 - 0.125) is nearer than the triangle
+        assert h["prim"][4] == 1
+    far = rays[[0]].copy()
+    far["org"] = [[0.0, 0.0, 2.0]]
+    h = o.trace(far, refcpu.TRACE_WT)
+    assert h["prim"][0] == 0 and h["t"][0] == 1.0              # the triangle at z = 3 before the sphere at 5.5

@@ -320,3 +320,37 @@ def test_textured_materials_match_oracle(gpu_ctx):
     ok = np.abs(d.mean(0)) <= 3 * se + 1e-6
     assert ok.mean() >= 0.99, ok.mean()
     assert (np.abs(d).max(-1) < 1e-3).mean() > 0.8
+
+
+def test_analytic_spheres_in_the_path_tracer(gpu_ctx):
+    """The Cornell room with two analytic spheres (one Lambertian, one small emitter): first-hit debug
+    modes equal to the oracle, the estimator statistically equal, and NEE (which does not sample
+    sphere emitters) unbiased against it."""
+    w = h = 40
+    sc = scenes.cornell_scene(0)
+    m_diff = sc.add_material(base_color=(0.7, 0.6, 0.3), metallic=0.0, roughness=1.0)
+    m_em = sc.add_material(base_color=(1, 1, 1), emissive=(4.0, 3.0, 2.0), metallic=0.0)
+    sc.add_sphere((-0.45, -0.45, 0.25), 0.25, m_diff)
+    sc.add_sphere((0.55, 0.5, 1.6), 0.12, m_em)
+    cam = scenes.cornell_camera(w, h)
+    r = Renderer(gpu_ctx, sc, cam, w, h)
+    o = refcpu.OracleScene(sc)
+    for mode in (spk.DEBUG_NORMAL, spk.DEBUG_ALBEDO, spk.DEBUG_DEPTH):
+        print_case("sphere debug mode", mode)
+        g, _, _ = r.render(4, debug=mode)
+        c, _ = o.render_frame(cam, w, h, frame_seed=4, debug_mode=mode)
+        assert (np.abs(g[..., :3] - c[..., :3]).max(-1) < 1e-4).mean() > 0.99
+    frames = 32
+    d = np.zeros((frames, h, w, 3))
+    for k in range(frames):
+        gg, _, _ = r.render(k, bounce=5)
+        cc, _ = o.render_frame(cam, w, h, frame_seed=k, max_bounce=5)
+        d[k] = gg[..., :3] - cc[..., :3]
+    se = d.std(0, ddof=1) / math.sqrt(frames)
+    assert (np.abs(d.mean(0)) <= 3 * se + 1e-6).mean() >= 0.99
+    n = 256
+    fp = np.array([r.render(k, bounce=5, limit=1e30)[0][..., :3].mean() for k in range(n)])
+    fn = np.array([r.render(k + 10 ** 6, bounce=5, flags=spk.FRAME_NEE | spk.FRAME_SEED64, limit=1e30)[0][..., :3].mean()
+                   for k in range(n)])
+    z = (fn.mean() - fp.mean()) / np.hypot(fn.std(ddof=1), fp.std(ddof=1)) * math.sqrt(n)
+    assert abs(z) < 3.5, z
