@@ -268,6 +268,11 @@ int spk_autotune(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t
 
 /* ---- path tracing ------------------------------------------------------------------------- */
 int spk_set_camera(spk_context *ctx, const spk_camera_desc *camera);
+/* Sky cube map (SkyRenderProxy::Evaluate with a sky map, SkyRenderProxy.cpp:50-56, Image2DCube::Sample,
+   io/Image.cpp:298-357): texture index + 1 of the faces +X, -X, +Y, -Y, +Z, -Z; all zero (or NULL)
+   for the gradient sky. Missed rays see it when spk_frame_desc.sky_enabled is set. A new texture
+   table clears it. */
+int spk_set_sky_cubemap(spk_context *ctx, const uint32_t faces[6]);
 /* One BasePass: one path per pixel; rgba = clamped colour and valid flag (+1/-1); normal =
    first-hit world normal (may be NULL). Device arrays of width*height*4 / *3 floats. */
 int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, float *normal_dev,

@@ -552,6 +552,7 @@ struct orcl_scene {
     std::vector<orcl::Texture> textures;
     std::vector<orcl::Emitter> emitters;  // instance order, face order
     std::vector<orcl::Sphere> spheres;
+    int32_t sky_faces[6] = {-1, -1, -1, -1, -1, -1};
     std::vector<float> emitter_cdf;       // running sum of area x mean emitted radiance
     orcl::Bvh tlas;
     uint64_t total_tris = 0;
@@ -962,6 +963,10 @@ orcl_scene *orcl_scene_create(const orcl_mesh *meshes, uint32_t num_meshes, cons
 }
 
 void orcl_scene_destroy(orcl_scene *scene) { delete scene; }
+
+void orcl_scene_set_sky_faces(orcl_scene *scene, const int32_t *faces) {
+    for (int f = 0; f < 6; ++f) scene->sky_faces[f] = faces ? faces[f] : -1;
+}
 
 void orcl_scene_set_spheres(orcl_scene *scene, const orcl_sphere *spheres, uint32_t num_spheres) {
     scene->spheres.resize(num_spheres);
@@ -1428,6 +1433,24 @@ struct PixelResult {
 
 static std::atomic<uint64_t> g_rays{0};
 
+// Image2DCube::DirectionToTextureCoordinate + Sample (io/Image.cpp:298-357)
+static V3 sky_cube_sample(const orcl_scene &sc, V3 dir) {
+    const float x = dir.x, y = dir.y, z = dir.z;
+    const float ax = std::fabs(x), ay = std::fabs(y), az = std::fabs(z);
+    int face;
+    float u, v;
+    if (ax >= ay && ax >= az) {
+        if (x > 0) { face = 0; u = -z / x; v = -y / x; } else { face = 1; u = z / -x; v = -y / -x; }
+    } else if (ay >= ax && ay >= az) {
+        if (y > 0) { face = 2; u = x / y; v = z / y; } else { face = 3; u = x / -y; v = -z / -y; }
+    } else {
+        if (z > 0) { face = 4; u = x / z; v = -y / z; } else { face = 5; u = -x / -z; v = -y / -z; }
+    }
+    u = (u + 1.f) * 0.5f;
+    v = (v + 1.f) * 0.5f;
+    return texture_sample(sc.textures[sc.sky_faces[face]], u, v);
+}
+
 // area pdf of a point on an emitter of this material under the emitter sampling below
 static float light_pdf_area(const orcl_scene &sc, const orcl_material &m) {
     return (m.emissive[0] + m.emissive[1] + m.emissive[2]) / 3.f / sc.emitter_cdf.back();
@@ -1500,8 +1523,13 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
         nrays++;
         if (!h.hit) {
             if (fr.sky_enabled) {
-                float t = 0.5f * d.z + 0.5f;
-                V3 sky = kOnes + (V3(fr.sky_color) - kOnes) * t;
+                V3 sky;
+                if (sc.sky_faces[0] >= 0) {
+                    sky = sky_cube_sample(sc, d);
+                } else {
+                    float t = 0.5f * d.z + 0.5f;
+                    sky = kOnes + (V3(fr.sky_color) - kOnes) * t;
+                }
                 res.color = res.color + cmul(sky, T);
             }
             break;

@@ -150,6 +150,8 @@ typedef struct orcl_sphere {
     uint32_t reserved[3];
 } orcl_sphere;
 void orcl_scene_set_spheres(orcl_scene *scene, const orcl_sphere *spheres, uint32_t num_spheres);
+/* Sky cube map faces (+X, -X, +Y, -Y, +Z, -Z) as texture indices; faces[0] < 0 = gradient sky. */
+void orcl_scene_set_sky_faces(orcl_scene *scene, const int32_t *faces);
 /* Texture table the materials' texture indices refer to (copied). */
 void orcl_scene_set_textures(orcl_scene *scene, const orcl_texture *textures, uint32_t num_textures);
 /* Image::Sample of texture `index` at (u, v): rgb into out[3]. */

@@ -87,6 +87,7 @@ def lib():
             "orcl_scene_create": (P, [P, U32, P, U32, P, U32]),
             "orcl_scene_set_textures": (None, [P, P, U32]),
             "orcl_scene_set_spheres": (None, [P, P, U32]),
+            "orcl_scene_set_sky_faces": (None, [P, P]),
             "orcl_texture_sample": (None, [P, ctypes.c_int, ctypes.c_float, ctypes.c_float, P]),
             "orcl_scene_destroy": (None, [P]),
             "orcl_scene_num_triangles": (U64, [P]),
@@ -181,6 +182,9 @@ class OracleScene:
                 arr[k].width, arr[k].height, arr[k].format = px.shape[1], px.shape[0], fmt
                 arr[k].pixels = px.ctypes.data
             L.orcl_scene_set_textures(self.h, arr, len(texs))
+        faces = getattr(scene, "sky_faces", None)
+        if faces is not None:
+            L.orcl_scene_set_sky_faces(self.h, (ctypes.c_int32 * 6)(*faces))
 
     def texture_sample(self, index, u, v):
         out = np.zeros(3, np.float32)

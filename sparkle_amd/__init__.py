@@ -181,6 +181,7 @@ def load_library():
         "spk_scene_stats_get": (I, [P, ctypes.POINTER(SceneStats)]),
         "spk_textures_upload": (I, [P, P, U32]),
         "spk_scene_set_spheres": (I, [P, P, U32]),
+        "spk_set_sky_cubemap": (I, [P, P]),
         "spk_build_check": (I, [P, U32, P, U32, P, ctypes.POINTER(SceneStats), ctypes.c_char_p, ctypes.c_size_t]),
         "spk_trace_closest": (I, [P, P, U64, P]),
         "spk_trace_closest_async": (I, [P, P, U64, P]),
@@ -240,6 +241,7 @@ class Scene:
         self.materials = []
         self.textures = []  # (pixels (H, W, 4), TEXTURE_* format)
         self.spheres = []   # (center (3,), radius, material): analytic spheres
+        self.sky_faces = None  # six texture indices (+X, -X, +Y, -Y, +Z, -Z) of a sky cube map
 
     def add_mesh(self, positions, indices, normals=None, tangents=None, uvs=None):
         m = dict(positions=np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3),
@@ -389,7 +391,14 @@ class Context:
                                               len(scene.materials), ctypes.byref(opts)))
         if getattr(scene, "spheres", None):
             self.set_spheres(scene.spheres)
+        self.set_sky_cubemap(getattr(scene, "sky_faces", None))
         return self.stats()
+
+    def set_sky_cubemap(self, faces):
+        """spk_set_sky_cubemap: six texture indices (+X, -X, +Y, -Y, +Z, -Z) or None."""
+        if faces is None:
+            return self._check(self.lib.spk_set_sky_cubemap(self.h, None))
+        return self._check(self.lib.spk_set_sky_cubemap(self.h, (ctypes.c_uint32 * 6)(*[f + 1 for f in faces])))
 
     def set_spheres(self, spheres):
         """spk_scene_set_spheres: spheres is a list of (center, radius, material)."""

@@ -471,6 +471,7 @@ int spk_textures_upload(spk_context *ctx, const spk_texture_desc *textures, uint
                                         hipMemcpyHostToDevice)) != hipSuccess))
         return hip_fail(ctx, e, "texture upload");
     ctx->num_textures = num_textures;
+    for (int f = 0; f < 6; ++f) ctx->sky_faces[f] = -1;  // the sky cube map is set again after a new table
     // materials of an uploaded scene or batch may name textures that no longer exist
     ctx->scene.valid = false;
     ctx->on_scene_changed();

@@ -106,6 +106,7 @@ struct PtArgs {
     uint32_t num_scenes, grid;
     uint32_t refill;  // extend: a wave takes new paths once this many of its lanes are idle (1..64)
     uint32_t npix, max_bounce, debug_mode, sky_enabled, flags, sort;
+    int32_t sky_faces[6];  // cube-map faces +X, -X, +Y, -Y, +Z, -Z in the texture table, or -1
     float sky[3];
     float output_limit;
     // learned radiance cache
@@ -371,6 +372,25 @@ __device__ __forceinline__ float4 lerp_attr(const float *p0, const float *p1, co
                        p0[3] + (p1[3] - p0[3]) * u + (p2[3] - p0[3]) * v);
 }
 
+// Image2DCube::Sample (io/Image.cpp:298-357): face by the largest axis, uv in [0, 1]^2, then the
+// face image's bilinear sample
+__device__ __forceinline__ F3 sample_sky_cube(const PtArgs &a, F3 dir) {
+    const float x = dir.x, y = dir.y, z = dir.z;
+    const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+    int face;
+    float u, v;
+    if (ax >= ay && ax >= az) {
+        if (x > 0.f) { face = 0; u = -z / x; v = -y / x; } else { face = 1; u = z / -x; v = -y / -x; }
+    } else if (ay >= ax && ay >= az) {
+        if (y > 0.f) { face = 2; u = x / y; v = z / y; } else { face = 3; u = x / -y; v = -z / -y; }
+    } else {
+        if (z > 0.f) { face = 4; u = x / z; v = -y / z; } else { face = 5; u = -x / -z; v = -y / -z; }
+    }
+    u = (u + 1.f) * 0.5f;
+    v = (v + 1.f) * 0.5f;
+    return tex_sample(a.tex_table, a.texels, a.sky_faces[face], u, v);
+}
+
 // GetPossibleMajorAxis (Utilities.h:222-234)
 __device__ __forceinline__ F3 major_axis(F3 n) {
     const float inv_sqrt3 = 0.577350269189625764509f;
@@ -441,8 +461,14 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
             F3 C = f3(C4.x, C4.y, C4.z);
             if (h.x == 0xFFFFFFFFu) {
                 if (a.sky_enabled) {
-                    const float t = 0.5f * d.z + 0.5f;
-                    const F3 sky = f3(1.f, 1.f, 1.f) + (f3(a.sky[0], a.sky[1], a.sky[2]) - f3(1.f, 1.f, 1.f)) * t;
+                    F3 sky;
+                    if (a.sky_faces[0] >= 0) {
+                        sky = sample_sky_cube(a, d);
+                    } else {
+                        // top: colour, bottom: white (SkyRenderProxy.cpp:58-61)
+                        const float t = 0.5f * d.z + 0.5f;
+                        sky = f3(1.f, 1.f, 1.f) + (f3(a.sky[0], a.sky[1], a.sky[2]) - f3(1.f, 1.f, 1.f)) * t;
+                    }
                     C = C + cmul(sky, T);
                 }
             } else {
@@ -1004,6 +1030,24 @@ void spk_context::release_all() {
 
 extern "C" {
 
+int spk_set_sky_cubemap(spk_context *ctx, const uint32_t faces[6]) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (!faces) {
+        for (int f = 0; f < 6; ++f) ctx->sky_faces[f] = -1;
+        return SPK_OK;
+    }
+    const bool none = faces[0] == 0;
+    for (int f = 0; f < 6; ++f) {
+        if ((faces[f] == 0) != none)
+            return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "either all six faces or none");
+        if (faces[f] > ctx->num_textures)
+            return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "face %d names texture %u of %u", f, faces[f] - 1,
+                             ctx->num_textures);
+    }
+    for (int f = 0; f < 6; ++f) ctx->sky_faces[f] = (int32_t)faces[f] - 1;
+    return SPK_OK;
+}
+
 int spk_set_camera(spk_context *ctx, const spk_camera_desc *camera) {
     if (!ctx || !camera) return SPK_ERR_INVALID_ARGUMENT;
     for (float v : {camera->position[0], camera->position[1], camera->position[2], camera->lens_radius})
@@ -1096,6 +1140,7 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     a.max_bounce = frame->max_bounce;
     a.debug_mode = frame->debug_mode;
     a.sky_enabled = frame->sky_enabled;
+    for (int f = 0; f < 6; ++f) a.sky_faces[f] = ctx->sky_faces[f];
     a.flags = frame->flags & ~SPK_FRAME_NO_SORT;
     for (int c = 0; c < 3; ++c) a.sky[c] = frame->sky_color[c];
     a.output_limit = frame->output_limit > 0.f ? frame->output_limit : SPK_OUTPUT_LIMIT;

@@ -76,6 +76,7 @@ struct spk_context {
     spk::DeviceScene scene;
     spk::DeviceBuffer texels, tex_table;  // texture table (spk_textures_upload)
     uint32_t num_textures = 0;
+    int32_t sky_faces[6] = {-1, -1, -1, -1, -1, -1};  // sky cube map faces in the texture table
     spk::HostScene host_scene;
     std::vector<spk::MaterialGpu> host_mats;
     spk_scene_stats stats{};

@@ -301,3 +301,32 @@ def test_oracle_nee_is_unbiased_and_reduces_variance():
     z = (stats[True][0] - stats[False][0]) / np.hypot(stats[True][1], stats[False][1])
     assert abs(z) < 3.5, (z, stats)
     assert stats[True][2] < 0.5 * stats[False][2], stats
+
+
+def _cube_scene():
+    """A far-away triangle (scenes need geometry) and a sky cube map of six constant faces."""
+    sc = spk.Scene()
+    m = sc.add_mesh(np.array([[100, 100, 100], [101, 100, 100], [100, 101, 100]], np.float32),
+                    np.array([[0, 1, 2]], np.uint32), np.array([[0, 0, 1]] * 3, np.float32))
+    sc.add_instance(m, sc.add_material())
+    faces = []
+    for f in range(6):
+        px = np.zeros((1, 1, 4), np.uint8)
+        px[0, 0, :3] = (40 * f + 10, 255 - 40 * f, 7 * f)
+        faces.append(sc.add_texture(px, spk.TEXTURE_RGBA8_UNORM))
+    sc.sky_faces = faces
+    return sc
+
+
+def test_sky_cube_map_face_selection():
+    """Image2DCube::DirectionToTextureCoordinate: a camera looking along each axis sees that axis's
+    face (order +X, -X, +Y, -Y, +Z, -Z), sampled from the face texture."""
+    sc = _cube_scene()
+    o = refcpu.OracleScene(sc)
+    dirs = [(1, 0, 0.01), (-1, 0, 0.01), (0, 1, 0.01), (0, -1, 0.01), (0.01, 0.0, 1), (0.01, 0.0, -1)]
+    for f, front in enumerate(dirs):
+        cam = scenes.camera_desc(position=(0.0, 0.0, 0.0), front=front, up_hint=(0, 1, 0) if f >= 4 else (0, 0, 1),
+                                 vertical_fov=0.2, focus_distance=1.0, aspect=1.0)
+        img, _ = o.render_frame(cam, 8, 8, frame_seed=0, max_bounce=1, sky=(0.5, 0.5, 0.5))
+        want = np.array([40 * f + 10, 255 - 40 * f, 7 * f], np.float32) / np.float32(255.0)
+        np.testing.assert_array_equal(img[..., :3].reshape(-1, 3), np.tile(want, (64, 1)))

@@ -354,3 +354,33 @@ def test_analytic_spheres_in_the_path_tracer(gpu_ctx):
                    for k in range(n)])
     z = (fn.mean() - fp.mean()) / np.hypot(fn.std(ddof=1), fp.std(ddof=1)) * math.sqrt(n)
     assert abs(z) < 3.5, z
+
+
+def test_sky_cube_map_matches_oracle(gpu_ctx):
+    """Missed rays sample the sky cube map (random 8x8 faces, sRGB and linear) exactly as the oracle
+    does; with geometry in view the estimator stays statistically equal."""
+    from tests.test_oracle_render import _cube_scene
+
+    sc = _cube_scene()
+    rng = np.random.Generator(np.random.PCG64(12))
+    sc.textures = [(rng.integers(0, 256, (8, 8, 4), dtype=np.uint8), spk.TEXTURE_RGBA8_SRGB if f % 2 else
+                    spk.TEXTURE_RGBA8_UNORM) for f in range(6)]
+    w, h = 64, 48
+    cam = scenes.camera_desc(position=(0.0, 0.0, 0.0), front=(0.3, 0.8, 0.5), vertical_fov=2.2, focus_distance=1.0,
+                             aspect=w / h)
+    r = Renderer(gpu_ctx, sc, cam, w, h)
+    o = refcpu.OracleScene(sc)
+    g, _, _ = r.render(3, bounce=1, sky=(0.5, 0.5, 0.5))
+    c, _ = o.render_frame(cam, w, h, frame_seed=3, max_bounce=1, sky=(0.5, 0.5, 0.5))
+    assert (np.abs(g[..., :3] - c[..., :3]).max(-1) < 1e-5).mean() > 0.99
+    hf = scenes.heightfield_scene(cells=32)
+    hf.textures, hf.sky_faces = sc.textures, sc.sky_faces
+    cam2 = spk.camera_desc(position=(0.5, -0.8, 0.6), front=(0.0, 1.0, -0.3), aperture_radius=0.0, aspect=w / h)
+    r2 = Renderer(gpu_ctx, hf, cam2, w, h)
+    o2 = refcpu.OracleScene(hf)
+    frames = 16
+    d = np.stack([r2.render(k, sky=(0.5, 0.5, 0.5))[0][..., :3] - o2.render_frame(cam2, w, h, frame_seed=k,
+                                                                                    sky=(0.5, 0.5, 0.5))[0][..., :3]
+                  for k in range(frames)])
+    se = d.std(0, ddof=1) / math.sqrt(frames)
+    assert (np.abs(d.mean(0)) <= 3 * se + 1e-6).mean() >= 0.99
