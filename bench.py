@@ -106,7 +106,8 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     after their graph-capture frame) and accumulate only the frames after warm_frac * T. The reference
     uses seeds from 1,000,000 on (independent of every estimator), summed in float64 on the device.
     All frames use the 64-bit seed composition: at 1080p the reference's uint32 j*W + i + f*W*H wraps
-    from frame 2072 on.
+    from frame 2072 on. No per-sample clamp anywhere: clamped estimators converge to different
+    (biased) images, so an equal-time comparison against one reference would measure the clamp.
     """
     import torch
 
@@ -126,7 +127,7 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     def frame(seed, flags=0):
         f = spk.FrameDesc()
         f.width, f.height, f.frame_seed, f.max_bounce, f.debug_mode = width, height, seed, max_bounce, 0
-        f.sky_enabled, f.flags, f.output_limit = 0, flags | spk.FRAME_SEED64, 0.0
+        f.sky_enabled, f.flags, f.output_limit = 0, flags | spk.FRAME_SEED64, 1e30
         return f
 
     ctx.render_frame(frame(999_999), rgba)  # graph capture outside every timed region
@@ -153,8 +154,9 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
         ctx.accumulate(rgba, acc, npx, k, 1)
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
+    img = acc.view(height, width, 4)[..., :3].double()
     results = {"plain": {"frames": spp, "accumulated_spp": spp, "seconds": round(T, 3),
-                         "relmse": relmse(acc.view(height, width, 4)[..., :3].double(), ref)}}
+                         "relmse": relmse(img, ref), "mean": float(img.mean())}}
 
     variants = [("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
                 ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
@@ -181,8 +183,9 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
                 used += 1
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        img = acc.view(height, width, 4)[..., :3].double()
         r = {"frames": frames, "accumulated_spp": used, "seconds": round(dt, 3),
-             "relmse": relmse(acc.view(height, width, 4)[..., :3].double(), ref) if used else None}
+             "relmse": relmse(img, ref) if used else None, "mean": float(img.mean())}
         if bounce is not None:
             r["terminate_bounce"] = bounce
             r["last_loss"] = ctx.cache_info()["last_loss"]
@@ -195,6 +198,8 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
                  f"max {max_bounce} bounces, Russian roulette from bounce 3",
         "relmse_definition": "mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)",
         "reference_spp": ref_spp,
+        "reference_mean": float(ref.mean()),
+        "per_sample_clamp": None,
         "reference_s": round(t_ref, 2),
         "equal_time_s": round(T, 3),
         "estimators": results,
