@@ -308,6 +308,54 @@ def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_
             "bvh": {"width": st.bvh_width, "depth": st.depth, "nodes": st.num_nodes, "build_s": round(t_build, 2)}}
 
 
+def c1_experiment(spk, scenes, stream, width=256, height=256, spp=16, max_bounce=4):
+    """The reference's own CPU-runnable configuration: the Cornell room (1,316 triangles) at 256x256,
+    16 spp, 4 bounces, rendered on the device and by the oracle's restatement of the reference CPU
+    path on the host's threads (same per-pixel seeds), both timed; samples/s and the relMSE between
+    the two images."""
+    import torch
+    from oracle import refcpu
+
+    sc = scenes.cornell_scene(0)
+    cam = scenes.cornell_camera(width, height)
+    ctx = spk.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    ctx.upload_scene(sc)
+    ctx.set_camera(cam)
+    npx = width * height
+    rgba = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+    acc = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+
+    def frame(seed):
+        f = spk.FrameDesc()
+        f.width, f.height, f.frame_seed, f.max_bounce = width, height, seed, max_bounce
+        return f
+
+    ctx.render_frame(frame(10 ** 6), rgba)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(spp):
+        ctx.render_frame(frame(k), rgba)
+        ctx.accumulate(rgba, acc, npx, k, 1)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    gimg = acc.view(height, width, 4)[..., :3].double().cpu().numpy()
+    ctx.close()
+    threads = min(os.cpu_count() or 1, 16)
+    o = refcpu.OracleScene(sc)
+    cimg = np.zeros((height, width, 4), np.float32)
+    t0 = time.perf_counter()
+    for k in range(spp):
+        img, _ = o.render_frame(cam, width, height, frame_seed=k, max_bounce=max_bounce, threads=threads)
+        refcpu.accumulate(img, cimg, k, 1)
+    t_cpu = time.perf_counter() - t0
+    return {"scene": "Cornell room 1,316 triangles, 256x256, 16 spp, 4 bounces",
+            "gpu_s": round(t_gpu, 4), "gpu_samples_per_s": round(npx * spp / t_gpu, 1),
+            "cpu_s": round(t_cpu, 3), "cpu_samples_per_s": round(npx * spp / t_cpu, 1), "cpu_threads": threads,
+            "cpu_kind": "port (the oracle's restatement of CPURenderer, same seeds)",
+            "relmse_gpu_vs_cpu": relmse(gimg, cimg[..., :3].astype(np.float64))}
+
+
 def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
     """Many scenes of uneven size in one launch: `count` many-instance scenes (triangle budgets
     log-uniform in [1k, 500k], images 64^2..512^2), `spp` frames of `max_bounce` bounces each, all
@@ -555,6 +603,7 @@ def main():
     ctx.close()
     out["cache_kernels"] = cache_kernel_bench(spk, scenes, stream)
     out["full_path"] = render_experiment(spk, scenes, stream)
+    out["c1"] = c1_experiment(spk, scenes, stream)
     if not args.no_batch:
         out["batch"] = batch_experiment(spk, scenes, stream)
     if args.relmse != "off":
