@@ -92,22 +92,26 @@ def relmse(img, ref, eps=1e-2):
     return float(((img - ref) ** 2 / (ref ** 2 + eps)).mean())
 
 
-def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_bounce=8, warm_frac=0.1):
+def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_bounce=8, warm_frac=0.1,
+                      budgets=(1.0, 4.0)):
     """Equal-time comparison of estimators on the many-instance scene (about 5.5M triangles, 3 area
-    lights) against a ref_spp plain reference.
+    lights) against a ref_spp reference.
 
-    plain        the reference's estimator: spp frames (seeds 0..spp-1) accumulated by spk_accumulate;
-                 its wall time T is the budget of every other estimator
+    plain        the reference's estimator (BSDF sampling only): spp frames (seeds 0..spp-1)
+                 accumulated by spk_accumulate; its wall time T is the unit of the time budgets
     plain_nee    plain + next-event estimation
     cache        learned radiance cache (terminate at bounce 2, online training every frame)
     cache_nee    cache + next-event estimation
     cache1_nee   cache from bounce 1 + next-event estimation
-    Each estimator renders frames until T has passed; the cache estimators start untrained (reset
-    after their graph-capture frame) and accumulate only the frames after warm_frac * T. The reference
-    uses seeds from 1,000,000 on (independent of every estimator), summed in float64 on the device.
-    All frames use the 64-bit seed composition: at 1080p the reference's uint32 j*W + i + f*W*H wraps
-    from frame 2072 on. No per-sample clamp anywhere: clamped estimators converge to different
-    (biased) images, so an equal-time comparison against one reference would measure the clamp.
+    For each budget b*T every estimator renders frames until b*T has passed (plain at b = 1 renders
+    exactly spp frames); the cache estimators start untrained (reset after their graph-capture frame)
+    and accumulate only the frames after warm_frac of the budget. The reference is ref_spp frames of
+    the plain tracer with next-event estimation (same expectation as the reference's estimator, about
+    3x lower variance per frame, so the reference's own noise adds less to every relMSE), seeds from
+    1,000,000 on, summed in float64 on the device. All frames use the 64-bit seed composition: at
+    1080p the reference's uint32 j*W + i + f*W*H wraps from frame 2072 on. No per-sample clamp
+    anywhere: clamped estimators converge to different (biased) images, so an equal-time comparison
+    against one reference would measure the clamp.
     """
     import torch
 
@@ -130,15 +134,13 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
         f.sky_enabled, f.flags, f.output_limit = 0, flags | spk.FRAME_SEED64, 1e30
         return f
 
-    ctx.render_frame(frame(999_999), rgba)  # graph capture outside every timed region
-    torch.cuda.synchronize()
-
     log(f"relMSE experiment: reference at {ref_spp} spp")
+    ctx.render_frame(frame(999_997, spk.FRAME_NEE), rgba)  # graph capture
     ref = torch.zeros(npx * 4, dtype=torch.float64, device="cuda")
     t0 = time.perf_counter()
     rays_ref = 0
     for k in range(ref_spp):
-        rays_ref += ctx.render_frame(frame(1_000_000 + k), rgba).rays_traced
+        rays_ref += ctx.render_frame(frame(1_000_000 + k, spk.FRAME_NEE), rgba).rays_traced
         ref += rgba.double()
         if (k + 1) % 512 == 0:
             log(f"  reference {k + 1}/{ref_spp} spp, {time.perf_counter() - t0:.1f} s")
@@ -147,6 +149,7 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     ref = (ref / ref_spp).view(height, width, 4)[..., :3]
 
     log(f"relMSE experiment: plain at {spp} spp")
+    ctx.render_frame(frame(999_999), rgba)  # graph capture for the plain flags
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(spp):
@@ -155,58 +158,69 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
     img = acc.view(height, width, 4)[..., :3].double()
-    results = {"plain": {"frames": spp, "accumulated_spp": spp, "seconds": round(T, 3),
-                         "relmse": relmse(img, ref), "mean": float(img.mean())}}
+    first = {"plain": {"frames": spp, "accumulated_spp": spp, "seconds": round(T, 3),
+                       "relmse": relmse(img, ref), "mean": float(img.mean())}}
 
-    variants = [("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
+    variants = [("plain", 0, None), ("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
                 ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
-    for name, flags, bounce in variants:
-        log(f"relMSE experiment: {name} for {T:.2f} s")
-        if bounce is not None:
-            ctx.cache_create(spk.cache_config(terminate_bounce=bounce))
-        ctx.render_frame(frame(999_998, flags), rgba)  # graph capture
-        if bounce is not None:
-            ctx.cache_reset()
-        torch.cuda.synchronize()
-        acc.zero_()
-        frames = used = 0
-        warm = warm_frac * T if bounce is not None else 0.0
-        t0 = time.perf_counter()
-        while True:
-            el = time.perf_counter() - t0
-            if el >= T:
-                break
-            ctx.render_frame(frame(frames, flags), rgba)
-            frames += 1
-            if el >= warm:
-                ctx.accumulate(rgba, acc, npx, used, 1)
-                used += 1
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        img = acc.view(height, width, 4)[..., :3].double()
-        r = {"frames": frames, "accumulated_spp": used, "seconds": round(dt, 3),
-             "relmse": relmse(img, ref) if used else None, "mean": float(img.mean())}
-        if bounce is not None:
-            r["terminate_bounce"] = bounce
-            r["last_loss"] = ctx.cache_info()["last_loss"]
-            ctx.cache_destroy()
-        results[name] = r
+    by_budget = []
+    for b in budgets:
+        budget = b * T
+        results = dict(first) if b == 1.0 else {}
+        for name, flags, bounce in variants:
+            if name in results:
+                continue
+            log(f"relMSE experiment: {name} for {budget:.2f} s")
+            if bounce is not None:
+                ctx.cache_create(spk.cache_config(terminate_bounce=bounce))
+            ctx.render_frame(frame(999_998, flags), rgba)  # graph capture
+            if bounce is not None:
+                ctx.cache_reset()
+            torch.cuda.synchronize()
+            acc.zero_()
+            frames = used = 0
+            warm = warm_frac * budget if bounce is not None else 0.0
+            t0 = time.perf_counter()
+            while True:
+                el = time.perf_counter() - t0
+                if el >= budget:
+                    break
+                ctx.render_frame(frame(frames, flags), rgba)
+                frames += 1
+                if el >= warm:
+                    ctx.accumulate(rgba, acc, npx, used, 1)
+                    used += 1
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            img = acc.view(height, width, 4)[..., :3].double()
+            r = {"frames": frames, "accumulated_spp": used, "seconds": round(dt, 3),
+                 "relmse": relmse(img, ref) if used else None, "mean": float(img.mean())}
+            if bounce is not None:
+                r["terminate_bounce"] = bounce
+                r["last_loss"] = ctx.cache_info()["last_loss"]
+                ctx.cache_destroy()
+            results[name] = r
+        e_plain, e_nee = results["plain"]["relmse"], results["plain_nee"]["relmse"]
+        by_budget.append({
+            "seconds": round(budget, 3), "estimators": results,
+            "ratio_plain_over": {k: round(e_plain / v["relmse"], 3) for k, v in results.items()
+                                 if k != "plain" and v["relmse"]},
+            "ratio_plain_nee_over": {k: round(e_nee / v["relmse"], 3) for k, v in results.items()
+                                     if k.startswith("cache") and v["relmse"]}})
 
-    e_plain = results["plain"]["relmse"]
     out = {
         "scene": f"many-instance generator seed 0: {st.num_triangles} triangles, 3 area lights, {width}x{height}, "
                  f"max {max_bounce} bounces, Russian roulette from bounce 3",
         "relmse_definition": "mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)",
         "reference_spp": ref_spp,
+        "reference_estimator": "plain + next-event estimation",
         "reference_mean": float(ref.mean()),
         "per_sample_clamp": None,
         "reference_s": round(t_ref, 2),
         "equal_time_s": round(T, 3),
-        "estimators": results,
-        "ratio_plain_over": {k: round(e_plain / v["relmse"], 3) for k, v in results.items()
-                             if k != "plain" and v["relmse"]},
-        "full_path_samples_per_s": round(npx * ref_spp / t_ref, 1),
-        "full_path_mrays_per_s": round(rays_ref / t_ref / 1e6, 1),
+        "budgets": by_budget,
+        "full_path_nee_samples_per_s": round(npx * ref_spp / t_ref, 1),
+        "full_path_nee_mrays_per_s": round(rays_ref / t_ref / 1e6, 1),
         "bvh_build_s": round(t_build, 2),
     }
     ctx.close()
@@ -263,13 +277,22 @@ def cache_kernel_bench(spk, scenes, stream, n_infer=1920 * 1080, n_train=1 << 16
             "bound": "hash-grid gathers (16 levels x 8 corners x 4 B per query) and table-gradient atomics"}
 
 
-def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_bounce=8, autotune=True):
-    """The full plain path trace of the many-instance scene (about 5.5M triangles, 400 instances,
+RENDER_MODES = {"plain": (0, None), "plain_nee": ("NEE", None), "cache_nee": ("NEE", 2), "cache1_nee": ("NEE", 1),
+                "cache": (0, 2)}
+
+
+def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_bounce=8, autotune=True,
+                      mode="plain"):
+    """The full path trace of the many-instance scene (about 5.5M triangles, 400 instances,
     3 area lights): spp frames of one path per pixel, max_bounce bounces, Russian roulette from
-    bounce 3, material-sorted queues, the frame graph replayed; schedule from spk_render_autotune."""
+    bounce 3, material-sorted queues, the frame graph replayed; schedule from spk_render_autotune.
+    mode "plain" is the reference's estimator; the others add next-event estimation and/or the
+    learned cache (terminating at bounce 2, or 1 for cache1_nee, trained every frame)."""
     import torch
 
-    log(f"render experiment: many-instance scene {width}x{height}, {spp} spp")
+    nee, cache_bounce = RENDER_MODES[mode]
+    flags = (spk.FRAME_NEE if nee else 0) | (spk.FRAME_CACHE if cache_bounce is not None else 0)
+    log(f"render experiment: many-instance scene {width}x{height}, {spp} spp, {mode}")
     sc, cam = scenes.many_instance_scene(seed=0, aspect=width / height)
     ctx = spk.Context(0)
     ctx.set_stream(stream.cuda_stream)
@@ -284,9 +307,11 @@ def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_
     def frame(seed):
         f = spk.FrameDesc()
         f.width, f.height, f.frame_seed, f.max_bounce = width, height, seed, max_bounce
-        f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, 0, 0.0
+        f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, flags, 0.0
         return f
 
+    if cache_bounce is not None:
+        ctx.cache_create(spk.cache_config(terminate_bounce=cache_bounce))
     sched = ctx.schedule()
     if autotune:
         sched, _ = ctx.autotune_render(frame(10 ** 6), rgba)
@@ -301,7 +326,7 @@ def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_
     dt = time.perf_counter() - t0
     ctx.close()
     return {"scene": f"many-instance generator seed 0: {st.num_triangles} triangles, {width}x{height}, "
-                     f"max {max_bounce} bounces", "spp": spp, "seconds": round(dt, 3),
+                     f"max {max_bounce} bounces", "mode": mode, "spp": spp, "seconds": round(dt, 3),
             "ms_per_frame": round(dt / spp * 1e3, 3), "samples_per_s": round(npx * spp / dt, 1),
             "mrays_per_s": round(rays / dt / 1e6, 1), "rays_per_sample": round(rays / (npx * spp), 3),
             "schedule": {"blocks_per_cu": sched[0], "material_sort": bool(sched[1]), "refill": sched[2]},
@@ -426,6 +451,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
     ap.add_argument("--render-only", type=int, default=0,
                     help="only run the full path trace of the many-instance scene for N spp (one JSON line)")
+    ap.add_argument("--render-mode", choices=sorted(RENDER_MODES), default="plain",
+                    help="estimator of --render-only")
     ap.add_argument("--cache-kernels", action="store_true",
                     help="only time the learned cache's inference and training kernels (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -449,7 +476,8 @@ def main():
 
         stream = torch.cuda.Stream()
         torch.cuda.set_stream(stream)
-        print(json.dumps({"full_path": render_experiment(spk, scenes, stream, spp=args.render_only)}), flush=True)
+        print(json.dumps({"full_path": render_experiment(spk, scenes, stream, spp=args.render_only,
+                                                         mode=args.render_mode)}), flush=True)
         return
     if args.cache_kernels:
         import torch

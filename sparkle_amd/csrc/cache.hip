@@ -29,9 +29,8 @@ constexpr int kQ = 64;        // queries per wave tile
 constexpr int kIn = 32;       // encoding width (levels * features)
 constexpr int kH = 64;        // hidden width
 constexpr int kOutPad = 16;   // output rows padded to one MFMA tile
-constexpr int kStrideX = kIn + 8;   // LDS row strides (bf16), padded against bank conflicts
-constexpr int kStrideH = kH + 8;
-constexpr int kStrideG = kH + 8;
+constexpr int kStrideH = kH + 8;   // LDS row stride (bf16), padded against bank conflicts
+constexpr int kStrideG = kStrideH;
 
 struct CacheDev {
     const __half2 *table;      // [L*T]
@@ -51,10 +50,12 @@ __device__ __forceinline__ uint32_t grid_index(const CacheDev &c, uint32_t l, ui
     return ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & ((1u << c.log2_table) - 1u);
 }
 
-// trilinear hash-grid features of one point in [0,1]^3 (16 levels x 2 features)
-__device__ __forceinline__ void encode_point(const CacheDev &c, float px, float py, float pz, float *feat) {
+// trilinear hash-grid features of one point in [0,1]^3 (16 levels x 2 features), written as BF16 to
+// row[0..31] (zeros when !valid)
+__device__ __forceinline__ void encode_point(const CacheDev &c, float px, float py, float pz, bool valid,
+                                             __bf16 *row) {
     const uint32_t T = 1u << c.log2_table;
-#pragma unroll 4
+#pragma unroll 2
     for (uint32_t l = 0; l < 16; ++l) {
         const float res = (float)c.res[l];
         float pos[3] = {px * res, py * res, pz * res};
@@ -77,15 +78,16 @@ __device__ __forceinline__ void encode_point(const CacheDev &c, float px, float 
             f0 += w * __low2float(v);
             f1 += w * __high2float(v);
         }
-        feat[2 * l] = f0;
-        feat[2 * l + 1] = f1;
+        row[2 * l] = (__bf16)(valid ? f0 : 0.f);
+        row[2 * l + 1] = (__bf16)(valid ? f1 : 0.f);
     }
 }
 
-// scatter d(loss)/d(features) of one point into the FP32 table gradient
-__device__ __forceinline__ void encode_backward(const CacheDev &c, float px, float py, float pz, const float *g,
+// scatter d(loss)/d(features) of one point (BF16 row g[0..31]) into the FP32 table gradient
+__device__ __forceinline__ void encode_backward(const CacheDev &c, float px, float py, float pz, const __bf16 *g,
                                                 float *gtable) {
     const uint32_t T = 1u << c.log2_table;
+#pragma unroll 1
     for (uint32_t l = 0; l < 16; ++l) {
         const float res = (float)c.res[l];
         float pos[3] = {px * res, py * res, pz * res};
@@ -97,7 +99,7 @@ __device__ __forceinline__ void encode_backward(const CacheDev &c, float px, flo
             cell[a] = (uint32_t)cf;
             fr[a] = pos[a] - cf;
         }
-        const float g0 = g[2 * l], g1 = g[2 * l + 1];
+        const float g0 = (float)g[2 * l], g1 = (float)g[2 * l + 1];
         if (g0 == 0.f && g1 == 0.f) continue;
         for (int k = 0; k < 8; ++k) {
             const uint32_t cx = cell[0] + (k & 1), cy = cell[1] + ((k >> 1) & 1), cz = cell[2] + ((k >> 2) & 1);
@@ -113,11 +115,13 @@ __device__ __forceinline__ void encode_backward(const CacheDev &c, float px, flo
 __device__ __forceinline__ bf16x8 ld8(const __bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 
 // D[M_TILES*16 x 64] = W[M_TILES*16 x K_STEPS*32] * Hin^T, Hin rows [query][k] in LDS.
-// The result is written to Hout rows [query][neuron] as BF16, after ReLU when RELU. With MASK the
-// result is multiplied by (Mask[query][neuron] > 0) (the ReLU derivative of the forward pass).
-template <int M_TILES, int K_STEPS, bool RELU, bool MASK>
+// The result is written to Hout rows [query][neuron] as BF16, after ReLU when RELU (or as FP32 to
+// out_f32 [query][m_valid]). RECORD sets bit n of mask_out[query] where the ReLU output is positive;
+// MASK multiplies by bit n of mask_in[query] (the ReLU derivative of the forward pass).
+template <int M_TILES, int K_STEPS, bool RELU, bool MASK, bool RECORD>
 __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ W, const __bf16 *Hin, int sin, __bf16 *Hout,
-                                          int sout, const __bf16 *Mask, int smask, float *out_f32, int m_valid) {
+                                          int sout, const uint64_t *mask_in, uint64_t *mask_out, float *out_f32,
+                                          int m_valid) {
     const int lane = __lane_id();
     const int r = lane & 15, g = lane >> 4;
     bf16x8 a[M_TILES][K_STEPS];
@@ -131,6 +135,8 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ W, const __
         bf16x8 b[K_STEPS];
 #pragma unroll
         for (int ks = 0; ks < K_STEPS; ++ks) b[ks] = ld8(Hin + q * sin + 32 * ks + 8 * g);
+        const uint64_t min_bits = MASK ? mask_in[q] : 0ull;
+        uint64_t bits = 0ull;
 #pragma unroll
         for (int mt = 0; mt < M_TILES; ++mt) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -143,7 +149,8 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ W, const __
                 const int n = 16 * mt + 4 * g + i;
                 float v = acc[i];
                 if (RELU) v = fmaxf(v, 0.f);
-                if (MASK) v = ((float)Mask[q * smask + n] > 0.f) ? v : 0.f;
+                if (MASK) v = ((min_bits >> n) & 1ull) ? v : 0.f;
+                if (RECORD && v > 0.f) bits |= 1ull << n;
                 if (out_f32) {
                     if (n < m_valid) out_f32[q * m_valid + n] = v;
                 } else {
@@ -151,6 +158,7 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ W, const __
                 }
             }
         }
+        if (RECORD) atomicOr((unsigned long long *)&mask_out[q], (unsigned long long)bits);
     }
 }
 
@@ -159,20 +167,23 @@ struct TrainDev {
     float *loss;           // accumulated (sum of per-element losses / (n*3))
     float *gtable;         // FP32 table gradient
     __bf16 *act_t;         // transposed activations: 5 blocks [feature][n] (X is 32 rows, H 64 rows)
-    __bf16 *grad_t;        // transposed output gradients: 5 blocks [neuron][n] (dH1..dH4 64 rows, dY 16 rows)
+    __bf16 *grad_t;        // transposed output gradients: 5 blocks [neuron][n] (dH0..dH3 64 rows, dY 16 rows)
     uint32_t n_pad;        // column stride of act_t / grad_t
 };
 
 // Fused encode + MLP forward (and, when TRAIN, loss and backward to the table) for 64 queries per wave.
+// LDS: one activation tile [64][72] BF16 updated in place (each 16-query slice of a layer reads its
+// inputs before it writes its outputs, and a wave's LDS operations complete in order), the outputs,
+// and (training) the ReLU masks of the four hidden layers as one 64-bit word per query; activations
+// and gradients also go to the transposed global copies for the weight-gradient GEMM as soon as they
+// are produced. About 10 KB of LDS per wave.
 template <bool TRAIN>
 __device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__restrict__ points, uint32_t n,
                                            int points_are_world, float *__restrict__ out, const TrainDev &td,
                                            uint32_t base) {
-    constexpr int kActs = TRAIN ? 4 : 2;
-    __shared__ __bf16 X[kQ * kStrideX];
-    __shared__ __bf16 H[kActs][kQ * kStrideH];
-    __shared__ __bf16 G[TRAIN ? 2 : 1][kQ * kStrideG];
+    __shared__ __bf16 P[kQ * kStrideH];
     __shared__ float Y[kQ * 3];
+    __shared__ uint64_t MB[TRAIN ? 4 : 1][kQ];
     const int lane = __lane_id();
     const uint32_t q = base + lane;
     const bool valid = q < n;
@@ -184,27 +195,36 @@ __device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__res
             p[a] = fminf(fmaxf(v, 0.f), 1.f);
         }
     }
-    {
-        float feat[kIn];
-        encode_point(c, p[0], p[1], p[2], feat);
+    __bf16 *p0 = P, *p1 = P;  // the same tile: every layer runs in place
+    encode_point(c, p[0], p[1], p[2], valid, p0 + lane * kStrideH);
+    if (TRAIN)
+        for (int l = 0; l < 4; ++l) MB[l][lane] = 0ull;
+    __syncthreads();
+    // transposed stores for the weight-gradient GEMMs
+    const size_t blk = (size_t)64 * td.n_pad;
+    auto store_t = [&](const __bf16 *src, int stride, int rows, __bf16 *dst) {
+#pragma unroll 1
+        for (int k = 0; k < rows; k += 8) {
+            const bf16x8 v = ld8(src + lane * stride + k);
 #pragma unroll
-        for (int k = 0; k < kIn; ++k) X[lane * kStrideX + k] = (__bf16)(valid ? feat[k] : 0.f);
-    }
+            for (int i = 0; i < 8; ++i) dst[(size_t)(k + i) * td.n_pad + base + lane] = v[i];
+        }
+    };
+    // forward: X (p0) -> H0 (p1) -> H1 (p0) -> H2 (p1) -> H3 (p0) -> Y
+    if (TRAIN) store_t(p0, kStrideH, 32, td.act_t);
+    mlp_layer<4, 1, true, false, TRAIN>(c.wf[0], p0, kStrideH, p1, kStrideH, nullptr, MB[0], nullptr, 0);
     __syncthreads();
-    // forward: X -> H0 -> H1 -> H2 -> H3 -> Y
-    __bf16 *h0 = H[0];
-    __bf16 *h1 = H[1];
-    __bf16 *h2 = TRAIN ? H[2] : H[0];
-    __bf16 *h3 = TRAIN ? H[3] : H[1];
-    mlp_layer<4, 1, true, false>(c.wf[0], X, kStrideX, h0, kStrideH, nullptr, 0, nullptr, 0);
+    if (TRAIN) store_t(p1, kStrideH, 64, td.act_t + blk);
+    mlp_layer<4, 2, true, false, TRAIN>(c.wf[1], p1, kStrideH, p0, kStrideH, nullptr, MB[TRAIN ? 1 : 0], nullptr, 0);
     __syncthreads();
-    mlp_layer<4, 2, true, false>(c.wf[1], h0, kStrideH, h1, kStrideH, nullptr, 0, nullptr, 0);
+    if (TRAIN) store_t(p0, kStrideH, 64, td.act_t + 2 * blk);
+    mlp_layer<4, 2, true, false, TRAIN>(c.wf[2], p0, kStrideH, p1, kStrideH, nullptr, MB[TRAIN ? 2 : 0], nullptr, 0);
     __syncthreads();
-    mlp_layer<4, 2, true, false>(c.wf[2], h1, kStrideH, h2, kStrideH, nullptr, 0, nullptr, 0);
+    if (TRAIN) store_t(p1, kStrideH, 64, td.act_t + 3 * blk);
+    mlp_layer<4, 2, true, false, TRAIN>(c.wf[3], p1, kStrideH, p0, kStrideH, nullptr, MB[TRAIN ? 3 : 0], nullptr, 0);
     __syncthreads();
-    mlp_layer<4, 2, true, false>(c.wf[3], h2, kStrideH, h3, kStrideH, nullptr, 0, nullptr, 0);
-    __syncthreads();
-    mlp_layer<1, 2, false, false>(c.wf[4], h3, kStrideH, nullptr, 0, nullptr, 0, Y, 3);
+    if (TRAIN) store_t(p0, kStrideH, 64, td.act_t + 4 * blk);
+    mlp_layer<1, 2, false, false, false>(c.wf[4], p0, kStrideH, nullptr, 0, nullptr, nullptr, Y, 3);
     __syncthreads();
     if (!TRAIN) {
         if (valid)
@@ -215,7 +235,7 @@ __device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__res
     const float inv = 1.f / ((float)n * 3.f);
     float lsum = 0.f;
     {
-        __bf16 *dy = G[0];
+        __bf16 *dy = p1;
         for (int o = 0; o < 32; ++o) dy[lane * kStrideG + o] = (__bf16)0.f;
         if (valid) {
             for (int o = 0; o < 3; ++o) {
@@ -232,37 +252,22 @@ __device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__res
     if (lane == 0) atomicAdd(td.loss, lsum);
     __syncthreads();
     // backward: dH3 = (W5^T dY) * relu'(H3), dH2, dH1, dH0, dX = W1^T dH0
-    __bf16 *gA = G[0], *gB = G[1];
-    // transposed stores for the weight-gradient GEMMs: activations X,H0..H3 and gradients dH0..dH3,dY
-    auto store_t = [&](const __bf16 *src, int stride, int rows, __bf16 *dst) {
-        for (int k = 0; k < rows; ++k) dst[(size_t)k * td.n_pad + base + lane] = src[lane * stride + k];
-    };
-    const size_t blk = (size_t)64 * td.n_pad;
-    store_t(X, kStrideX, 32, td.act_t);
-    store_t(h0, kStrideH, 64, td.act_t + blk);
-    store_t(h1, kStrideH, 64, td.act_t + 2 * blk);
-    store_t(h2, kStrideH, 64, td.act_t + 3 * blk);
-    store_t(h3, kStrideH, 64, td.act_t + 4 * blk);
-    store_t(gA, kStrideG, 16, td.grad_t + 4 * blk);  // dY (3 rows used, 16 stored)
-    mlp_layer<4, 1, false, true>(c.wb[4], gA, kStrideG, gB, kStrideG, h3, kStrideH, nullptr, 0);  // dH3
+    store_t(p1, kStrideG, 16, td.grad_t + 4 * blk);  // dY (3 rows used, 16 stored)
+    mlp_layer<4, 1, false, true, false>(c.wb[4], p1, kStrideG, p0, kStrideG, MB[3], nullptr, nullptr, 0);  // dH3
     __syncthreads();
-    store_t(gB, kStrideG, 64, td.grad_t + 3 * blk);
-    mlp_layer<4, 2, false, true>(c.wb[3], gB, kStrideG, gA, kStrideG, h2, kStrideH, nullptr, 0);  // dH2
+    store_t(p0, kStrideG, 64, td.grad_t + 3 * blk);
+    mlp_layer<4, 2, false, true, false>(c.wb[3], p0, kStrideG, p1, kStrideG, MB[2], nullptr, nullptr, 0);  // dH2
     __syncthreads();
-    store_t(gA, kStrideG, 64, td.grad_t + 2 * blk);
-    mlp_layer<4, 2, false, true>(c.wb[2], gA, kStrideG, gB, kStrideG, h1, kStrideH, nullptr, 0);  // dH1
+    store_t(p1, kStrideG, 64, td.grad_t + 2 * blk);
+    mlp_layer<4, 2, false, true, false>(c.wb[2], p1, kStrideG, p0, kStrideG, MB[1], nullptr, nullptr, 0);  // dH1
     __syncthreads();
-    store_t(gB, kStrideG, 64, td.grad_t + 1 * blk);
-    mlp_layer<4, 2, false, true>(c.wb[1], gB, kStrideG, gA, kStrideG, h0, kStrideH, nullptr, 0);  // dH0
+    store_t(p0, kStrideG, 64, td.grad_t + 1 * blk);
+    mlp_layer<4, 2, false, true, false>(c.wb[1], p0, kStrideG, p1, kStrideG, MB[0], nullptr, nullptr, 0);  // dH0
     __syncthreads();
-    store_t(gA, kStrideG, 64, td.grad_t);
-    mlp_layer<2, 2, false, false>(c.wb[0], gA, kStrideG, gB, kStrideG, nullptr, 0, nullptr, 0);  // dX (32)
+    store_t(p1, kStrideG, 64, td.grad_t);
+    mlp_layer<2, 2, false, false, false>(c.wb[0], p1, kStrideG, p0, kStrideG, nullptr, nullptr, nullptr, 0);  // dX
     __syncthreads();
-    if (valid && td.gtable) {
-        float gx[kIn];
-        for (int k = 0; k < kIn; ++k) gx[k] = (float)gB[lane * kStrideG + k];
-        encode_backward(c, p[0], p[1], p[2], gx, td.gtable);
-    }
+    if (valid && td.gtable) encode_backward(c, p[0], p[1], p[2], p0 + lane * kStrideG, td.gtable);
 }
 
 // Persistent over 64-query tiles; the query count comes from n_dev (device) when given, capped at n.
