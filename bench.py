@@ -329,7 +329,8 @@ def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_
                      f"max {max_bounce} bounces", "mode": mode, "spp": spp, "seconds": round(dt, 3),
             "ms_per_frame": round(dt / spp * 1e3, 3), "samples_per_s": round(npx * spp / dt, 1),
             "mrays_per_s": round(rays / dt / 1e6, 1), "rays_per_sample": round(rays / (npx * spp), 3),
-            "schedule": {"blocks_per_cu": sched[0], "material_sort": bool(sched[1]), "refill": sched[2]},
+            "schedule": {"blocks_per_cu": sched[0], "material_sort": bool(sched[1]), "refill": sched[2],
+                         "tail_bounce": sched[3]},
             "bvh": {"width": st.bvh_width, "depth": st.depth, "nodes": st.num_nodes, "build_s": round(t_build, 2)}}
 
 
@@ -404,7 +405,7 @@ def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
         f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, 0, 0.0
         return f
 
-    (blocks, srt, refill), tune_ms = ctx.autotune_batch(frame(0), rgba)
+    (blocks, srt, refill, tail), tune_ms = ctx.autotune_batch(frame(0), rgba)
     torch.cuda.synchronize()
     rays = 0
     t0 = time.perf_counter()
@@ -426,7 +427,7 @@ def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
         "seconds": round(dt, 3),
         "samples_per_s": round(npx * spp / dt, 1),
         "mrays_per_s": round(rays / dt / 1e6, 1),
-        "schedule": {"blocks_per_cu": blocks, "material_sort": bool(srt), "refill": refill,
+        "schedule": {"blocks_per_cu": blocks, "material_sort": bool(srt), "refill": refill, "tail_bounce": tail,
                      "tuned_frame_ms": round(tune_ms, 3)},
         "build_s": round(t_build, 2),
         "finite": finite,

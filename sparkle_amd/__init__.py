@@ -142,10 +142,10 @@ class FrameStats(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("material_sort", ctypes.c_uint32), ("refill", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("tail_bounce", ctypes.c_uint32)]
 
     def as_tuple(self):
-        return (self.blocks_per_cu, self.material_sort, self.refill)
+        return (self.blocks_per_cu, self.material_sort, self.refill, self.tail_bounce)
 
 
 class BatchScene(ctypes.Structure):
@@ -508,14 +508,14 @@ class Context:
         return st
 
     def autotune_batch(self, frame, rgba_dev):
-        """spk_batch_autotune: returns ((blocks_per_cu, material_sort, refill), best frame ms)."""
+        """spk_batch_autotune: returns ((blocks_per_cu, material_sort, refill, tail_bounce), best frame ms)."""
         sc, ms = Schedule(), ctypes.c_double()
         self._check(self.lib.spk_batch_autotune(self.h, ctypes.byref(frame), _ptr(rgba_dev), ctypes.byref(sc),
                                                 ctypes.byref(ms)))
         return sc.as_tuple(), ms.value
 
     def autotune_render(self, frame, rgba_dev):
-        """spk_render_autotune: returns ((blocks_per_cu, material_sort, refill), best frame ms)."""
+        """spk_render_autotune: returns ((blocks_per_cu, material_sort, refill, tail_bounce), best frame ms)."""
         sc, ms = Schedule(), ctypes.c_double()
         self._check(self.lib.spk_render_autotune(self.h, ctypes.byref(frame), _ptr(rgba_dev), ctypes.byref(sc),
                                                  ctypes.byref(ms)))
@@ -526,8 +526,9 @@ class Context:
         self._check(self.lib.spk_get_schedule(self.h, 1 if batch else 0, ctypes.byref(sc)))
         return sc.as_tuple()
 
-    def set_schedule(self, blocks_per_cu, material_sort, refill, batch=False):
-        sc = Schedule(blocks_per_cu, material_sort, refill, 0)
+    def set_schedule(self, blocks_per_cu, material_sort, refill, tail_bounce=0, batch=False):
+        """spk_set_schedule; tail_bounce > 0 runs the segments from that index on one path per lane."""
+        sc = Schedule(blocks_per_cu, material_sort, refill, tail_bounce)
         return self._check(self.lib.spk_set_schedule(self.h, 1 if batch else 0, ctypes.byref(sc)))
 
     def accumulate(self, rgba_dev, accum_dev, num_pixels, cumulated, spp=1):

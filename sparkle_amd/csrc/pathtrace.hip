@@ -49,6 +49,8 @@ enum : uint32_t {
     GCTR_TRAIN = 1,     // training samples recorded (may exceed the capacity)
     GCTR_APPLY = 2,     // cache apply work fetch
     GCTR_TARGETS = 3,   // target work fetch
+    GCTR_TAIL_RAYS = 4,     // segments traced by the tail kernel
+    GCTR_TAIL_SHADOWS = 5,  // shadow rays traced by the tail kernel
 };
 enum : uint32_t {
     PF_NEE_PREV = 1u,  // the previous vertex sampled the emitters directly (MIS-weight emission hits)
@@ -105,6 +107,7 @@ struct PtArgs {
     uint32_t *path_scene;    // scene of each path (written only when num_scenes > 1)
     uint32_t num_scenes, grid;
     uint32_t refill;  // extend: a wave takes new paths once this many of its lanes are idle (1..64)
+    uint32_t tail_from;  // segments from this index on run in k_tail (max_bounce: none)
     uint32_t npix, max_bounce, debug_mode, sky_enabled, flags, sort;
     int32_t sky_faces[6];  // cube-map faces +X, -X, +Y, -Y, +Z, -Z in the texture table, or -1
     float sky[3];
@@ -130,13 +133,15 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *ctr) {
     return __shfl(base, 0);
 }
 
-// wave-aggregated append: returns the slot for lanes with `want`
+// wave-aggregated append over the lanes that execute it: returns the slot for lanes with `want`
+// (the lowest active lane takes the slots, so it also works inside divergent code)
 __device__ __forceinline__ uint32_t wave_append(uint32_t *ctr, bool want) {
     const uint64_t mask = __ballot(want);
     const uint32_t cnt = __popcll(mask);
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)__ballot(true)) - 1u;
     uint32_t base = 0;
-    if (lane_id() == 0 && cnt) base = atomicAdd(ctr, cnt);
-    base = __shfl(base, 0);
+    if (lane_id() == leader && cnt) base = atomicAdd(ctr, cnt);
+    base = __shfl(base, (int)leader);
     const uint64_t below = lane_id() == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane_id())));
     return base + (uint32_t)__popcll(below);
 }
@@ -233,6 +238,17 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
     if (tid == 0) a.counters[CTR_QUEUE] = a.npix;
 }
 
+// hit record of a finished closest-hit query: leaf triangle (or SPK_SPHERE_BIT | sphere), t and the
+// barycentrics, or a miss; spheres are tested after the BVH
+__device__ __forceinline__ uint4 hit_record(const PtArgs &a, const RayIn &r, BestHit &best) {
+    if (a.num_spheres) spheres_closest(a.spheres, a.num_spheres, a.sphere_base, r, best);
+    if (best.prim == 0xFFFFFFFFu) return make_uint4(0xFFFFFFFFu, __float_as_uint(INFINITY), 0u, 0u);
+    const float t = final_t(a.tris, r, best);
+    if (best.tri & SPK_SPHERE_BIT) return make_uint4(best.tri, __float_as_uint(t), 0u, 0u);
+    return make_uint4(best.tri, __float_as_uint(t), __float_as_uint(best.V / best.det),
+                      __float_as_uint(best.W / best.det));
+}
+
 // ---- extend: closest hit for every queued path + material histogram ----
 // Lanes trace one path each and a wave takes new queue slots (one atomic) once `refill` of its
 // lanes are idle, so a wave does not wait for its slowest ray before fetching more work.
@@ -279,23 +295,13 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
             continue;
         }
         if (active && !tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
-            uint4 h;
             uint32_t key = a.nkeys - 1u;  // misses sort last
             const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
             const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
-            if (a.num_spheres) spheres_closest(a.spheres, a.num_spheres, a.sphere_base, r, tr.best);
-            if (tr.best.prim != 0xFFFFFFFFu) {
-                const float t = final_t(a.tris, r, tr.best);
-                if (tr.best.tri & SPK_SPHERE_BIT) {
-                    h = make_uint4(tr.best.tri, __float_as_uint(t), 0u, 0u);
-                } else {
-                    h = make_uint4(tr.best.tri, __float_as_uint(t), __float_as_uint(tr.best.V / tr.best.det),
-                                   __float_as_uint(tr.best.W / tr.best.det));
-                }
-                key = hit_material(a, tr.best.tri);
+            const uint4 h = hit_record(a, r, tr.best);
+            if (h.x != 0xFFFFFFFFu) {
+                key = hit_material(a, h.x);
                 if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
-            } else {
-                h = make_uint4(0xFFFFFFFFu, __float_as_uint(INFINITY), 0u, 0u);
             }
             a.hits[slot] = h;
             if (a.sort) atomicAdd(&lhist[key], 1u);
@@ -433,6 +439,245 @@ __device__ __forceinline__ MatEval eval_material(const PtArgs &a, const Material
 }
 
 // ---- shade: SamplePixel's loop body (CPURenderer.cpp:261-368) ----
+// What shading one path segment produced: whether the path continues, its shadow ray (NEE), and
+// the cache query or training vertex it recorded.
+struct ShadeOut {
+    bool cont, shadow, cache_q, train;
+    float4 sh0, sh1, sh2;
+    F3 loc, Tv, Cv;
+};
+
+// Shades path `path` at hit record h of segment `bounce`: adds emission/sky to its colour, samples the
+// next direction (rays/thr/rng/prev_pdf written when it continues) and prepares the NEE shadow ray.
+// The colour is written back before the shadow ray's contribution is added, so the sum order per
+// path is the same whichever kernel runs the segment.
+__device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, bool last, bool nee, uint32_t path,
+                                           const uint4 h, ShadeOut &o) {
+    bool &cont = o.cont, &shadow = o.shadow, &cache_q = o.cache_q, &train = o.train;
+    float4 &sh0 = o.sh0, &sh1 = o.sh1, &sh2 = o.sh2;
+    F3 &loc = o.loc, &Tv = o.Tv, &Cv = o.Cv;
+    cont = shadow = cache_q = train = false;
+    sh0 = sh1 = sh2 = make_float4(0.f, 0.f, 0.f, 0.f);
+    loc = Tv = Cv = f3(0.f, 0.f, 0.f);
+    const float4 rd4 = a.rays[2 * path + 1], ro4 = a.rays[2 * path];
+    const F3 d = f3(rd4.x, rd4.y, rd4.z);
+    const float4 T4 = a.thr[path];
+    F3 T = f3(T4.x, T4.y, T4.z);
+    const uint32_t pf = __float_as_uint(T4.w);
+    float4 C4 = a.col[path];
+    F3 C = f3(C4.x, C4.y, C4.z);
+    if (h.x == 0xFFFFFFFFu) {
+        if (a.sky_enabled) {
+            F3 sky;
+            if (a.sky_faces[0] >= 0) {
+                sky = sample_sky_cube(a, d);
+            } else {
+                // top: colour, bottom: white (SkyRenderProxy.cpp:58-61)
+                const float t = 0.5f * d.z + 0.5f;
+                sky = f3(1.f, 1.f, 1.f) + (f3(a.sky[0], a.sky[1], a.sky[2]) - f3(1.f, 1.f, 1.f)) * t;
+            }
+            C = C + cmul(sky, T);
+        }
+    } else {
+        const float u = __uint_as_float(h.z), v = __uint_as_float(h.w), t = __uint_as_float(h.y);
+        loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
+        const bool is_sphere = (h.x & SPK_SPHERE_BIT) != 0u;
+        F3 normal, tangent;
+        float tu = 0.f, tv = 0.f;
+        MaterialGpu mat;
+        if (is_sphere) {
+            // SphereRenderProxy::GetIntersection (SphereRenderProxy.h:22-31): uv 0, no normal map
+            const uint32_t k = h.x & ~SPK_SPHERE_BIT;
+            const float4 sc = a.spheres[2 * k];
+            mat = a.mats[__float_as_uint(a.spheres[2 * k + 1].x)];
+            normal = normalized(loc - f3(sc.x, sc.y, sc.z));
+            tangent = major_axis(normal);
+        } else {
+            const TriShade &s = a.shade[h.x];
+            const float4 n4 = lerp_attr(s.n0, s.n1, s.n2, u, v);
+            const float4 t4 = lerp_attr(s.t0, s.t1, s.t2, u, v);
+            // texture coordinate (Mesh.h:119-132)
+            tu = s.uv0[0] + (s.uv1[0] - s.uv0[0]) * u + (s.uv2[0] - s.uv0[0]) * v;
+            tv = s.uv0[1] + (s.uv1[1] - s.uv0[1]) * u + (s.uv2[1] - s.uv0[1]) * v;
+            mat = a.mats[s.material];
+            F3 nrm = f3(n4.x, n4.y, n4.z);
+            tangent = f3(t4.x, t4.y, t4.z);
+            if (mat.tex_normal >= 0) {
+                // normal map in the tangent frame (MeshRenderProxy.cpp:350-356, Utilities.h:398-404)
+                const F3 tn =
+                    tex_sample(a.tex_table, a.texels, mat.tex_normal, tu, tv) * 2.f - f3(1.f, 1.f, 1.f);
+                const F3 bit = cross(nrm, tangent) * t4.w;
+                nrm = normalized(tangent * tn.x + bit * tn.y + nrm * tn.z);
+            }
+            normal = normalized(nrm);
+        }
+        const MatEval me = eval_material(a, mat, tu, tv);
+        if (bounce == 0u && a.normal_out) {
+            a.normal_out[3 * (size_t)path] = normal.x;
+            a.normal_out[3 * (size_t)path + 1] = normal.y;
+            a.normal_out[3 * (size_t)path + 2] = normal.z;
+        }
+        // the cache is a function of position only, so it stands in for the path tail only at
+        // vertices whose reflected radiance barely depends on direction (non-metal PBR); paths
+        // continue through metal and glass vertices to the next such vertex
+        const bool cache_vertex = a.cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
+                                  me.metallic < 0.5f;
+        if (me.emits) {
+            if ((pf & PF_NEE_PREV) && !is_sphere) {  // spheres are not among the sampled emitters
+                // the previous vertex also sampled this emitter: balance-heuristic weight of
+                // the BSDF-sampled direction against light sampling of this point
+                const uint4 *tp = a.tris + (size_t)h.x * 3u;
+                const uint4 va = tp[0], vb = tp[1], vc = tp[2];
+                const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
+                const F3 ng = cross(f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z)) - p0,
+                                    f3(__uint_as_float(vc.x), __uint_as_float(vc.y), __uint_as_float(vc.z)) - p0);
+                const float cos_y = fabsf(dot(ng, d)) / sqrtf(sqnorm(ng));
+                const float p_light = light_pdf_area(a, mat) * t * t / cos_y;
+                const float p_bsdf = a.prev_pdf[path];
+                if (cos_y > 0.f) C = C + cmul(me.emissive, T) * (p_bsdf / (p_bsdf + p_light));
+            } else {
+                C = C + cmul(me.emissive, T);
+            }
+        } else if (cache_vertex && !(pf & PF_TRAIN)) {
+            // terminate into the cache: radiance leaving this vertex comes from the network
+            cache_q = true;
+            Tv = T;
+        } else {
+            if (cache_vertex) {  // a training path: record the vertex, its target follows
+                train = true;
+                Tv = T;
+                Cv = C;
+            }
+            uint4 rs = a.rng[path];
+            Rng rng{rs.x, rs.y, rs.z, rs.w};
+            F3 wi;
+            bool diffuse = false;
+            const F3 f = sample_surface(rng, mat, me.base, me.metallic, me.roughness, d, normal, tangent, wi,
+                                        diffuse);
+            bool done = false;
+            if (a.debug_mode != SPK_DEBUG_COLOR && bounce == 0u) {
+                switch (a.debug_mode) {
+                    case 3: C = (normal + f3(1.f, 1.f, 1.f)) * 0.5f; break;
+                    case 4: C = (wi + f3(1.f, 1.f, 1.f)) * 0.5f; break;
+                    case 7: C = f3(1.f, 1.f, 1.f) * me.metallic; break;
+                    case 8: C = f3(1.f, 1.f, 1.f) * me.roughness; break;
+                    case 9: C = me.base; break;
+                    case 10: C = me.emissive; break;
+                    case 11: {
+                        const spk_camera_desc &cam = scene_of_path(a, path).cam;
+                        C = f3(1.f, 1.f, 1.f) * (dot(loc - ld3(cam.position), ld3(cam.front)) / cam.far_plane);
+                        break;
+                    }
+                    default: break;
+                }
+                done = true;
+            }
+            if (!done && nee && diffuse && !last) {
+                // next-event estimation for the Lambertian lobe: one area-sampled point on an
+                // emissive triangle, Lambert BRDF base/pi, two-sided emitters. Not at the last
+                // vertex: the BSDF-only estimator never reaches an emitter from there.
+                uint4 ns = a.nee_rng[path];
+                Rng nr{ns.x, ns.y, ns.z, ns.w};
+                const float r0 = rng_unit(nr) * a.emitter_total;
+                uint32_t lo = 0, hi = a.num_emitters - 1u;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (a.emitter_cdf[mid] < r0) lo = mid + 1u; else hi = mid;
+                }
+                const uint32_t et = a.emitters[lo];
+                const float w_prev = lo ? a.emitter_cdf[lo - 1u] : 0.f;
+                const float w_tri = a.emitter_cdf[lo] - w_prev;
+                const float r1 = rng_unit(nr), r2 = rng_unit(nr);
+                a.nee_rng[path] = make_uint4(nr.s0, nr.s1, nr.s2, nr.s3);
+                const uint4 *tp = a.tris + (size_t)et * 3u;
+                const uint4 va = tp[0], vb = tp[1], vc = tp[2];
+                const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
+                const F3 p1 = f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z));
+                const F3 p2 = f3(__uint_as_float(vc.x), __uint_as_float(vc.y), __uint_as_float(vc.z));
+                const float su = sqrtf(r1);
+                const float b0 = 1.f - su, b1 = r2 * su;
+                const F3 y = p0 * b0 + p1 * b1 + p2 * (1.f - b0 - b1);
+                const F3 ng = cross(p1 - p0, p2 - p0);
+                const float area2 = sqrtf(sqnorm(ng));
+                const F3 dl = y - loc;
+                const float dist2 = sqnorm(dl);
+                const float dist = sqrtf(dist2);
+                const F3 wl = dl / dist;
+                const float cos_x = dot(normal, wl);
+                const float cos_y = fabsf(dot(ng, wl)) / area2;
+                const MaterialGpu &em = a.mats[vb.w];  // v1.w carries the material id
+                // emitted radiance at the sampled point (uv from its barycentrics)
+                const TriShade &es = a.shade[et];
+                const float bu = b1, bv = 1.f - b0 - b1;
+                const MatEval lme = eval_material(a, em, es.uv0[0] + (es.uv1[0] - es.uv0[0]) * bu +
+                                                             (es.uv2[0] - es.uv0[0]) * bv,
+                                                  es.uv0[1] + (es.uv1[1] - es.uv0[1]) * bu +
+                                                      (es.uv2[1] - es.uv0[1]) * bv);
+                if (lme.emits && cos_x > 0.f && cos_y > 0.f && dist > 2e-4f && w_tri > 0.f) {
+                    // pdf over area: (w_tri / total) / area; contribution f*cos_x*Le*cos_y/(d^2 pdf),
+                    // weighted by the balance heuristic against the cosine-sampled direction
+                    const float area = 0.5f * area2;
+                    const float p_light = light_pdf_area(a, em) * dist2 / cos_y;
+                    const float p_bsdf = cos_x / SPK_PI;
+                    const float w_mis = p_light / (p_light + p_bsdf);
+                    const float g = cos_x * cos_y / dist2 * area * a.emitter_total / w_tri / SPK_PI * w_mis;
+                    const F3 base3 = me.base;
+                    const F3 Le = lme.emissive;
+                    const F3 contrib = cmul(cmul(T, base3), Le) * g;
+                    shadow = true;
+                    sh0 = make_float4(loc.x + wl.x * SPK_TOLERANCE, loc.y + wl.y * SPK_TOLERANCE,
+                                      loc.z + wl.z * SPK_TOLERANCE, dist - 2.f * SPK_TOLERANCE);
+                    sh1 = make_float4(wl.x, wl.y, wl.z, __uint_as_float(path));
+                    sh2 = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
+                }
+            }
+            if (!done) {
+                T = cmul(T, f);
+                if (sqnorm(T) < SPK_EPS) {
+                    C4.w = -1.f;
+                } else {
+                    bool alive = true;
+                    if (bounce >= 3u) {
+                        float p = fmaxf(fmaxf(T.x, T.y), T.z);
+                        p = fminf(fmaxf(p, 0.05f), 1.0f);
+                        if (rng_unit(rng) > p) alive = false; else T = T / p;
+                    }
+                    if (alive && !last) {
+                        cont = true;
+                        const F3 o2 = loc + wi * SPK_TOLERANCE;
+                        const uint32_t npf = (pf & PF_TRAIN) | ((nee && diffuse) ? PF_NEE_PREV : 0u);
+                        if (nee && diffuse) a.prev_pdf[path] = fmaxf(dot(wi, normal), 0.f) / SPK_PI;
+                        a.rays[2 * path] = make_float4(o2.x, o2.y, o2.z, SPK_EPS);
+                        a.rays[2 * path + 1] = make_float4(wi.x, wi.y, wi.z, INFINITY);
+                        a.thr[path] = make_float4(T.x, T.y, T.z, __uint_as_float(npf));
+                        a.rng[path] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+                    }
+                }
+            }
+        }
+    }
+    a.col[path] = make_float4(C.x, C.y, C.z, C4.w);
+}
+
+// cache query / training vertex records of a shaded segment (wave-aggregated over the active lanes)
+__device__ __forceinline__ void append_cache_records(const PtArgs &a, uint32_t path, const ShadeOut &o) {
+    const uint32_t cs = wave_append(a.counters + kCtrGlobal + GCTR_CACHE_Q, o.cache_q);
+    if (o.cache_q) {
+        a.cq_points[3 * (size_t)cs] = o.loc.x;
+        a.cq_points[3 * (size_t)cs + 1] = o.loc.y;
+        a.cq_points[3 * (size_t)cs + 2] = o.loc.z;
+        a.cq_meta[cs] = make_float4(o.Tv.x, o.Tv.y, o.Tv.z, __uint_as_float(path));
+    }
+    const uint32_t ts = wave_append(a.counters + kCtrGlobal + GCTR_TRAIN, o.train);
+    if (o.train && ts < a.train_cap) {
+        a.tr_points[3 * (size_t)ts] = o.loc.x;
+        a.tr_points[3 * (size_t)ts + 1] = o.loc.y;
+        a.tr_points[3 * (size_t)ts + 2] = o.loc.z;
+        a.tr_meta0[ts] = make_float4(o.Cv.x, o.Cv.y, o.Cv.z, __uint_as_float(path));
+        a.tr_meta1[ts] = make_float4(o.Tv.x, o.Tv.y, o.Tv.z, 0.f);
+    }
+}
+
 __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
     uint32_t *ctr = a.counters + bounce * kCtrStride;
     const uint32_t count = ctr[CTR_QUEUE];
@@ -444,239 +689,48 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
         const uint32_t base = wave_fetch(ctr + CTR_SHADE);
         if (base >= count) break;
         const uint32_t idx = base + lane_id();
-        bool cont = false, shadow = false, cache_q = false, train = false;
         uint32_t path = 0;
-        float4 sh0 = make_float4(0.f, 0.f, 0.f, 0.f), sh1 = sh0, sh2 = sh0;
-        F3 loc = f3(0.f, 0.f, 0.f), Tv = loc, Cv = loc;
+        ShadeOut o;
+        o.cont = o.shadow = o.cache_q = o.train = false;
         if (idx < count) {
             const uint32_t slot = a.sort ? a.sorted[idx] : idx;
             path = queue[slot];
-            const uint4 h = a.hits[slot];
-            const float4 rd4 = a.rays[2 * path + 1], ro4 = a.rays[2 * path];
-            const F3 d = f3(rd4.x, rd4.y, rd4.z);
-            const float4 T4 = a.thr[path];
-            F3 T = f3(T4.x, T4.y, T4.z);
-            const uint32_t pf = __float_as_uint(T4.w);
-            float4 C4 = a.col[path];
-            F3 C = f3(C4.x, C4.y, C4.z);
-            if (h.x == 0xFFFFFFFFu) {
-                if (a.sky_enabled) {
-                    F3 sky;
-                    if (a.sky_faces[0] >= 0) {
-                        sky = sample_sky_cube(a, d);
-                    } else {
-                        // top: colour, bottom: white (SkyRenderProxy.cpp:58-61)
-                        const float t = 0.5f * d.z + 0.5f;
-                        sky = f3(1.f, 1.f, 1.f) + (f3(a.sky[0], a.sky[1], a.sky[2]) - f3(1.f, 1.f, 1.f)) * t;
-                    }
-                    C = C + cmul(sky, T);
-                }
-            } else {
-                const float u = __uint_as_float(h.z), v = __uint_as_float(h.w), t = __uint_as_float(h.y);
-                loc = f3(ro4.x, ro4.y, ro4.z) + d * t;
-                const bool is_sphere = (h.x & SPK_SPHERE_BIT) != 0u;
-                F3 normal, tangent;
-                float tu = 0.f, tv = 0.f;
-                MaterialGpu mat;
-                if (is_sphere) {
-                    // SphereRenderProxy::GetIntersection (SphereRenderProxy.h:22-31): uv 0, no normal map
-                    const uint32_t k = h.x & ~SPK_SPHERE_BIT;
-                    const float4 sc = a.spheres[2 * k];
-                    mat = a.mats[__float_as_uint(a.spheres[2 * k + 1].x)];
-                    normal = normalized(loc - f3(sc.x, sc.y, sc.z));
-                    tangent = major_axis(normal);
-                } else {
-                    const TriShade &s = a.shade[h.x];
-                    const float4 n4 = lerp_attr(s.n0, s.n1, s.n2, u, v);
-                    const float4 t4 = lerp_attr(s.t0, s.t1, s.t2, u, v);
-                    // texture coordinate (Mesh.h:119-132)
-                    tu = s.uv0[0] + (s.uv1[0] - s.uv0[0]) * u + (s.uv2[0] - s.uv0[0]) * v;
-                    tv = s.uv0[1] + (s.uv1[1] - s.uv0[1]) * u + (s.uv2[1] - s.uv0[1]) * v;
-                    mat = a.mats[s.material];
-                    F3 nrm = f3(n4.x, n4.y, n4.z);
-                    tangent = f3(t4.x, t4.y, t4.z);
-                    if (mat.tex_normal >= 0) {
-                        // normal map in the tangent frame (MeshRenderProxy.cpp:350-356, Utilities.h:398-404)
-                        const F3 tn =
-                            tex_sample(a.tex_table, a.texels, mat.tex_normal, tu, tv) * 2.f - f3(1.f, 1.f, 1.f);
-                        const F3 bit = cross(nrm, tangent) * t4.w;
-                        nrm = normalized(tangent * tn.x + bit * tn.y + nrm * tn.z);
-                    }
-                    normal = normalized(nrm);
-                }
-                const MatEval me = eval_material(a, mat, tu, tv);
-                if (bounce == 0u && a.normal_out) {
-                    a.normal_out[3 * (size_t)path] = normal.x;
-                    a.normal_out[3 * (size_t)path + 1] = normal.y;
-                    a.normal_out[3 * (size_t)path + 2] = normal.z;
-                }
-                // the cache is a function of position only, so it stands in for the path tail only at
-                // vertices whose reflected radiance barely depends on direction (non-metal PBR); paths
-                // continue through metal and glass vertices to the next such vertex
-                const bool cache_vertex = a.cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
-                                          me.metallic < 0.5f;
-                if (me.emits) {
-                    if ((pf & PF_NEE_PREV) && !is_sphere) {  // spheres are not among the sampled emitters
-                        // the previous vertex also sampled this emitter: balance-heuristic weight of
-                        // the BSDF-sampled direction against light sampling of this point
-                        const uint4 *tp = a.tris + (size_t)h.x * 3u;
-                        const uint4 va = tp[0], vb = tp[1], vc = tp[2];
-                        const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
-                        const F3 ng = cross(f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z)) - p0,
-                                            f3(__uint_as_float(vc.x), __uint_as_float(vc.y), __uint_as_float(vc.z)) - p0);
-                        const float cos_y = fabsf(dot(ng, d)) / sqrtf(sqnorm(ng));
-                        const float p_light = light_pdf_area(a, mat) * t * t / cos_y;
-                        const float p_bsdf = a.prev_pdf[path];
-                        if (cos_y > 0.f) C = C + cmul(me.emissive, T) * (p_bsdf / (p_bsdf + p_light));
-                    } else {
-                        C = C + cmul(me.emissive, T);
-                    }
-                } else if (cache_vertex && !(pf & PF_TRAIN)) {
-                    // terminate into the cache: radiance leaving this vertex comes from the network
-                    cache_q = true;
-                    Tv = T;
-                } else {
-                    if (cache_vertex) {  // a training path: record the vertex, its target follows
-                        train = true;
-                        Tv = T;
-                        Cv = C;
-                    }
-                    uint4 rs = a.rng[path];
-                    Rng rng{rs.x, rs.y, rs.z, rs.w};
-                    F3 wi;
-                    bool diffuse = false;
-                    const F3 f = sample_surface(rng, mat, me.base, me.metallic, me.roughness, d, normal, tangent, wi,
-                                                diffuse);
-                    bool done = false;
-                    if (a.debug_mode != SPK_DEBUG_COLOR && bounce == 0u) {
-                        switch (a.debug_mode) {
-                            case 3: C = (normal + f3(1.f, 1.f, 1.f)) * 0.5f; break;
-                            case 4: C = (wi + f3(1.f, 1.f, 1.f)) * 0.5f; break;
-                            case 7: C = f3(1.f, 1.f, 1.f) * me.metallic; break;
-                            case 8: C = f3(1.f, 1.f, 1.f) * me.roughness; break;
-                            case 9: C = me.base; break;
-                            case 10: C = me.emissive; break;
-                            case 11: {
-                                const spk_camera_desc &cam = scene_of_path(a, path).cam;
-                                C = f3(1.f, 1.f, 1.f) * (dot(loc - ld3(cam.position), ld3(cam.front)) / cam.far_plane);
-                                break;
-                            }
-                            default: break;
-                        }
-                        done = true;
-                    }
-                    if (!done && nee && diffuse && !last) {
-                        // next-event estimation for the Lambertian lobe: one area-sampled point on an
-                        // emissive triangle, Lambert BRDF base/pi, two-sided emitters. Not at the last
-                        // vertex: the BSDF-only estimator never reaches an emitter from there.
-                        uint4 ns = a.nee_rng[path];
-                        Rng nr{ns.x, ns.y, ns.z, ns.w};
-                        const float r0 = rng_unit(nr) * a.emitter_total;
-                        uint32_t lo = 0, hi = a.num_emitters - 1u;
-                        while (lo < hi) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (a.emitter_cdf[mid] < r0) lo = mid + 1u; else hi = mid;
-                        }
-                        const uint32_t et = a.emitters[lo];
-                        const float w_prev = lo ? a.emitter_cdf[lo - 1u] : 0.f;
-                        const float w_tri = a.emitter_cdf[lo] - w_prev;
-                        const float r1 = rng_unit(nr), r2 = rng_unit(nr);
-                        a.nee_rng[path] = make_uint4(nr.s0, nr.s1, nr.s2, nr.s3);
-                        const uint4 *tp = a.tris + (size_t)et * 3u;
-                        const uint4 va = tp[0], vb = tp[1], vc = tp[2];
-                        const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
-                        const F3 p1 = f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z));
-                        const F3 p2 = f3(__uint_as_float(vc.x), __uint_as_float(vc.y), __uint_as_float(vc.z));
-                        const float su = sqrtf(r1);
-                        const float b0 = 1.f - su, b1 = r2 * su;
-                        const F3 y = p0 * b0 + p1 * b1 + p2 * (1.f - b0 - b1);
-                        const F3 ng = cross(p1 - p0, p2 - p0);
-                        const float area2 = sqrtf(sqnorm(ng));
-                        const F3 dl = y - loc;
-                        const float dist2 = sqnorm(dl);
-                        const float dist = sqrtf(dist2);
-                        const F3 wl = dl / dist;
-                        const float cos_x = dot(normal, wl);
-                        const float cos_y = fabsf(dot(ng, wl)) / area2;
-                        const MaterialGpu &em = a.mats[vb.w];  // v1.w carries the material id
-                        // emitted radiance at the sampled point (uv from its barycentrics)
-                        const TriShade &es = a.shade[et];
-                        const float bu = b1, bv = 1.f - b0 - b1;
-                        const MatEval lme = eval_material(a, em, es.uv0[0] + (es.uv1[0] - es.uv0[0]) * bu +
-                                                                     (es.uv2[0] - es.uv0[0]) * bv,
-                                                          es.uv0[1] + (es.uv1[1] - es.uv0[1]) * bu +
-                                                              (es.uv2[1] - es.uv0[1]) * bv);
-                        if (lme.emits && cos_x > 0.f && cos_y > 0.f && dist > 2e-4f && w_tri > 0.f) {
-                            // pdf over area: (w_tri / total) / area; contribution f*cos_x*Le*cos_y/(d^2 pdf),
-                            // weighted by the balance heuristic against the cosine-sampled direction
-                            const float area = 0.5f * area2;
-                            const float p_light = light_pdf_area(a, em) * dist2 / cos_y;
-                            const float p_bsdf = cos_x / SPK_PI;
-                            const float w_mis = p_light / (p_light + p_bsdf);
-                            const float g = cos_x * cos_y / dist2 * area * a.emitter_total / w_tri / SPK_PI * w_mis;
-                            const F3 base3 = me.base;
-                            const F3 Le = lme.emissive;
-                            const F3 contrib = cmul(cmul(T, base3), Le) * g;
-                            shadow = true;
-                            sh0 = make_float4(loc.x + wl.x * SPK_TOLERANCE, loc.y + wl.y * SPK_TOLERANCE,
-                                              loc.z + wl.z * SPK_TOLERANCE, dist - 2.f * SPK_TOLERANCE);
-                            sh1 = make_float4(wl.x, wl.y, wl.z, __uint_as_float(path));
-                            sh2 = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
-                        }
-                    }
-                    if (!done) {
-                        T = cmul(T, f);
-                        if (sqnorm(T) < SPK_EPS) {
-                            C4.w = -1.f;
-                        } else {
-                            bool alive = true;
-                            if (bounce >= 3u) {
-                                float p = fmaxf(fmaxf(T.x, T.y), T.z);
-                                p = fminf(fmaxf(p, 0.05f), 1.0f);
-                                if (rng_unit(rng) > p) alive = false; else T = T / p;
-                            }
-                            if (alive && !last) {
-                                cont = true;
-                                const F3 o2 = loc + wi * SPK_TOLERANCE;
-                                const uint32_t npf = (pf & PF_TRAIN) | ((nee && diffuse) ? PF_NEE_PREV : 0u);
-                                if (nee && diffuse) a.prev_pdf[path] = fmaxf(dot(wi, normal), 0.f) / SPK_PI;
-                                a.rays[2 * path] = make_float4(o2.x, o2.y, o2.z, SPK_EPS);
-                                a.rays[2 * path + 1] = make_float4(wi.x, wi.y, wi.z, INFINITY);
-                                a.thr[path] = make_float4(T.x, T.y, T.z, __uint_as_float(npf));
-                                a.rng[path] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
-                            }
-                        }
-                    }
-                }
-            }
-            a.col[path] = make_float4(C.x, C.y, C.z, C4.w);
+            shade_path(a, bounce, last, nee, path, a.hits[slot], o);
         }
-        const uint32_t qs = wave_append(a.counters + (bounce + 1u) * kCtrStride + CTR_QUEUE, cont);
-        if (cont) next_queue[qs] = path;
-        const uint32_t ss = wave_append(ctr + CTR_SHADOWS, shadow);
-        if (shadow) {
-            a.shadow[3 * (size_t)ss] = sh0;
-            a.shadow[3 * (size_t)ss + 1] = sh1;
-            a.shadow[3 * (size_t)ss + 2] = sh2;
+        const uint32_t qs = wave_append(a.counters + (bounce + 1u) * kCtrStride + CTR_QUEUE, o.cont);
+        if (o.cont) next_queue[qs] = path;
+        const uint32_t ss = wave_append(ctr + CTR_SHADOWS, o.shadow);
+        if (o.shadow) {
+            a.shadow[3 * (size_t)ss] = o.sh0;
+            a.shadow[3 * (size_t)ss + 1] = o.sh1;
+            a.shadow[3 * (size_t)ss + 2] = o.sh2;
         }
-        if (a.cache_on) {
-            const uint32_t cs = wave_append(a.counters + kCtrGlobal + GCTR_CACHE_Q, cache_q);
-            if (cache_q) {
-                a.cq_points[3 * (size_t)cs] = loc.x;
-                a.cq_points[3 * (size_t)cs + 1] = loc.y;
-                a.cq_points[3 * (size_t)cs + 2] = loc.z;
-                a.cq_meta[cs] = make_float4(Tv.x, Tv.y, Tv.z, __uint_as_float(path));
-            }
-            const uint32_t ts = wave_append(a.counters + kCtrGlobal + GCTR_TRAIN, train);
-            if (train && ts < a.train_cap) {
-                a.tr_points[3 * (size_t)ts] = loc.x;
-                a.tr_points[3 * (size_t)ts + 1] = loc.y;
-                a.tr_points[3 * (size_t)ts + 2] = loc.z;
-                a.tr_meta0[ts] = make_float4(Cv.x, Cv.y, Cv.z, __uint_as_float(path));
-                a.tr_meta1[ts] = make_float4(Tv.x, Tv.y, Tv.z, 0.f);
-            }
-        }
+        if (a.cache_on) append_cache_records(a, path, o);
     }
+}
+
+// occlusion query of a NEE segment (s0 = origin, tmax; s1 = direction): true when nothing blocks it
+template <int WIDTH, int STACK>
+__device__ __forceinline__ bool shadow_unoccluded(const PtArgs &a, const float4 s0, const float4 s1, uint2 *stack) {
+    RayIn r{s0.x, s0.y, s0.z, SPK_EPS, s1.x, s1.y, s1.z, s0.w};
+    BestHit best;
+    best.t = r.tmax;
+    best.prim = 0xFFFFFFFFu;
+    best.tri = 0u;
+    best.U = best.V = best.W = 0.f;
+    best.det = 1.f;
+    TraverseCounters c{0u, 0u};
+    traverse<WIDTH, STACK, false, true>(a.nodes, a.tris, 0u, r, stack, kPtBlock, best, c);
+    if (best.prim == 0xFFFFFFFFu && a.num_spheres) spheres_closest<true>(a.spheres, a.num_spheres, a.sphere_base, r, best);
+    return best.prim == 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void add_color(const PtArgs &a, uint32_t path, const float4 c) {
+    float4 C = a.col[path];
+    C.x += c.x;
+    C.y += c.y;
+    C.z += c.z;
+    a.col[path] = C;
 }
 
 // ---- shadow: occlusion of the NEE segments; unoccluded contributions added to the path colour ----
@@ -690,25 +744,79 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) 
         if (base >= count) break;
         const uint32_t i = base + lane_id();
         if (i >= count) continue;
-        const float4 s0 = a.shadow[3 * (size_t)i], s1 = a.shadow[3 * (size_t)i + 1], s2 = a.shadow[3 * (size_t)i + 2];
-        RayIn r{s0.x, s0.y, s0.z, SPK_EPS, s1.x, s1.y, s1.z, s0.w};
-        BestHit best;
-        best.t = r.tmax;
-        best.prim = 0xFFFFFFFFu;
-        best.tri = 0u;
-        best.U = best.V = best.W = 0.f;
-        best.det = 1.f;
-        TraverseCounters c{0u, 0u};
-        traverse<WIDTH, STACK, false, true>(a.nodes, a.tris, 0u, r, stack + threadIdx.x, kPtBlock, best, c);
-        if (best.prim == 0xFFFFFFFFu && a.num_spheres) spheres_closest<true>(a.spheres, a.num_spheres, a.sphere_base, r, best);
-        if (best.prim == 0xFFFFFFFFu) {
-            const uint32_t path = __float_as_uint(s1.w);
-            float4 C = a.col[path];
-            C.x += s2.x;
-            C.y += s2.y;
-            C.z += s2.z;
-            a.col[path] = C;
+        const float4 s0 = a.shadow[3 * (size_t)i], s1 = a.shadow[3 * (size_t)i + 1];
+        if (shadow_unoccluded<WIDTH, STACK>(a, s0, s1, stack + threadIdx.x))
+            add_color(a, __float_as_uint(s1.w), a.shadow[3 * (size_t)i + 2]);
+    }
+}
+
+// ---- tail: the segments from `from` on, one path per lane until the path ends ----
+// Late segments hold few paths with long incoherent rays, and a bounce-synchronous frame waits for
+// the slowest ray of every bounce; here a lane carries its path through extend, shade and shadow and
+// takes the next queued path when it ends, so the frame waits for the slowest path instead. Every
+// segment runs the wavefront kernels' per-path code in the same order (closest hit, shading with
+// the colour written back, then the shadow contribution), so images do not depend on `from`.
+template <int WIDTH, int STACK>
+__global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
+    __shared__ uint2 stack[STACK * kPtBlock];
+    uint32_t *ctr = a.counters + from * kCtrStride;
+    const uint32_t count = ctr[CTR_QUEUE];
+    const uint32_t *queue = (from & 1u) ? a.queue1 : a.queue0;
+    const bool nee = (a.flags & SPK_FRAME_NEE) && a.num_emitters > 0;
+    const uint32_t lane = lane_id();
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+    uint32_t path = 0u, bounce = from, rays = 0u, shadows = 0u;
+    bool active = false, drained = false;
+    for (;;) {
+        if (!drained) {
+            const uint64_t idle = __ballot(!active);
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            if (nidle >= a.refill) {
+                uint32_t base = 0u;
+                if (lane == 0u) base = atomicAdd(ctr + CTR_EXTEND, nidle);
+                base = __shfl(base, 0);
+                drained = base + nidle >= count;
+                if (!active) {
+                    const uint32_t my = base + (uint32_t)__popcll(idle & below);
+                    if (my < count) {
+                        path = queue[my];
+                        bounce = from;
+                        active = true;
+                    }
+                }
+            }
         }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
+            const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
+            const uint32_t root = a.num_scenes > 1u ? a.scenes[a.path_scene[path]].root : 0u;
+            Traversal<WIDTH, STACK, false> tr;
+            tr.init(r, root);
+            while (tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
+            }
+            const uint4 h = hit_record(a, r, tr.best);
+            ShadeOut o;
+            shade_path(a, bounce, bounce + 1u >= a.max_bounce, nee, path, h, o);
+            ++rays;
+            if (o.shadow) {
+                ++shadows;
+                if (shadow_unoccluded<WIDTH, STACK>(a, o.sh0, o.sh1, stack + threadIdx.x)) add_color(a, path, o.sh2);
+            }
+            if (a.cache_on) append_cache_records(a, path, o);
+            if (o.cont) ++bounce; else active = false;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rays += __shfl_xor(rays, off);
+        shadows += __shfl_xor(shadows, off);
+    }
+    if (lane == 0u) {
+        if (rays) atomicAdd(a.counters + kCtrGlobal + GCTR_TAIL_RAYS, rays);
+        if (shadows) atomicAdd(a.counters + kCtrGlobal + GCTR_TAIL_SHADOWS, shadows);
     }
 }
 
@@ -841,15 +949,14 @@ __global__ void k_spatial_fill(const float4 *__restrict__ rgba, const float *__r
 using ExtendFn = void (*)(PtArgs, uint32_t);
 struct PtEntry {
     int width, stack;
-    ExtendFn extend, shadow;
+    ExtendFn extend, shadow, tail;
 };
+#define SPK_PT(W, S) {W, S, k_extend<W, S>, k_shadow<W, S>, k_tail<W, S>}
 const PtEntry kPtEntries[] = {
-    {8, 12, k_extend<8, 12>, k_shadow<8, 12>}, {4, 12, k_extend<4, 12>, k_shadow<4, 12>},
-    {8, 16, k_extend<8, 16>, k_shadow<8, 16>}, {8, 24, k_extend<8, 24>, k_shadow<8, 24>},
-    {8, 32, k_extend<8, 32>, k_shadow<8, 32>}, {8, 64, k_extend<8, 64>, k_shadow<8, 64>},
-    {4, 16, k_extend<4, 16>, k_shadow<4, 16>}, {4, 24, k_extend<4, 24>, k_shadow<4, 24>},
-    {4, 32, k_extend<4, 32>, k_shadow<4, 32>}, {4, 64, k_extend<4, 64>, k_shadow<4, 64>},
+    SPK_PT(8, 12), SPK_PT(4, 12), SPK_PT(8, 16), SPK_PT(8, 24), SPK_PT(8, 32),
+    SPK_PT(8, 64), SPK_PT(4, 16), SPK_PT(4, 24), SPK_PT(4, 32), SPK_PT(4, 64),
 };
+#undef SPK_PT
 
 struct PathState {
     DeviceBuffer rays, thr, col, rng, nee_rng, queue0, queue1, hits, sorted, shadow, counters, hist, path_scene;
@@ -972,7 +1079,7 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
 static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t s, int grid, CacheState *cache,
                                 int cus) {
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kPtBlock), 0, s, a);
-    for (uint32_t b = 0; b < a.max_bounce; ++b) {
+    for (uint32_t b = 0; b < a.tail_from; ++b) {
         hipLaunchKernelGGL(pe.extend, dim3(grid), dim3(kPtBlock), 0, s, a, b);
         if (a.sort) {
             hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, a, b);
@@ -982,6 +1089,7 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
         if ((a.flags & SPK_FRAME_NEE) && a.num_emitters)
             hipLaunchKernelGGL(pe.shadow, dim3(grid), dim3(kPtBlock), 0, s, a, b);
     }
+    if (a.tail_from < a.max_bounce) hipLaunchKernelGGL(pe.tail, dim3(grid), dim3(kPtBlock), 0, s, a, a.tail_from);
     if (a.cache_on) {
         hipError_t e = cache_infer_queue(*cache, a.cq_points, a.counters + kCtrGlobal + GCTR_CACHE_Q, a.npix, a.cq_out,
                                          cus * 8, s);
@@ -1071,6 +1179,7 @@ struct FrameSource {
     uint32_t blocks_per_cu;
     uint32_t sort;
     uint32_t refill;
+    uint32_t tail;  // first segment run by k_tail (0: none)
 };
 
 static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameSource &src, float *rgba_dev,
@@ -1138,6 +1247,7 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     a.refill = std::min(std::max(src.refill, 1u), 64u);
     a.npix = npix;
     a.max_bounce = frame->max_bounce;
+    a.tail_from = src.tail && src.tail < frame->max_bounce ? src.tail : frame->max_bounce;
     a.debug_mode = frame->debug_mode;
     a.sky_enabled = frame->sky_enabled;
     for (int f = 0; f < 6; ++f) a.sky_faces[f] = ctx->sky_faces[f];
@@ -1226,8 +1336,8 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
         std::vector<uint32_t> c(kCtrTotal);
         if ((e = hipMemcpy(c.data(), ps.counters.ptr, c.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
             return hip_fail(ctx, e, "frame counters");
-        uint64_t rays = 0, shadows = 0;
-        for (uint32_t b = 0; b < a.max_bounce; ++b) {
+        uint64_t rays = c[kCtrGlobal + GCTR_TAIL_RAYS], shadows = c[kCtrGlobal + GCTR_TAIL_SHADOWS];
+        for (uint32_t b = 0; b < a.tail_from; ++b) {
             rays += c[b * kCtrStride + CTR_QUEUE];
             shadows += c[b * kCtrStride + CTR_SHADOWS];
         }
@@ -1282,6 +1392,7 @@ int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     src.blocks_per_cu = ctx->schedule.blocks_per_cu;
     src.sort = ctx->schedule.material_sort;
     src.refill = ctx->schedule.refill;
+    src.tail = ctx->schedule.tail_bounce;
     return run_frame(ctx, frame, src, rgba_dev, normal_dev, stats);
 }
 
@@ -1410,12 +1521,15 @@ int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
     src.blocks_per_cu = bs.schedule.blocks_per_cu;
     src.sort = bs.schedule.material_sort;
     src.refill = bs.schedule.refill;
+    src.tail = bs.schedule.tail_bounce;
     return run_frame(ctx, frame, src, rgba_dev, nullptr, stats);
 }
 
 static int check_schedule(spk_context *ctx, const spk_schedule &sc) {
-    if (sc.blocks_per_cu < 1 || sc.blocks_per_cu > 32 || sc.material_sort > 1 || sc.refill < 1 || sc.refill > 64)
-        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "schedule out of range (blocks 1..32, sort 0/1, refill 1..64)");
+    if (sc.blocks_per_cu < 1 || sc.blocks_per_cu > 32 || sc.material_sort > 1 || sc.refill < 1 || sc.refill > 64 ||
+        sc.tail_bounce > (uint32_t)kMaxBounce)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT,
+                         "schedule out of range (blocks 1..32, sort 0/1, refill 1..64, tail 0..max bounces)");
     return SPK_OK;
 }
 
@@ -1439,7 +1553,6 @@ int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule) 
     int rc = check_schedule(ctx, *schedule);
     if (rc != SPK_OK) return rc;
     *sc = *schedule;
-    sc->reserved = 0u;
     return SPK_OK;
 }
 
@@ -1468,6 +1581,24 @@ static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *
                     best_s = *target;
                 }
             }
+        }
+    }
+    // then the first segment of the per-path tail kernel, on the best grid / sort / refill
+    static const uint32_t kTail[] = {2u, 3u, 4u, 5u};
+    const spk_schedule base = best_s;
+    for (uint32_t t : kTail) {
+        if (t >= frame->max_bounce) continue;
+        *target = spk_schedule{base.blocks_per_cu, base.material_sort, base.refill, t};
+        spk_frame_stats st{};
+        int rc = batch ? spk_batch_render(ctx, frame, rgba_dev, nullptr)
+                       : spk_render_frame(ctx, frame, rgba_dev, nullptr, nullptr);
+        if (rc == SPK_OK)
+            rc = batch ? spk_batch_render(ctx, frame, rgba_dev, &st)
+                       : spk_render_frame(ctx, frame, rgba_dev, nullptr, &st);
+        if (rc != SPK_OK) return rc;
+        if (st.milliseconds < best) {
+            best = st.milliseconds;
+            best_s = *target;
         }
     }
     *target = best_s;

@@ -121,25 +121,76 @@ def test_material_sort_does_not_change_results(gpu_ctx):
 
 
 def test_schedule_and_autotune_do_not_change_results(gpu_ctx):
-    """Every launch schedule (grid size, material sort, lane refill) gives the same image; the
-    autotuner keeps one of its candidates; out-of-range schedules are rejected."""
+    """Every launch schedule (grid size, material sort, lane refill, per-path tail) gives the same
+    image; the autotuner keeps one of its candidates; out-of-range schedules are rejected."""
     w, h = 40, 30
     r = Renderer(gpu_ctx, scenes.cornell_scene(), scenes.cornell_camera(w, h), w, h)
     ref, _, _ = r.render(9, bounce=6)
-    for sched in ((2, 0, 1), (16, 1, 64), (4, 1, 16), (1, 0, 33)):
+    for sched in ((2, 0, 1), (16, 1, 64), (4, 1, 16), (1, 0, 33), (8, 1, 16, 1), (2, 0, 64, 3), (8, 1, 16, 5),
+                  (8, 1, 16, 6), (8, 1, 16, 16)):
         print_case("schedule", sched)
         gpu_ctx.set_schedule(*sched)
         img, _, _ = r.render(9, bounce=6)
         np.testing.assert_array_equal(img, ref)
-    (b, srt, refill), ms = gpu_ctx.autotune_render(frame(w, h, 9, bounce=6), r.rgba)
-    assert b in (2, 4, 8, 16) and srt in (0, 1) and refill in (16, 64) and ms > 0
+    (b, srt, refill, tail), ms = gpu_ctx.autotune_render(frame(w, h, 9, bounce=6), r.rgba)
+    assert b in (2, 4, 8, 16) and srt in (0, 1) and refill in (16, 64) and tail in (0, 2, 3, 4, 5) and ms > 0
     img, _, _ = r.render(9, bounce=6)
     np.testing.assert_array_equal(img, ref)
     with pytest.raises(spk.SpkError):
         gpu_ctx.set_schedule(0, 1, 16)
     with pytest.raises(spk.SpkError):
         gpu_ctx.set_schedule(8, 1, 65)
+    with pytest.raises(spk.SpkError):
+        gpu_ctx.set_schedule(8, 1, 16, 65)  # beyond the 64-segment maximum
     gpu_ctx.set_schedule(8, 1, 16)
+
+
+def test_per_path_tail_matches_the_wavefront(gpu_ctx):
+    """Segments run one path per lane (k_tail) from bounce 1, 2 or 4 on: the same image bit for bit
+    and the same extend and shadow ray counts as the bounce-synchronous frame, with NEE, spheres,
+    textures, a sky, Russian roulette (from bounce 3) and every material type, over several frames."""
+    import torch
+
+    w, h = 48, 40
+    sc = scenes.textured_cornell_scene(seed=2)
+    glass = sc.add_material(base_color=(1, 1, 1), eta=1.5, type=spk.MATERIAL_DIELECTRIC)
+    sc.add_sphere((0.3, -0.2, 0.35), 0.12, material=glass)
+    sc.add_sphere((-0.25, 0.25, 0.2), 0.1, material=0)
+    r = Renderer(gpu_ctx, sc, scenes.cornell_camera(w, h), w, h)
+    try:
+        for flags in (0, spk.FRAME_NEE):
+            for seed in (3, 4):
+                gpu_ctx.set_schedule(8, 1, 16, 0)
+                f = frame(w, h, seed, bounce=8, flags=flags, sky=(0.2, 0.3, 0.6))
+                f.output_limit = 1e30
+                ref_st = gpu_ctx.render_frame(f, r.rgba)
+                ref = r.rgba.cpu().numpy().copy()
+                for tail in (1, 2, 4):
+                    print_case("tail", tail, "flags", flags, "seed", seed)
+                    gpu_ctx.set_schedule(8, 1, 16, tail)
+                    st = gpu_ctx.render_frame(f, r.rgba)
+                    np.testing.assert_array_equal(r.rgba.cpu().numpy(), ref)
+                    assert st.rays_traced == ref_st.rays_traced and st.shadow_rays == ref_st.shadow_rays
+                    assert ref_st.rays_traced > w * h
+                    assert flags == 0 or ref_st.shadow_rays > 0
+        # the cache's first frame after a reset: the same queries, so the same image
+        gpu_ctx.cache_create(spk.cache_config(terminate_bounce=2))
+        f = frame(w, h, 5, bounce=8, flags=spk.FRAME_CACHE | spk.FRAME_NEE)
+        imgs = []
+        for tail in (0, 1, 3):
+            print_case("cache tail", tail)
+            gpu_ctx.set_schedule(8, 1, 16, tail)
+            gpu_ctx.render_frame(f, r.rgba)  # capture
+            gpu_ctx.cache_reset()
+            gpu_ctx.render_frame(f, r.rgba)
+            imgs.append(r.rgba.cpu().numpy().copy())
+        gpu_ctx.cache_destroy()
+        for img in imgs[1:]:
+            np.testing.assert_array_equal(img, imgs[0])
+        assert np.isfinite(imgs[0]).all()
+    finally:
+        gpu_ctx.set_schedule(8, 1, 16)
+    torch.cuda.synchronize()
 
 
 def test_sky_and_open_scene(gpu_ctx):
