@@ -126,12 +126,50 @@ struct PtArgs {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// one atomic per wave: the wave's lanes take consecutive items
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t *ctr) {
+// Device-scope atomics on one counter serialise (a few ns each), so work distribution avoids one
+// atomic per 64 items: kernels with even per-item cost stride statically over 64-item chunks
+// (wave w takes chunks w, w + waves, ...); traversal kernels take kFetchChunk slots per atomic.
+constexpr uint32_t kFetchChunk = 128;
+
+__device__ __forceinline__ uint32_t grid_wave() { return (blockIdx.x * kPtBlock + threadIdx.x) >> 6; }
+__device__ __forceinline__ uint32_t grid_waves() { return gridDim.x * (uint32_t)(kPtBlock >> 6); }
+
+// one atomic per wave for n items: the wave's lanes take consecutive items from the returned base
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t *ctr, uint32_t n) {
     uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(ctr, 64u);
+    if (lane_id() == 0) base = atomicAdd(ctr, n);
     return __shfl(base, 0);
 }
+
+// A wave's private run of queue slots [next, end), refilled kFetchChunk at a time from the shared
+// counter. take() hands one slot to each idle lane (lanes in `idle`, rank = idle lanes below it);
+// a slot >= count means no work. Wave-uniform state; every lane of the wave must call take().
+struct SlotRun {
+    uint32_t next = 0u, end = 0u;
+    bool global_done = false;
+    __device__ __forceinline__ uint32_t take(uint32_t *ctr, uint32_t count, uint32_t nidle, uint32_t rank) {
+        const uint32_t avail = end > next ? end - next : 0u;
+        uint32_t fresh = 0u;
+        const bool fetch = nidle > avail && !global_done;
+        if (fetch) {
+            fresh = wave_fetch(ctr, kFetchChunk);
+            global_done = fresh + kFetchChunk >= count;
+        }
+        uint32_t my = 0xFFFFFFFFu;
+        if (rank < avail) my = next + rank;
+        else if (fetch) my = fresh + (rank - avail);
+        if (nidle <= avail) {
+            next += nidle;
+        } else if (fetch) {
+            next = fresh + (nidle - avail);
+            end = min(fresh + kFetchChunk, count);
+        } else {
+            next = end;
+        }
+        return my;
+    }
+    __device__ __forceinline__ bool drained() const { return global_done && next >= end; }
+};
 
 // wave-aggregated append over the lanes that execute it: returns the slot for lanes with `want`
 // (the lowest active lane takes the slots, so it also works inside divergent code)
@@ -266,18 +304,15 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
     Traversal<WIDTH, STACK, false> tr;
     uint32_t slot = 0u, path = 0u;
-    bool active = false, drained = false;
+    bool active = false;
+    SlotRun run;
     for (;;) {
-        if (!drained) {
+        if (!run.drained()) {
             const uint64_t idle = __ballot(!active);
             const uint32_t nidle = (uint32_t)__popcll(idle);
             if (nidle >= a.refill) {
-                uint32_t base = 0u;
-                if (lane == 0u) base = atomicAdd(ctr + CTR_EXTEND, nidle);
-                base = __shfl(base, 0);
-                drained = base + nidle >= count;
+                const uint32_t my = run.take(ctr + CTR_EXTEND, count, nidle, (uint32_t)__popcll(idle & below));
                 if (!active) {
-                    const uint32_t my = base + (uint32_t)__popcll(idle & below);
                     if (my < count) {
                         path = queue[my];
                         const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
@@ -291,7 +326,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
             }
         }
         if (!__any(active)) {
-            if (drained) break;
+            if (run.drained()) break;
             continue;
         }
         if (active && !tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
@@ -341,9 +376,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sort_scatter(PtArgs a, uint32_t bo
     uint32_t *offs = a.hist + (size_t)bounce * 2u * a.nkeys + a.nkeys;
     const uint32_t lane = lane_id();
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
-    for (;;) {
-        const uint32_t base = wave_fetch(ctr + CTR_SCATTER);
-        if (base >= count) break;
+    for (uint32_t base = grid_wave() * 64u; base < count; base += grid_waves() * 64u) {
         const uint32_t slot = base + lane;
         const bool valid = slot < count;
         uint32_t key = a.nkeys - 1u;
@@ -685,9 +718,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
     uint32_t *next_queue = (bounce & 1u) ? a.queue0 : a.queue1;
     const bool last = bounce + 1u >= a.max_bounce;
     const bool nee = (a.flags & SPK_FRAME_NEE) && a.num_emitters > 0;
-    for (;;) {
-        const uint32_t base = wave_fetch(ctr + CTR_SHADE);
-        if (base >= count) break;
+    for (uint32_t base = grid_wave() * 64u; base < count; base += grid_waves() * 64u) {
         const uint32_t idx = base + lane_id();
         uint32_t path = 0;
         ShadeOut o;
@@ -740,13 +771,15 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) 
     uint32_t *ctr = a.counters + bounce * kCtrStride;
     const uint32_t count = ctr[CTR_SHADOWS];
     for (;;) {
-        const uint32_t base = wave_fetch(ctr + CTR_SHADOWW);
-        if (base >= count) break;
-        const uint32_t i = base + lane_id();
-        if (i >= count) continue;
-        const float4 s0 = a.shadow[3 * (size_t)i], s1 = a.shadow[3 * (size_t)i + 1];
-        if (shadow_unoccluded<WIDTH, STACK>(a, s0, s1, stack + threadIdx.x))
-            add_color(a, __float_as_uint(s1.w), a.shadow[3 * (size_t)i + 2]);
+        const uint32_t chunk = wave_fetch(ctr + CTR_SHADOWW, kFetchChunk);
+        if (chunk >= count) break;
+        for (uint32_t base = chunk; base < min(chunk + kFetchChunk, count); base += 64u) {
+            const uint32_t i = base + lane_id();
+            if (i >= count) continue;
+            const float4 s0 = a.shadow[3 * (size_t)i], s1 = a.shadow[3 * (size_t)i + 1];
+            if (shadow_unoccluded<WIDTH, STACK>(a, s0, s1, stack + threadIdx.x))
+                add_color(a, __float_as_uint(s1.w), a.shadow[3 * (size_t)i + 2]);
+        }
     }
 }
 
@@ -766,18 +799,15 @@ __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
     const uint32_t lane = lane_id();
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
     uint32_t path = 0u, bounce = from, rays = 0u, shadows = 0u;
-    bool active = false, drained = false;
+    bool active = false;
+    SlotRun run;
     for (;;) {
-        if (!drained) {
+        if (!run.drained()) {
             const uint64_t idle = __ballot(!active);
             const uint32_t nidle = (uint32_t)__popcll(idle);
             if (nidle >= a.refill) {
-                uint32_t base = 0u;
-                if (lane == 0u) base = atomicAdd(ctr + CTR_EXTEND, nidle);
-                base = __shfl(base, 0);
-                drained = base + nidle >= count;
+                const uint32_t my = run.take(ctr + CTR_EXTEND, count, nidle, (uint32_t)__popcll(idle & below));
                 if (!active) {
-                    const uint32_t my = base + (uint32_t)__popcll(idle & below);
                     if (my < count) {
                         path = queue[my];
                         bounce = from;
@@ -787,7 +817,7 @@ __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
             }
         }
         if (!__any(active)) {
-            if (drained) break;
+            if (run.drained()) break;
             continue;
         }
         if (active) {
@@ -823,9 +853,7 @@ __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
 // ---- cache: add throughput x predicted radiance; build training targets ----
 __global__ __launch_bounds__(kPtBlock) void k_cache_apply(PtArgs a) {
     const uint32_t count = min(a.counters[kCtrGlobal + GCTR_CACHE_Q], a.npix);
-    for (;;) {
-        const uint32_t base = wave_fetch(a.counters + kCtrGlobal + GCTR_APPLY);
-        if (base >= count) break;
+    for (uint32_t base = grid_wave() * 64u; base < count; base += grid_waves() * 64u) {
         const uint32_t i = base + lane_id();
         if (i >= count) continue;
         const float4 m = a.cq_meta[i];
@@ -841,9 +869,7 @@ __global__ __launch_bounds__(kPtBlock) void k_cache_apply(PtArgs a) {
 // target = radiance gathered after the vertex / throughput at the vertex (clamped to [0, 64])
 __global__ __launch_bounds__(kPtBlock) void k_cache_targets(PtArgs a) {
     const uint32_t count = min(a.counters[kCtrGlobal + GCTR_TRAIN], a.train_cap);
-    for (;;) {
-        const uint32_t base = wave_fetch(a.counters + kCtrGlobal + GCTR_TARGETS);
-        if (base >= count) break;
+    for (uint32_t base = grid_wave() * 64u; base < count; base += grid_waves() * 64u) {
         const uint32_t i = base + lane_id();
         if (i >= count) continue;
         const float4 m0 = a.tr_meta0[i], m1 = a.tr_meta1[i];
