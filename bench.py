@@ -71,18 +71,25 @@ def cpu_baseline(scene, rays, seconds_target=2.0, runs=5):
 
 
 def load_traffic(config):
-    """HBM-side bytes per launch and L2 hit rate from the committed rocprofv3 counter summary, when it
-    was taken with the kernel configuration this run uses."""
+    """HBM-side bytes per launch and L2 hit rate from the committed rocprofv3 counter summary, for the
+    kernel configuration this run uses (the summary holds one `config:` block per profiled
+    configuration)."""
     path = os.path.join(ROOT, "profiles", "trace_incoherent_2m_pmc.md")
     if not os.path.exists(path):
         return None, None, None
-    fields = {}
+    blocks, cur = {}, None
     for line in open(path):
-        if ":" in line and line.split(":")[0] in ("config", "hbm_bytes_per_launch", "l2_hit_rate"):
-            k, v = line.split(":", 1)
-            fields[k.strip()] = v.strip()
-    if fields.get("config") != config:
-        return None, None, f"profiles/trace_incoherent_2m_pmc.md was taken with config {fields.get('config')}"
+        if ":" not in line:
+            continue
+        k, v = (x.strip() for x in line.split(":", 1))
+        if k == "config":
+            cur = blocks.setdefault(v, {})
+        elif k in ("hbm_bytes_per_launch", "l2_hit_rate") and cur is not None:
+            cur[k] = v
+    fields = blocks.get(config)
+    if not fields or "hbm_bytes_per_launch" not in fields:
+        return None, None, f"profiles/trace_incoherent_2m_pmc.md has no counters for config {config} " \
+                           f"(profiled: {', '.join(sorted(blocks))})"
     l2 = float(fields["l2_hit_rate"]) if "l2_hit_rate" in fields else None
     return int(float(fields["hbm_bytes_per_launch"])), l2, "profiles/trace_incoherent_2m_pmc.md"
 
@@ -105,7 +112,8 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     cache1_nee   cache from bounce 1 + next-event estimation
     For each budget b*T every estimator renders frames until b*T has passed (plain at b = 1 renders
     exactly spp frames); the cache estimators start untrained (reset after their graph-capture frame)
-    and accumulate only the frames after warm_frac of the budget. The reference is ref_spp frames of
+    and accumulate only the frames after warm_frac of the budget. Each estimator runs on the launch
+    schedule spk_render_autotune picks for it (tuned once, before any timed region). The reference is ref_spp frames of
     the plain tracer with next-event estimation (same expectation as the reference's estimator, about
     3x lower variance per frame, so the reference's own noise adds less to every relMSE), seeds from
     1,000,000 on, summed in float64 on the device. All frames use the 64-bit seed composition: at
@@ -134,7 +142,19 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
         f.sky_enabled, f.flags, f.output_limit = 0, flags | spk.FRAME_SEED64, 1e30
         return f
 
+    variants = [("plain", 0, None), ("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
+                ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
+    schedules = {}
+    for name, flags, bounce in variants:
+        if bounce is not None:
+            ctx.cache_create(spk.cache_config(terminate_bounce=bounce))
+        schedules[name], _ = ctx.autotune_render(frame(999_990, flags), rgba)
+        if bounce is not None:
+            ctx.cache_destroy()
+    log(f"relMSE experiment: schedules {schedules}")
+
     log(f"relMSE experiment: reference at {ref_spp} spp")
+    ctx.set_schedule(*schedules["plain_nee"])
     ctx.render_frame(frame(999_997, spk.FRAME_NEE), rgba)  # graph capture
     ref = torch.zeros(npx * 4, dtype=torch.float64, device="cuda")
     t0 = time.perf_counter()
@@ -149,6 +169,7 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     ref = (ref / ref_spp).view(height, width, 4)[..., :3]
 
     log(f"relMSE experiment: plain at {spp} spp")
+    ctx.set_schedule(*schedules["plain"])
     ctx.render_frame(frame(999_999), rgba)  # graph capture for the plain flags
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -161,8 +182,6 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     first = {"plain": {"frames": spp, "accumulated_spp": spp, "seconds": round(T, 3),
                        "relmse": relmse(img, ref), "mean": float(img.mean())}}
 
-    variants = [("plain", 0, None), ("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
-                ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
     by_budget = []
     for b in budgets:
         budget = b * T
@@ -171,6 +190,7 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
             if name in results:
                 continue
             log(f"relMSE experiment: {name} for {budget:.2f} s")
+            ctx.set_schedule(*schedules[name])
             if bounce is not None:
                 ctx.cache_create(spk.cache_config(terminate_bounce=bounce))
             ctx.render_frame(frame(999_998, flags), rgba)  # graph capture
@@ -219,6 +239,8 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
         "reference_s": round(t_ref, 2),
         "equal_time_s": round(T, 3),
         "budgets": by_budget,
+        "schedules": {k: dict(zip(("blocks_per_cu", "material_sort", "refill", "tail_bounce"), v))
+                      for k, v in schedules.items()},
         "full_path_nee_samples_per_s": round(npx * ref_spp / t_ref, 1),
         "full_path_nee_mrays_per_s": round(rays_ref / t_ref / 1e6, 1),
         "bvh_build_s": round(t_build, 2),
@@ -450,6 +472,8 @@ def main():
     ap.add_argument("--relmse", type=str, default="1920x1080,64,4096",
                     help="equal-time relMSE experiment WxH,spp,reference-spp ('off' to skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
+    ap.add_argument("--trace-only", action="store_true",
+                    help="only the closest-hit measurement (for counter runs): no cache, path-trace, batch or relMSE")
     ap.add_argument("--render-only", type=int, default=0,
                     help="only run the full path trace of the many-instance scene for N spp (one JSON line)")
     ap.add_argument("--render-mode", choices=sorted(RENDER_MODES), default="plain",
@@ -630,6 +654,9 @@ def main():
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     ctx.close()
+    if args.trace_only:
+        print(json.dumps(out), flush=True)
+        return
     out["cache_kernels"] = cache_kernel_bench(spk, scenes, stream)
     out["full_path"] = render_experiment(spk, scenes, stream)
     out["c1"] = c1_experiment(spk, scenes, stream)

@@ -46,7 +46,8 @@ def _png_rgba(pixels):
 
 
 def _png_decode(data):
-    """Decodes an 8-bit RGB/RGBA PNG without interlace (enough for the writer's textures)."""
+    """Decodes an 8-bit grey, RGB or RGBA PNG without interlace into RGBA: grey replicates into RGB
+    and a missing alpha is 255 (the ExpandToRgba rule of USDLoader.cpp:126-163)."""
     if data[:8] != b"\x89PNG\r\n\x1a\n":
         raise ValueError("not a PNG image")
     pos, idat, hdr = 8, b"", None
@@ -60,9 +61,9 @@ def _png_decode(data):
             idat += body
         pos += 12 + n
     w, h, depth, ctype, _, _, interlace = hdr
-    if depth != 8 or ctype not in (2, 6) or interlace:
-        raise NotImplementedError("only 8-bit RGB/RGBA PNG without interlace")
-    ch = 4 if ctype == 6 else 3
+    if depth != 8 or ctype not in (0, 2, 6) or interlace:
+        raise NotImplementedError("only 8-bit grey/RGB/RGBA PNG without interlace")
+    ch = {0: 1, 2: 3, 6: 4}[ctype]
     raw = zlib.decompress(idat)
     out = np.zeros((h, w, ch), np.uint8)
     prev = np.zeros(w * ch, np.int32)
@@ -89,7 +90,9 @@ def _png_decode(data):
             cur[i] = (line[i] + p) & 0xFF
         out[j] = cur.reshape(w, ch)
         prev = cur
-    if ch == 3:
+    if ch == 1:
+        out = np.repeat(out, 3, axis=-1)
+    if ch in (1, 3):
         out = np.concatenate([out, np.full((h, w, 1), 255, np.uint8)], -1)
     return out
 

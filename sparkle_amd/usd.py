@@ -1,0 +1,870 @@
+"""USD scenes in the ASCII format (.usda): a reader with the rules of Sparkle's USD loader and a
+writer shaped like its exporter, for the subset the two exchange.
+
+The reference reads USD through tinyusdz's Tydra render-scene converter (a git submodule,
+thirdparty/tinyusdz, not vendored) and maps the result in libraries/source/io/scene/USDLoader.cpp;
+its exporter (USDExporter.cpp) writes Xform prims with one `xformOp:transform` each, meshes,
+a `/Materials` scope of UsdPreviewSurface networks, cameras and lights. This module parses the USDA
+text itself and applies the loader's rules:
+  * the scene is the default prim's subtree; an Xform's local matrix composes with its parent's
+    (`LoadNode`, :493-530); a Mesh, Camera or light takes its parent Xform's world transform (their
+    own xformOps are not read: only Xform nodes carry transforms in the loader);
+  * USD matrices are row-vector (`v' = v M`), Sparkle's are column-vector: the loader transposes
+    (`MatrixCast`, :29-41); xformOps compose first-listed outermost (USD's rule);
+  * meshes: points, faceVertexCounts/Indices (polygons fanned into triangles), vertex normals
+    (face-varying or uniform attributes split the vertices, as Tydra's vertex-index building does),
+    the primary uv set `primvars:st` unflipped, `primvars:tangents` or fallback tangents
+    `(n x a) x n` with `a = GetPossibleMajorAxis(n)` and w = 1 (:330-347); a mesh without a bound
+    material is skipped (:268-272);
+  * materials (UsdPreviewSurface, :357-428): diffuseColor / emissiveColor / normal / metallic+
+    roughness from values or UsdUVTexture connections (metallic and roughness must share one
+    image: B is metallic, G roughness, as the exporter writes them); an emissive texture sets the
+    emissive factor to 1; `opacity` authored below 1 (not a texture) makes the material dielectric
+    with eta = `ior`; UsdPreviewSurface defaults where unauthored (diffuse 0.18, roughness 0.5,
+    metallic 0, ior 1.5);
+  * textures: 8-bit images are sRGB when the colour space is `sRGB`/`srgb_texture` (or `auto`, the
+    USD default for 8-bit data), linear for `raw`/`lin_srgb` (:192-206); grey images replicate into
+    RGB with alpha 1 (`ExpandToRgba`, :126-163); paths are relative to the .usda file;
+  * camera (perspective only, :80-124): focalLength and verticalAperture in millimetres -> metres,
+    fStop and focusDistance only when positive, Sparkle's defaults otherwise (35 mm, 24 mm, f/22,
+    1 m; CameraComponent.h:16-23); the camera node stores the world-to-camera rotation, so right,
+    front and up are the rows of its world matrix (Transform::ExtractLocalBasis, OrbitCamera
+    `SetupFromTransform`);
+  * lights: colour x intensity x 2^exposure (:454-457); a DomeLight is the sky light (the only
+    light the CPU path tracer evaluates, CPURenderer.cpp:251-268), a DistantLight the directional
+    light (reported, not used by the path); a textured dome is reported and not loaded.
+PNG images are decoded by `sparkle_amd.gltf`'s decoder (8-bit grey, RGB or RGBA); float textures
+travel as Radiance .hdr files (RGBE, the exporter's choice for float images, USDExporter.cpp:103-106),
+decoded as R x 2^(E - 136) with alpha 1 and encoded with a frexp mantissa scaled by 256, so a round
+trip through a file rounds each channel to the shared-exponent precision.
+"""
+import math
+import os
+import re
+
+import numpy as np
+
+from . import (MATERIAL_DIELECTRIC, MATERIAL_PBR, TEXTURE_RGBA32F, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA8_UNORM,
+               CameraDesc, Scene)
+from .gltf import _png_decode, _png_rgba
+
+
+class UsdError(ValueError):
+    pass
+
+
+# ---------------------------------------------------------------------------------------------
+# Radiance RGBE (.hdr) images for float textures
+# ---------------------------------------------------------------------------------------------
+def hdr_encode(rgb):
+    """(H, W, >=3) float -> .hdr bytes (flat RGBE scanlines)."""
+    a = np.asarray(rgb, np.float32)[..., :3]
+    h, w = a.shape[:2]
+    m = a.max(axis=-1)
+    mant, expo = np.frexp(m)
+    scale = np.where(m > 1e-32, mant * np.float32(256.0) / np.where(m > 1e-32, m, 1.0), 0.0).astype(np.float32)
+    rgbe = np.zeros((h, w, 4), np.uint8)
+    rgbe[..., :3] = np.clip(a * scale[..., None], 0, 255).astype(np.uint8)
+    rgbe[..., 3] = np.where(m > 1e-32, expo + 128, 0).astype(np.uint8)
+    head = f"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y {h} +X {w}\n".encode()
+    return head + rgbe.tobytes()
+
+
+def hdr_decode(data):
+    """.hdr bytes (flat or run-length scanlines) -> (H, W, 4) float32 with alpha 1."""
+    if not (data.startswith(b"#?RADIANCE") or data.startswith(b"#?RGBE")):
+        raise UsdError("not a Radiance .hdr image")
+    pos = 0
+    while True:  # header lines end with an empty line
+        end = data.index(b"\n", pos)
+        line = data[pos:end]
+        pos = end + 1
+        if line == b"":
+            break
+        if line.startswith(b"FORMAT=") and line != b"FORMAT=32-bit_rle_rgbe":
+            raise UsdError("only 32-bit_rle_rgbe .hdr images")
+    end = data.index(b"\n", pos)
+    dims = data[pos:end].split()
+    pos = end + 1
+    if len(dims) != 4 or dims[0] != b"-Y" or dims[2] != b"+X":
+        raise UsdError("only -Y H +X W .hdr images")
+    h, w = int(dims[1]), int(dims[3])
+    rgbe = np.zeros((h, w, 4), np.uint8)
+    for j in range(h):
+        if 8 <= w < 32768 and data[pos] == 2 and data[pos + 1] == 2 and not data[pos + 2] & 0x80:
+            if (data[pos + 2] << 8 | data[pos + 3]) != w:
+                raise UsdError("bad .hdr scanline width")
+            pos += 4
+            for c in range(4):
+                i = 0
+                while i < w:
+                    n = data[pos]
+                    pos += 1
+                    if n > 128:
+                        rgbe[j, i:i + n - 128, c] = data[pos]
+                        pos += 1
+                        i += n - 128
+                    else:
+                        rgbe[j, i:i + n, c] = np.frombuffer(data, np.uint8, n, pos)
+                        pos += n
+                        i += n
+        else:
+            rgbe[j] = np.frombuffer(data, np.uint8, w * 4, pos).reshape(w, 4)
+            pos += w * 4
+    e = rgbe[..., 3].astype(np.int32)
+    f = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136), 0.0).astype(np.float32)
+    out = np.ones((h, w, 4), np.float32)
+    out[..., :3] = rgbe[..., :3].astype(np.float32) * f[..., None]
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# USDA text: tokens and a small recursive-descent parser
+# ---------------------------------------------------------------------------------------------
+class Asset(str):
+    """An asset path (`@file@`)."""
+
+
+class SdfPath(str):
+    """A scene path (`</A/B.prop>`)."""
+
+
+_TOKEN = re.compile(r'''
+    (?P<ws>[ \t\r\n]+|\#[^\n]*)
+  | (?P<str3>"""(?:.|\n)*?"""|\'\'\'(?:.|\n)*?\'\'\')
+  | (?P<str>"(?:[^"\\\n]|\\.)*"|'(?:[^'\\\n]|\\.)*')
+  | (?P<asset>@@@.*?@@@|@[^@\n]*@)
+  | (?P<path><[^>\n]*>)
+  | (?P<num>[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?(?![A-Za-z_]))
+  | (?P<ident>[-+]?[A-Za-z_][A-Za-z0-9_:.]*)
+  | (?P<punct>[()\[\]{}=,;:])
+''', re.VERBOSE)
+
+
+def _tokens(text):
+    if not text.startswith("#usda"):
+        raise UsdError("not a USDA file (no '#usda' header)")
+    text = text[text.index("\n") + 1:] if "\n" in text else ""
+    out, pos = [], 0
+    while pos < len(text):
+        m = _TOKEN.match(text, pos)
+        if not m:
+            raise UsdError(f"cannot parse USDA near {text[pos:pos + 40]!r}")
+        pos = m.end()
+        kind = m.lastgroup
+        if kind == "ws":
+            continue
+        v = m.group(kind)
+        if kind == "str3":
+            out.append(("str", v[3:-3]))
+        elif kind == "str":
+            out.append(("str", bytes(v[1:-1], "utf-8").decode("unicode_escape")))
+        elif kind == "asset":
+            out.append(("asset", Asset(v.strip("@"))))
+        elif kind == "path":
+            out.append(("path", SdfPath(v[1:-1])))
+        elif kind == "num":
+            out.append(("num", float(v) if any(c in v for c in ".eE") else int(v)))
+        else:
+            out.append((kind, v))
+    return out
+
+
+class _Parser:
+    def __init__(self, toks):
+        self.t, self.i = toks, 0
+
+    def peek(self, k=0):
+        return self.t[self.i + k] if self.i + k < len(self.t) else (None, None)
+
+    def next(self):
+        tok = self.peek()
+        self.i += 1
+        return tok
+
+    def expect(self, value):
+        kind, v = self.next()
+        if v != value:
+            raise UsdError(f"expected {value!r}, found {v!r}")
+
+    def value(self):
+        kind, v = self.next()
+        if kind in ("num", "str", "asset", "path"):
+            return v
+        if kind == "ident":
+            low = v.lower()
+            if low in ("inf", "+inf"):
+                return math.inf
+            if low == "-inf":
+                return -math.inf
+            if low == "nan":
+                return math.nan
+            if v == "None":
+                return None
+            if v in ("true", "false"):
+                return v == "true"
+            return v
+        if v == "(":
+            return tuple(self.seq(")"))
+        if v == "[":
+            return self.seq("]")
+        if v == "{":  # dictionary / timeSamples: keys and values, kept as a list of pairs
+            items = []
+            while self.peek()[1] != "}":
+                if self.peek()[0] == "ident" and self.peek(1)[0] in ("ident", "str") and self.peek(2)[1] == "=":
+                    self.next()  # value type of a dictionary entry
+                key = self.next()[1]
+                self.expect(":") if self.peek()[1] == ":" else self.expect("=")
+                items.append((key, self.value()))
+                if self.peek()[1] in (",", ";"):
+                    self.next()
+            self.next()
+            return items
+        raise UsdError(f"unexpected {v!r} in a value")
+
+    def seq(self, close):
+        out = []
+        while self.peek()[1] != close:
+            out.append(self.value())
+            if self.peek()[1] == ",":
+                self.next()
+        self.next()
+        return out
+
+    def metadata(self):
+        """`( key = value ... )` after a prim or property; a bare string is the doc."""
+        meta = {}
+        self.expect("(")
+        while self.peek()[1] != ")":
+            kind, v = self.peek()
+            if kind == "str":
+                meta["doc"] = self.next()[1]
+            else:
+                self.next()
+                if v in ("prepend", "append", "add", "delete", "reorder"):
+                    v = self.next()[1]
+                self.expect("=")
+                meta[v] = self.value()
+            if self.peek()[1] in (",", ";"):
+                self.next()
+        self.next()
+        return meta
+
+    def prim(self, parent_path):
+        spec = self.next()[1]
+        type_name = self.next()[1] if self.peek()[0] == "ident" else ""
+        name = self.next()[1]
+        path = f"{parent_path}/{name}" if parent_path != "/" else f"/{name}"
+        p = {"spec": spec, "type": type_name, "name": name, "path": path, "meta": {}, "attrs": {}, "conns": {},
+             "rels": {}, "children": []}
+        if self.peek()[1] == "(":
+            p["meta"] = self.metadata()
+        self.expect("{")
+        while self.peek()[1] != "}":
+            kind, v = self.peek()
+            if v in ("def", "over", "class"):
+                p["children"].append(self.prim(path))
+            elif v == "variantSet":
+                raise UsdError(f"{path}: variant sets are not supported")
+            else:
+                self.property(p)
+        self.next()
+        return p
+
+    def property(self, p):
+        words = []
+        while self.peek()[1] in ("custom", "uniform", "varying", "prepend", "append", "add", "delete"):
+            words.append(self.next()[1])
+        kind, v = self.next()
+        if v == "rel":
+            name = self.next()[1]
+            targets = []
+            if self.peek()[1] == "=":
+                self.next()
+                val = self.value()
+                targets = [t for t in (val if isinstance(val, list) else [val]) if t is not None]
+            if self.peek()[1] == "(":
+                self.metadata()
+            p["rels"][name] = [SdfPath(t) for t in targets]
+            return
+        type_name = v
+        if self.peek()[1] == "[":
+            self.next()
+            self.expect("]")
+            type_name += "[]"
+        name = self.next()[1]
+        value, meta = None, {}
+        if self.peek()[1] == "=":
+            self.next()
+            value = self.value()
+        if self.peek()[1] == "(":
+            meta = self.metadata()
+        if name.endswith(".connect"):
+            tgt = value if not isinstance(value, list) else (value[0] if value else None)
+            p["conns"][name[:-len(".connect")]] = SdfPath(tgt) if tgt else None
+        elif name.endswith(".timeSamples"):
+            base = name[:-len(".timeSamples")]
+            samples = value or []
+            if base not in p["attrs"] and samples:  # the first sample stands in for the default
+                p["attrs"][base] = {"type": type_name, "value": samples[0][1], "meta": meta, "uniform": False}
+        else:
+            p["attrs"][name] = {"type": type_name, "value": value, "meta": meta, "uniform": "uniform" in words}
+
+
+def parse_usda(text):
+    """Parses USDA text into (stage metadata, root prims); each prim is a dict with its type, name,
+    path, metadata, attributes, connections, relationships and children."""
+    ps = _Parser(_tokens(text))
+    meta = ps.metadata() if ps.peek()[1] == "(" else {}
+    roots = []
+    while ps.peek()[0] is not None:
+        roots.append(ps.prim("/"))
+    return meta, roots
+
+
+# ---------------------------------------------------------------------------------------------
+# transforms (column-vector 4x4, float64)
+# ---------------------------------------------------------------------------------------------
+def _rot(axis, deg):
+    a = math.radians(float(deg))
+    c, s = math.cos(a), math.sin(a)
+    m = np.eye(4)
+    i, j = {"X": (1, 2), "Y": (2, 0), "Z": (0, 1)}[axis]
+    m[i, i], m[i, j], m[j, i], m[j, j] = c, -s, s, c
+    return m
+
+
+def _quat(q):
+    w, x, y, z = (float(c) for c in q)  # USDA writes quaternions real part first
+    n = math.sqrt(w * w + x * x + y * y + z * z) or 1.0
+    w, x, y, z = w / n, x / n, y / n, z / n
+    m = np.eye(4)
+    m[:3, :3] = [[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                 [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                 [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]]
+    return m
+
+
+def _op_matrix(kind, v):
+    if kind == "transform":
+        return np.asarray(v, np.float64).reshape(4, 4).T  # row-vector USD matrix -> column-vector
+    if kind == "translate":
+        m = np.eye(4)
+        m[:3, 3] = [float(c) for c in v]
+        return m
+    if kind == "scale":
+        return np.diag([float(c) for c in v] + [1.0])
+    if kind in ("rotateX", "rotateY", "rotateZ"):
+        return _rot(kind[-1], v)
+    if kind.startswith("rotate") and len(kind) == 9:  # rotateXYZ etc.: first letter applied first
+        m = np.eye(4)
+        for axis, ang in zip(kind[6:], v):
+            m = _rot(axis, ang) @ m
+        return m
+    if kind == "orient":
+        return _quat(v)
+    raise UsdError(f"unsupported xformOp {kind}")
+
+
+def local_matrix(prim):
+    """The prim's local transform from xformOpOrder (column-vector; first op outermost)."""
+    order = prim["attrs"].get("xformOpOrder", {}).get("value") or []
+    m = np.eye(4)
+    for name in order:
+        inv = name.startswith("!invert!")
+        key = name[len("!invert!"):] if inv else name
+        if key == "!resetXformStack!":
+            continue
+        if key not in prim["attrs"]:
+            raise UsdError(f"{prim['path']}: xformOpOrder names {key}, which is not authored")
+        op = key.split(":")[1]
+        om = _op_matrix(op, prim["attrs"][key]["value"])
+        m = m @ (np.linalg.inv(om) if inv else om)
+    return m
+
+
+# ---------------------------------------------------------------------------------------------
+# the loader's mapping
+# ---------------------------------------------------------------------------------------------
+_SRGB_SPACES = ("sRGB", "srgb", "sRGB_Texture", "srgb_texture", "auto", "")
+_PREVIEW_DEFAULTS = {"diffuseColor": (0.18, 0.18, 0.18), "emissiveColor": (0.0, 0.0, 0.0), "metallic": 0.0,
+                     "roughness": 0.5, "opacity": 1.0, "ior": 1.5}
+# CameraComponent::Attribute defaults (CameraComponent.h:16-23)
+_CAMERA_DEFAULTS = {"focal_length": 0.035, "sensor_height": 0.024, "aperture": 22.0, "focus_distance": 1.0}
+
+
+class UsdScene:
+    """What the reader produces: the Scene (meshes, instances, materials, textures), the camera
+    (world position, right/front/up rows, physical attributes) or None, the sky colour (DomeLight)
+    or None, the directional light colour or None, and notes on what was skipped."""
+
+    def __init__(self):
+        self.scene = Scene()
+        self.camera = None
+        self.sky = None
+        self.directional = None
+        self.skipped = []
+        self.mesh_names = []
+
+    def camera_desc(self, aspect, far_plane=1000.0):
+        """The render camera of `camera` (CameraComponent::UpdateRenderData and the focus plane of
+        CameraRenderProxy.cpp:24-27, 113-136), in float32."""
+        if self.camera is None:
+            raise UsdError("the scene has no camera")
+        c = self.camera
+        return camera_from_basis(c["position"], c["right"], c["front"], c["up"], c["focal_length"],
+                                 c["sensor_height"], c["aperture"], c["focus_distance"], aspect, far_plane)
+
+
+def camera_from_basis(position, right, front, up, focal_length, sensor_height, aperture, focus_distance, aspect,
+                      far_plane=1000.0):
+    f32 = np.float32
+    fov = f32(2.0) * f32(math.atan(f32(sensor_height) / (f32(2.0) * f32(focal_length))))
+    lens_radius = f32(focal_length) / f32(aperture) * f32(0.5)
+    h = f32(math.tan(fov / f32(2.0)))
+    height = f32(2.0) * h * f32(focus_distance)
+    width = f32(aspect) * height
+    P, R, F, U = (np.asarray(v, f32) for v in (position, right, front, up))
+    mu = R * width
+    mv = U * height
+    ll = P + F * f32(focus_distance) - mu * f32(0.5) - mv * f32(0.5)
+    cd = CameraDesc()
+    cd.position[:] = [float(x) for x in P]
+    cd.right[:] = [float(x) for x in R]
+    cd.up[:] = [float(x) for x in U]
+    cd.front[:] = [float(x) for x in F]
+    cd.lower_left[:] = [float(x) for x in ll]
+    cd.max_u[:] = [float(x) for x in mu]
+    cd.max_v[:] = [float(x) for x in mv]
+    cd.lens_radius = float(lens_radius)
+    cd.far_plane = float(far_plane)
+    return cd
+
+
+def _major_axis(n):
+    """GetPossibleMajorAxis (core/math/Utilities.h:222-234): Right, Front or Up."""
+    inv_sqrt3 = 1.0 / math.sqrt(3.0)
+    out = np.tile(np.array([0.0, 0.0, 1.0], np.float32), (n.shape[0], 1))
+    ax = np.abs(n[:, 0]) < inv_sqrt3
+    ay = ~ax & (np.abs(n[:, 1]) < inv_sqrt3)
+    out[ax] = (1.0, 0.0, 0.0)
+    out[ay] = (0.0, 1.0, 0.0)
+    return out
+
+
+def _fallback_tangents(normals):
+    n = normals.astype(np.float32)
+    t = np.cross(np.cross(n, _major_axis(n)), n)
+    return np.concatenate([t, np.ones((n.shape[0], 1), np.float32)], 1).astype(np.float32)
+
+
+def _interp(attr):
+    return (attr.get("meta") or {}).get("interpolation", "vertex")
+
+
+def _mesh_arrays(prim):
+    """Triangle arrays of a Mesh prim: positions, indices, normals, tangents or None, uvs or None."""
+    A = prim["attrs"]
+    for key in ("points", "faceVertexCounts", "faceVertexIndices"):
+        if A.get(key, {}).get("value") is None:
+            raise UsdError(f"{prim['path']}: {key} is not authored")
+    pts = np.asarray(A["points"]["value"], np.float64).reshape(-1, 3)
+    counts = np.asarray(A["faceVertexCounts"]["value"], np.int64)
+    fvi = np.asarray(A["faceVertexIndices"]["value"], np.int64)
+    if counts.sum() != fvi.size or (counts < 3).any():
+        raise UsdError(f"{prim['path']}: face counts and indices disagree")
+    if fvi.size and (fvi.min() < 0 or fvi.max() >= pts.shape[0]):
+        raise UsdError(f"{prim['path']}: face vertex index out of range")
+    face_of = np.repeat(np.arange(counts.size), counts)
+    # triangle fan per polygon, as corner numbers into the face-vertex list
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    tri = []
+    for s, c in zip(starts, counts):
+        for k in range(1, c - 1):
+            tri.extend((s, s + k, s + k + 1))
+    tri = np.asarray(tri, np.int64).reshape(-1, 3)
+
+    attrs = {}
+    n = A.get("normals") or A.get("primvars:normals")
+    if n is not None and n.get("value") is not None:
+        attrs["normals"] = (np.asarray(n["value"], np.float64).reshape(-1, 3), _interp(n))
+    st = A.get("primvars:st")
+    if st is not None and st.get("value") is not None:
+        vals = np.asarray(st["value"], np.float64).reshape(-1, 2)
+        idx = A.get("primvars:st:indices")
+        if idx is not None and idx.get("value") is not None:
+            vals = vals[np.asarray(idx["value"], np.int64)]
+        attrs["uv"] = (vals, _interp(st))
+    tg = A.get("primvars:tangents")
+    if tg is not None and tg.get("value") is not None:
+        v = np.asarray(tg["value"], np.float64)
+        v = v.reshape(-1, 4) if v.size % 4 == 0 and np.asarray(tg["value"]).ndim == 2 and np.asarray(
+            tg["value"]).shape[1] == 4 else v.reshape(-1, 3)
+        attrs["tangents"] = (v, _interp(tg))
+
+    def per_corner(vals, interp):
+        if interp in ("vertex", "varying"):
+            return vals[fvi]
+        if interp == "faceVarying":
+            return vals
+        if interp == "uniform":
+            return vals[face_of]
+        if interp == "constant":
+            return np.repeat(vals[:1], fvi.size, 0)
+        raise UsdError(f"{prim['path']}: interpolation {interp} is not supported")
+
+    split = any(i not in ("vertex", "varying") for _, i in attrs.values())
+    if not split:  # single-indexable as authored: points are the vertices
+        pos, index = pts, fvi[tri.reshape(-1)]
+        got = {k: v for k, (v, _) in attrs.items()}
+        for k, v in got.items():
+            if v.shape[0] != pts.shape[0]:
+                raise UsdError(f"{prim['path']}: {k} has {v.shape[0]} values for {pts.shape[0]} points")
+    else:  # every face corner becomes a vertex (the vertex-index building of the converter)
+        pos, index = pts[fvi], tri.reshape(-1)
+        got = {k: per_corner(v, i) for k, (v, i) in attrs.items()}
+    if "normals" not in got:
+        # no authored normals: area-weighted vertex normals of the triangles
+        p = pos
+        fn = np.cross(p[index[1::3]] - p[index[0::3]], p[index[2::3]] - p[index[0::3]])
+        vn = np.zeros_like(p)
+        for k in range(3):
+            np.add.at(vn, index[k::3], fn)
+        ln = np.linalg.norm(vn, axis=1, keepdims=True)
+        got["normals"] = np.where(ln > 0, vn / np.maximum(ln, 1e-30), np.array([0.0, 0.0, 1.0]))
+    nrm = got["normals"].astype(np.float32)
+    tan = got.get("tangents")
+    if tan is not None and tan.shape[1] == 3:
+        tan = np.concatenate([tan, np.ones((tan.shape[0], 1))], 1)
+    tan = tan.astype(np.float32) if tan is not None else _fallback_tangents(nrm)
+    uv = got["uv"].astype(np.float32) if "uv" in got else None
+    return pos.astype(np.float32), index.astype(np.uint32), nrm, tan, uv
+
+
+class _Reader:
+    def __init__(self, path):
+        self.dir = os.path.dirname(os.path.abspath(path))
+        with open(path, "r", encoding="utf-8") as f:
+            self.meta, self.roots = parse_usda(f.read())
+        self.by_path = {}
+        for r in self.roots:
+            self._index(r)
+        self.out = UsdScene()
+        self.materials = {}
+        self.textures = {}
+
+    def _index(self, p):
+        self.by_path[p["path"]] = p
+        for c in p["children"]:
+            self._index(c)
+
+    def prim_at(self, path):
+        return self.by_path.get(str(path).split(".")[0])
+
+    # ---- materials ----
+    def texture(self, shader, srgb_default):
+        f = shader["attrs"].get("inputs:file")
+        if f is None or not f.get("value"):
+            raise UsdError(f"{shader['path']}: UsdUVTexture without inputs:file")
+        file = str(f["value"])
+        space = (f.get("meta") or {}).get("colorSpace")
+        if space is None:
+            sc = shader["attrs"].get("inputs:sourceColorSpace")
+            space = sc["value"] if sc is not None and sc.get("value") else "auto"
+        srgb = space in _SRGB_SPACES if srgb_default else space in ("sRGB", "srgb", "sRGB_Texture", "srgb_texture")
+        fmt = TEXTURE_RGBA8_SRGB if srgb else TEXTURE_RGBA8_UNORM
+        full = os.path.normpath(os.path.join(self.dir, file))
+        if full.lower().endswith(".hdr"):
+            fmt = TEXTURE_RGBA32F
+        key = (full, fmt)
+        if key not in self.textures:
+            with open(full, "rb") as fh:
+                data = fh.read()
+            img = hdr_decode(data) if fmt == TEXTURE_RGBA32F else _png_decode(data)
+            self.textures[key] = self.out.scene.add_texture(img, fmt)
+        return self.textures[key]
+
+    def surface_input(self, surf, name):
+        """(value, texture shader prim, output channel) of a UsdPreviewSurface input."""
+        conn = surf["conns"].get(f"inputs:{name}")
+        if conn:
+            src = self.prim_at(conn)
+            if src is None:
+                raise UsdError(f"{surf['path']}: inputs:{name} connects to missing {conn}")
+            shader_id = src["attrs"].get("info:id", {}).get("value")
+            if shader_id == "UsdUVTexture":
+                return None, src, str(conn).split(".")[-1]
+            raise UsdError(f"{surf['path']}: inputs:{name} connects to a {shader_id} shader")
+        a = surf["attrs"].get(f"inputs:{name}")
+        authored = a is not None and a.get("value") is not None
+        return (a["value"] if authored else _PREVIEW_DEFAULTS.get(name)), None, authored
+
+    def material(self, mat_path):
+        if mat_path in self.materials:
+            return self.materials[mat_path]
+        mp = self.prim_at(mat_path)
+        if mp is None or mp["type"] != "Material":
+            raise UsdError(f"material binding {mat_path} does not name a Material prim")
+        # MaterialResource defaults (io/Material.h:9-23) unless a preview surface is found
+        kw = dict(base_color=(1.0, 1.0, 1.0), emissive=(0.0, 0.0, 0.0), metallic=1.0, roughness=1.0)
+        mtype, eta = MATERIAL_PBR, 0.0
+        sconn = mp["conns"].get("outputs:surface")
+        surf = self.prim_at(sconn) if sconn else None
+        if surf is not None and surf["attrs"].get("info:id", {}).get("value") == "UsdPreviewSurface":
+            v, tex, _ = self.surface_input(surf, "diffuseColor")
+            if tex is not None:
+                kw["base_color_texture"] = self.texture(tex, True)
+            else:
+                kw["base_color"] = tuple(float(np.float32(c)) for c in v)
+            v, tex, _ = self.surface_input(surf, "emissiveColor")
+            if tex is not None:
+                kw["emissive_texture"] = self.texture(tex, True)
+                kw["emissive"] = (1.0, 1.0, 1.0)  # the factor multiplies the texture
+            else:
+                kw["emissive"] = tuple(float(np.float32(c)) for c in v)
+            v, tex, _ = self.surface_input(surf, "normal")
+            if tex is not None:
+                kw["normal_texture"] = self.texture(tex, False)
+            vm, tm, _ = self.surface_input(surf, "metallic")
+            vr, tr, _ = self.surface_input(surf, "roughness")
+            if tm is not None:
+                if tr is None or self.texture(tm, False) != self.texture(tr, False):
+                    raise UsdError(f"{surf['path']}: metallic and roughness must come from one texture")
+                kw["metallic_roughness_texture"] = self.texture(tm, False)
+            else:
+                kw["metallic"] = float(np.float32(vm))
+                kw["roughness"] = float(np.float32(vr))
+            vo, to, _ = self.surface_input(surf, "opacity")
+            if to is None and float(vo) < 1.0:
+                mtype = MATERIAL_DIELECTRIC
+                eta = float(np.float32(self.surface_input(surf, "ior")[0]))
+        mi = self.out.scene.add_material(type=mtype, eta=eta, **kw)
+        self.materials[mat_path] = mi
+        return mi
+
+    # ---- nodes ----
+    def mesh(self, prim, world):
+        rel = prim["rels"].get("material:binding")
+        if not rel:
+            self.out.skipped.append(f"{prim['path']}: mesh without material")
+            return
+        mi = self.material(str(rel[0]))
+        pos, idx, nrm, tan, uv = _mesh_arrays(prim)
+        k = self.out.scene.add_mesh(pos, idx, nrm, tan, uv)
+        self.out.scene.add_instance(k, mi, world[:3, :].astype(np.float32))
+        self.out.mesh_names.append(prim["name"])
+
+    def camera(self, prim, world):
+        A = prim["attrs"]
+        proj = A.get("projection", {}).get("value", "perspective")
+        if proj != "perspective":
+            raise UsdError(f"{prim['path']}: only perspective cameras are supported")
+        attr = dict(_CAMERA_DEFAULTS)
+
+        def val(key):
+            a = A.get(key)
+            return None if a is None or a.get("value") is None else float(a["value"])
+
+        fl = val("focalLength")
+        attr["focal_length"] = (fl if fl is not None else 50.0) * 0.001  # USD's schema default is 50 mm
+        va = val("verticalAperture")
+        attr["sensor_height"] = (va if va is not None else attr["sensor_height"] * 1000.0) * 0.001
+        fs = val("fStop")
+        if fs is not None and fs > 0.0:
+            attr["aperture"] = fs
+        fd = val("focusDistance")
+        if fd is not None and fd > 0.0:
+            attr["focus_distance"] = fd
+        rows = world[:3, :3]
+        unit = [r / (np.linalg.norm(r) or 1.0) for r in rows]
+        attr.update(position=world[:3, 3].copy(), right=unit[0], front=unit[1], up=unit[2])
+        self.out.camera = attr
+
+    def light_color(self, prim):
+        A = prim["attrs"]
+
+        def val(key, default):
+            a = A.get(key) or A.get(key.replace("inputs:", ""))
+            return default if a is None or a.get("value") is None else a["value"]
+
+        color = np.asarray(val("inputs:color", (1.0, 1.0, 1.0)), np.float64)
+        return tuple(float(np.float32(c)) for c in color * float(val("inputs:intensity", 1.0)) *
+                     math.pow(2.0, float(val("inputs:exposure", 0.0))))
+
+    def node(self, prim, parent):
+        world = parent @ local_matrix(prim)
+        for c in prim["children"]:
+            if c["spec"] != "def":
+                continue
+            t = c["type"]
+            if t == "Xform":
+                self.node(c, world)
+            elif t == "Mesh":
+                self.mesh(c, world)
+            elif t == "Camera":
+                self.camera(c, world)
+            elif t == "DomeLight":
+                if c["attrs"].get("inputs:texture:file", {}).get("value"):
+                    self.out.skipped.append(f"{c['path']}: textured dome light")
+                self.out.sky = self.light_color(c)
+            elif t == "DistantLight":
+                self.out.directional = self.light_color(c)
+            elif t in ("Scope", "Material", "Shader", "GeomSubset"):
+                continue
+            else:
+                self.out.skipped.append(f"{c['path']}: unsupported prim type {t or '(none)'}")
+
+    def read(self):
+        default = self.meta.get("defaultPrim")
+        root = next((r for r in self.roots if r["name"] == default), None) if default else None
+        if root is None:
+            root = next((r for r in self.roots if r["spec"] == "def"), None)
+        if root is None:
+            raise UsdError("the stage has no root prim")
+        if root["type"] == "Mesh":  # a lone mesh as default prim: its own node
+            self.mesh(root, np.eye(4))
+        else:
+            self.node(root, np.eye(4))
+        return self.out
+
+
+def read_usda(path):
+    """Reads a .usda file into a UsdScene with the reference loader's rules."""
+    return _Reader(path).read()
+
+
+# ---------------------------------------------------------------------------------------------
+# writer
+# ---------------------------------------------------------------------------------------------
+def _fmt(x):
+    return repr(float(np.float32(x)))
+
+
+def _tuple(v):
+    return "(" + ", ".join(_fmt(c) for c in v) + ")"
+
+
+def _valid_name(s):
+    s = re.sub(r"[^A-Za-z0-9_]", "_", s or "")
+    return s if s and not s[0].isdigit() else "_" + s
+
+
+def write_usda(path, scene, camera=None, sky=None):
+    """Writes `scene` as USDA in the exporter's layout: a root Xform "Root" holding one Xform per
+    instance (its transform as `xformOp:transform`) with the mesh in object space, a `/Materials`
+    scope with one UsdPreviewSurface network per material (textures as PNG, float ones as .hdr, in
+    `textures/` beside the file, read through a shared `st_reader`), an optional camera Xform (`camera`: dict of
+    position, right, front, up rows and focal_length / sensor_height / aperture / focus_distance in
+    metres and f-stops) and an optional DomeLight of colour `sky`."""
+    base = os.path.dirname(os.path.abspath(path))
+    os.makedirs(os.path.join(base, "textures"), exist_ok=True)
+    tex_files = {}
+
+    def tex_file(k):
+        if k not in tex_files:
+            px, fmt = scene.textures[k]
+            rel = f"textures/{k}.hdr" if fmt == TEXTURE_RGBA32F else f"textures/{k}.png"
+            with open(os.path.join(base, rel), "wb") as f:
+                f.write(hdr_encode(px) if fmt == TEXTURE_RGBA32F else _png_rgba(px))
+            tex_files[k] = (rel, fmt == TEXTURE_RGBA8_SRGB)
+        return tex_files[k]
+
+    L = ['#usda 1.0', '(', '    defaultPrim = "Root"', '    metersPerUnit = 1', '    upAxis = "Z"', ')', '',
+         'def Xform "Root"', '{']
+
+    def matrix_line(m, ind):
+        rows = np.asarray(m, np.float64).T  # column-vector -> USD row-vector
+        return (f'{ind}matrix4d xformOp:transform = (' + ", ".join(
+            "(" + ", ".join(repr(float(x)) for x in r) + ")" for r in rows) + ")\n"
+                f'{ind}uniform token[] xformOpOrder = ["xformOp:transform"]')
+
+    for k, inst in enumerate(scene.instances):
+        mesh = scene.meshes[inst["mesh"]]
+        T = np.eye(4)
+        T[:3, :] = np.asarray(inst["M"], np.float64).reshape(3, 4)
+        L += [f'    def Xform "node_{k}"', '    {', matrix_line(T, '        '), '',
+              f'        def Mesh "mesh_{k}" (', '            prepend apiSchemas = ["MaterialBindingAPI"]', '        )',
+              '        {']
+        idx = np.asarray(mesh["indices"]).reshape(-1)
+        L.append(f'            int[] faceVertexCounts = [{", ".join(["3"] * (idx.size // 3))}]')
+        L.append(f'            int[] faceVertexIndices = [{", ".join(str(int(i)) for i in idx)}]')
+        L.append(f'            rel material:binding = </Materials/material_{inst["material"]}>')
+        pos = np.asarray(mesh["positions"], np.float32).reshape(-1, 3)
+        if mesh.get("normals") is not None:
+            nrm = np.asarray(mesh["normals"], np.float32).reshape(-1, 3)
+            L.append('            normal3f[] normals = [' + ", ".join(_tuple(v) for v in nrm) + '] (')
+            L += ['                interpolation = "vertex"', '            )']
+        L.append('            point3f[] points = [' + ", ".join(_tuple(v) for v in pos) + ']')
+        if mesh.get("tangents") is not None:
+            tg = np.asarray(mesh["tangents"], np.float32).reshape(-1, 4)
+            L.append('            float4[] primvars:tangents = [' + ", ".join(_tuple(v) for v in tg) + '] (')
+            L += ['                interpolation = "vertex"', '            )']
+        if mesh.get("uvs") is not None:
+            uv = np.asarray(mesh["uvs"], np.float32).reshape(-1, 2)
+            L.append('            texCoord2f[] primvars:st = [' + ", ".join(_tuple(v) for v in uv) + '] (')
+            L += ['                interpolation = "vertex"', '            )']
+        L += ['            uniform token subdivisionScheme = "none"', '        }', '    }', '']
+    if camera is not None:
+        T = np.eye(4)
+        T[0, :3], T[1, :3], T[2, :3] = camera["right"], camera["front"], camera["up"]
+        T[:3, 3] = camera["position"]
+        L += ['    def Xform "camera_node"', '    {', matrix_line(T, '        '), '',
+              '        def Camera "camera"', '        {',
+              f'            float focalLength = {_fmt(camera["focal_length"] * 1000.0)}',
+              f'            float focusDistance = {_fmt(camera["focus_distance"])}',
+              f'            float fStop = {_fmt(camera["aperture"])}',
+              '            token projection = "perspective"',
+              f'            float verticalAperture = {_fmt(camera["sensor_height"] * 1000.0)}',
+              '        }', '    }', '']
+    if sky is not None:
+        L += ['    def DomeLight "sky"', '    {', f'        color3f inputs:color = {_tuple(sky)}',
+              '        float inputs:intensity = 1', '    }']
+    L += ['}', '', 'def Scope "Materials"', '{']
+    for k, m in enumerate(scene.materials):
+        mp = f"/Materials/material_{k}"
+        L += [f'    def Material "material_{k}"', '    {',
+              f'        token outputs:surface.connect = <{mp}/surface.outputs:surface>', '',
+              '        def Shader "st_reader"', '        {', '            uniform token info:id = "UsdPrimvarReader_float2"',
+              '            string inputs:varname = "st"', '            float2 outputs:result', '        }', '']
+        surf = ['        def Shader "surface"', '        {', '            uniform token info:id = "UsdPreviewSurface"']
+
+        def add_tex(name, k_tex):
+            rel, srgb = tex_file(k_tex)
+            L.extend([f'        def Shader "{name}"', '        {', '            uniform token info:id = "UsdUVTexture"',
+                      f'            asset inputs:file = @{rel}@',
+                      f'            token inputs:sourceColorSpace = "{"sRGB" if srgb else "raw"}"',
+                      f'            float2 inputs:st.connect = <{mp}/st_reader.outputs:result>',
+                      '            token inputs:wrapS = "repeat"', '            token inputs:wrapT = "repeat"',
+                      '            float outputs:b', '            float outputs:g', '            float3 outputs:rgb',
+                      '        }', ''])
+            return f"{mp}/{name}"
+
+        def tex_index(key):
+            v = m.get(key, -1)
+            return v if v is not None and v >= 0 else None
+
+        if tex_index("base_color_texture") is not None:
+            surf.append(f'            color3f inputs:diffuseColor.connect = <{add_tex("base_color_tex", m["base_color_texture"])}.outputs:rgb>')
+        else:
+            surf.append(f'            color3f inputs:diffuseColor = {_tuple(m["base_color"])}')
+        if tex_index("emissive_texture") is not None:
+            surf.append(f'            color3f inputs:emissiveColor.connect = <{add_tex("emissive_tex", m["emissive_texture"])}.outputs:rgb>')
+        else:
+            surf.append(f'            color3f inputs:emissiveColor = {_tuple(m["emissive"])}')
+        if tex_index("normal_texture") is not None:
+            surf.append(f'            normal3f inputs:normal.connect = <{add_tex("normal_tex", m["normal_texture"])}.outputs:rgb>')
+        if tex_index("metallic_roughness_texture") is not None:
+            p = add_tex("metallic_roughness_tex", m["metallic_roughness_texture"])
+            surf.append(f'            float inputs:metallic.connect = <{p}.outputs:b>')
+            surf.append(f'            float inputs:roughness.connect = <{p}.outputs:g>')
+        else:
+            surf.append(f'            float inputs:metallic = {_fmt(m["metallic"])}')
+            surf.append(f'            float inputs:roughness = {_fmt(m["roughness"])}')
+        if m["type"] == MATERIAL_DIELECTRIC:
+            surf.append(f'            float inputs:ior = {_fmt(m["eta"])}')
+            surf.append('            float inputs:opacity = 0')
+        surf += ['            int inputs:useSpecularWorkflow = 0', '            token outputs:surface', '        }']
+        L += surf + ['    }', '']
+    L += ['}', '']
+    with open(path, "w", encoding="utf-8") as f:
+        f.write("\n".join(L))
