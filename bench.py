@@ -356,6 +356,86 @@ def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_
             "bvh": {"width": st.bvh_width, "depth": st.depth, "nodes": st.num_nodes, "build_s": round(t_build, 2)}}
 
 
+def two_level_experiment(spk, scenes, stream, width=1920, height=1080, spp=16, n_rays=1 << 22, max_bounce=8):
+    """Instanced form of the many-instance scene (icospheres and tori as shared object-space
+    prototypes, displaced boxes as their own meshes, each placed by a transform): uploaded flattened
+    and two-level (TLAS over the instances, one BLAS per mesh); for each, the BVH build time, device
+    bytes, closest-hit rate on n_rays incoherent rays in the scene box (autotuned variant) and the
+    plain path-trace frame time (autotuned schedule); for the two-level scene, the host time of
+    spk_scene_update_instances moving every instance."""
+    import torch
+
+    log("two-level experiment: instanced many-instance scene")
+    sc, cam = scenes.many_instance_scene(seed=0, aspect=width / height, shared=True)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=n_rays, seed=0)
+    d_rays = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()
+    d_hits = torch.empty(n_rays * 8, dtype=torch.uint8, device="cuda")
+    npx = width * height
+    rgba = torch.zeros(npx * 4, dtype=torch.float32, device="cuda")
+    out = {"scene": f"many-instance generator seed 0, instanced: {len(sc.meshes)} meshes "
+                    f"({sum(m['indices'].shape[0] for m in sc.meshes)} object-space triangles), "
+                    f"{len(sc.instances)} instances, {sc.num_triangles} world triangles",
+           "rays": n_rays}
+    hits = {}
+    for two in (False, True):
+        ctx = spk.Context(0)
+        ctx.set_stream(stream.cuda_stream)
+        t0 = time.perf_counter()
+        st = ctx.upload_scene(sc, two_level=two)
+        up = time.perf_counter() - t0
+        cfg, _ = ctx.autotune(d_rays, min(n_rays, 1 << 20))
+        st = ctx.stats()
+        for _ in range(2):
+            ctx.trace(d_rays, d_hits, n_rays, sync=False)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(10):
+            ctx.trace(d_rays, d_hits, n_rays, sync=False)
+        b.record(stream)
+        torch.cuda.synchronize()
+        trace_ms = a.elapsed_time(b) / 10
+        hits[two] = d_hits.cpu().numpy().view(spk.HIT_DTYPE)["prim"].copy()
+        ctx.set_camera(cam)
+
+        def frame(seed):
+            f = spk.FrameDesc()
+            f.width, f.height, f.frame_seed, f.max_bounce = width, height, seed, max_bounce
+            f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, 0, 0.0
+            return f
+
+        sched, _ = ctx.autotune_render(frame(10 ** 6), rgba)
+        ctx.render_frame(frame(10 ** 6 + 1), rgba)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(spp):
+            ctx.render_frame(frame(k), rgba)
+        torch.cuda.synchronize()
+        frame_ms = (time.perf_counter() - t0) / spp * 1e3
+        r = {"upload_s": round(up, 3), "build_s": round(st.build_seconds, 3), "nodes": st.num_nodes,
+             "depth": st.depth, "device_bytes": st.device_bytes,
+             "trace_config": {"bvh_width": cfg[0], "stack": cfg[1], "block": cfg[2], "refill": cfg[3]},
+             "trace_ms": round(trace_ms, 4), "trace_mrays_per_s": round(n_rays / trace_ms / 1e3, 1),
+             "frame_ms": round(frame_ms, 3), "samples_per_s": round(npx / frame_ms * 1e3, 1),
+             "schedule": dict(zip(("blocks_per_cu", "material_sort", "refill", "tail_bounce"), sched))}
+        if two:
+            xfs = np.stack([i["M"] for i in sc.instances]).astype(np.float32)
+            times = []
+            for k in range(3):
+                moved = xfs.copy()
+                moved[:, 2, 3] += np.float32(0.01 * (k + 1))  # every instance moves up
+                t0 = time.perf_counter()
+                ctx.update_instances(moved)
+                times.append(time.perf_counter() - t0)
+            ctx.update_instances(xfs)
+            r["update_all_instances_s"] = round(float(np.median(times)), 4)
+        ctx.close()
+        out["two_level" if two else "flattened"] = r
+    out["same_ids_fraction"] = float((hits[False] == hits[True]).mean())
+    return out
+
+
 def c1_experiment(spk, scenes, stream, width=256, height=256, spp=16, max_bounce=4):
     """The reference's own CPU-runnable configuration: the Cornell room (1,316 triangles) at 256x256,
     16 spp, 4 bounces, rendered on the device and by the oracle's restatement of the reference CPU
@@ -478,6 +558,8 @@ def main():
                     help="only run the full path trace of the many-instance scene for N spp (one JSON line)")
     ap.add_argument("--render-mode", choices=sorted(RENDER_MODES), default="plain",
                     help="estimator of --render-only")
+    ap.add_argument("--two-level", action="store_true",
+                    help="only the flattened vs two-level comparison on the instanced scene (one JSON line)")
     ap.add_argument("--cache-kernels", action="store_true",
                     help="only time the learned cache's inference and training kernels (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -503,6 +585,15 @@ def main():
         torch.cuda.set_stream(stream)
         print(json.dumps({"full_path": render_experiment(spk, scenes, stream, spp=args.render_only,
                                                          mode=args.render_mode)}), flush=True)
+        return
+    if args.two_level:
+        import torch
+        import sparkle_amd as spk
+        from sparkle_amd import scenes
+
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        print(json.dumps({"two_level": two_level_experiment(spk, scenes, stream)}), flush=True)
         return
     if args.cache_kernels:
         import torch
@@ -660,6 +751,7 @@ def main():
     out["cache_kernels"] = cache_kernel_bench(spk, scenes, stream)
     out["full_path"] = render_experiment(spk, scenes, stream)
     out["c1"] = c1_experiment(spk, scenes, stream)
+    out["two_level"] = two_level_experiment(spk, scenes, stream)
     if not args.no_batch:
         out["batch"] = batch_experiment(spk, scenes, stream)
     if args.relmse != "off":

@@ -80,7 +80,8 @@ typedef struct spk_mesh_desc {
 } spk_mesh_desc;
 
 /* Instance of a mesh with a material. world_from_local is row-major 3x4. Instances are baked
-   into world space at upload (the hot path works on flattened geometry). */
+   into world space at upload (one flattened tree), or kept as TLAS leaves over object-space BLASes
+   with spk_build_options.two_level. */
 typedef struct spk_instance_desc {
     uint32_t mesh;
     uint32_t material;
@@ -126,6 +127,10 @@ typedef struct spk_build_options {
     uint32_t bvh_width;     /* 8 (default) or 4 */
     uint32_t max_leaf_tris; /* triangles per leaf slot, 1..3 (default 3) */
     uint32_t num_threads;   /* host build threads, 0 = all */
+    uint32_t two_level;     /* 1: one BLAS per mesh in object space under a TLAS over the instances
+                               (SceneRenderProxy's TLAS / MeshRenderProxy's BLAS), so that
+                               spk_scene_update_instances moves instances without rebuilding any
+                               BLAS; 0: one tree over the flattened triangles (default, fastest) */
 } spk_build_options;
 
 typedef struct spk_scene_stats {
@@ -203,6 +208,13 @@ int spk_scene_upload(spk_context *ctx, const spk_mesh_desc *meshes, uint32_t num
                      const spk_material_desc *materials, uint32_t num_materials,
                      const spk_build_options *options);
 int spk_scene_stats_get(spk_context *ctx, spk_scene_stats *out);
+/* Moves the instances of a two-level scene (spk_build_options.two_level): world_from_local holds
+   num_instances row-major 3x4 transforms in upload order (num_instances = the uploaded count).
+   Replaces SceneRenderProxy::UpdateBVH's TLAS rebuild on primitive moves (SceneRenderProxy.cpp:
+   269-297): the TLAS is rebuilt over the moved world boxes, every BLAS is kept, and the world-space
+   shading data of the instances whose transform changed is recomputed. Hits after the update are
+   those of a fresh upload of the moved scene. SPK_ERR_UNSUPPORTED for a flattened scene. */
+int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, uint32_t num_instances);
 /* Analytic sphere (SpherePrimitive / SphereRenderProxy.cpp:5-48, SphereRenderProxy.h:22-36):
    hit t = -half_b -+ sqrt(half_b^2 - c) with half_b = d.(o - center), c = |o - center|^2 - r^2
    (near root from outside, far root from inside; ray directions are taken as unit length, as in
@@ -320,8 +332,8 @@ int spk_batch_offsets(spk_context *ctx, uint64_t *offsets, uint32_t max_count);
 typedef struct spk_schedule {
     uint32_t blocks_per_cu; /* persistent grid: 128-thread blocks per CU, 1..32 (default 8) */
     uint32_t material_sort; /* 1: shade each bounce in material order (counting sort; default 1) */
-    uint32_t refill;        /* extend: a wave takes new paths once this many of its 64 lanes are
-                               idle, 1..64 (default 16; 64 = whole waves) */
+    uint32_t refill;        /* extend after the camera rays (which refill whole waves): a wave takes
+                               new paths once this many of its 64 lanes are idle, 1..64 (default 16) */
     uint32_t tail_bounce;   /* segments from this index on run one path per lane to its end (closest
                                hit, shading and shadow ray in one persistent kernel) instead of one
                                bounce-synchronous launch per segment; 0 = none (default), <= 64 */
@@ -331,7 +343,7 @@ typedef struct spk_schedule {
 int spk_get_schedule(spk_context *ctx, int batch, spk_schedule *out);
 int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule);
 /* Schedule autotuners: time blocks_per_cu {2, 4, 8, 16} x material_sort {0, 1} x refill {16, 64} on
-   one frame (after a capture frame per candidate), then tail_bounce {2, 3, 4, 5} (below max_bounce)
+   one frame (after a capture frame per candidate), then tail_bounce {1, 2, 3, 4, 5} (below max_bounce)
    on the fastest of those; keep and return the fastest. */
 int spk_render_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,
                         double *best_ms);

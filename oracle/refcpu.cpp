@@ -997,6 +997,37 @@ void orcl_scene_set_textures(orcl_scene *scene, const orcl_texture *textures, ui
         }
     }
 }
+// Twin of the device's two-level traversal (no counterpart in the reference): the library uploads,
+// per instance, object_from_world = the inverse of the 3x3 part computed in double and rounded once,
+// with translation -inverse * t in double, rounded once; the device transforms the world ray with it
+// in the order ((a0 x + a1 y) + a2 z) + a3 and tests the instance's object-space triangles. The
+// closest hit (t, then global id) over all instances does not depend on the TLAS or BLAS shapes.
+static void object_from_world(const Xf &w, float r[12]) {
+    const float *M = w.m;
+    const double a[9] = {M[0], M[1], M[2], M[4], M[5], M[6], M[8], M[9], M[10]};
+    const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                       a[2] * (a[3] * a[7] - a[4] * a[6]);
+    const double inv[9] = {(a[4] * a[8] - a[5] * a[7]) / det, (a[2] * a[7] - a[1] * a[8]) / det,
+                           (a[1] * a[5] - a[2] * a[4]) / det, (a[5] * a[6] - a[3] * a[8]) / det,
+                           (a[0] * a[8] - a[2] * a[6]) / det, (a[2] * a[3] - a[0] * a[5]) / det,
+                           (a[3] * a[7] - a[4] * a[6]) / det, (a[1] * a[6] - a[0] * a[7]) / det,
+                           (a[0] * a[4] - a[1] * a[3]) / det};
+    const double t[3] = {M[3], M[7], M[11]};
+    for (int row = 0; row < 3; ++row) {
+        r[4 * row] = (float)inv[3 * row];
+        r[4 * row + 1] = (float)inv[3 * row + 1];
+        r[4 * row + 2] = (float)inv[3 * row + 2];
+        r[4 * row + 3] = (float)-(inv[3 * row] * t[0] + inv[3 * row + 1] * t[1] + inv[3 * row + 2] * t[2]);
+    }
+}
+
+static inline void object_ray_twin(const float *r, V3 wo, V3 wd, V3 &o, V3 &d) {
+    o = V3(((r[0] * wo.x + r[1] * wo.y) + r[2] * wo.z) + r[3], ((r[4] * wo.x + r[5] * wo.y) + r[6] * wo.z) + r[7],
+           ((r[8] * wo.x + r[9] * wo.y) + r[10] * wo.z) + r[11]);
+    d = V3((r[0] * wd.x + r[1] * wd.y) + r[2] * wd.z, (r[4] * wd.x + r[5] * wd.y) + r[6] * wd.z,
+           (r[8] * wd.x + r[9] * wd.y) + r[10] * wd.z);
+}
+
 uint64_t orcl_scene_num_triangles(const orcl_scene *scene) { return scene->total_tris; }
 
 void orcl_set_fp64_eps(double eps_edge, double eps_tie, double eps_graze) {
@@ -1011,13 +1042,57 @@ int orcl_trace(const orcl_scene *scene, const orcl_ray *rays, uint64_t num_rays,
                orcl_hit *hits, uint32_t *flags, double *t64) {
     if (!scene || (!rays && num_rays) || (!hits && num_rays)) return -1;
     const orcl_scene &sc = *scene;
+    std::vector<float> inv(mode == ORCL_TRACE_WT_TWO ? 12 * sc.instances.size() : 0);
+    for (size_t k = 0; 12 * k < inv.size(); ++k) object_from_world(sc.instances[k].to_world, &inv[12 * k]);
     parallel_for(num_rays, num_threads, [&](uint64_t b, uint64_t e) {
         for (uint64_t i = b; i < e; ++i) {
             const orcl_ray &r = rays[i];
             V3 o(r.org), d(r.dir);
             hits[i].prim = 0xFFFFFFFFu;
             hits[i].t = std::numeric_limits<float>::infinity();
-            if (mode == ORCL_TRACE_REF_MT) {
+            if (mode == ORCL_TRACE_WT_TWO) {
+                bool ok = std::isfinite(o.x) && std::isfinite(o.y) && std::isfinite(o.z) && std::isfinite(d.x) &&
+                          std::isfinite(d.y) && std::isfinite(d.z) && (d.x != 0.f || d.y != 0.f || d.z != 0.f);
+                if (!ok || std::isnan(r.tmin) || std::isnan(r.tmax)) continue;
+                WtBest best;
+                best.t = r.tmax;
+                const float tmin = std::max(r.tmin, 0.f);
+                for (size_t ii = 0; ii < sc.instances.size(); ++ii) {
+                    const Instance &ins = sc.instances[ii];
+                    const Mesh &m = sc.meshes[ins.mesh];
+                    V3 lo, ld;
+                    object_ray_twin(&inv[12 * ii], o, d, lo, ld);
+                    WtRay wr = wt_setup(lo, ld);
+                    for (uint32_t f = 0; f < m.ntri; ++f) {
+                        V3 a, bb, c;
+                        m.tri(f, a, bb, c);
+                        float t, U, V, W, det;
+                        if (tri_wt(wr, a, bb, c, t, U, V, W, det))
+                            wt_consider(best, t, tmin, ins.prim_offset + f, (int)ii, f, U, V, W, det);
+                    }
+                }
+                for (size_t k = 0; k < sc.spheres.size(); ++k) {
+                    float t;
+                    const uint64_t gid = sc.total_tris + k;
+                    if (sphere_t(sc.spheres[k], o, d, t) && t > tmin && t > 0.f &&
+                        (t < best.t || (t == best.t && gid < best.gid))) {
+                        best.t = t;
+                        best.gid = gid;
+                        best.sphere = (int)k;
+                    }
+                }
+                if (best.sphere >= 0) {
+                    hits[i].prim = (uint32_t)best.gid;
+                    hits[i].t = best.t;
+                } else if (best.gid != ~0ull) {
+                    const Instance &ins = sc.instances[best.inst];
+                    V3 lo, ld, a, bb, c;
+                    object_ray_twin(&inv[12 * (size_t)best.inst], o, d, lo, ld);
+                    sc.meshes[ins.mesh].tri(best.face, a, bb, c);
+                    hits[i].prim = (uint32_t)best.gid;
+                    hits[i].t = refine_t(lo, ld, a, bb, c, best.t);
+                }
+            } else if (mode == ORCL_TRACE_REF_MT) {
                 Candidate c;
                 scene_intersect_ref(sc, o, d, c);
                 if (c.inst >= 0 || c.sphere >= 0) {

@@ -126,7 +126,9 @@ enum {
     ORCL_TRACE_REF_MT = 0,   /* reference semantics: MT, |det|<1e-6 cutoff, no tmax culling */
     ORCL_TRACE_WT = 1,       /* watertight FP32 (GPU twin), order-independent ties */
     ORCL_TRACE_WT_BRUTE = 2, /* watertight FP32 over every triangle */
-    ORCL_TRACE_FP64 = 3      /* FP64 oracle with flags */
+    ORCL_TRACE_FP64 = 3,     /* FP64 oracle with flags */
+    ORCL_TRACE_WT_TWO = 4    /* watertight FP32 twin of a two-level scene: every instance's triangles
+                                in object space, ray through the float inverse transform */
 };
 
 enum {

@@ -15,6 +15,7 @@ TRACE_REF_MT = 0
 TRACE_WT = 1
 TRACE_WT_BRUTE = 2
 TRACE_FP64 = 3
+TRACE_WT_TWO = 4
 FLAG_EDGE, FLAG_TIE, FLAG_GRAZE = 1, 2, 4
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])

@@ -85,7 +85,8 @@ class TextureDesc(ctypes.Structure):
 
 
 class BuildOptions(ctypes.Structure):
-    _fields_ = [("bvh_width", ctypes.c_uint32), ("max_leaf_tris", ctypes.c_uint32), ("num_threads", ctypes.c_uint32)]
+    _fields_ = [("bvh_width", ctypes.c_uint32), ("max_leaf_tris", ctypes.c_uint32), ("num_threads", ctypes.c_uint32),
+                ("two_level", ctypes.c_uint32)]
 
 
 class SceneStats(ctypes.Structure):
@@ -197,6 +198,7 @@ def load_library():
         "spk_tonemap": (I, [P, P, P, U32, U32, F]),
         "spk_spatial_fill": (I, [P, P, P, P, U32, U32]),
         "spk_batch_upload": (I, [P, P, U32, ctypes.POINTER(BuildOptions)]),
+        "spk_scene_update_instances": (I, [P, ctypes.POINTER(F), U32]),
         "spk_batch_render": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(FrameStats)]),
         "spk_batch_offsets": (I, [P, ctypes.POINTER(U64), U32]),
         "spk_batch_autotune": (I, [P, ctypes.POINTER(FrameDesc), P, ctypes.POINTER(Schedule),
@@ -379,20 +381,28 @@ class Context:
             arr[k].pixels = px.ctypes.data
         return self._check(self.lib.spk_textures_upload(self.h, arr, len(textures)))
 
-    def upload_scene(self, scene, bvh_width=8, max_leaf_tris=3, num_threads=0):
+    def upload_scene(self, scene, bvh_width=8, max_leaf_tris=3, num_threads=0, two_level=False):
         """spk_textures_upload (the scene's textures, when it has any or the table is not empty)
-        followed by spk_scene_upload."""
+        followed by spk_scene_upload; two_level keeps one BLAS per mesh under a TLAS of instances
+        (then update_instances moves them)."""
         if scene.textures or getattr(self, "_has_textures", False):
             self.upload_textures(scene.textures)
             self._has_textures = bool(scene.textures)
         ms, ins, mats = scene.c_arrays()
-        opts = BuildOptions(bvh_width, max_leaf_tris, num_threads)
+        opts = BuildOptions(bvh_width, max_leaf_tris, num_threads, 1 if two_level else 0)
         self._check(self.lib.spk_scene_upload(self.h, ms, len(scene.meshes), ins, len(scene.instances), mats,
                                               len(scene.materials), ctypes.byref(opts)))
         if getattr(scene, "spheres", None):
             self.set_spheres(scene.spheres)
         self.set_sky_cubemap(getattr(scene, "sky_faces", None))
         return self.stats()
+
+    def update_instances(self, transforms):
+        """spk_scene_update_instances: transforms is (num_instances, 3, 4) (or 12 floats each),
+        world_from_local of every instance in upload order."""
+        m = np.ascontiguousarray(transforms, dtype=np.float32).reshape(-1, 12)
+        return self._check(self.lib.spk_scene_update_instances(
+            self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), m.shape[0]))
 
     def set_sky_cubemap(self, faces):
         """spk_set_sky_cubemap: six texture indices (+X, -X, +Y, -Y, +Z, -Z) or None."""

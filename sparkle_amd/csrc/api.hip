@@ -56,10 +56,23 @@ hipError_t DeviceBuffer::ensure(size_t want) {
     return hipSuccess;
 }
 
-// Flattens instances into world-space triangles and builds the wide BVH on the host.
+// World position of an object-space point: M (row-major 3x4) x p, copied exactly for the identity.
+static inline void transform_point(const float *M, const float *p, float *dst) {
+    static const float I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    if (std::memcmp(M, I, sizeof(I)) == 0) {
+        dst[0] = p[0], dst[1] = p[1], dst[2] = p[2];
+        return;
+    }
+    dst[0] = M[0] * p[0] + M[1] * p[1] + M[2] * p[2] + M[3];
+    dst[1] = M[4] * p[0] + M[5] * p[1] + M[6] * p[2] + M[7];
+    dst[2] = M[8] * p[0] + M[9] * p[1] + M[10] * p[2] + M[11];
+}
+
+// Flattens instances into world-space triangles and (unless `build` is false) builds the wide BVH
+// over them on the host.
 static int flatten_and_build(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_instance_desc *instances,
                              uint32_t num_instances, const spk_build_options *options, HostScene &hs,
-                             std::string &err) {
+                             std::string &err, bool build = true) {
     if (!meshes || num_meshes == 0 || !instances || num_instances == 0) {
         err = "scene needs at least one mesh and one instance";
         return SPK_ERR_INVALID_ARGUMENT;
@@ -104,20 +117,10 @@ static int flatten_and_build(const spk_mesh_desc *meshes, uint32_t num_meshes, c
         const spk_instance_desc &ins = instances[i];
         const spk_mesh_desc &md = meshes[ins.mesh];
         const float *M = ins.world_from_local;
-        static const float I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-        const bool ident = std::memcmp(M, I, sizeof(I)) == 0;
         for (uint32_t f = 0; f < md.num_triangles; ++f, ++g) {
-            for (int k = 0; k < 3; ++k) {
-                const float *p = md.positions + 3 * (size_t)md.indices[3 * (size_t)f + k];
-                float *dst = &hs.verts[9 * (size_t)g + 3 * k];
-                if (ident) {
-                    dst[0] = p[0], dst[1] = p[1], dst[2] = p[2];
-                } else {
-                    dst[0] = M[0] * p[0] + M[1] * p[1] + M[2] * p[2] + M[3];
-                    dst[1] = M[4] * p[0] + M[5] * p[1] + M[6] * p[2] + M[7];
-                    dst[2] = M[8] * p[0] + M[9] * p[1] + M[10] * p[2] + M[11];
-                }
-            }
+            for (int k = 0; k < 3; ++k)
+                transform_point(M, md.positions + 3 * (size_t)md.indices[3 * (size_t)f + k],
+                                &hs.verts[9 * (size_t)g + 3 * k]);
             hs.global_ids[g] = g;
             hs.materials[g] = ins.material;
             hs.local_ids[g] = f;
@@ -134,28 +137,346 @@ static int flatten_and_build(const spk_mesh_desc *meshes, uint32_t num_meshes, c
     in.max_leaf_tris = options && options->max_leaf_tris ? options->max_leaf_tris : 3u;
     in.num_threads = options ? options->num_threads : 0u;
     hs.input = in;
+    if (!build) return SPK_OK;
     err = build_wide_bvh(in, hs.bvh);
     if (!err.empty()) return SPK_ERR_INVALID_SCENE;
     return SPK_OK;
 }
 
+// Shading attributes of face f of mesh md placed by M (row-major 3x4): normals and tangents through
+// the 3x3 part, the geometric normal of the world triangle v9 when the mesh has no normals, the
+// default tangent (1, 0, 0) through the 3x3 part when it has no tangents.
+static void shade_triangle(const spk_mesh_desc &md, const float *M, uint32_t f, const float *v, uint32_t material,
+                           TriShade &s) {
+    std::memset(&s, 0, sizeof(s));
+    float *ns[3] = {s.n0, s.n1, s.n2}, *ts[3] = {s.t0, s.t1, s.t2}, *uvs[3] = {s.uv0, s.uv1, s.uv2};
+    const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+    float gn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const float gl = std::sqrt(gn[0] * gn[0] + gn[1] * gn[1] + gn[2] * gn[2]);
+    if (gl > 0.f) gn[0] /= gl, gn[1] /= gl, gn[2] /= gl;
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t vi = md.indices[3 * (size_t)f + c];
+        if (md.normals) {
+            const float *nn = md.normals + 3 * (size_t)vi;
+            for (int r = 0; r < 3; ++r) ns[c][r] = M[4 * r] * nn[0] + M[4 * r + 1] * nn[1] + M[4 * r + 2] * nn[2];
+        } else {
+            for (int r = 0; r < 3; ++r) ns[c][r] = gn[r];
+        }
+        if (md.tangents) {
+            const float *tt = md.tangents + 4 * (size_t)vi;
+            for (int r = 0; r < 3; ++r) ts[c][r] = M[4 * r] * tt[0] + M[4 * r + 1] * tt[1] + M[4 * r + 2] * tt[2];
+            ts[c][3] = tt[3];
+        } else {
+            ts[c][0] = M[0], ts[c][1] = M[4], ts[c][2] = M[8];
+            ts[c][3] = 1.f;
+        }
+        if (md.uvs) {
+            uvs[c][0] = md.uvs[2 * (size_t)vi];
+            uvs[c][1] = md.uvs[2 * (size_t)vi + 1];
+        }
+    }
+    s.material = material;
+}
+
+static spk_mesh_desc mesh_view(const MeshCopy &mc) {
+    spk_mesh_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.positions = mc.positions.data();
+    d.normals = mc.normals.empty() ? nullptr : mc.normals.data();
+    d.tangents = mc.tangents.empty() ? nullptr : mc.tangents.data();
+    d.uvs = mc.uvs.empty() ? nullptr : mc.uvs.data();
+    d.indices = mc.indices.data();
+    d.num_vertices = mc.num_vertices;
+    d.num_triangles = mc.num_triangles;
+    return d;
+}
+
+// ---- two-level scenes ----
+// BLAS per mesh over its object-space triangles (v0.w = face index), built once.
+static int build_blases(HostScene &hs, uint32_t width, uint32_t max_leaf, uint32_t threads, std::string &err) {
+    const uint32_t nm = (uint32_t)hs.mesh_copies.size();
+    std::vector<char> used(nm, 0);
+    for (uint32_t m : hs.inst_mesh) used[m] = 1;
+    hs.blas.assign(nm, BuildOutput());
+    for (uint32_t m = 0; m < nm; ++m) {
+        const MeshCopy &mc = hs.mesh_copies[m];
+        if (!used[m] || mc.num_triangles == 0) continue;
+        std::vector<float> v(9 * (size_t)mc.num_triangles);
+        std::vector<uint32_t> ids(mc.num_triangles), zero(mc.num_triangles, 0u);
+        for (uint32_t f = 0; f < mc.num_triangles; ++f) {
+            ids[f] = f;
+            for (int k = 0; k < 3; ++k)
+                std::memcpy(&v[9 * (size_t)f + 3 * k], &mc.positions[3 * (size_t)mc.indices[3 * (size_t)f + k]], 12);
+        }
+        BuildInput in;
+        in.tri_verts = v.data();
+        in.global_ids = ids.data();
+        in.materials = zero.data();
+        in.local_ids = ids.data();
+        in.num_tris = mc.num_triangles;
+        in.width = width;
+        in.max_leaf_tris = max_leaf;
+        in.num_threads = threads;
+        err = build_wide_bvh(in, hs.blas[m]);
+        if (!err.empty()) {
+            err = "mesh " + std::to_string(m) + ": " + err;
+            return SPK_ERR_INVALID_SCENE;
+        }
+    }
+    return SPK_OK;
+}
+
+// TLAS over the instances' world boxes (each box as the pseudo-triangle lo, hi, hi, one per leaf
+// slot), then the device arrays: TLAS nodes first, every BLAS after them with its child and
+// triangle bases rebased, and one instance record per TLAS leaf slot.
+static int assemble_two_level(HostScene &hs, uint32_t width, uint32_t threads, std::string &err) {
+    const uint32_t ni = (uint32_t)hs.inst_mesh.size();
+    std::vector<uint32_t> live;  // instances with triangles
+    std::vector<float> pv;
+    for (uint32_t i = 0; i < ni; ++i) {
+        const uint32_t nt = hs.mesh_copies[hs.inst_mesh[i]].num_triangles;
+        if (nt == 0) continue;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const float *v = &hs.verts[9 * (size_t)hs.inst_id_base[i]];
+        for (size_t k = 0; k < 3 * (size_t)nt; ++k)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], v[3 * k + a]);
+                hi[a] = std::max(hi[a], v[3 * k + a]);
+            }
+        // the BLAS is hit with a rounded object-space ray: pad the world box by a few ulps of its
+        // coordinates so that the TLAS never culls a hit the object-space test reports
+        for (int a = 0; a < 3; ++a) {
+            const float pad = 1e-5f * std::max({std::fabs(lo[a]), std::fabs(hi[a]), hi[a] - lo[a], 1e-6f});
+            lo[a] -= pad;
+            hi[a] += pad;
+        }
+        live.push_back(i);
+        pv.insert(pv.end(), {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], hi[0], hi[1], hi[2]});
+    }
+    std::vector<uint32_t> ids(live.size()), zero(live.size(), 0u);
+    for (uint32_t k = 0; k < live.size(); ++k) ids[k] = k;
+    BuildInput tin;
+    tin.tri_verts = pv.data();
+    tin.global_ids = ids.data();
+    tin.materials = zero.data();
+    tin.local_ids = ids.data();
+    tin.num_tris = (uint32_t)live.size();
+    tin.width = width;
+    tin.max_leaf_tris = 1;
+    tin.num_threads = threads;
+    err = build_wide_bvh(tin, hs.tlas);
+    if (!err.empty()) {
+        err = "TLAS: " + err;
+        return SPK_ERR_INVALID_SCENE;
+    }
+    const uint32_t nb = hs.tlas.node_bytes;
+    const uint32_t slots = width;
+    // the traversal keeps a TLAS node's leaf slots as one bit mask below bit 24
+    for (uint32_t k = 0; k < hs.tlas.num_nodes; ++k) {
+        const uint8_t *meta = &hs.tlas.nodes[(size_t)k * nb + 24];
+        for (uint32_t sl = 0; sl < slots; ++sl)
+            if (meta[sl] && (meta[sl] & 31u) + (meta[sl] >> 5) > 24u) {
+                err = "TLAS leaf layout exceeds the traversal's pending-instance mask";
+                return SPK_ERR_INVALID_SCENE;
+            }
+    }
+    uint32_t total_nodes = hs.tlas.num_nodes, total_tris = 0, max_depth = 0;
+    std::vector<uint32_t> root(hs.blas.size(), 0u), tri_off(hs.blas.size(), 0u);
+    for (size_t m = 0; m < hs.blas.size(); ++m) {
+        if (hs.blas[m].num_nodes == 0) continue;
+        root[m] = total_nodes;
+        tri_off[m] = total_tris;
+        total_nodes += hs.blas[m].num_nodes;
+        total_tris += (uint32_t)hs.blas[m].tris.size();
+        max_depth = std::max(max_depth, hs.blas[m].depth);
+    }
+    hs.nodes2.assign((size_t)total_nodes * nb, 0);
+    std::memcpy(hs.nodes2.data(), hs.tlas.nodes.data(), (size_t)hs.tlas.num_nodes * nb);
+    hs.tris2.resize(total_tris);
+    for (size_t m = 0; m < hs.blas.size(); ++m) {
+        const BuildOutput &b = hs.blas[m];
+        if (b.num_nodes == 0) continue;
+        uint8_t *dst = &hs.nodes2[(size_t)root[m] * nb];
+        std::memcpy(dst, b.nodes.data(), (size_t)b.num_nodes * nb);
+        for (uint32_t k = 0; k < b.num_nodes; ++k) {
+            uint32_t cb, tb;
+            std::memcpy(&cb, dst + (size_t)k * nb + 16, 4);
+            std::memcpy(&tb, dst + (size_t)k * nb + 20, 4);
+            cb += root[m];
+            tb += tri_off[m];
+            std::memcpy(dst + (size_t)k * nb + 16, &cb, 4);
+            std::memcpy(dst + (size_t)k * nb + 20, &tb, 4);
+        }
+        std::memcpy(&hs.tris2[tri_off[m]], b.tris.data(), b.tris.size() * sizeof(TriRecord));
+    }
+    hs.inst_rec.assign(16 * hs.tlas.tris.size(), 0.f);
+    for (size_t j = 0; j < hs.tlas.tris.size(); ++j) {
+        const uint32_t i = live[hs.tlas.tri_input[j]];
+        const float *M = &hs.inst_xf[12 * (size_t)i];
+        // object_from_world = inverse of [A | t] in double, rounded once
+        const double a[9] = {M[0], M[1], M[2], M[4], M[5], M[6], M[8], M[9], M[10]};
+        const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                           a[2] * (a[3] * a[7] - a[4] * a[6]);
+        if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
+            err = "instance " + std::to_string(i) + " has a singular transform";
+            return SPK_ERR_INVALID_SCENE;
+        }
+        const double inv[9] = {(a[4] * a[8] - a[5] * a[7]) / det, (a[2] * a[7] - a[1] * a[8]) / det,
+                               (a[1] * a[5] - a[2] * a[4]) / det, (a[5] * a[6] - a[3] * a[8]) / det,
+                               (a[0] * a[8] - a[2] * a[6]) / det, (a[2] * a[3] - a[0] * a[5]) / det,
+                               (a[3] * a[7] - a[4] * a[6]) / det, (a[1] * a[6] - a[0] * a[7]) / det,
+                               (a[0] * a[4] - a[1] * a[3]) / det};
+        const double t[3] = {M[3], M[7], M[11]};
+        float *r = &hs.inst_rec[16 * j];
+        for (int row = 0; row < 3; ++row) {
+            r[4 * row] = (float)inv[3 * row];
+            r[4 * row + 1] = (float)inv[3 * row + 1];
+            r[4 * row + 2] = (float)inv[3 * row + 2];
+            r[4 * row + 3] = (float)-(inv[3 * row] * t[0] + inv[3 * row + 1] * t[1] + inv[3 * row + 2] * t[2]);
+        }
+        const uint32_t tail[4] = {root[hs.inst_mesh[i]], hs.inst_id_base[i], i, 0u};
+        std::memcpy(r + 12, tail, 16);
+    }
+    hs.num_nodes2 = total_nodes;
+    hs.depth2 = hs.tlas.depth + max_depth + 2u;
+    return SPK_OK;
+}
+
+// Two-level form of a validated, flattened scene: object-space mesh copies, BLASes and the TLAS.
+static int build_two_level(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_instance_desc *instances,
+                           uint32_t num_instances, const spk_build_options *options, HostScene &hs,
+                           std::string &err) {
+    hs.two_level = true;
+    hs.mesh_copies.assign(num_meshes, MeshCopy());
+    for (uint32_t m = 0; m < num_meshes; ++m) {
+        const spk_mesh_desc &md = meshes[m];
+        MeshCopy &mc = hs.mesh_copies[m];
+        mc.num_vertices = md.num_vertices;
+        mc.num_triangles = md.num_triangles;
+        if (md.num_triangles == 0) continue;
+        mc.positions.assign(md.positions, md.positions + 3 * (size_t)md.num_vertices);
+        if (md.normals) mc.normals.assign(md.normals, md.normals + 3 * (size_t)md.num_vertices);
+        if (md.tangents) mc.tangents.assign(md.tangents, md.tangents + 4 * (size_t)md.num_vertices);
+        if (md.uvs) mc.uvs.assign(md.uvs, md.uvs + 2 * (size_t)md.num_vertices);
+        mc.indices.assign(md.indices, md.indices + 3 * (size_t)md.num_triangles);
+    }
+    hs.inst_mesh.resize(num_instances);
+    hs.inst_material.resize(num_instances);
+    hs.inst_id_base.resize(num_instances);
+    hs.inst_xf.resize(12 * (size_t)num_instances);
+    uint32_t g = 0;
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        hs.inst_mesh[i] = instances[i].mesh;
+        hs.inst_material[i] = instances[i].material;
+        hs.inst_id_base[i] = g;
+        std::memcpy(&hs.inst_xf[12 * (size_t)i], instances[i].world_from_local, 48);
+        g += meshes[instances[i].mesh].num_triangles;
+    }
+    const uint32_t width = options && options->bvh_width ? options->bvh_width : 8u;
+    const uint32_t leaf = options && options->max_leaf_tris ? options->max_leaf_tris : 3u;
+    const uint32_t threads = options ? options->num_threads : 0u;
+    hs.input.width = width;
+    hs.input.max_leaf_tris = leaf;
+    hs.input.num_threads = threads;
+    int rc = build_blases(hs, width, leaf, threads, err);
+    if (rc != SPK_OK) return rc;
+    return assemble_two_level(hs, width, threads, err);
+}
+
 static void fill_stats(const HostScene &hs, spk_scene_stats *st, uint64_t device_bytes) {
     std::memset(st, 0, sizeof(*st));
-    st->num_triangles = hs.bvh.tris.size();
-    st->num_nodes = hs.bvh.num_nodes;
+    st->num_triangles = hs.global_ids.size();
     st->bvh_width = hs.input.width;
-    st->depth = hs.bvh.depth;
-    st->node_bytes = hs.bvh.node_bytes;
     st->triangle_bytes = sizeof(TriRecord);
     st->device_bytes = device_bytes;
+    if (hs.two_level) {
+        // nodes of the TLAS and every BLAS; depth = TLAS depth + deepest BLAS; costs summed
+        st->num_nodes = hs.num_nodes2;
+        st->depth = hs.tlas.depth + (hs.depth2 - hs.tlas.depth - 2u);
+        st->node_bytes = hs.tlas.node_bytes;
+        st->sah_cost = hs.tlas.sah_cost;
+        st->build_seconds = hs.tlas.seconds;
+        for (const BuildOutput &b : hs.blas) {
+            st->degenerate_tris += b.degenerate;
+            st->build_seconds += b.seconds;
+        }
+        return;
+    }
+    st->num_nodes = hs.bvh.num_nodes;
+    st->depth = hs.bvh.depth;
+    st->node_bytes = hs.bvh.node_bytes;
     st->degenerate_tris = hs.bvh.degenerate;
     st->sah_cost = hs.bvh.sah_cost;
     st->build_seconds = hs.bvh.seconds;
 }
 
+// World triangles in global-id order with the flattened records' tags (v0.w global id, v1.w
+// material, v2.w face index): what a two-level scene's shading reads instead of leaf-order records.
+static std::vector<TriRecord> world_records(const HostScene &hs, uint32_t first, uint32_t count) {
+    std::vector<TriRecord> out(count);
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t g = first + k;
+        TriRecord &r = out[k];
+        const float *v = &hs.verts[9 * (size_t)g];
+        float *dst[3] = {r.v0, r.v1, r.v2};
+        for (int c = 0; c < 3; ++c) dst[c][0] = v[3 * c], dst[c][1] = v[3 * c + 1], dst[c][2] = v[3 * c + 2];
+        std::memcpy(&r.v0[3], &g, 4);
+        std::memcpy(&r.v1[3], &hs.shade_in[g].material, 4);
+        std::memcpy(&r.v2[3], &hs.shade_src[g].face, 4);
+    }
+    return out;
+}
+
+static int upload_two_level(spk_context *ctx) {
+    HostScene &hs = ctx->host_scene;
+    const std::vector<MaterialGpu> &mats = ctx->host_mats;
+    const uint32_t n = (uint32_t)hs.global_ids.size();
+    const std::vector<TriRecord> wt = world_records(hs, 0, n);
+    hipError_t e;
+    DeviceScene &ds = ctx->scene;
+    ds.valid = false;
+    if ((e = ds.nodes.ensure(hs.nodes2.size())) != hipSuccess ||
+        (e = ds.tris.ensure(std::max<size_t>(1, hs.tris2.size()) * sizeof(TriRecord))) != hipSuccess ||
+        (e = ds.insts.ensure(hs.inst_rec.size() * 4)) != hipSuccess ||
+        (e = ds.wtris.ensure(wt.size() * sizeof(TriRecord))) != hipSuccess ||
+        (e = ds.shade.ensure(hs.shade_in.size() * sizeof(TriShade))) != hipSuccess ||
+        (e = ds.materials.ensure(mats.size() * sizeof(MaterialGpu))) != hipSuccess)
+        return hip_fail(ctx, e, "scene allocation");
+    if ((e = hipMemcpy(ds.nodes.ptr, hs.nodes2.data(), hs.nodes2.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(ds.tris.ptr, hs.tris2.data(), hs.tris2.size() * sizeof(TriRecord), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(ds.insts.ptr, hs.inst_rec.data(), hs.inst_rec.size() * 4, hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(ds.wtris.ptr, wt.data(), wt.size() * sizeof(TriRecord), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(ds.shade.ptr, hs.shade_in.data(), hs.shade_in.size() * sizeof(TriShade),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(ds.materials.ptr, mats.data(), mats.size() * sizeof(MaterialGpu), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+        return hip_fail(ctx, e, "scene upload");
+    ds.two_level = true;
+    ds.num_tris = n;
+    ds.num_nodes = hs.num_nodes2;
+    ds.num_materials = (uint32_t)mats.size();
+    ds.width = (int)hs.input.width;
+    ds.depth = (int)hs.depth2;
+    ds.variant = TraceVariant{ds.width, pick_trace_stack((int)hs.depth2, true), 128};
+    // emissive triangles by global id for next-event estimation
+    ds.host_emitters.clear();
+    for (uint32_t g = 0; g < n; ++g)
+        if (mats[hs.shade_in[g].material].emissive_flag) ds.host_emitters.push_back(g);
+    ds.num_spheres = 0;
+    ds.valid = true;
+    fill_stats(hs, &ctx->stats,
+               ds.nodes.bytes + ds.tris.bytes + ds.insts.bytes + ds.wtris.bytes + ds.shade.bytes + ds.materials.bytes);
+    return SPK_OK;
+}
+
 // Uploads the host scene's current BVH (nodes, triangles, leaf-order shading) and materials.
 static int upload_built(spk_context *ctx, int stack) {
     HostScene &hs = ctx->host_scene;
+    if (hs.two_level) return upload_two_level(ctx);
+    ctx->scene.two_level = false;
     const std::vector<MaterialGpu> &mats = ctx->host_mats;
     const uint32_t n = (uint32_t)hs.bvh.tris.size();
     const std::vector<TriShade> shade = leaf_order_shade(hs);
@@ -244,54 +565,28 @@ int prepare_scene(const spk_mesh_desc *meshes, uint32_t num_meshes, const spk_in
         }
     }
 
-    int rc = flatten_and_build(meshes, num_meshes, instances, num_instances, options, hs, err);
+    const bool two = options && options->two_level;
+    int rc = flatten_and_build(meshes, num_meshes, instances, num_instances, options, hs, err, !two);
     if (rc != SPK_OK) return rc;
-    if (pick_trace_stack((int)hs.bvh.depth) < 0) {
-        err = "BVH depth " + std::to_string(hs.bvh.depth) + " exceeds the largest traversal stack (" +
+    if (two) {
+        rc = build_two_level(meshes, num_meshes, instances, num_instances, options, hs, err);
+        if (rc != SPK_OK) return rc;
+    }
+    const uint32_t need = two ? hs.depth2 : hs.bvh.depth;
+    if (pick_trace_stack((int)need, two) < 0) {
+        err = "BVH depth " + std::to_string(need) + " exceeds the largest traversal stack (" +
               std::to_string(max_trace_stack()) + ")";
         return SPK_ERR_STACK_DEPTH;
     }
 
-    // shading attributes in input order (permuted to leaf order at upload)
-    const uint32_t n = (uint32_t)hs.bvh.tris.size();
+    // shading attributes in input (global id) order; a flattened scene permutes them to leaf order
+    const uint32_t n = (uint32_t)hs.global_ids.size();
     hs.shade_in.assign(n, TriShade());
     for (uint32_t src = 0; src < n; ++src) {
         const uint32_t inst = hs.shade_src[src].inst, f = hs.shade_src[src].face;
         const spk_instance_desc &ins = instances[inst];
-        const spk_mesh_desc &md = meshes[ins.mesh];
-        const float *M = ins.world_from_local;
-        TriShade &s = hs.shade_in[src];
-        std::memset(&s, 0, sizeof(s));
-        float *ns[3] = {s.n0, s.n1, s.n2}, *ts[3] = {s.t0, s.t1, s.t2}, *uvs[3] = {s.uv0, s.uv1, s.uv2};
-        const float *v = &hs.verts[9 * (size_t)src];
-        // geometric normal when the mesh has none
-        const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
-        float gn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-        const float gl = std::sqrt(gn[0] * gn[0] + gn[1] * gn[1] + gn[2] * gn[2]);
-        if (gl > 0.f) gn[0] /= gl, gn[1] /= gl, gn[2] /= gl;
-        for (int c = 0; c < 3; ++c) {
-            const uint32_t vi = md.indices[3 * (size_t)f + c];
-            if (md.normals) {
-                const float *nn = md.normals + 3 * (size_t)vi;
-                for (int r = 0; r < 3; ++r) ns[c][r] = M[4 * r] * nn[0] + M[4 * r + 1] * nn[1] + M[4 * r + 2] * nn[2];
-            } else {
-                for (int r = 0; r < 3; ++r) ns[c][r] = gn[r];
-            }
-            if (md.tangents) {
-                const float *tt = md.tangents + 4 * (size_t)vi;
-                for (int r = 0; r < 3; ++r) ts[c][r] = M[4 * r] * tt[0] + M[4 * r + 1] * tt[1] + M[4 * r + 2] * tt[2];
-                ts[c][3] = tt[3];
-            } else {
-                // default tangent (1, 0, 0) through the 3x3 part
-                ts[c][0] = M[0], ts[c][1] = M[4], ts[c][2] = M[8];
-                ts[c][3] = 1.f;
-            }
-            if (md.uvs) {
-                uvs[c][0] = md.uvs[2 * (size_t)vi];
-                uvs[c][1] = md.uvs[2 * (size_t)vi + 1];
-            }
-        }
-        s.material = ins.material;
+        shade_triangle(meshes[ins.mesh], ins.world_from_local, f, &hs.verts[9 * (size_t)src], ins.material,
+                       hs.shade_in[src]);
     }
     return SPK_OK;
 }
@@ -516,6 +811,81 @@ int spk_scene_stats_get(spk_context *ctx, spk_scene_stats *out) {
     return SPK_OK;
 }
 
+int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, uint32_t num_instances) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
+    HostScene &hs = ctx->host_scene;
+    if (!hs.two_level)
+        return set_error(ctx, SPK_ERR_UNSUPPORTED, "instances move only in two-level scenes (spk_build_options)");
+    if (!world_from_local || num_instances != (uint32_t)hs.inst_mesh.size())
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "expected %zu transforms", hs.inst_mesh.size());
+    for (size_t k = 0; k < 12 * (size_t)num_instances; ++k)
+        if (!std::isfinite(world_from_local[k]))
+            return set_error(ctx, SPK_ERR_INVALID_SCENE, "instance %zu has a non-finite transform", k / 12);
+    (void)hipSetDevice(ctx->device);
+    // world geometry and shading of the moved instances, as a fresh upload would compute them
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        const float *M = world_from_local + 12 * (size_t)i;
+        const double det = (double)M[0] * ((double)M[5] * M[10] - (double)M[6] * M[9]) -
+                           (double)M[1] * ((double)M[4] * M[10] - (double)M[6] * M[8]) +
+                           (double)M[2] * ((double)M[4] * M[9] - (double)M[5] * M[8]);
+        if (!(std::fabs(det) > 0.0))
+            return set_error(ctx, SPK_ERR_INVALID_SCENE, "instance %u has a singular transform", i);
+    }
+    std::vector<uint32_t> moved;
+    const std::vector<float> old_xf = hs.inst_xf;
+    for (uint32_t i = 0; i < num_instances; ++i) {
+        const float *M = world_from_local + 12 * (size_t)i;
+        if (std::memcmp(M, &hs.inst_xf[12 * (size_t)i], 48) == 0) continue;
+        moved.push_back(i);
+        std::memcpy(&hs.inst_xf[12 * (size_t)i], M, 48);
+        const MeshCopy &mc = hs.mesh_copies[hs.inst_mesh[i]];
+        const spk_mesh_desc md = mesh_view(mc);
+        const uint32_t g0 = hs.inst_id_base[i];
+        for (uint32_t f = 0; f < mc.num_triangles; ++f) {
+            float *v = &hs.verts[9 * (size_t)(g0 + f)];
+            for (int k = 0; k < 3; ++k)
+                transform_point(M, &mc.positions[3 * (size_t)mc.indices[3 * (size_t)f + k]], v + 3 * k);
+            shade_triangle(md, M, f, v, hs.inst_material[i], hs.shade_in[g0 + f]);
+        }
+    }
+    std::string err;
+    int rc = assemble_two_level(hs, hs.input.width, hs.input.num_threads, err);
+    if (rc != SPK_OK) {
+        // keep the previous (valid) state on the device; restore the host copy of the transforms
+        hs.inst_xf = old_xf;
+        return set_error(ctx, rc, "%s", err.c_str());
+    }
+    DeviceScene &ds = ctx->scene;
+    hipError_t e;
+    if ((e = ds.nodes.ensure(hs.nodes2.size())) != hipSuccess || (e = ds.insts.ensure(hs.inst_rec.size() * 4)) != hipSuccess)
+        return hip_fail(ctx, e, "scene allocation");
+    if ((e = hipMemcpy(ds.nodes.ptr, hs.nodes2.data(), hs.nodes2.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(ds.insts.ptr, hs.inst_rec.data(), hs.inst_rec.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(ctx, e, "TLAS upload");
+    for (uint32_t i : moved) {
+        const uint32_t g0 = hs.inst_id_base[i], nt = hs.mesh_copies[hs.inst_mesh[i]].num_triangles;
+        if (nt == 0) continue;
+        const std::vector<TriRecord> wt = world_records(hs, g0, nt);
+        if ((e = hipMemcpy((TriRecord *)ds.wtris.ptr + g0, wt.data(), nt * sizeof(TriRecord),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy((TriShade *)ds.shade.ptr + g0, &hs.shade_in[g0], nt * sizeof(TriShade),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(ctx, e, "instance shading upload");
+    }
+    ds.num_nodes = hs.num_nodes2;
+    ds.depth = (int)hs.depth2;
+    if (ds.variant.stack < ds.depth) {
+        const int stack = pick_trace_stack(ds.depth, true);
+        if (stack < 0) return set_error(ctx, SPK_ERR_STACK_DEPTH, "two-level depth %d too deep", ds.depth);
+        ds.variant = TraceVariant{ds.width, stack, 128};
+    }
+    fill_stats(hs, &ctx->stats,
+               ds.nodes.bytes + ds.tris.bytes + ds.insts.bytes + ds.wtris.bytes + ds.shade.bytes + ds.materials.bytes);
+    ctx->on_scene_changed();  // frame graphs hold the node pointer; emitter areas may have changed
+    return SPK_OK;
+}
+
 int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev) {
     if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
     if (n && (!rays_dev || !hits_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null ray or hit array");
@@ -523,7 +893,7 @@ int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t 
     const DeviceScene &ds = ctx->scene;
     hipError_t e = launch_trace(ds.variant, rays_dev, n, hits_dev, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(),
                                 nullptr, nullptr, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
-                                ctx->stream);
+                                ds.two_level ? ds.insts.ptr : nullptr, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
     return SPK_OK;
 }
@@ -563,7 +933,7 @@ int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint3
     const DeviceScene &ds = ctx->scene;
     e = launch_trace(ds.variant, rays_dev, n, ctx->tmp_hits.ptr, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(),
                      nodes_dev, tris_dev, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
-                     ctx->stream);
+                     ds.two_level ? ds.insts.ptr : nullptr, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "count launch");
     return spk_synchronize(ctx);
 }
@@ -596,7 +966,7 @@ int spk_list_trace_configs(spk_context *ctx, spk_trace_config *out, uint32_t max
     if (!ctx || !count) return SPK_ERR_INVALID_ARGUMENT;
     if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
     TraceVariant vs[kMaxVariants];
-    const int nv = std::min(kMaxVariants, trace_variants(vs, kMaxVariants, ctx->scene.width));
+    const int nv = std::min(kMaxVariants, trace_variants(vs, kMaxVariants, ctx->scene.width, ctx->scene.two_level));
     uint32_t k = 0;
     for (int i = 0; i < nv; ++i) {
         if (vs[i].stack < ctx->scene.depth) continue;
@@ -617,10 +987,19 @@ int spk_set_trace_config(spk_context *ctx, const spk_trace_config *config) {
     if ((int)config->bvh_width != ctx->scene.width) {
         HostScene &hs = ctx->host_scene;
         hs.input.width = config->bvh_width;
-        std::string err = build_wide_bvh(hs.input, hs.bvh);
-        if (!err.empty()) return set_error(ctx, SPK_ERR_INVALID_SCENE, "%s", err.c_str());
-        const int stack = pick_trace_stack((int)hs.bvh.depth);
-        if (stack < 0) return set_error(ctx, SPK_ERR_STACK_DEPTH, "BVH depth %u too deep", hs.bvh.depth);
+        std::string err;
+        int stack;
+        if (hs.two_level) {
+            int rc = build_blases(hs, hs.input.width, hs.input.max_leaf_tris, hs.input.num_threads, err);
+            if (rc == SPK_OK) rc = assemble_two_level(hs, hs.input.width, hs.input.num_threads, err);
+            if (rc != SPK_OK) return set_error(ctx, rc, "%s", err.c_str());
+            stack = pick_trace_stack((int)hs.depth2, true);
+        } else {
+            err = build_wide_bvh(hs.input, hs.bvh);
+            if (!err.empty()) return set_error(ctx, SPK_ERR_INVALID_SCENE, "%s", err.c_str());
+            stack = pick_trace_stack((int)hs.bvh.depth);
+        }
+        if (stack < 0) return set_error(ctx, SPK_ERR_STACK_DEPTH, "BVH too deep for the compiled stacks");
         int rc = upload_built(ctx, stack);
         if (rc != SPK_OK) return rc;
         ctx->on_scene_changed();
@@ -629,7 +1008,7 @@ int spk_set_trace_config(spk_context *ctx, const spk_trace_config *config) {
         return set_error(ctx, SPK_ERR_STACK_DEPTH, "stack %u is smaller than the tree depth %d", config->stack,
                          ctx->scene.depth);
     TraceVariant vs[kMaxVariants];
-    const int nv = std::min(kMaxVariants, trace_variants(vs, kMaxVariants, ctx->scene.width));
+    const int nv = std::min(kMaxVariants, trace_variants(vs, kMaxVariants, ctx->scene.width, ctx->scene.two_level));
     for (int i = 0; i < nv; ++i) {
         if (vs[i].stack == (int)config->stack && vs[i].block == (int)config->block &&
             vs[i].refill == (int)config->refill) {
@@ -656,7 +1035,8 @@ static int time_variant(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, d
         if (e == hipSuccess)
             e = launch_trace(ctx->scene.variant, rays_dev, n, ctx->tmp_hits.ptr, ctx->scene.nodes.ptr,
                              ctx->scene.tris.ptr, 0u, ctx->invalid_counter(), nullptr, nullptr, ctx->work_counter(),
-                             ctx->scene.spheres.ptr, ctx->scene.num_spheres, ctx->scene.num_tris, ctx->stream);
+                             ctx->scene.spheres.ptr, ctx->scene.num_spheres, ctx->scene.num_tris,
+                             ctx->scene.two_level ? ctx->scene.insts.ptr : nullptr, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float t = 0.f;

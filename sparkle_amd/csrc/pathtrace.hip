@@ -75,7 +75,9 @@ struct FrameDev {
 
 struct PtArgs {
     const uint4 *nodes;
-    const uint4 *tris;
+    const uint4 *tris;   // traversal triangles (a two-level scene's are in object space)
+    const uint4 *wtris;  // world triangles indexed like `shade` (the same array as tris when flattened)
+    const uint4 *insts;  // two-level scenes: instance records of the TLAS leaves
     const TriShade *shade;
     const MaterialGpu *mats;
     const uint32_t *emitters;
@@ -276,21 +278,23 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
     if (tid == 0) a.counters[CTR_QUEUE] = a.npix;
 }
 
-// hit record of a finished closest-hit query: leaf triangle (or SPK_SPHERE_BIT | sphere), t and the
+// hit record of a finished closest-hit query: the triangle's shading index (leaf order in a
+// flattened scene, global id in a two-level one) or SPK_SPHERE_BIT | sphere, t and the
 // barycentrics, or a miss; spheres are tested after the BVH
+template <bool TWO>
 __device__ __forceinline__ uint4 hit_record(const PtArgs &a, const RayIn &r, BestHit &best) {
     if (a.num_spheres) spheres_closest(a.spheres, a.num_spheres, a.sphere_base, r, best);
     if (best.prim == 0xFFFFFFFFu) return make_uint4(0xFFFFFFFFu, __float_as_uint(INFINITY), 0u, 0u);
-    const float t = final_t(a.tris, r, best);
+    const float t = final_t(a.tris, r, best, TWO ? a.insts : nullptr);
     if (best.tri & SPK_SPHERE_BIT) return make_uint4(best.tri, __float_as_uint(t), 0u, 0u);
-    return make_uint4(best.tri, __float_as_uint(t), __float_as_uint(best.V / best.det),
+    return make_uint4(TWO ? best.prim : best.tri, __float_as_uint(t), __float_as_uint(best.V / best.det),
                       __float_as_uint(best.W / best.det));
 }
 
 // ---- extend: closest hit for every queued path + material histogram ----
 // Lanes trace one path each and a wave takes new queue slots (one atomic) once `refill` of its
 // lanes are idle, so a wave does not wait for its slowest ray before fetching more work.
-template <int WIDTH, int STACK>
+template <int WIDTH, int STACK, bool TWO>
 __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) {
     __shared__ uint2 stack[STACK * kPtBlock];
     __shared__ uint32_t lhist[kMaxSortKeys];
@@ -302,7 +306,9 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
     __syncthreads();
     const uint32_t lane = lane_id();
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
-    Traversal<WIDTH, STACK, false> tr;
+    // camera rays of neighbouring pixels are coherent: their waves refill whole, so lanes stay in step
+    const uint32_t refill = bounce == 0u ? 64u : a.refill;
+    Traversal<WIDTH, STACK, false, false, TWO> tr;
     uint32_t slot = 0u, path = 0u;
     bool active = false;
     SlotRun run;
@@ -310,7 +316,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
         if (!run.drained()) {
             const uint64_t idle = __ballot(!active);
             const uint32_t nidle = (uint32_t)__popcll(idle);
-            if (nidle >= a.refill) {
+            if (nidle >= refill) {
                 const uint32_t my = run.take(ctr + CTR_EXTEND, count, nidle, (uint32_t)__popcll(idle & below));
                 if (!active) {
                     if (my < count) {
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
                         const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
                         const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
                         const uint32_t root = a.num_scenes > 1u ? a.scenes[a.path_scene[path]].root : 0u;
-                        tr.init(r, root);
+                        tr.init(r, root, a.insts);
                         slot = my;
                         active = true;
                     }
@@ -333,7 +339,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
             uint32_t key = a.nkeys - 1u;  // misses sort last
             const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
             const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
-            const uint4 h = hit_record(a, r, tr.best);
+            const uint4 h = hit_record<TWO>(a, r, tr.best);
             if (h.x != 0xFFFFFFFFu) {
                 key = hit_material(a, h.x);
                 if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
@@ -559,7 +565,7 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
             if ((pf & PF_NEE_PREV) && !is_sphere) {  // spheres are not among the sampled emitters
                 // the previous vertex also sampled this emitter: balance-heuristic weight of
                 // the BSDF-sampled direction against light sampling of this point
-                const uint4 *tp = a.tris + (size_t)h.x * 3u;
+                const uint4 *tp = a.wtris + (size_t)h.x * 3u;
                 const uint4 va = tp[0], vb = tp[1], vc = tp[2];
                 const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
                 const F3 ng = cross(f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z)) - p0,
@@ -622,7 +628,7 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
                 const float w_tri = a.emitter_cdf[lo] - w_prev;
                 const float r1 = rng_unit(nr), r2 = rng_unit(nr);
                 a.nee_rng[path] = make_uint4(nr.s0, nr.s1, nr.s2, nr.s3);
-                const uint4 *tp = a.tris + (size_t)et * 3u;
+                const uint4 *tp = a.wtris + (size_t)et * 3u;
                 const uint4 va = tp[0], vb = tp[1], vc = tp[2];
                 const F3 p0 = f3(__uint_as_float(va.x), __uint_as_float(va.y), __uint_as_float(va.z));
                 const F3 p1 = f3(__uint_as_float(vb.x), __uint_as_float(vb.y), __uint_as_float(vb.z));
@@ -741,7 +747,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
 }
 
 // occlusion query of a NEE segment (s0 = origin, tmax; s1 = direction): true when nothing blocks it
-template <int WIDTH, int STACK>
+template <int WIDTH, int STACK, bool TWO>
 __device__ __forceinline__ bool shadow_unoccluded(const PtArgs &a, const float4 s0, const float4 s1, uint2 *stack) {
     RayIn r{s0.x, s0.y, s0.z, SPK_EPS, s1.x, s1.y, s1.z, s0.w};
     BestHit best;
@@ -750,8 +756,9 @@ __device__ __forceinline__ bool shadow_unoccluded(const PtArgs &a, const float4 
     best.tri = 0u;
     best.U = best.V = best.W = 0.f;
     best.det = 1.f;
+    best.inst = 0u;
     TraverseCounters c{0u, 0u};
-    traverse<WIDTH, STACK, false, true>(a.nodes, a.tris, 0u, r, stack, kPtBlock, best, c);
+    traverse<WIDTH, STACK, false, true, TWO>(a.nodes, a.tris, 0u, r, stack, kPtBlock, best, c, a.insts);
     if (best.prim == 0xFFFFFFFFu && a.num_spheres) spheres_closest<true>(a.spheres, a.num_spheres, a.sphere_base, r, best);
     return best.prim == 0xFFFFFFFFu;
 }
@@ -765,7 +772,7 @@ __device__ __forceinline__ void add_color(const PtArgs &a, uint32_t path, const 
 }
 
 // ---- shadow: occlusion of the NEE segments; unoccluded contributions added to the path colour ----
-template <int WIDTH, int STACK>
+template <int WIDTH, int STACK, bool TWO>
 __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) {
     __shared__ uint2 stack[STACK * kPtBlock];
     uint32_t *ctr = a.counters + bounce * kCtrStride;
@@ -777,7 +784,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) 
             const uint32_t i = base + lane_id();
             if (i >= count) continue;
             const float4 s0 = a.shadow[3 * (size_t)i], s1 = a.shadow[3 * (size_t)i + 1];
-            if (shadow_unoccluded<WIDTH, STACK>(a, s0, s1, stack + threadIdx.x))
+            if (shadow_unoccluded<WIDTH, STACK, TWO>(a, s0, s1, stack + threadIdx.x))
                 add_color(a, __float_as_uint(s1.w), a.shadow[3 * (size_t)i + 2]);
         }
     }
@@ -789,7 +796,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) 
 // takes the next queued path when it ends, so the frame waits for the slowest path instead. Every
 // segment runs the wavefront kernels' per-path code in the same order (closest hit, shading with
 // the colour written back, then the shadow contribution), so images do not depend on `from`.
-template <int WIDTH, int STACK>
+template <int WIDTH, int STACK, bool TWO>
 __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
     __shared__ uint2 stack[STACK * kPtBlock];
     uint32_t *ctr = a.counters + from * kCtrStride;
@@ -824,17 +831,17 @@ __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
             const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
             const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
             const uint32_t root = a.num_scenes > 1u ? a.scenes[a.path_scene[path]].root : 0u;
-            Traversal<WIDTH, STACK, false> tr;
-            tr.init(r, root);
+            Traversal<WIDTH, STACK, false, false, TWO> tr;
+            tr.init(r, root, a.insts);
             while (tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
             }
-            const uint4 h = hit_record(a, r, tr.best);
+            const uint4 h = hit_record<TWO>(a, r, tr.best);
             ShadeOut o;
             shade_path(a, bounce, bounce + 1u >= a.max_bounce, nee, path, h, o);
             ++rays;
             if (o.shadow) {
                 ++shadows;
-                if (shadow_unoccluded<WIDTH, STACK>(a, o.sh0, o.sh1, stack + threadIdx.x)) add_color(a, path, o.sh2);
+                if (shadow_unoccluded<WIDTH, STACK, TWO>(a, o.sh0, o.sh1, stack + threadIdx.x)) add_color(a, path, o.sh2);
             }
             if (a.cache_on) append_cache_records(a, path, o);
             if (o.cont) ++bounce; else active = false;
@@ -975,12 +982,17 @@ __global__ void k_spatial_fill(const float4 *__restrict__ rgba, const float *__r
 using ExtendFn = void (*)(PtArgs, uint32_t);
 struct PtEntry {
     int width, stack;
+    bool two;
     ExtendFn extend, shadow, tail;
 };
-#define SPK_PT(W, S) {W, S, k_extend<W, S>, k_shadow<W, S>, k_tail<W, S>}
+#define SPK_PT(W, S, T) {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_tail<W, S, T>}
 const PtEntry kPtEntries[] = {
-    SPK_PT(8, 12), SPK_PT(4, 12), SPK_PT(8, 16), SPK_PT(8, 24), SPK_PT(8, 32),
-    SPK_PT(8, 64), SPK_PT(4, 16), SPK_PT(4, 24), SPK_PT(4, 32), SPK_PT(4, 64),
+    SPK_PT(8, 12, false), SPK_PT(4, 12, false), SPK_PT(8, 16, false), SPK_PT(8, 24, false),
+    SPK_PT(8, 32, false), SPK_PT(8, 64, false), SPK_PT(4, 16, false), SPK_PT(4, 24, false),
+    SPK_PT(4, 32, false), SPK_PT(4, 64, false),
+    // two-level scenes
+    SPK_PT(8, 24, true),  SPK_PT(8, 32, true),  SPK_PT(8, 64, true),  SPK_PT(4, 24, true),
+    SPK_PT(4, 32, true),  SPK_PT(4, 64, true),
 };
 #undef SPK_PT
 
@@ -1060,15 +1072,25 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
         const std::vector<uint32_t> &em = ctx->scene.host_emitters;
         std::vector<float> cdf(em.size());
         double acc = 0.0;
+        const HostScene &hs = ctx->host_scene;
         for (size_t k = 0; k < em.size(); ++k) {
-            const TriRecord &r = ctx->host_scene.bvh.tris[em[k]];
-            const double e1[3] = {(double)r.v1[0] - r.v0[0], (double)r.v1[1] - r.v0[1], (double)r.v1[2] - r.v0[2]};
-            const double e2[3] = {(double)r.v2[0] - r.v0[0], (double)r.v2[1] - r.v0[1], (double)r.v2[2] - r.v0[2]};
+            // world vertices and material: leaf-order records, or global-id order in a two-level scene
+            float v0[3], v1[3], v2[3];
+            uint32_t mat;
+            if (hs.two_level) {
+                const float *v = &hs.verts[9 * (size_t)em[k]];
+                for (int c = 0; c < 3; ++c) v0[c] = v[c], v1[c] = v[3 + c], v2[c] = v[6 + c];
+                mat = hs.shade_in[em[k]].material;
+            } else {
+                const TriRecord &r = hs.bvh.tris[em[k]];
+                for (int c = 0; c < 3; ++c) v0[c] = r.v0[c], v1[c] = r.v1[c], v2[c] = r.v2[c];
+                std::memcpy(&mat, &r.v1[3], 4);
+            }
+            const double e1[3] = {(double)v1[0] - v0[0], (double)v1[1] - v0[1], (double)v1[2] - v0[2]};
+            const double e2[3] = {(double)v2[0] - v0[0], (double)v2[1] - v0[1], (double)v2[2] - v0[2]};
             const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
                          cz = e1[0] * e2[1] - e1[1] * e2[0];
             const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-            uint32_t mat;
-            std::memcpy(&mat, &r.v1[3], 4);
             const MaterialGpu &m = ctx->host_mats[mat];
             const double lum = (m.emissive[0] + m.emissive[1] + m.emissive[2]) / 3.0;
             acc += area * lum;
@@ -1226,13 +1248,16 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     const DeviceScene &ds = *src.ds;
     const PtEntry *pe = nullptr;
     for (const PtEntry &e : kPtEntries)
-        if (e.width == ds.width && e.stack == pick_trace_stack(ds.depth)) pe = &e;
+        if (e.width == ds.width && e.two == ds.two_level && e.stack == pick_trace_stack(ds.depth, ds.two_level))
+            pe = &e;
     if (!pe) return set_error(ctx, SPK_ERR_STACK_DEPTH, "no path-tracer kernel for depth %d", ds.depth);
 
     PtArgs a;
     std::memset(&a, 0, sizeof(a));
     a.nodes = (const uint4 *)ds.nodes.ptr;
     a.tris = (const uint4 *)ds.tris.ptr;
+    a.wtris = ds.two_level ? (const uint4 *)ds.wtris.ptr : a.tris;
+    a.insts = ds.two_level ? (const uint4 *)ds.insts.ptr : nullptr;
     a.shade = (const TriShade *)ds.shade.ptr;
     a.mats = (const MaterialGpu *)ds.materials.ptr;
     a.tex_table = (const TexRecord *)ctx->tex_table.ptr;
@@ -1426,6 +1451,8 @@ int spk_batch_upload(spk_context *ctx, const spk_batch_scene *scenes, uint32_t n
                      const spk_build_options *options) {
     if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
     if (!scenes || num_scenes == 0) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "batch needs at least one scene");
+    if (options && options->two_level)
+        return set_error(ctx, SPK_ERR_UNSUPPORTED, "batches are flattened (two_level is for spk_scene_upload)");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete ctx->batch;
@@ -1610,7 +1637,7 @@ static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *
         }
     }
     // then the first segment of the per-path tail kernel, on the best grid / sort / refill
-    static const uint32_t kTail[] = {2u, 3u, 4u, 5u};
+    static const uint32_t kTail[] = {1u, 2u, 3u, 4u, 5u};
     const spk_schedule base = best_s;
     for (uint32_t t : kTail) {
         if (t >= frame->max_bounce) continue;

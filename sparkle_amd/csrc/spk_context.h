@@ -32,6 +32,13 @@ struct ShadeSource {
 };
 
 // Host copy of the flattened scene and its BVH (kept for statistics and re-validation).
+// Object-space copy of one mesh (two-level scenes re-flatten their shading data when instances move).
+struct MeshCopy {
+    std::vector<float> positions, normals, tangents, uvs;
+    std::vector<uint32_t> indices;
+    uint32_t num_vertices = 0, num_triangles = 0;
+};
+
 struct HostScene {
     std::vector<float> verts;
     std::vector<uint32_t> global_ids, materials, local_ids;
@@ -39,10 +46,24 @@ struct HostScene {
     std::vector<TriShade> shade_in;  // per input triangle
     BuildInput input;
     BuildOutput bvh;
+    // two-level scenes (spk_build_options.two_level): one BLAS per mesh in object space under a TLAS
+    // whose leaf slots are instances; shading data stays per global triangle id in world space
+    bool two_level = false;
+    std::vector<MeshCopy> mesh_copies;
+    std::vector<BuildOutput> blas;          // per mesh (empty for meshes no instance uses)
+    BuildOutput tlas;                       // over the instances' world boxes, one per leaf slot
+    std::vector<uint32_t> inst_mesh, inst_material, inst_id_base;
+    std::vector<float> inst_xf;             // world_from_local, 12 floats per instance
+    std::vector<uint8_t> nodes2;            // TLAS nodes, then every BLAS (child and triangle bases rebased)
+    std::vector<TriRecord> tris2;           // object-space BLAS triangles, v0.w = face index in the mesh
+    std::vector<float> inst_rec;            // 16 floats per TLAS leaf slot (see traverse.h)
+    uint32_t num_nodes2 = 0, depth2 = 0;    // depth2: stack entries a two-level query may need
 };
 
 struct DeviceScene {
     DeviceBuffer nodes, tris, shade, materials, emitters;
+    DeviceBuffer insts, wtris;  // two-level: instance records; world triangles in global-id order
+    bool two_level = false;
     DeviceBuffer spheres;  // 2 x float4 per sphere: (center, r^2), (material bits, 0, 0, 0)
     uint32_t num_tris = 0, num_nodes = 0, num_materials = 0, num_emitters = 0, num_spheres = 0;
     int width = 8, depth = 0;

@@ -12,13 +12,14 @@ struct SphereSet {
     uint32_t count, id_base;
 };
 
-template <int WIDTH, int STACK, int BLOCK, bool COUNT>
+// TWO: a two-level scene (TLAS over the instance records `insts`, one BLAS per mesh).
+template <int WIDTH, int STACK, int BLOCK, bool COUNT, bool TWO>
 __global__ __launch_bounds__(BLOCK) void k_trace(const float4 *__restrict__ rays, uint64_t n,
                                                  uint2 *__restrict__ hits, const uint4 *__restrict__ nodes,
                                                  const uint4 *__restrict__ tris, uint32_t root,
                                                  uint32_t *__restrict__ invalid_count,
                                                  uint32_t *__restrict__ cnt_nodes, uint32_t *__restrict__ cnt_tris,
-                                                 SphereSet sph) {
+                                                 SphereSet sph, const uint4 *__restrict__ insts) {
     __shared__ uint2 stack[STACK * BLOCK];
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -35,9 +36,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const float4 *__restrict__ rays
         best.tri = 0u;
         best.U = best.V = best.W = 0.f;
         best.det = 1.f;
-        traverse<WIDTH, STACK, COUNT>(nodes, tris, root, r, stack + threadIdx.x, BLOCK, best, c);
+        best.inst = 0u;
+        traverse<WIDTH, STACK, COUNT, false, TWO>(nodes, tris, root, r, stack + threadIdx.x, BLOCK, best, c, insts);
         if (sph.count) spheres_closest(sph.spheres, sph.count, sph.id_base, r, best);
-        if (best.prim != 0xFFFFFFFFu) out = make_uint2(best.prim, __float_as_uint(final_t(tris, r, best)));
+        if (best.prim != 0xFFFFFFFFu)
+            out = make_uint2(best.prim, __float_as_uint(final_t(tris, r, best, TWO ? insts : nullptr)));
     }
     hits[i] = out;
     if (COUNT) {
@@ -123,20 +126,26 @@ __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 namespace {
 using TraceFn = void (*)(const float4 *, uint64_t, uint2 *, const uint4 *, const uint4 *, uint32_t, uint32_t *,
-                         uint32_t *, uint32_t *, SphereSet);
+                         uint32_t *, uint32_t *, SphereSet, const uint4 *);
 using PersistentFn = void (*)(const float4 *, uint64_t, uint2 *, const uint4 *, const uint4 *, uint32_t, uint32_t *,
                               uint32_t *, uint32_t *, unsigned long long *, uint32_t, SphereSet);
 struct Entry {
     int width, stack, block;
     bool count;
     TraceFn fn;
-    PersistentFn pfn;
+    PersistentFn pfn;  // null: no persistent variant (two-level scenes)
+    bool two;
 };
 
 #define SPK_TV(W, S, B)                                                                                   \
-    {W, S, B, false, k_trace<W, S, B, false>, k_trace_persistent<W, S, B, false>},                        \
+    {W, S, B, false, k_trace<W, S, B, false, false>, k_trace_persistent<W, S, B, false>, false},          \
     {                                                                                                     \
-        W, S, B, true, k_trace<W, S, B, true>, k_trace_persistent<W, S, B, true>                          \
+        W, S, B, true, k_trace<W, S, B, true, false>, k_trace_persistent<W, S, B, true>, false            \
+    }
+#define SPK_TV2(W, S, B)                                                                                  \
+    {W, S, B, false, k_trace<W, S, B, false, true>, nullptr, true},                                       \
+    {                                                                                                     \
+        W, S, B, true, k_trace<W, S, B, true, true>, nullptr, true                                        \
     }
 
 // LDS per block = stack * block * 8 bytes, kept at or below 64 KiB
@@ -146,8 +155,12 @@ const Entry kEntries[] = {
     SPK_TV(8, 64, 128), SPK_TV(4, 16, 64),  SPK_TV(4, 16, 128), SPK_TV(4, 16, 256), SPK_TV(4, 24, 64),
     SPK_TV(4, 24, 128), SPK_TV(4, 24, 256), SPK_TV(4, 32, 64),  SPK_TV(4, 32, 128), SPK_TV(4, 32, 256),
     SPK_TV(4, 64, 64),  SPK_TV(4, 64, 128),
+    // two-level scenes: the stack also holds a TLAS level's group and pending instances
+    SPK_TV2(8, 24, 64), SPK_TV2(8, 24, 128), SPK_TV2(8, 32, 64), SPK_TV2(8, 32, 128), SPK_TV2(8, 64, 64),
+    SPK_TV2(4, 24, 64), SPK_TV2(4, 24, 128), SPK_TV2(4, 32, 64), SPK_TV2(4, 32, 128), SPK_TV2(4, 64, 64),
 };
 #undef SPK_TV
+#undef SPK_TV2
 constexpr int kNumEntries = (int)(sizeof(kEntries) / sizeof(kEntries[0]));
 int g_resident_blocks[kNumEntries];  // resident blocks per CU of each persistent kernel, filled on first use
 
@@ -158,16 +171,18 @@ const int kRefill[] = {1, 16, 32};
 hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, void *hits, const void *nodes,
                         const void *tris, uint32_t root, uint32_t *invalid_count, uint32_t *cnt_nodes,
                         uint32_t *cnt_tris, unsigned long long *work, const void *spheres, uint32_t num_spheres,
-                        uint32_t sphere_id_base, hipStream_t stream) {
+                        uint32_t sphere_id_base, const void *insts, hipStream_t stream) {
     const SphereSet sph{(const float4 *)spheres, spheres ? num_spheres : 0u, sphere_id_base};
     const bool count = cnt_nodes != nullptr || cnt_tris != nullptr;
+    const bool two = insts != nullptr;
     for (int k = 0; k < kNumEntries; ++k) {
         const Entry &e = kEntries[k];
-        if (e.width != v.width || e.stack != v.stack || e.block != v.block || e.count != count) continue;
+        if (e.width != v.width || e.stack != v.stack || e.block != v.block || e.count != count || e.two != two)
+            continue;
         if (n == 0) return hipSuccess;
         const uint64_t blocks = (n + (uint64_t)v.block - 1) / (uint64_t)v.block;
         if (v.refill > 0) {
-            if (!work) return hipErrorInvalidValue;
+            if (!work || !e.pfn) return hipErrorInvalidValue;
             if (g_resident_blocks[k] == 0) {
                 int nb = 0;
                 hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, e.pfn, v.block, 0);
@@ -189,7 +204,7 @@ hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, voi
         if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
         hipLaunchKernelGGL(e.fn, dim3((uint32_t)blocks), dim3(v.block), 0, stream, (const float4 *)rays, n,
                            (uint2 *)hits, (const uint4 *)nodes, (const uint4 *)tris, root, invalid_count, cnt_nodes,
-                           cnt_tris, sph);
+                           cnt_tris, sph, (const uint4 *)insts);
         return hipGetLastError();
     }
     return hipErrorInvalidValue;
@@ -197,19 +212,20 @@ hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, voi
 
 int max_trace_stack() { return 64; }
 
-int pick_trace_stack(int depth) {
+int pick_trace_stack(int depth, bool two) {
     const int sizes[] = {12, 16, 24, 32, 64};
     for (int s : sizes)
-        if (s >= depth) return s;
+        if (s >= depth && (!two || s >= 24)) return s;
     return -1;
 }
 
-int trace_variants(TraceVariant *out, int max_out, int width) {
+int trace_variants(TraceVariant *out, int max_out, int width, bool two) {
     int k = 0;
     for (const Entry &e : kEntries) {
-        if (e.count || e.width != width) continue;
+        if (e.count || e.width != width || e.two != two) continue;
         if (k < max_out) out[k] = TraceVariant{e.width, e.stack, e.block, 0};
         ++k;
+        if (!e.pfn) continue;
         for (int r : kRefill) {
             if (k < max_out) out[k] = TraceVariant{e.width, e.stack, e.block, r};
             ++k;

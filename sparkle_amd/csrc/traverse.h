@@ -232,7 +232,32 @@ struct BestHit {
     uint32_t prim;     // global triangle id, 0xFFFFFFFF when none
     uint32_t tri;      // leaf-order index of the best triangle
     float U, V, W, det;
+    uint32_t inst;     // two-level scenes: instance record of the best hit
 };
+
+// ---- two-level scenes: a TLAS over instances, one BLAS per mesh in object space ----
+// Instance record (64 bytes, TLAS leaf order): rows 0-2 = object_from_world (3x4, float bits),
+// row 3 = (BLAS root node, first global triangle id of the instance, instance index, 0).
+// A TLAS leaf slot holds one instance; the ray enters its BLAS transformed to object space with an
+// unnormalised direction, so the ray parameter t, box culling and the tie rule carry over.
+#define SPK_INST_ENTRY 0x80000000u  // stack entry flag: pending instances {leaf base, bits}
+
+__device__ __forceinline__ void object_ray(const uint4 r0, const uint4 r1, const uint4 r2, float wox, float woy,
+                                           float woz, float wdx, float wdy, float wdz, float &ox, float &oy,
+                                           float &oz, float &dx, float &dy, float &dz) {
+    const float a00 = __uint_as_float(r0.x), a01 = __uint_as_float(r0.y), a02 = __uint_as_float(r0.z),
+                a03 = __uint_as_float(r0.w);
+    const float a10 = __uint_as_float(r1.x), a11 = __uint_as_float(r1.y), a12 = __uint_as_float(r1.z),
+                a13 = __uint_as_float(r1.w);
+    const float a20 = __uint_as_float(r2.x), a21 = __uint_as_float(r2.y), a22 = __uint_as_float(r2.z),
+                a23 = __uint_as_float(r2.w);
+    ox = ((a00 * wox + a01 * woy) + a02 * woz) + a03;
+    oy = ((a10 * wox + a11 * woy) + a12 * woz) + a13;
+    oz = ((a20 * wox + a21 * woy) + a22 * woz) + a23;
+    dx = (a00 * wdx + a01 * wdy) + a02 * wdz;
+    dy = (a10 * wdx + a11 * wdy) + a12 * wdz;
+    dz = (a20 * wdx + a21 * wdy) + a22 * wdz;
+}
 
 struct TraverseCounters {
     uint32_t nodes;
@@ -255,7 +280,11 @@ __device__ __forceinline__ float safe_inv(float d) {
 // step() pops a group when the current one is exhausted, visits one node (all child boxes) and
 // tests that node's leaf triangles; traverse() runs steps until the ray is done, and the
 // persistent closest-hit kernel interleaves steps with refilling finished lanes.
-template <int WIDTH, int STACK, bool COUNT, bool ANYHIT = false>
+// TWO: the tree rooted at `root` is a TLAS whose leaf slots are instances (records in `insts`).
+// Reaching TLAS leaves pushes the node's remaining group and one entry for its other instances,
+// then enters the first instance's BLAS with the object-space ray; when the stack is back at its
+// depth on entry the BLAS is done and the world ray is restored.
+template <int WIDTH, int STACK, bool COUNT, bool ANYHIT = false, bool TWO = false>
 struct Traversal {
     float ox, oy, oz, ix, iy, iz, box_tmin;
     WtSetup wt;
@@ -265,13 +294,51 @@ struct Traversal {
     int sp;
     BestHit best;
     TraverseCounters cnt;
+    // two-level state: the world ray, the level (0 TLAS, 1 BLAS), the stack depth on entering the
+    // BLAS, the instance's first global triangle id and its record
+    float wox, woy, woz, wdx, wdy, wdz;
+    uint32_t level, blas_sp, id_base, cur;
+    const uint4 *insts;
 
-    __device__ __forceinline__ void init(const RayIn &ray, uint32_t root) {
-        ox = ray.ox, oy = ray.oy, oz = ray.oz;
-        wt = wt_setup(ray.dx, ray.dy, ray.dz);
-        ix = safe_inv(ray.dx), iy = safe_inv(ray.dy), iz = safe_inv(ray.dz);
-        oct = (ray.dx < 0.f ? 1u : 0u) | (ray.dy < 0.f ? 2u : 0u) | (ray.dz < 0.f ? 4u : 0u);
+    __device__ __forceinline__ void set_ray(float px, float py, float pz, float dx, float dy, float dz) {
+        ox = px, oy = py, oz = pz;
+        wt = wt_setup(dx, dy, dz);
+        ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+        oct = (dx < 0.f ? 1u : 0u) | (dy < 0.f ? 2u : 0u) | (dz < 0.f ? 4u : 0u);
+    }
+
+    __device__ __forceinline__ void enter(uint32_t slot) {
+        const uint4 *ip = insts + (size_t)slot * 4u;
+        const uint4 r0 = ip[0], r1 = ip[1], r2 = ip[2], r3 = ip[3];
+        float px, py, pz, dx, dy, dz;
+        object_ray(r0, r1, r2, wox, woy, woz, wdx, wdy, wdz, px, py, pz, dx, dy, dz);
+        set_ray(px, py, pz, dx, dy, dz);
+        level = 1u;
+        blas_sp = (uint32_t)sp;
+        id_base = r3.y;
+        cur = slot;
+        g_base = r3.x;
+        g_imask = 1u;
+        g_hits = WIDTH == 8 ? 1u << oct : 1u;
+        g_oct = WIDTH == 8 ? oct : 0u;
+    }
+
+    __device__ __forceinline__ void leave() {
+        set_ray(wox, woy, woz, wdx, wdy, wdz);
+        level = 0u;
+    }
+
+    __device__ __forceinline__ void init(const RayIn &ray, uint32_t root, const uint4 *inst_records = nullptr) {
+        set_ray(ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
         box_tmin = fmaxf(ray.tmin, 0.f);
+        if (TWO) {
+            wox = ray.ox, woy = ray.oy, woz = ray.oz, wdx = ray.dx, wdy = ray.dy, wdz = ray.dz;
+            level = 0u;
+            blas_sp = 0u;
+            id_base = 0u;
+            cur = 0u;
+            insts = inst_records;
+        }
         g_base = root;
         g_imask = 1u;
         g_hits = WIDTH == 8 ? 1u << oct : 1u;  // 4-wide: rank order is resolved per node below
@@ -282,6 +349,7 @@ struct Traversal {
         best.tri = 0xFFFFFFFFu;
         best.U = best.V = best.W = 0.f;
         best.det = 1.f;
+        best.inst = 0u;
         cnt.nodes = 0u;
         cnt.tris = 0u;
     }
@@ -290,13 +358,26 @@ struct Traversal {
     __device__ __forceinline__ bool step(const uint4 *__restrict__ nodes, const uint4 *__restrict__ tris,
                                          uint2 *stack, int stride) {
         if (g_hits == 0u) {
+            if (TWO && level != 0u && sp == (int)blas_sp) leave();  // the instance's BLAS is done
             if (sp == 0) return false;
             --sp;
             const uint2 e = stack[sp * stride];
-            g_base = e.x;
-            g_imask = (e.y >> 8) & 0xffu;
-            g_hits = (e.y >> 16) & 0xffu;  // pushed entries always hold a pending hit
-            g_oct = e.y & 0xffu;
+            if (TWO && (e.y & SPK_INST_ENTRY)) {
+                // pending instances of a TLAS node: enter the next one, keep the rest
+                uint32_t bits = e.y & 0xFFFFFFu;
+                const uint32_t k = __builtin_ctz(bits);
+                bits &= bits - 1u;
+                if (bits != 0u) {
+                    stack[sp * stride] = make_uint2(e.x, SPK_INST_ENTRY | bits);
+                    ++sp;
+                }
+                enter(e.x + k);
+            } else {
+                g_base = e.x;
+                g_imask = (e.y >> 8) & 0xffu;
+                g_hits = (e.y >> 16) & 0xffu;  // pushed entries always hold a pending hit
+                g_oct = e.y & 0xffu;
+            }
         }
         const uint32_t r = __builtin_ctz(g_hits);
         g_hits &= g_hits - 1u;
@@ -393,6 +474,26 @@ struct Traversal {
         g_hits = new_hits;
         g_oct = node_oct;
 
+        if (TWO && level == 0u) {
+            // TLAS leaves are instances: keep this node's inner children and its other instances on
+            // the stack and enter the first instance
+            if (tri_bits != 0u) {
+                if (g_hits != 0u) {
+                    stack[sp * stride] = make_uint2(g_base, (g_imask << 8) | (g_hits << 16) | g_oct);
+                    ++sp;
+                    g_hits = 0u;
+                }
+                const uint32_t k = __builtin_ctz(tri_bits);
+                const uint32_t rest = tri_bits & (tri_bits - 1u);
+                if (rest != 0u) {
+                    stack[sp * stride] = make_uint2(tri_base, SPK_INST_ENTRY | rest);
+                    ++sp;
+                }
+                enter(tri_base + k);
+            }
+            return true;
+        }
+
         // leaf triangles of this node
         while (tri_bits != 0u) {
             const uint32_t k = __builtin_ctz(tri_bits);
@@ -406,11 +507,12 @@ struct Traversal {
             const float4 fc = make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), 0.f);
             const TriTest tt = tri_wt(wt, ox, oy, oz, fa, fb, fc);
             if (tt.hit && tt.t > box_tmin) {
-                const uint32_t gid = a.w;
+                const uint32_t gid = TWO ? id_base + a.w : a.w;
                 if (tt.t < best.t || (tt.t == best.t && gid < best.prim)) {
                     best.t = tt.t;
                     best.prim = gid;
                     best.tri = ti;
+                    if (TWO) best.inst = cur;
                     best.U = tt.U;
                     best.V = tt.V;
                     best.W = tt.W;
@@ -428,12 +530,12 @@ struct Traversal {
 };
 
 // One closest-hit (or, with ANYHIT, occlusion) query. best.t must hold tmax on entry.
-template <int WIDTH, int STACK, bool COUNT, bool ANYHIT = false>
+template <int WIDTH, int STACK, bool COUNT, bool ANYHIT = false, bool TWO = false>
 __device__ __forceinline__ void traverse(const uint4 *__restrict__ nodes, const uint4 *__restrict__ tris,
                                          uint32_t root, const RayIn &ray, uint2 *stack, int stride, BestHit &best,
-                                         TraverseCounters &cnt) {
-    Traversal<WIDTH, STACK, COUNT, ANYHIT> tr;
-    tr.init(ray, root);
+                                         TraverseCounters &cnt, const uint4 *insts = nullptr) {
+    Traversal<WIDTH, STACK, COUNT, ANYHIT, TWO> tr;
+    tr.init(ray, root, insts);
     tr.best.t = best.t;
     while (tr.step(nodes, tris, stack, stride)) {
     }
@@ -474,14 +576,22 @@ __device__ __forceinline__ void spheres_closest(const float4 *__restrict__ spher
     }
 }
 
-// refined distance of the final hit (bit-identical to the oracle's)
-__device__ __forceinline__ float final_t(const uint4 *__restrict__ tris, const RayIn &ray, const BestHit &best) {
+// refined distance of the final hit (bit-identical to the oracle's); in a two-level scene on the
+// object-space triangle with the instance's object-space ray (the same t parameter)
+__device__ __forceinline__ float final_t(const uint4 *__restrict__ tris, const RayIn &ray, const BestHit &best,
+                                         const uint4 *insts = nullptr) {
     if (best.tri & SPK_SPHERE_BIT) return best.t;  // closed form already
     const uint4 *tp = tris + (size_t)best.tri * 3u;
     const uint4 a = tp[0], b = tp[1], c = tp[2];
     const float4 fa = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), 0.f);
     const float4 fb = make_float4(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), 0.f);
     const float4 fc = make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), 0.f);
+    if (insts) {
+        const uint4 *ip = insts + (size_t)best.inst * 4u;
+        float px, py, pz, dx, dy, dz;
+        object_ray(ip[0], ip[1], ip[2], ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, px, py, pz, dx, dy, dz);
+        return refine_t(px, py, pz, dx, dy, dz, fa, fb, fc, best.t);
+    }
     return refine_t(ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, fa, fb, fc, best.t);
 }
 

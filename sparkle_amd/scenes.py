@@ -368,12 +368,16 @@ def displaced_box(n, amp, rng):
             np.concatenate(tan_all), np.concatenate(uv_all))
 
 
-def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance=25.0, aspect=1920 / 1080):
+def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance=25.0, aspect=1920 / 1080,
+                        shared=False):
     """`count` seed-placed instances flattened to world space (about `target_tris` triangles):
     icospheres (subdivision 2-5, capped by the per-instance budget), tori (R/r = 3) and noise-displaced
     boxes on a 20 x 20 ground plane; materials 60% Lambertian, 30% GGX metal (roughness U[0.05, 0.6]),
     10% dielectric (IOR 1.5); three emissive area lights. Returns (scene, camera with the given
-    aspect ratio, 1920x1080 by default)."""
+    aspect ratio, 1920x1080 by default).
+    shared: the same placement as instances instead: one object-space mesh per icosphere subdivision
+    and one torus, shared by every instance of that kind, and each displaced box as its own mesh,
+    all placed by world_from_local transforms (scale, rotation, translation)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sc = Scene()
     ground = sc.add_material(base_color=(0.5, 0.5, 0.5), metallic=0.0, roughness=1.0)
@@ -385,31 +389,51 @@ def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance
         max_sub += 1
     side = int(math.ceil(math.sqrt(count)))
     cell = 19.0 / side
+    protos = {}
+
+    def proto(key, make):
+        if key not in protos:
+            protos[key] = sc.add_mesh(*make())
+        return protos[key]
+
     for k in range(count):
         gx, gy = k % side, k // side
         cx = -9.5 + (gx + 0.5) * cell + rng.uniform(-0.2, 0.2) * cell
         cy = -9.5 + (gy + 0.5) * cell + rng.uniform(-0.2, 0.2) * cell
         size = rng.uniform(0.18, 0.38) * cell
         kind = rng.integers(3)
+        M = None  # world_from_local of the shared form
         if kind == 0:
             sub = min(int(rng.integers(2, 6)), max_sub)
-            p, i, n, t, uv = _unit_icosphere(sub)
-            p = p * size + np.array([cx, cy, size])
+            if shared:
+                m = proto(("ico", sub), lambda: _unit_icosphere(sub))
+                M = np.array([[size, 0, 0, cx], [0, size, 0, cy], [0, 0, size, size]])
+            else:
+                p, i, n, t, uv = _unit_icosphere(sub)
+                p = p * size + np.array([cx, cy, size])
         elif kind == 1:
             r = size / 4.0
             nmaj = int(max(16, math.sqrt(per * 2)))
             nmin = max(8, nmaj // 2)
-            p, i, n, t, uv = torus(3 * r, r, nmaj, nmin, center=(cx, cy, r * 1.05))
+            if shared:
+                m = proto(("torus", nmaj, nmin), lambda: torus(3.0, 1.0, nmaj, nmin, center=(0.0, 0.0, 0.0)))
+                M = np.array([[r, 0, 0, cx], [0, r, 0, cy], [0, 0, r, r * 1.05]])
+            else:
+                p, i, n, t, uv = torus(3 * r, r, nmaj, nmin, center=(cx, cy, r * 1.05))
         else:
             nb = int(max(4, math.sqrt(per / 12)))
             p, i, n, t, uv = displaced_box(nb, 0.25, rng)
             ang = rng.uniform(0, math.pi)
             c, s = math.cos(ang), math.sin(ang)
             R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
-            p = (p * size) @ R.T + np.array([cx, cy, size * 1.3])
-            n = n @ R.T
-            t = t.copy()
-            t[:, :3] = t[:, :3] @ R.T
+            if shared:
+                m = sc.add_mesh(p, i, n, t, uv)
+                M = np.concatenate([R * size, np.array([[cx], [cy], [size * 1.3]])], 1)
+            else:
+                p = (p * size) @ R.T + np.array([cx, cy, size * 1.3])
+                n = n @ R.T
+                t = t.copy()
+                t[:, :3] = t[:, :3] @ R.T
         u = rng.random()
         if u < 0.6:
             mat = sc.add_material(base_color=tuple(rng.uniform(0.2, 0.8, 3)), metallic=0.0, roughness=1.0)
@@ -419,7 +443,10 @@ def many_instance_scene(seed=0, count=400, target_tris=5_000_000, light_radiance
         else:
             mat = sc.add_material(base_color=(1.0, 1.0, 1.0), metallic=0.0, roughness=0.0, eta=1.5,
                                   type=MATERIAL_DIELECTRIC)
-        _add(sc, (p, i, n, t, uv), mat)
+        if shared:
+            sc.add_instance(m, mat, M.astype(np.float32))
+        else:
+            _add(sc, (p, i, n, t, uv), mat)
     for (lx, ly) in ((-5.0, -3.0), (4.0, -4.0), (0.0, 5.0)):
         light = sc.add_material(base_color=(1, 1, 1), emissive=(light_radiance,) * 3, metallic=0.0)
         h = 6.0

@@ -56,3 +56,49 @@ def multi_mesh_scene(seed=0):
     sc.add_instance(m1, mat)
     sc.add_instance(m0, mat, np.array([[1, 0, 0, 0.3], [0, 1, 0, -0.1], [0, 0, 1, 0.2]], dtype=np.float32))
     return sc
+
+
+def rotation(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    x, y, z = axis
+    c, s_ = np.cos(angle), np.sin(angle)
+    C = 1 - c
+    return np.array([[c + x * x * C, x * y * C - z * s_, x * z * C + y * s_],
+                     [y * x * C + z * s_, c + y * y * C, y * z * C - x * s_],
+                     [z * x * C - y * s_, z * y * C + x * s_, c + z * z * C]])
+
+
+def instanced_scene(seed=0, count=40, mirror=True):
+    """Three meshes (icosphere, box, heightfield patch) placed `count` times with random rotations,
+    non-uniform scales (one mirrored instance when `mirror`) and translations in [0, 4]^3, plus an
+    emissive quad; returns the scene and the transforms of its instances."""
+    from sparkle_amd import scenes
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = spk.Scene()
+    mats = [sc.add_material(base_color=tuple(rng.uniform(0.2, 0.8, 3)), metallic=0.0, roughness=1.0)
+            for _ in range(3)]
+    light = sc.add_material(base_color=(1, 1, 1), emissive=(8.0, 8.0, 8.0), metallic=0.0)
+    meshes = [sc.add_mesh(*scenes.icosphere(2, radius=0.3, center=(0.0, 0.0, 0.0))),
+              sc.add_mesh(*scenes.box_mesh((0.0, 0.0, 0.0), (0.25, 0.15, 0.2), 0.4))]
+    hf = scenes.heightfield_scene(cells=12)
+    meshes.append(sc.add_mesh(**{k: v for k, v in hf.meshes[0].items()}))
+    xfs = []
+    for k in range(count):
+        R = rotation(rng.normal(size=3), rng.uniform(0, 2 * np.pi))
+        S = np.diag(rng.uniform(0.5, 1.6, 3))
+        if mirror and k == 3:
+            S[0, 0] = -S[0, 0]
+        M = np.zeros((3, 4))
+        M[:, :3] = R @ S
+        M[:, 3] = rng.uniform(0.0, 4.0, 3)
+        M = M.astype(np.float32)
+        sc.add_instance(meshes[k % 3], mats[k % 3], M)
+        xfs.append(M)
+    q = np.array([[1.5, 1.5, 4.6], [2.5, 1.5, 4.6], [2.5, 2.5, 4.6], [1.5, 2.5, 4.6]], np.float32)
+    lm = sc.add_mesh(q, np.array([[0, 2, 1], [0, 3, 2]], np.uint32))
+    M = np.eye(3, 4, dtype=np.float32)
+    sc.add_instance(lm, light, M)
+    xfs.append(M)
+    return sc, np.stack(xfs)

@@ -93,3 +93,21 @@ def test_batch_scene_generator_spans_the_requested_sizes():
     # the many-instance scene itself is unchanged by the per-instance budget cap
     sc, _ = scenes.many_instance_scene(seed=0)
     assert sc.num_triangles == 5_533_044
+
+
+def test_instanced_many_instance_scene_places_the_same_geometry():
+    """The shared form of the many-instance generator (prototype meshes placed by transforms) gives
+    the flattened form's world triangles up to float rounding, instance by instance."""
+    from sparkle_amd import scenes
+
+    flat, _ = scenes.many_instance_scene(seed=2, count=40, target_tris=200_000)
+    shared, _ = scenes.many_instance_scene(seed=2, count=40, target_tris=200_000, shared=True)
+    assert len(shared.instances) == len(flat.instances) and shared.num_triangles == flat.num_triangles
+    assert len(shared.meshes) < len(flat.meshes)
+    for a, b in zip(flat.instances, shared.instances):
+        ma, mb = flat.meshes[a["mesh"]], shared.meshes[b["mesh"]]
+        np.testing.assert_array_equal(ma["indices"], mb["indices"])
+        pa = ma["positions"].astype(np.float64) @ a["M"][:, :3].T.astype(np.float64) + a["M"][:, 3]
+        pb = mb["positions"].astype(np.float64) @ b["M"][:, :3].T.astype(np.float64) + b["M"][:, 3]
+        np.testing.assert_allclose(pa, pb, atol=2e-5)
+        assert a["material"] == b["material"]
