@@ -548,7 +548,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-coherent", action="store_true")
-    ap.add_argument("--config", type=str, default=None, help="traversal variant width,stack,block[,refill]")
+    ap.add_argument("--config", type=str, default=None,
+                    help="traversal variant width,stack,block[,refill[,ray_sort]]")
+    ap.add_argument("--leaf", type=int, default=3, help="triangles per BVH leaf slot (1..3)")
     ap.add_argument("--relmse", type=str, default="1920x1080,64,4096",
                     help="equal-time relMSE experiment WxH,spp,reference-spp ('off' to skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
@@ -626,7 +628,7 @@ def main():
     ctx.set_stream(stream.cuda_stream)
     assert ctx.stream() == stream.cuda_stream
     log("building BVH and uploading")
-    st = ctx.upload_scene(sc)
+    st = ctx.upload_scene(sc, max_leaf_tris=args.leaf)
     d_rays = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()
     d_hits = torch.empty(n * 8, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
@@ -636,14 +638,15 @@ def main():
         parts = [int(x) for x in args.config.split(",")]
         w_, s_, b_ = parts[:3]
         r_ = parts[3] if len(parts) > 3 else 0
-        ctx.set_trace_config(w_, s_, b_, r_)
+        o_ = parts[4] if len(parts) > 4 else 0
+        ctx.set_trace_config(w_, s_, b_, r_, o_)
         st = ctx.stats()
-        tune = {"bvh_width": w_, "stack": s_, "block": b_, "refill": r_, "fixed": True}
+        tune = {"bvh_width": w_, "stack": s_, "block": b_, "refill": r_, "ray_sort": o_, "fixed": True}
     elif not args.no_autotune:
         log("autotuning traversal variants")
-        sub = min(n, 1 << 21)
-        cfg, ms = ctx.autotune(d_rays, sub, tune_width=True)
-        tune = {"bvh_width": cfg[0], "stack": cfg[1], "block": cfg[2], "refill": cfg[3]}
+        # the whole batch: the coherence order pays off only with enough rays per bin
+        cfg, ms = ctx.autotune(d_rays, n, tune_width=True)
+        tune = {"bvh_width": cfg[0], "stack": cfg[1], "block": cfg[2], "refill": cfg[3], "ray_sort": cfg[4]}
         st = ctx.stats()
 
     # traversal counts for the algorithmic-bytes model
@@ -653,6 +656,12 @@ def main():
     mean_nodes = cnt_nodes.double().mean().item()
     mean_tris = cnt_tris.double().mean().item()
     bytes_per_ray = 32 + 8 + st.node_bytes * mean_nodes + st.triangle_bytes * mean_tris
+    sort_bytes_per_ray = 0
+    if ctx.trace_config()[4]:
+        # the traversal kernel reads its ray's index in the coherence order; the sort kernels before it
+        # read each ray once more to bin it and write and read a 4-byte key and a 4-byte index
+        bytes_per_ray += 4
+        sort_bytes_per_ray = 32 + 16
     bytes_per_launch = bytes_per_ray * n
 
     for _ in range(args.warmup):
@@ -662,6 +671,9 @@ def main():
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    # events on the context stream around the traversal kernel of each timed call (a coherence sort,
+    # when the configuration has one, runs before it and stays outside)
+    ctx.trace_kernel_timing(args.steps)
     t0 = time.perf_counter()
     for k in range(args.steps):
         starts[k].record(stream)
@@ -672,7 +684,9 @@ def main():
     ctx.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    kernel_ms = ctx.trace_kernel_times()
+    assert len(kernel_ms) == args.steps
+    ctx.trace_kernel_timing(0)
     avg_ms = float(np.mean(kernel_ms))
     region_ms = starts[0].elapsed_time(ends[-1])
     value = n * args.steps / (region_ms / 1e3) / 1e6
@@ -698,7 +712,7 @@ def main():
         coherent_mrays = round(rays_a.shape[0] * 10 / (ea.elapsed_time(eb) / 1e3) / 1e6, 1)
 
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
-    cur = ctx.trace_config()
+    cur = ctx.trace_config()[:4] if not ctx.trace_config()[4] else ctx.trace_config()
     traffic, l2_hit, traffic_src = load_traffic(",".join(str(x) for x in cur))
     out = {
         "metric": METRIC,
@@ -736,7 +750,10 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_ray": round(bytes_per_ray, 1),
             "compulsory_bytes_per_ray": 40,
+            "sort_bytes_per_ray": sort_bytes_per_ray,
             "kernel_avg_ms": round(avg_ms, 4),
+            "kernel": f"k_trace<{ctx.trace_config()[0]}, {ctx.trace_config()[1]}, {ctx.trace_config()[2]}, false, "
+                      f"false>" if not ctx.trace_config()[3] else "k_trace_persistent",
             "traffic_source": traffic_src,
             "l2_hit_rate": l2_hit,
         },

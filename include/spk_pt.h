@@ -255,6 +255,15 @@ int spk_trace_closest_host(spk_context *ctx, const spk_ray *rays, uint64_t n, sp
    uint32, either may be NULL). */
 int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t *nodes_dev,
                     uint32_t *tris_dev);
+/* Kernel timing, a diagnostic with no counterpart in the reference (its rays are traced on the CPU):
+   records HIP events on the context stream around the closest-hit kernel of each of the next
+   `capacity` spk_trace_closest(_async) calls, leaving the coherence sort of a ray_sort configuration
+   outside, so that the traversal kernel can be priced against its memory traffic on its own.
+   capacity 0 switches it off; a new capacity forgets earlier calls. */
+int spk_trace_kernel_timing(spk_context *ctx, uint32_t capacity);
+/* Waits for the timed calls and writes each one's kernel time in milliseconds, in call order
+   (up to max); count receives the number of timed calls. */
+int spk_trace_kernel_times(spk_context *ctx, float *ms, uint32_t max, uint32_t *count);
 
 /* Traversal kernel configuration: BVH width (4 or 8), LDS stack entries per lane, threads per
    block. The stack must hold the tree depth (spk_scene_stats.depth). */
@@ -264,6 +273,9 @@ typedef struct spk_trace_config {
     uint32_t block;
     uint32_t refill; /* 0: one ray per thread; k > 0: persistent lanes, a wave takes new rays once k of
                         its 64 lanes are idle */
+    uint32_t ray_sort; /* 1 (one ray per thread only): trace the rays in the order of a counting sort
+                          by direction bin (cube face x 4 x 4) and origin cell (8^3 of the scene box),
+                          so a wave's rays start close and point alike; hits keep the input order */
 } spk_trace_config;
 
 #define SPK_TUNE_WIDTH 1u /* also try the other BVH width (rebuilds the tree on the host) */

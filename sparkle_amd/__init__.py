@@ -98,10 +98,10 @@ class SceneStats(ctypes.Structure):
 
 class TraceConfig(ctypes.Structure):
     _fields_ = [("bvh_width", ctypes.c_uint32), ("stack", ctypes.c_uint32), ("block", ctypes.c_uint32),
-                ("refill", ctypes.c_uint32)]
+                ("refill", ctypes.c_uint32), ("ray_sort", ctypes.c_uint32)]
 
     def as_tuple(self):
-        return (self.bvh_width, self.stack, self.block, self.refill)
+        return (self.bvh_width, self.stack, self.block, self.refill, self.ray_sort)
 
 
 TUNE_WIDTH = 1
@@ -188,6 +188,8 @@ def load_library():
         "spk_trace_closest_async": (I, [P, P, U64, P]),
         "spk_trace_closest_host": (I, [P, P, U64, P]),
         "spk_trace_count": (I, [P, P, U64, P, P]),
+        "spk_trace_kernel_timing": (I, [P, U32]),
+        "spk_trace_kernel_times": (I, [P, ctypes.POINTER(F), U32, ctypes.POINTER(U32)]),
         "spk_get_trace_config": (I, [P, ctypes.POINTER(TraceConfig)]),
         "spk_set_trace_config": (I, [P, ctypes.POINTER(TraceConfig)]),
         "spk_list_trace_configs": (I, [P, P, U32, ctypes.POINTER(U32)]),
@@ -446,14 +448,28 @@ class Context:
             n = rays_dev.numel() * rays_dev.element_size() // 32
         return self._check(self.lib.spk_trace_count(self.h, _ptr(rays_dev), n, _ptr(nodes_dev), _ptr(tris_dev)))
 
+    def trace_kernel_timing(self, capacity):
+        """Time the closest-hit kernel (not the coherence sort before it) of the next `capacity`
+        trace calls with HIP events on the context stream; 0 switches timing off."""
+        return self._check(self.lib.spk_trace_kernel_timing(self.h, capacity))
+
+    def trace_kernel_times(self):
+        """Kernel milliseconds of the calls timed since trace_kernel_timing, in call order."""
+        cnt = ctypes.c_uint32(0)
+        self._check(self.lib.spk_trace_kernel_times(self.h, None, 0, ctypes.byref(cnt)))
+        out = (ctypes.c_float * max(cnt.value, 1))()
+        self._check(self.lib.spk_trace_kernel_times(self.h, out, cnt.value, ctypes.byref(cnt)))
+        return [float(out[k]) for k in range(cnt.value)]
+
     def trace_config(self):
         c = TraceConfig()
         self._check(self.lib.spk_get_trace_config(self.h, ctypes.byref(c)))
         return c.as_tuple()
 
-    def set_trace_config(self, bvh_width, stack, block, refill=0):
-        """refill 0: one ray per thread; k > 0: persistent lanes refilled once k lanes of a wave are idle."""
-        c = TraceConfig(bvh_width, stack, block, refill)
+    def set_trace_config(self, bvh_width, stack, block, refill=0, ray_sort=0):
+        """refill 0: one ray per thread; k > 0: persistent lanes refilled once k lanes of a wave are idle;
+        ray_sort 1: rays traced in a coherence order (counting sort by direction bin and origin cell)."""
+        c = TraceConfig(bvh_width, stack, block, refill, ray_sort)
         return self._check(self.lib.spk_set_trace_config(self.h, ctypes.byref(c)))
 
     def list_trace_configs(self):

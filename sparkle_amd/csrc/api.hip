@@ -2,11 +2,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/spk_pt.h"
@@ -189,6 +191,21 @@ static spk_mesh_desc mesh_view(const MeshCopy &mc) {
     d.num_vertices = mc.num_vertices;
     d.num_triangles = mc.num_triangles;
     return d;
+}
+
+// Scene box of the world vertices (the ray sort's origin cells), kept with the device scene.
+static void set_scene_box(const HostScene &hs, DeviceScene &ds) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < hs.verts.size(); i += 3)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], hs.verts[i + a]);
+            hi[a] = std::max(hi[a], hs.verts[i + a]);
+        }
+    for (int a = 0; a < 3; ++a) {
+        ds.box_lo[a] = std::isfinite(lo[a]) ? lo[a] : 0.f;
+        const float ext = std::isfinite(hi[a] - lo[a]) ? hi[a] - lo[a] : 0.f;
+        ds.box_inv[a] = ext > 0.f ? 1.f / ext : 1.f;
+    }
 }
 
 // ---- two-level scenes ----
@@ -455,6 +472,7 @@ static int upload_two_level(spk_context *ctx) {
             hipSuccess)
         return hip_fail(ctx, e, "scene upload");
     ds.two_level = true;
+    set_scene_box(hs, ds);
     ds.num_tris = n;
     ds.num_nodes = hs.num_nodes2;
     ds.num_materials = (uint32_t)mats.size();
@@ -497,6 +515,7 @@ static int upload_built(spk_context *ctx, int stack) {
         (e = hipMemcpy(ds.materials.ptr, mats.data(), mats.size() * sizeof(MaterialGpu), hipMemcpyHostToDevice)) !=
             hipSuccess)
         return hip_fail(ctx, e, "scene upload");
+    set_scene_box(hs, ds);
     ds.num_tris = n;
     ds.num_nodes = hs.bvh.num_nodes;
     ds.num_materials = (uint32_t)mats.size();
@@ -839,16 +858,31 @@ int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, 
         if (std::memcmp(M, &hs.inst_xf[12 * (size_t)i], 48) == 0) continue;
         moved.push_back(i);
         std::memcpy(&hs.inst_xf[12 * (size_t)i], M, 48);
-        const MeshCopy &mc = hs.mesh_copies[hs.inst_mesh[i]];
-        const spk_mesh_desc md = mesh_view(mc);
-        const uint32_t g0 = hs.inst_id_base[i];
-        for (uint32_t f = 0; f < mc.num_triangles; ++f) {
-            float *v = &hs.verts[9 * (size_t)(g0 + f)];
-            for (int k = 0; k < 3; ++k)
-                transform_point(M, &mc.positions[3 * (size_t)mc.indices[3 * (size_t)f + k]], v + 3 * k);
-            shade_triangle(md, M, f, v, hs.inst_material[i], hs.shade_in[g0 + f]);
-        }
     }
+    // world vertices and shading of each moved instance (independent ranges, spread over threads)
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t m; (m = next.fetch_add(1)) < moved.size();) {
+            const uint32_t i = moved[m];
+            const float *M = &hs.inst_xf[12 * (size_t)i];
+            const MeshCopy &mc = hs.mesh_copies[hs.inst_mesh[i]];
+            const spk_mesh_desc md = mesh_view(mc);
+            const uint32_t g0 = hs.inst_id_base[i];
+            for (uint32_t f = 0; f < mc.num_triangles; ++f) {
+                float *v = &hs.verts[9 * (size_t)(g0 + f)];
+                for (int k = 0; k < 3; ++k)
+                    transform_point(M, &mc.positions[3 * (size_t)mc.indices[3 * (size_t)f + k]], v + 3 * k);
+                shade_triangle(md, M, f, v, hs.inst_material[i], hs.shade_in[g0 + f]);
+            }
+        }
+    };
+    // the build's thread count (spk_build_options.num_threads; 0: every hardware thread)
+    const unsigned hw = hs.input.num_threads ? hs.input.num_threads : std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = std::min<unsigned>(hw, std::max<unsigned>(1u, (unsigned)moved.size()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool) t.join();
     std::string err;
     int rc = assemble_two_level(hs, hs.input.width, hs.input.num_threads, err);
     if (rc != SPK_OK) {
@@ -875,6 +909,7 @@ int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, 
     }
     ds.num_nodes = hs.num_nodes2;
     ds.depth = (int)hs.depth2;
+    set_scene_box(hs, ds);  // the ray sort's origin cells follow the moved instances
     if (ds.variant.stack < ds.depth) {
         const int stack = pick_trace_stack(ds.depth, true);
         if (stack < 0) return set_error(ctx, SPK_ERR_STACK_DEPTH, "two-level depth %d too deep", ds.depth);
@@ -886,15 +921,78 @@ int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, 
     return SPK_OK;
 }
 
+// Scratch of the coherence order for n rays (grown on demand); null when the variant does not sort.
+static hipError_t sort_scratch(spk_context *ctx, uint64_t n, RaySortScratch &s, const RaySortScratch *&out) {
+    out = nullptr;
+    if (!ctx->scene.variant.sort) return hipSuccess;
+    hipError_t e;
+    const uint64_t chunks = (std::max<uint64_t>(n, 1) + kRaySortChunk - 1) / kRaySortChunk;
+    if ((e = ctx->sort_keys.ensure(std::max<uint64_t>(n, 1) * 4)) != hipSuccess ||
+        (e = ctx->sort_perm.ensure(std::max<uint64_t>(n, 1) * 4)) != hipSuccess ||
+        (e = ctx->sort_counts.ensure(chunks * kRaySortBins * 4)) != hipSuccess ||
+        (e = ctx->sort_hist.ensure((size_t)kRaySortBins * 4)) != hipSuccess)
+        return e;
+    s.keys = (uint32_t *)ctx->sort_keys.ptr;
+    s.counts = (uint32_t *)ctx->sort_counts.ptr;
+    s.perm = (uint32_t *)ctx->sort_perm.ptr;
+    s.hist = (uint32_t *)ctx->sort_hist.ptr;
+    for (int a = 0; a < 3; ++a) s.lo[a] = ctx->scene.box_lo[a], s.inv[a] = ctx->scene.box_inv[a];
+    out = &s;
+    return hipSuccess;
+}
+
 int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev) {
     if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
     if (n && (!rays_dev || !hits_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null ray or hit array");
     if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
     const DeviceScene &ds = ctx->scene;
-    hipError_t e = launch_trace(ds.variant, rays_dev, n, hits_dev, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(),
-                                nullptr, nullptr, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
-                                ds.two_level ? ds.insts.ptr : nullptr, ctx->stream);
+    RaySortScratch ss;
+    const RaySortScratch *sp = nullptr;
+    hipError_t e = sort_scratch(ctx, n, ss, sp);
+    if (e != hipSuccess) return hip_fail(ctx, e, "ray sort buffers");
+    const hipEvent_t *timed = nullptr;
+    if (n && 2 * (size_t)ctx->kernel_timed < ctx->kernel_events.size())
+        timed = &ctx->kernel_events[2 * (size_t)ctx->kernel_timed++];
+    e = launch_trace(ds.variant, rays_dev, n, hits_dev, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(), nullptr,
+                     nullptr, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
+                     ds.two_level ? ds.insts.ptr : nullptr, sp, ctx->stream, timed);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    return SPK_OK;
+}
+
+int spk_trace_kernel_timing(spk_context *ctx, uint32_t capacity) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (capacity > (1u << 16)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "at most 65536 timed calls");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipStreamSynchronize(ctx->stream);  // no recorded event may still be pending
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    for (hipEvent_t ev : ctx->kernel_events) (void)hipEventDestroy(ev);
+    ctx->kernel_events.clear();
+    ctx->kernel_timed = 0;
+    for (uint32_t k = 0; k < 2 * capacity; ++k) {
+        hipEvent_t ev;
+        if ((e = hipEventCreate(&ev)) != hipSuccess) {
+            for (hipEvent_t x : ctx->kernel_events) (void)hipEventDestroy(x);
+            ctx->kernel_events.clear();
+            return hip_fail(ctx, e, "hipEventCreate");
+        }
+        ctx->kernel_events.push_back(ev);
+    }
+    return SPK_OK;
+}
+
+int spk_trace_kernel_times(spk_context *ctx, float *ms, uint32_t max, uint32_t *count) {
+    if (!ctx || (max && !ms)) return SPK_ERR_INVALID_ARGUMENT;
+    const uint32_t n = ctx->kernel_timed;
+    if (count) *count = n;
+    if (n == 0) return SPK_OK;
+    hipError_t e = hipEventSynchronize(ctx->kernel_events[2 * (size_t)n - 1]);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventSynchronize");
+    for (uint32_t k = 0; k < std::min(n, max); ++k) {
+        if ((e = hipEventElapsedTime(&ms[k], ctx->kernel_events[2 * (size_t)k], ctx->kernel_events[2 * (size_t)k + 1])) !=
+            hipSuccess)
+            return hip_fail(ctx, e, "hipEventElapsedTime");
+    }
     return SPK_OK;
 }
 
@@ -931,9 +1029,12 @@ int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint3
     hipError_t e;
     if ((e = ctx->tmp_hits.ensure(n * sizeof(spk_hit))) != hipSuccess) return hip_fail(ctx, e, "hit buffer");
     const DeviceScene &ds = ctx->scene;
+    RaySortScratch ss;
+    const RaySortScratch *sp = nullptr;
+    if ((e = sort_scratch(ctx, n, ss, sp)) != hipSuccess) return hip_fail(ctx, e, "ray sort buffers");
     e = launch_trace(ds.variant, rays_dev, n, ctx->tmp_hits.ptr, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(),
                      nodes_dev, tris_dev, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
-                     ds.two_level ? ds.insts.ptr : nullptr, ctx->stream);
+                     ds.two_level ? ds.insts.ptr : nullptr, sp, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "count launch");
     return spk_synchronize(ctx);
 }
@@ -959,6 +1060,7 @@ int spk_get_trace_config(spk_context *ctx, spk_trace_config *out) {
     out->stack = (uint32_t)ctx->scene.variant.stack;
     out->block = (uint32_t)ctx->scene.variant.block;
     out->refill = (uint32_t)ctx->scene.variant.refill;
+    out->ray_sort = (uint32_t)ctx->scene.variant.sort;
     return SPK_OK;
 }
 
@@ -972,7 +1074,7 @@ int spk_list_trace_configs(spk_context *ctx, spk_trace_config *out, uint32_t max
         if (vs[i].stack < ctx->scene.depth) continue;
         if (out && k < max_out)
             out[k] = spk_trace_config{(uint32_t)vs[i].width, (uint32_t)vs[i].stack, (uint32_t)vs[i].block,
-                                      (uint32_t)vs[i].refill};
+                                      (uint32_t)vs[i].refill, (uint32_t)vs[i].sort};
         ++k;
     }
     *count = k;
@@ -1011,14 +1113,14 @@ int spk_set_trace_config(spk_context *ctx, const spk_trace_config *config) {
     const int nv = std::min(kMaxVariants, trace_variants(vs, kMaxVariants, ctx->scene.width, ctx->scene.two_level));
     for (int i = 0; i < nv; ++i) {
         if (vs[i].stack == (int)config->stack && vs[i].block == (int)config->block &&
-            vs[i].refill == (int)config->refill) {
+            vs[i].refill == (int)config->refill && vs[i].sort == (int)(config->ray_sort != 0)) {
             ctx->scene.variant = vs[i];
             ctx->on_scene_changed();
             return SPK_OK;
         }
     }
-    return set_error(ctx, SPK_ERR_UNSUPPORTED, "no compiled variant with stack %u, block %u and refill %u",
-                     config->stack, config->block, config->refill);
+    return set_error(ctx, SPK_ERR_UNSUPPORTED, "no compiled variant with stack %u, block %u, refill %u, ray sort %u",
+                     config->stack, config->block, config->refill, config->ray_sort);
 }
 
 static int time_variant(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, double &ms) {
@@ -1029,6 +1131,9 @@ static int time_variant(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, d
         return set_error(ctx, SPK_ERR_HIP, "hipEventCreate");
     }
     hipError_t e = ctx->tmp_hits.ensure(n * sizeof(spk_hit));
+    RaySortScratch ss;
+    const RaySortScratch *sp = nullptr;
+    if (e == hipSuccess) e = sort_scratch(ctx, n, ss, sp);
     double best = 1e300;
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
         e = hipEventRecord(a, ctx->stream);
@@ -1036,7 +1141,7 @@ static int time_variant(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, d
             e = launch_trace(ctx->scene.variant, rays_dev, n, ctx->tmp_hits.ptr, ctx->scene.nodes.ptr,
                              ctx->scene.tris.ptr, 0u, ctx->invalid_counter(), nullptr, nullptr, ctx->work_counter(),
                              ctx->scene.spheres.ptr, ctx->scene.num_spheres, ctx->scene.num_tris,
-                             ctx->scene.two_level ? ctx->scene.insts.ptr : nullptr, ctx->stream);
+                             ctx->scene.two_level ? ctx->scene.insts.ptr : nullptr, sp, ctx->stream);
         if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float t = 0.f;
@@ -1057,10 +1162,10 @@ int spk_autotune(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t
     if (!rays_dev || n == 0) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "autotune needs rays");
     const int widths[2] = {ctx->scene.width, ctx->scene.width == 8 ? 4 : 8};
     const int nw = (flags & SPK_TUNE_WIDTH) ? 2 : 1;
-    spk_trace_config best{0, 0, 0, 0};
+    spk_trace_config best{0, 0, 0, 0, 0};
     double best_t = 1e300;
     for (int wi = 0; wi < nw; ++wi) {
-        spk_trace_config cfg{(uint32_t)widths[wi], 64u, 64u, 0u};
+        spk_trace_config cfg{(uint32_t)widths[wi], 64u, 64u, 0u, 0u};
         if ((int)cfg.bvh_width != ctx->scene.width) {
             int rc = spk_set_trace_config(ctx, &cfg);
             if (rc != SPK_OK) return rc;

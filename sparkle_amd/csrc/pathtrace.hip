@@ -1182,6 +1182,9 @@ void spk_context::release_all() {
     texels.release();
     tex_table.release();
     num_textures = 0;
+    for (hipEvent_t ev : kernel_events) (void)hipEventDestroy(ev);
+    kernel_events.clear();
+    kernel_timed = 0;
 }
 
 extern "C" {

@@ -64,6 +64,7 @@ struct DeviceScene {
     DeviceBuffer nodes, tris, shade, materials, emitters;
     DeviceBuffer insts, wtris;  // two-level: instance records; world triangles in global-id order
     bool two_level = false;
+    float box_lo[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};  // scene box (origin cells of the ray sort)
     DeviceBuffer spheres;  // 2 x float4 per sphere: (center, r^2), (material bits, 0, 0, 0)
     uint32_t num_tris = 0, num_nodes = 0, num_materials = 0, num_emitters = 0, num_spheres = 0;
     int width = 8, depth = 0;
@@ -94,6 +95,9 @@ struct spk_context {
     std::string error;
     spk::DeviceBuffer counters;  // [0] invalid rays, [2..3] ray counter of the persistent kernels
     spk::DeviceBuffer tmp_rays, tmp_hits;
+    spk::DeviceBuffer sort_keys, sort_counts, sort_hist, sort_perm;  // coherence order of spk_trace_closest
+    std::vector<hipEvent_t> kernel_events;  // spk_trace_kernel_timing: begin / end per timed call
+    uint32_t kernel_timed = 0;              // calls recorded so far
     spk::DeviceScene scene;
     spk::DeviceBuffer texels, tex_table;  // texture table (spk_textures_upload)
     uint32_t num_textures = 0;

@@ -11,7 +11,17 @@ struct TraceVariant {
     int stack;  // LDS stack entries per lane
     int block;  // threads per block
     int refill; // 0: one ray per thread; k > 0: persistent lanes, a wave refills when k lanes are idle
+    int sort = 0;  // 1: rays traced in coherence order (counting sort by direction bin and origin cell)
 };
+
+// Device scratch of the coherence order: a bin per ray, the per-chunk bin counts (then offsets),
+// the bin totals (then bases), the permutation; lo / inv: the scene box (origin cells are eighths).
+struct RaySortScratch {
+    uint32_t *keys, *counts, *hist, *perm;
+    float lo[3], inv[3];
+};
+constexpr uint32_t kRaySortBins = 96u * 512u;
+constexpr uint32_t kRaySortChunk = 32768u;  // rays per sorting block (16-bit LDS counters)
 
 // Closest hit of n rays (32-byte records) into n 8-byte hits. invalid_count (device) is
 // incremented once per NaN/inf/zero-direction ray. cnt_nodes/cnt_tris (device, may be null)
@@ -20,10 +30,13 @@ struct TraceVariant {
 // spheres (device, may be null): 2 float4 per sphere, tested after the BVH with ids id_base + k.
 // insts (device, may be null): instance records of a two-level scene (root = TLAS root); only
 // one-ray-per-thread variants (refill 0) exist for two-level scenes.
+// sort (may be null for variants without ray_sort): scratch of the coherence order.
+// timed (may be null): two events recorded just before and after the traversal kernel.
 hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, void *hits, const void *nodes,
                         const void *tris, uint32_t root, uint32_t *invalid_count, uint32_t *cnt_nodes,
                         uint32_t *cnt_tris, unsigned long long *work, const void *spheres, uint32_t num_spheres,
-                        uint32_t sphere_id_base, const void *insts, hipStream_t stream);
+                        uint32_t sphere_id_base, const void *insts, const RaySortScratch *sort,
+                        hipStream_t stream, const hipEvent_t *timed = nullptr);
 
 // Largest stack depth any compiled variant supports.
 int max_trace_stack();

@@ -13,16 +13,19 @@ struct SphereSet {
 };
 
 // TWO: a two-level scene (TLAS over the instance records `insts`, one BLAS per mesh).
+// perm (may be null): thread k traces ray perm[k] (a coherence order) and writes its slot.
 template <int WIDTH, int STACK, int BLOCK, bool COUNT, bool TWO>
 __global__ __launch_bounds__(BLOCK) void k_trace(const float4 *__restrict__ rays, uint64_t n,
                                                  uint2 *__restrict__ hits, const uint4 *__restrict__ nodes,
                                                  const uint4 *__restrict__ tris, uint32_t root,
                                                  uint32_t *__restrict__ invalid_count,
                                                  uint32_t *__restrict__ cnt_nodes, uint32_t *__restrict__ cnt_tris,
-                                                 SphereSet sph, const uint4 *__restrict__ insts) {
+                                                 SphereSet sph, const uint4 *__restrict__ insts,
+                                                 const uint32_t *__restrict__ perm) {
     __shared__ uint2 stack[STACK * BLOCK];
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const uint64_t i = perm ? (uint64_t)perm[k] : k;
     const float4 a = rays[2 * i], b = rays[2 * i + 1];
     RayIn r{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     uint2 out = make_uint2(0xFFFFFFFFu, 0x7f800000u);
@@ -124,9 +127,154 @@ __global__ __launch_bounds__(BLOCK) void k_trace_persistent(
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
+// ---- coherence order: counting sort of ray indices by (direction bin, origin cell) ----
+// Direction bin: the cube face of the major axis (6) x a 4 x 4 grid of the other two components
+// divided by the major one; origin cell: 8^3 cells of the scene box (clamped). Invalid rays go to
+// bin 0 (the trace kernel reports them as before).
+__device__ __forceinline__ uint32_t ray_bin(const RayIn &r, const RaySortScratch &s) {
+    if (!ray_valid(r)) return 0u;
+    const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+    uint32_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float u = fminf(fmaxf((o[a] - s.lo[a]) * s.inv[a] * 8.f, 0.f), 7.f);
+        c[a] = (uint32_t)u;
+    }
+    const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+    const int m = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    const float dm = m == 0 ? d[0] : (m == 1 ? d[1] : d[2]);
+    const float am = fabsf(dm);
+    const float s1 = (m == 0 ? d[1] : (m == 1 ? d[2] : d[0])) / am, s2 = (m == 0 ? d[2] : (m == 1 ? d[0] : d[1])) / am;
+    const uint32_t i1 = (uint32_t)fminf(fmaxf((s1 + 1.f) * 2.f, 0.f), 3.f);
+    const uint32_t i2 = (uint32_t)fminf(fmaxf((s2 + 1.f) * 2.f, 0.f), 3.f);
+    const uint32_t face = (uint32_t)m * 2u + (dm < 0.f ? 1u : 0u);
+    return ((face * 4u + i1) * 4u + i2) * 512u + (c[0] + 8u * (c[1] + 8u * c[2]));
+}
+
+// Device-scope atomics bypass the per-XCD L2s and serialise, so the sort counts in LDS: block j
+// owns rays [j * kRaySortChunk, (j + 1) * kRaySortChunk), counts its bins with LDS atomics on
+// 16-bit halves (two bins per word; a chunk holds fewer than 2^16 rays) and writes its row of
+// counts[j][bin]. A column scan turns the rows into per-chunk offsets and totals, a scan of the
+// totals gives the bin bases, and the scatter ranks each ray within its chunk's bin with LDS
+// atomics again and writes its index to its place. No global atomics. Both passes hold four rays
+// per thread in flight: one 96 KiB block per CU leaves 16 waves to hide the memory latency.
+// (Copying the rays themselves into bin order, so that the trace reads them coalesced, costs the
+// scatter more in scattered 16-byte writes than it saves the trace.)
+constexpr uint32_t kSortBatch = 4;
+
+__global__ __launch_bounds__(1024) void k_ray_bins(const float4 *__restrict__ rays, uint64_t n, RaySortScratch s) {
+    __shared__ uint32_t cnt[kRaySortBins / 2];
+    for (uint32_t b = threadIdx.x; b < kRaySortBins / 2; b += 1024u) cnt[b] = 0u;
+    __syncthreads();
+    const uint64_t first = (uint64_t)blockIdx.x * kRaySortChunk;
+    const uint64_t last = min(first + kRaySortChunk, n);
+    for (uint64_t base = first + threadIdx.x; base < last; base += 1024u * kSortBatch) {
+        float4 a[kSortBatch], b[kSortBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kSortBatch; ++q) {
+            const uint64_t i = base + 1024u * q;
+            if (i < last) a[q] = rays[2 * i], b[q] = rays[2 * i + 1];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kSortBatch; ++q) {
+            const uint64_t i = base + 1024u * q;
+            if (i >= last) continue;
+            const uint32_t key = ray_bin(RayIn{a[q].x, a[q].y, a[q].z, a[q].w, b[q].x, b[q].y, b[q].z, b[q].w}, s);
+            s.keys[i] = key;
+            atomicAdd(&cnt[key >> 1], (key & 1u) ? 0x10000u : 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t *row = s.counts + (size_t)blockIdx.x * kRaySortBins;
+    for (uint32_t b = threadIdx.x; b < kRaySortBins / 2; b += 1024u) {
+        const uint32_t w = cnt[b];
+        row[2 * b] = w & 0xFFFFu;
+        row[2 * b + 1] = w >> 16;
+    }
+}
+
+// per bin: counts[j][bin] -> sum of counts[j' < j][bin]; hist[bin] = the bin's total
+__global__ __launch_bounds__(256) void k_ray_bin_columns(RaySortScratch s, uint32_t chunks) {
+    const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+    if (b >= kRaySortBins) return;
+    uint32_t run = 0;
+    uint32_t j = 0;
+    for (; j + 8u <= chunks; j += 8u) {  // eight independent loads in flight per thread
+        uint32_t c[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q) c[q] = s.counts[(size_t)(j + q) * kRaySortBins + b];
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q) {
+            s.counts[(size_t)(j + q) * kRaySortBins + b] = run;
+            run += c[q];
+        }
+    }
+    for (; j < chunks; ++j) {
+        const size_t k = (size_t)j * kRaySortBins + b;
+        const uint32_t c = s.counts[k];
+        s.counts[k] = run;
+        run += c;
+    }
+    s.hist[b] = run;
+}
+
+// exclusive scan of the bin counts in place (one block; each thread owns a contiguous run)
+__global__ __launch_bounds__(1024) void k_ray_bin_scan(uint32_t *hist) {
+    __shared__ uint32_t part[1024];
+    constexpr uint32_t per = kRaySortBins / 1024u;
+    const uint32_t t = threadIdx.x;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k) sum += hist[t * per + k];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t add = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t c = hist[t * per + k];
+        hist[t * per + k] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_ray_scatter(uint64_t n, RaySortScratch s) {
+    __shared__ uint32_t cnt[kRaySortBins / 2];
+    for (uint32_t b = threadIdx.x; b < kRaySortBins / 2; b += 1024u) cnt[b] = 0u;
+    __syncthreads();
+    const uint64_t first = (uint64_t)blockIdx.x * kRaySortChunk;
+    const uint64_t last = min(first + kRaySortChunk, n);
+    const uint32_t *row = s.counts + (size_t)blockIdx.x * kRaySortBins;
+    for (uint64_t base = first + threadIdx.x; base < last; base += 1024u * kSortBatch) {
+        uint32_t key[kSortBatch], pos[kSortBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kSortBatch; ++q) {
+            const uint64_t i = base + 1024u * q;
+            key[q] = i < last ? s.keys[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kSortBatch; ++q) {
+            pos[q] = 0u;
+            if (base + 1024u * q < last) {
+                const uint32_t k = key[q];
+                const uint32_t old = atomicAdd(&cnt[k >> 1], (k & 1u) ? 0x10000u : 1u);
+                pos[q] = s.hist[k] + row[k] + ((k & 1u) ? old >> 16 : old & 0xFFFFu);
+            }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kSortBatch; ++q) {
+            const uint64_t i = base + 1024u * q;
+            if (i < last) s.perm[pos[q]] = (uint32_t)i;
+        }
+    }
+}
+
 namespace {
 using TraceFn = void (*)(const float4 *, uint64_t, uint2 *, const uint4 *, const uint4 *, uint32_t, uint32_t *,
-                         uint32_t *, uint32_t *, SphereSet, const uint4 *);
+                         uint32_t *, uint32_t *, SphereSet, const uint4 *, const uint32_t *);
 using PersistentFn = void (*)(const float4 *, uint64_t, uint2 *, const uint4 *, const uint4 *, uint32_t, uint32_t *,
                               uint32_t *, uint32_t *, unsigned long long *, uint32_t, SphereSet);
 struct Entry {
@@ -171,7 +319,8 @@ const int kRefill[] = {1, 16, 32};
 hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, void *hits, const void *nodes,
                         const void *tris, uint32_t root, uint32_t *invalid_count, uint32_t *cnt_nodes,
                         uint32_t *cnt_tris, unsigned long long *work, const void *spheres, uint32_t num_spheres,
-                        uint32_t sphere_id_base, const void *insts, hipStream_t stream) {
+                        uint32_t sphere_id_base, const void *insts, const RaySortScratch *sort,
+                        hipStream_t stream, const hipEvent_t *timed) {
     const SphereSet sph{(const float4 *)spheres, spheres ? num_spheres : 0u, sphere_id_base};
     const bool count = cnt_nodes != nullptr || cnt_tris != nullptr;
     const bool two = insts != nullptr;
@@ -182,7 +331,7 @@ hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, voi
         if (n == 0) return hipSuccess;
         const uint64_t blocks = (n + (uint64_t)v.block - 1) / (uint64_t)v.block;
         if (v.refill > 0) {
-            if (!work || !e.pfn) return hipErrorInvalidValue;
+            if (!work || !e.pfn || v.sort) return hipErrorInvalidValue;
             if (g_resident_blocks[k] == 0) {
                 int nb = 0;
                 hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, e.pfn, v.block, 0);
@@ -196,16 +345,31 @@ hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, voi
             const uint64_t grid = std::min<uint64_t>(blocks, (uint64_t)g_resident_blocks[k] * (uint64_t)cus);
             const uint32_t refill = (uint32_t)std::min(std::max(v.refill, 1), 64);
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, stream, work);
+            if (timed && (de = hipEventRecord(timed[0], stream)) != hipSuccess) return de;
             hipLaunchKernelGGL(e.pfn, dim3((uint32_t)grid), dim3(v.block), 0, stream, (const float4 *)rays, n,
                                (uint2 *)hits, (const uint4 *)nodes, (const uint4 *)tris, root, invalid_count,
                                cnt_nodes, cnt_tris, work, refill, sph);
-            return hipGetLastError();
+            if ((de = hipGetLastError()) != hipSuccess) return de;
+            return timed ? hipEventRecord(timed[1], stream) : hipSuccess;
         }
         if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+        const uint32_t *perm = nullptr;
+        if (v.sort) {
+            if (!sort || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+            const uint32_t chunks = (uint32_t)((n + kRaySortChunk - 1) / kRaySortChunk);
+            hipLaunchKernelGGL(k_ray_bins, dim3(chunks), dim3(1024), 0, stream, (const float4 *)rays, n, *sort);
+            hipLaunchKernelGGL(k_ray_bin_columns, dim3(kRaySortBins / 256u), dim3(256), 0, stream, *sort, chunks);
+            hipLaunchKernelGGL(k_ray_bin_scan, dim3(1), dim3(1024), 0, stream, sort->hist);
+            hipLaunchKernelGGL(k_ray_scatter, dim3(chunks), dim3(1024), 0, stream, n, *sort);
+            perm = sort->perm;
+        }
+        hipError_t te;
+        if (timed && (te = hipEventRecord(timed[0], stream)) != hipSuccess) return te;
         hipLaunchKernelGGL(e.fn, dim3((uint32_t)blocks), dim3(v.block), 0, stream, (const float4 *)rays, n,
                            (uint2 *)hits, (const uint4 *)nodes, (const uint4 *)tris, root, invalid_count, cnt_nodes,
-                           cnt_tris, sph, (const uint4 *)insts);
-        return hipGetLastError();
+                           cnt_tris, sph, (const uint4 *)insts, perm);
+        if ((te = hipGetLastError()) != hipSuccess) return te;
+        return timed ? hipEventRecord(timed[1], stream) : hipSuccess;
     }
     return hipErrorInvalidValue;
 }
@@ -223,7 +387,9 @@ int trace_variants(TraceVariant *out, int max_out, int width, bool two) {
     int k = 0;
     for (const Entry &e : kEntries) {
         if (e.count || e.width != width || e.two != two) continue;
-        if (k < max_out) out[k] = TraceVariant{e.width, e.stack, e.block, 0};
+        if (k < max_out) out[k] = TraceVariant{e.width, e.stack, e.block, 0, 0};
+        ++k;
+        if (k < max_out) out[k] = TraceVariant{e.width, e.stack, e.block, 0, 1};
         ++k;
         if (!e.pfn) continue;
         for (int r : kRefill) {

@@ -38,6 +38,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(spk.FrameDesc) == 44
     assert ctypes.sizeof(spk.CacheConfig) == 40
     assert ctypes.sizeof(spk.BuildOptions) == 16
+    assert ctypes.sizeof(spk.TraceConfig) == 20
     assert ctypes.sizeof(spk.Schedule) == 16
     assert spk.RAY_DTYPE.itemsize == 32 and spk.HIT_DTYPE.itemsize == 8
 

@@ -187,7 +187,7 @@ def test_persistent_kernels_match_counts_and_invalid_rays(gpu_ctx):
         for refill in (1, 16, 32):
             print_case("persistent", width, refill)
             gpu_ctx.set_trace_config(width, 16, 64, refill)
-            assert gpu_ctx.trace_config() == (width, 16, 64, refill)
+            assert gpu_ctx.trace_config() == (width, 16, 64, refill, 0)
             rc, h = gpu_trace(gpu_ctx, rays)
             assert rc == spk.SPK_ERR_INVALID_RAY
             np.testing.assert_array_equal(h["prim"], ref["prim"])
@@ -198,6 +198,21 @@ def test_persistent_kernels_match_counts_and_invalid_rays(gpu_ctx):
             assert rc1 == spk.SPK_ERR_INVALID_RAY
             np.testing.assert_array_equal(cn.cpu().numpy(), ref_counts[0])
             np.testing.assert_array_equal(ct.cpu().numpy(), ref_counts[1])
+        # coherence order: seven sorting chunks, the last one ragged, invalid rays in bin 0
+        print_case("ray sort", width)
+        gpu_ctx.set_trace_config(width, 16, 64, 0, 1)
+        assert gpu_ctx.trace_config() == (width, 16, 64, 0, 1)
+        rc, h = gpu_trace(gpu_ctx, rays)
+        assert rc == spk.SPK_ERR_INVALID_RAY
+        np.testing.assert_array_equal(h["prim"], ref["prim"])
+        np.testing.assert_array_equal(h["t"].view(np.uint32), ref["t"].view(np.uint32))
+        cn.fill_(-1)
+        ct.fill_(-1)
+        rc1 = gpu_ctx.lib.spk_trace_count(gpu_ctx.h, d.data_ptr(), n, cn.data_ptr(), ct.data_ptr())
+        assert rc1 == spk.SPK_ERR_INVALID_RAY
+        np.testing.assert_array_equal(cn.cpu().numpy(), ref_counts[0])
+        np.testing.assert_array_equal(ct.cpu().numpy(), ref_counts[1])
+        gpu_ctx.set_trace_config(width, 16, 64, 0, 0)
         rc, _ = gpu_trace(gpu_ctx, rays[100:1100])  # no invalid ray in this slice
         assert rc == spk.SPK_OK
 
@@ -277,3 +292,29 @@ def test_analytic_spheres_match_oracle(gpu_ctx):
         assert rc == spk.SPK_OK
         np.testing.assert_array_equal(h["prim"], want["prim"])
         np.testing.assert_array_equal(h["t"].view(np.uint32), want["t"].view(np.uint32))
+
+
+def test_kernel_timing_records_each_call(gpu_ctx):
+    """spk_trace_kernel_timing times the traversal kernel of each of the next `capacity` calls, with
+    and without the coherence sort before it; later calls and capacity 0 record nothing."""
+    import torch
+
+    sc = scenes.heightfield_scene(cells=128)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=100_000, seed=31)
+    gpu_ctx.upload_scene(sc)
+    d = to_dev(rays)
+    hits = torch.empty(rays.shape[0] * 8, dtype=torch.uint8, device="cuda")
+    w, stack, block = gpu_ctx.trace_config()[:3]
+    for sort in (0, 1):
+        print_case("kernel timing", sort)
+        gpu_ctx.set_trace_config(w, stack, block, 0, sort)
+        gpu_ctx.trace_kernel_timing(3)
+        for _ in range(5):
+            assert gpu_ctx.trace(d, hits, sync=False) == spk.SPK_OK
+        t = gpu_ctx.trace_kernel_times()
+        assert len(t) == 3 and all(0.0 < x < 1000.0 for x in t), t
+    gpu_ctx.trace_kernel_timing(0)
+    assert gpu_ctx.trace(d, hits) == spk.SPK_OK
+    assert gpu_ctx.trace_kernel_times() == []
+    gpu_ctx.set_trace_config(w, stack, block, 0, 0)
