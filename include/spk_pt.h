@@ -344,8 +344,10 @@ int spk_batch_offsets(spk_context *ctx, uint64_t *offsets, uint32_t max_count);
 typedef struct spk_schedule {
     uint32_t blocks_per_cu; /* persistent grid: 128-thread blocks per CU, 1..32 (default 8) */
     uint32_t material_sort; /* 1: shade each bounce in material order (counting sort; default 1) */
-    uint32_t refill;        /* extend after the camera rays (which refill whole waves): a wave takes
-                               new paths once this many of its 64 lanes are idle, 1..64 (default 16) */
+    uint32_t refill;        /* extend after the camera rays (which run one path per thread): a wave
+                               takes new paths once this many of its 64 lanes are idle, 1..64
+                               (default 16); 0: one path per thread over a full grid, like the
+                               camera rays (the per-path tail then refills at 16) */
     uint32_t tail_bounce;   /* segments from this index on run one path per lane to its end (closest
                                hit, shading and shadow ray in one persistent kernel) instead of one
                                bounce-synchronous launch per segment; 0 = none (default), <= 64 */
@@ -354,8 +356,8 @@ typedef struct spk_schedule {
 /* batch = 0: the schedule of spk_render_frame; batch != 0: of spk_batch_render. */
 int spk_get_schedule(spk_context *ctx, int batch, spk_schedule *out);
 int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule);
-/* Schedule autotuners: time blocks_per_cu {2, 4, 8, 16} x material_sort {0, 1} x refill {16, 64} on
-   one frame (after a capture frame per candidate), then tail_bounce {1, 2, 3, 4, 5} (below max_bounce)
+/* Schedule autotuners: time blocks_per_cu {2, 4, 8, 16} x material_sort {0, 1} x refill {0, 16, 64} on
+   the fastest of three frames (after a capture frame per candidate), then tail_bounce {1, 2, 3, 4, 5} (below max_bounce)
    on the fastest of those; keep and return the fastest. */
 int spk_render_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,
                         double *best_ms);

@@ -109,6 +109,7 @@ struct PtArgs {
     uint32_t *path_scene;    // scene of each path (written only when num_scenes > 1)
     uint32_t num_scenes, grid;
     uint32_t refill;  // extend: a wave takes new paths once this many of its lanes are idle (1..64)
+    uint32_t direct;  // 1: the wavefront extends run one queue slot per thread over a full grid
     uint32_t tail_from;  // segments from this index on run in k_tail (max_bounce: none)
     uint32_t npix, max_bounce, debug_mode, sky_enabled, flags, sort;
     int32_t sky_faces[6];  // cube-map faces +X, -X, +Y, -Y, +Z, -Z in the texture table, or -1
@@ -218,6 +219,16 @@ __device__ __forceinline__ const SceneRec &scene_of_path(const PtArgs &a, uint32
 // ---- generate: reseed per pixel, jitter, lens, camera ray (CPURenderer.cpp:220-238, 398-445) ----
 // Pixel p of the frame is pixel p - pix_base of its scene's image; seeds, jitter and camera are the
 // scene's own, so a scene renders the same alone or in a batch.
+// Position of pixel (i, j) of a W x H image in 8x8-tile order (tiles row by row, pixels row by row
+// inside a tile; tiles on the right and bottom edges are narrower or shorter, so the order is a
+// bijection onto [0, W*H)). Bounce 0 takes its paths in this order, so a wave's 64 camera rays
+// leave one tile of the image instead of a 64x1 strip.
+__device__ __forceinline__ uint32_t tile_order(uint32_t i, uint32_t j, uint32_t W, uint32_t H) {
+    const uint32_t tx = i >> 3, ty = j >> 3;
+    const uint32_t w = min(8u, W - (tx << 3)), h = min(8u, H - (ty << 3));
+    return (ty << 3) * W + (tx << 3) * h + (j & 7u) * w + (i & 7u);
+}
+
 __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
     const uint32_t tid = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t stride = gridDim.x * kPtBlock;
@@ -268,7 +279,7 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
             const Rng r2 = rng_seed(seed ^ 0x5bd1e9955bd1e995ull);
             a.nee_rng[p] = make_uint4(r2.s0, r2.s1, r2.s2, r2.s3);
         }
-        a.queue0[p] = p;
+        a.queue0[S.pix_base + tile_order(i, j, W, H)] = p;
         if (a.normal_out) {
             a.normal_out[3 * (size_t)p] = 0.f;
             a.normal_out[3 * (size_t)p + 1] = 0.f;
@@ -348,6 +359,48 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
             if (a.sort) atomicAdd(&lhist[key], 1u);
             active = false;
         }
+    }
+    if (a.sort) {
+        __syncthreads();
+        uint32_t *gh = a.hist + (size_t)bounce * 2u * a.nkeys;
+        for (uint32_t k = threadIdx.x; k < a.nkeys; k += kPtBlock)
+            if (lhist[k]) atomicAdd(&gh[k], lhist[k]);
+    }
+}
+
+// Extend without the persistent grid: one queue slot per thread over a grid that covers every
+// pixel (a queue never holds more paths). Always used for bounce 0: camera rays of a tile are
+// coherent, so lane refill buys nothing there, and a full grid keeps more waves resident than the
+// persistent one. Later bounces use it when the schedule's refill is 0. Same hit records and
+// histogram as k_extend.
+template <int WIDTH, int STACK, bool TWO>
+__global__ __launch_bounds__(kPtBlock) void k_extend_first(PtArgs a, uint32_t bounce) {
+    __shared__ uint2 stack[STACK * kPtBlock];
+    __shared__ uint32_t lhist[kMaxSortKeys];
+    const uint32_t *ctr = a.counters + bounce * kCtrStride;
+    const uint32_t count = ctr[CTR_QUEUE];
+    const uint32_t *queue = (bounce & 1u) ? a.queue1 : a.queue0;
+    if (a.sort)
+        for (uint32_t k = threadIdx.x; k < a.nkeys; k += kPtBlock) lhist[k] = 0u;
+    __syncthreads();
+    const uint32_t slot = blockIdx.x * kPtBlock + threadIdx.x;
+    if (slot < count) {
+        const uint32_t path = queue[slot];
+        const float4 ro = a.rays[2 * path], rd = a.rays[2 * path + 1];
+        const RayIn r{ro.x, ro.y, ro.z, ro.w, rd.x, rd.y, rd.z, rd.w};
+        const uint32_t root = a.num_scenes > 1u ? a.scenes[a.path_scene[path]].root : 0u;
+        Traversal<WIDTH, STACK, false, false, TWO> tr;
+        tr.init(r, root, a.insts);
+        while (tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
+        }
+        uint32_t key = a.nkeys - 1u;  // misses sort last
+        const uint4 h = hit_record<TWO>(a, r, tr.best);
+        if (h.x != 0xFFFFFFFFu) {
+            key = hit_material(a, h.x);
+            if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
+        }
+        a.hits[slot] = h;
+        if (a.sort) atomicAdd(&lhist[key], 1u);
     }
     if (a.sort) {
         __syncthreads();
@@ -983,9 +1036,9 @@ using ExtendFn = void (*)(PtArgs, uint32_t);
 struct PtEntry {
     int width, stack;
     bool two;
-    ExtendFn extend, shadow, tail;
+    ExtendFn extend, shadow, tail, first;
 };
-#define SPK_PT(W, S, T) {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_tail<W, S, T>}
+#define SPK_PT(W, S, T) {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_tail<W, S, T>, k_extend_first<W, S, T>}
 const PtEntry kPtEntries[] = {
     SPK_PT(8, 12, false), SPK_PT(4, 12, false), SPK_PT(8, 16, false), SPK_PT(8, 24, false),
     SPK_PT(8, 32, false), SPK_PT(8, 64, false), SPK_PT(4, 16, false), SPK_PT(4, 24, false),
@@ -1128,7 +1181,10 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
                                 int cus) {
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kPtBlock), 0, s, a);
     for (uint32_t b = 0; b < a.tail_from; ++b) {
-        hipLaunchKernelGGL(pe.extend, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+        if (b == 0u || a.direct)  // a full grid, one queue slot per thread
+            hipLaunchKernelGGL(pe.first, dim3((a.npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, s, a, b);
+        else
+            hipLaunchKernelGGL(pe.extend, dim3(grid), dim3(kPtBlock), 0, s, a, b);
         if (a.sort) {
             hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, a, b);
             hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(kPtBlock), 0, s, a, b);
@@ -1298,7 +1354,9 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     const int cus = device_cus(ctx);
     const int grid = cus * (int)src.blocks_per_cu;
     a.grid = (uint32_t)grid;
-    a.refill = std::min(std::max(src.refill, 1u), 64u);
+    // refill 0: direct extends; the per-path tail then refills at 16 idle lanes
+    a.refill = src.refill ? std::min(src.refill, 64u) : 16u;
+    a.direct = src.refill == 0u ? 1u : 0u;
     a.npix = npix;
     a.max_bounce = frame->max_bounce;
     a.tail_from = src.tail && src.tail < frame->max_bounce ? src.tail : frame->max_bounce;
@@ -1582,10 +1640,10 @@ int spk_batch_render(spk_context *ctx, const spk_frame_desc *frame, float *rgba_
 }
 
 static int check_schedule(spk_context *ctx, const spk_schedule &sc) {
-    if (sc.blocks_per_cu < 1 || sc.blocks_per_cu > 32 || sc.material_sort > 1 || sc.refill < 1 || sc.refill > 64 ||
+    if (sc.blocks_per_cu < 1 || sc.blocks_per_cu > 32 || sc.material_sort > 1 || sc.refill > 64 ||
         sc.tail_bounce > (uint32_t)kMaxBounce)
         return set_error(ctx, SPK_ERR_INVALID_ARGUMENT,
-                         "schedule out of range (blocks 1..32, sort 0/1, refill 1..64, tail 0..max bounces)");
+                         "schedule out of range (blocks 1..32, sort 0/1, refill 0..64, tail 0..max bounces)");
     return SPK_OK;
 }
 
@@ -1613,27 +1671,37 @@ int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule) 
 }
 
 // Times every candidate schedule on one frame (a capture frame first, then a timed one).
+// Frame time of the current schedule: one frame to capture its graph, then the fastest of three
+// (a single frame varies by a few percent, as much as neighbouring schedules differ).
+static int time_schedule(spk_context *ctx, int batch, const spk_frame_desc *frame, float *rgba_dev, double &ms) {
+    ms = 1e30;
+    for (int k = 0; k < 4; ++k) {
+        spk_frame_stats st{};
+        const int rc = batch ? spk_batch_render(ctx, frame, rgba_dev, k ? &st : nullptr)
+                             : spk_render_frame(ctx, frame, rgba_dev, nullptr, k ? &st : nullptr);
+        if (rc != SPK_OK) return rc;
+        if (k) ms = std::min(ms, (double)st.milliseconds);
+    }
+    return SPK_OK;
+}
+
 static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *frame, float *rgba_dev,
                              spk_schedule *chosen, double *best_ms) {
     spk_schedule *target = schedule_of(ctx, batch);
     if (!target) return set_error(ctx, SPK_ERR_NO_SCENE, "no batch uploaded");
     static const uint32_t kBlocks[] = {2u, 4u, 8u, 16u};
-    static const uint32_t kRefill[] = {16u, 64u};
+    static const uint32_t kRefill[] = {0u, 16u, 64u};
     double best = 1e30;
     spk_schedule best_s = *target;
     for (uint32_t srt = 0; srt < 2; ++srt) {
         for (uint32_t b : kBlocks) {
             for (uint32_t r : kRefill) {
                 *target = spk_schedule{b, srt, r, 0u};
-                spk_frame_stats st{};
-                int rc = batch ? spk_batch_render(ctx, frame, rgba_dev, nullptr)
-                               : spk_render_frame(ctx, frame, rgba_dev, nullptr, nullptr);
-                if (rc == SPK_OK)
-                    rc = batch ? spk_batch_render(ctx, frame, rgba_dev, &st)
-                               : spk_render_frame(ctx, frame, rgba_dev, nullptr, &st);
+                double ms = 0.0;
+                const int rc = time_schedule(ctx, batch, frame, rgba_dev, ms);
                 if (rc != SPK_OK) return rc;
-                if (st.milliseconds < best) {
-                    best = st.milliseconds;
+                if (ms < best) {
+                    best = ms;
                     best_s = *target;
                 }
             }
@@ -1645,15 +1713,11 @@ static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *
     for (uint32_t t : kTail) {
         if (t >= frame->max_bounce) continue;
         *target = spk_schedule{base.blocks_per_cu, base.material_sort, base.refill, t};
-        spk_frame_stats st{};
-        int rc = batch ? spk_batch_render(ctx, frame, rgba_dev, nullptr)
-                       : spk_render_frame(ctx, frame, rgba_dev, nullptr, nullptr);
-        if (rc == SPK_OK)
-            rc = batch ? spk_batch_render(ctx, frame, rgba_dev, &st)
-                       : spk_render_frame(ctx, frame, rgba_dev, nullptr, &st);
+        double ms = 0.0;
+        const int rc = time_schedule(ctx, batch, frame, rgba_dev, ms);
         if (rc != SPK_OK) return rc;
-        if (st.milliseconds < best) {
-            best = st.milliseconds;
+        if (ms < best) {
+            best = ms;
             best_s = *target;
         }
     }

@@ -100,7 +100,7 @@ def test_batch_schedule_does_not_change_results():
         ref = rgba.cpu().numpy().copy()
         print_case("batch autotune")
         (blocks, srt, refill, tail), ms = ctx.autotune_batch(f, rgba)
-        assert blocks in (2, 4, 8, 16) and srt in (0, 1) and refill in (16, 64) and tail in (0, 1, 2, 3, 4, 5) and ms > 0
+        assert blocks in (2, 4, 8, 16) and srt in (0, 1) and refill in (0, 16, 64) and tail in (0, 1, 2, 3, 4, 5) and ms > 0
         assert ctx.schedule(batch=True) == (blocks, srt, refill, tail)
         ctx.render_batch(f, rgba)
         assert np.array_equal(rgba.cpu().numpy(), ref)
@@ -109,6 +109,10 @@ def test_batch_schedule_does_not_change_results():
             ctx.set_schedule(8, 1, 16, tail, batch=True)
             ctx.render_batch(f, rgba)
             assert np.array_equal(rgba.cpu().numpy(), ref)
+        print_case("batch direct extends")
+        ctx.set_schedule(4, 1, 0, 0, batch=True)
+        ctx.render_batch(f, rgba)
+        assert np.array_equal(rgba.cpu().numpy(), ref)
         print_case("batch unsorted")
         ctx.render_batch(frame(0, 0, 1, bounce=6, flags=spk.FRAME_NO_SORT), rgba)
         assert np.array_equal(rgba.cpu().numpy(), ref)

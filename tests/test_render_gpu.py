@@ -44,14 +44,14 @@ class Renderer:
                 self.nrm.cpu().numpy().reshape(self.h, self.w, 3).copy(), st)
 
 
-def test_first_hit_debug_modes_match(gpu_ctx):
-    w = h = 64
+@pytest.mark.parametrize("w,h", [(64, 64), (61, 37)])  # 61x37: partial 8x8 tiles on both edges
+def test_first_hit_debug_modes_match(gpu_ctx, w, h):
     sc = scenes.cornell_scene()
     cam = scenes.cornell_camera(w, h)
     r = Renderer(gpu_ctx, sc, cam, w, h)
     o = refcpu.OracleScene(sc)
     for mode in (spk.DEBUG_NORMAL, spk.DEBUG_ALBEDO, spk.DEBUG_DEPTH):
-        print_case("debug mode", mode)
+        print_case("debug mode", mode, w, h)
         g, gn, _ = r.render(3, debug=mode)
         c, cn = o.render_frame(cam, w, h, frame_seed=3, debug_mode=mode)
         diff = np.abs(g[..., :3] - c[..., :3]).max(-1)
@@ -127,13 +127,13 @@ def test_schedule_and_autotune_do_not_change_results(gpu_ctx):
     r = Renderer(gpu_ctx, scenes.cornell_scene(), scenes.cornell_camera(w, h), w, h)
     ref, _, _ = r.render(9, bounce=6)
     for sched in ((2, 0, 1), (16, 1, 64), (4, 1, 16), (1, 0, 33), (8, 1, 16, 1), (2, 0, 64, 3), (8, 1, 16, 5),
-                  (8, 1, 16, 6), (8, 1, 16, 16)):
+                  (8, 1, 16, 6), (8, 1, 16, 16), (8, 1, 0), (2, 0, 0, 3)):
         print_case("schedule", sched)
         gpu_ctx.set_schedule(*sched)
         img, _, _ = r.render(9, bounce=6)
         np.testing.assert_array_equal(img, ref)
     (b, srt, refill, tail), ms = gpu_ctx.autotune_render(frame(w, h, 9, bounce=6), r.rgba)
-    assert b in (2, 4, 8, 16) and srt in (0, 1) and refill in (16, 64) and tail in (0, 1, 2, 3, 4, 5) and ms > 0
+    assert b in (2, 4, 8, 16) and srt in (0, 1) and refill in (0, 16, 64) and tail in (0, 1, 2, 3, 4, 5) and ms > 0
     img, _, _ = r.render(9, bounce=6)
     np.testing.assert_array_equal(img, ref)
     with pytest.raises(spk.SpkError):
@@ -165,9 +165,10 @@ def test_per_path_tail_matches_the_wavefront(gpu_ctx):
                 f.output_limit = 1e30
                 ref_st = gpu_ctx.render_frame(f, r.rgba)
                 ref = r.rgba.cpu().numpy().copy()
-                for tail in (1, 2, 4):
-                    print_case("tail", tail, "flags", flags, "seed", seed)
-                    gpu_ctx.set_schedule(8, 1, 16, tail)
+                # (refill 0: the wavefront extends run one path per thread over a full grid)
+                for refill, tail in ((16, 1), (16, 2), (16, 4), (0, 0), (0, 2)):
+                    print_case("refill", refill, "tail", tail, "flags", flags, "seed", seed)
+                    gpu_ctx.set_schedule(8, 1, refill, tail)
                     st = gpu_ctx.render_frame(f, r.rgba)
                     np.testing.assert_array_equal(r.rgba.cpu().numpy(), ref)
                     assert st.rays_traced == ref_st.rays_traced and st.shadow_rays == ref_st.shadow_rays
