@@ -658,10 +658,11 @@ def main():
     bytes_per_ray = 32 + 8 + st.node_bytes * mean_nodes + st.triangle_bytes * mean_tris
     sort_bytes_per_ray = 0
     if ctx.trace_config()[4]:
-        # the traversal kernel reads its ray's index in the coherence order; the sort kernels before it
-        # read each ray once more to bin it and write and read a 4-byte key and a 4-byte index
+        # the traversal kernel reads its ray's index in the coherence order; the two sort passes before
+        # it read each ray once more and write its key (36 B), then move the key and index by
+        # direction bin (4 in, 8 out), count the keys again (4) and move the index by full key (8 in, 4 out)
         bytes_per_ray += 4
-        sort_bytes_per_ray = 32 + 16
+        sort_bytes_per_ray = 36 + 12 + 4 + 12
     bytes_per_launch = bytes_per_ray * n
 
     for _ in range(args.warmup):

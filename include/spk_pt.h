@@ -275,7 +275,8 @@ typedef struct spk_trace_config {
                         its 64 lanes are idle */
     uint32_t ray_sort; /* 1 (one ray per thread only): trace the rays in the order of a counting sort
                           by direction bin (cube face x 4 x 4) and origin cell (8^3 of the scene box),
-                          so a wave's rays start close and point alike; hits keep the input order */
+                          so a wave's rays start close and point alike; hits keep the input order.
+                          Calls of 2^32 rays or more trace in input order. */
 } spk_trace_config;
 
 #define SPK_TUNE_WIDTH 1u /* also try the other BVH width (rebuilds the tree on the host) */

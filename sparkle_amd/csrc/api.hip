@@ -924,18 +924,23 @@ int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, 
 // Scratch of the coherence order for n rays (grown on demand); null when the variant does not sort.
 static hipError_t sort_scratch(spk_context *ctx, uint64_t n, RaySortScratch &s, const RaySortScratch *&out) {
     out = nullptr;
-    if (!ctx->scene.variant.sort) return hipSuccess;
+    if (!ctx->scene.variant.sort || n > 0xFFFFFFFFull) return hipSuccess;  // such calls trace unsorted
     hipError_t e;
-    const uint64_t chunks = (std::max<uint64_t>(n, 1) + kRaySortChunk - 1) / kRaySortChunk;
-    if ((e = ctx->sort_keys.ensure(std::max<uint64_t>(n, 1) * 4)) != hipSuccess ||
-        (e = ctx->sort_perm.ensure(std::max<uint64_t>(n, 1) * 4)) != hipSuccess ||
-        (e = ctx->sort_counts.ensure(chunks * kRaySortBins * 4)) != hipSuccess ||
-        (e = ctx->sort_hist.ensure((size_t)kRaySortBins * 4)) != hipSuccess)
+    const uint64_t m = std::max<uint64_t>(n, 1);
+    const uint64_t chunks = (m + kRaySortChunk - 1) / kRaySortChunk;
+    if ((e = ctx->sort_keys.ensure(m * 4)) != hipSuccess || (e = ctx->sort_keys1.ensure(m * 4)) != hipSuccess ||
+        (e = ctx->sort_perm1.ensure(m * 4)) != hipSuccess || (e = ctx->sort_perm.ensure(m * 4)) != hipSuccess ||
+        (e = ctx->sort_offs.ensure(chunks * kRaySortBins * 4)) != hipSuccess ||
+        (e = ctx->sort_offs1.ensure((m + kRaySortChunk1 - 1) / kRaySortChunk1 * kRayDirBins * 4)) != hipSuccess ||
+        (e = ctx->sort_total.ensure((size_t)(kRayDirBins + kRaySortBins) * 4)) != hipSuccess)
         return e;
     s.keys = (uint32_t *)ctx->sort_keys.ptr;
-    s.counts = (uint32_t *)ctx->sort_counts.ptr;
+    s.keys1 = (uint32_t *)ctx->sort_keys1.ptr;
+    s.perm1 = (uint32_t *)ctx->sort_perm1.ptr;
     s.perm = (uint32_t *)ctx->sort_perm.ptr;
-    s.hist = (uint32_t *)ctx->sort_hist.ptr;
+    s.offs = (uint32_t *)ctx->sort_offs.ptr;
+    s.offs1 = (uint32_t *)ctx->sort_offs1.ptr;
+    s.total = (uint32_t *)ctx->sort_total.ptr;
     for (int a = 0; a < 3; ++a) s.lo[a] = ctx->scene.box_lo[a], s.inv[a] = ctx->scene.box_inv[a];
     out = &s;
     return hipSuccess;

@@ -14,14 +14,20 @@ struct TraceVariant {
     int sort = 0;  // 1: rays traced in coherence order (counting sort by direction bin and origin cell)
 };
 
-// Device scratch of the coherence order: a bin per ray, the per-chunk bin counts (then offsets),
-// the bin totals (then bases), the permutation; lo / inv: the scene box (origin cells are eighths).
+// Device scratch of the coherence order (a two-pass counting sort): the key of each ray, the keys
+// and input indices in direction-bin order (pass 1), the final permutation (pass 2), per chunk the
+// offset of its run in each bin (pass 1: rows of kRayDirBins per kRaySortChunk1 rays; pass 2: rows
+// of kRaySortBins per kRaySortChunk rays), the bin totals then bases (kRayDirBins for pass 1, then
+// kRaySortBins); lo / inv: the scene box (origin cells are eighths of it).
+constexpr uint32_t kRayDirBins = 96u;    // cube face of the major axis x 4 x 4
+constexpr uint32_t kRayCellBins = 512u;  // 8^3 origin cells
+constexpr uint32_t kRaySortBins = kRayDirBins * kRayCellBins;
 struct RaySortScratch {
-    uint32_t *keys, *counts, *hist, *perm;
+    uint32_t *keys, *keys1, *perm1, *perm, *offs1, *offs, *total;
     float lo[3], inv[3];
 };
-constexpr uint32_t kRaySortBins = 96u * 512u;
-constexpr uint32_t kRaySortChunk = 32768u;  // rays per sorting block (16-bit LDS counters)
+constexpr uint32_t kRaySortChunk = 32768u;  // rays per block of pass 2 (16-bit LDS counters)
+constexpr uint32_t kRaySortChunk1 = 8192u;   // rays per block of pass 1 (96 bins: more blocks per CU)
 
 // Closest hit of n rays (32-byte records) into n 8-byte hits. invalid_count (device) is
 // incremented once per NaN/inf/zero-direction ray. cnt_nodes/cnt_tris (device, may be null)

@@ -148,83 +148,84 @@ __device__ __forceinline__ uint32_t ray_bin(const RayIn &r, const RaySortScratch
     const uint32_t i1 = (uint32_t)fminf(fmaxf((s1 + 1.f) * 2.f, 0.f), 3.f);
     const uint32_t i2 = (uint32_t)fminf(fmaxf((s2 + 1.f) * 2.f, 0.f), 3.f);
     const uint32_t face = (uint32_t)m * 2u + (dm < 0.f ? 1u : 0u);
-    return ((face * 4u + i1) * 4u + i2) * 512u + (c[0] + 8u * (c[1] + 8u * c[2]));
+    return ((face * 4u + i1) * 4u + i2) * kRayCellBins + (c[0] + 8u * (c[1] + 8u * c[2]));
 }
 
-// Device-scope atomics bypass the per-XCD L2s and serialise, so the sort counts in LDS: block j
-// owns rays [j * kRaySortChunk, (j + 1) * kRaySortChunk), counts its bins with LDS atomics on
-// 16-bit halves (two bins per word; a chunk holds fewer than 2^16 rays) and writes its row of
-// counts[j][bin]. A column scan turns the rows into per-chunk offsets and totals, a scan of the
-// totals gives the bin bases, and the scatter ranks each ray within its chunk's bin with LDS
-// atomics again and writes its index to its place. No global atomics. Both passes hold four rays
-// per thread in flight: one 96 KiB block per CU leaves 16 waves to hide the memory latency.
-// (Copying the rays themselves into bin order, so that the trace reads them coalesced, costs the
-// scatter more in scattered 16-byte writes than it saves the trace.)
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) p[i] = v;
+}
+
+// The sort runs in two counting passes over chunks of rays (kRaySortChunk1 in pass 1,
+// kRaySortChunk in pass 2), one 1024-thread block per chunk. Pass 1 orders the rays by direction bin alone (96 bins), pass 2 orders that sequence by the
+// full key (direction bin x origin cell, 49,152 bins). A chunk of pass 1 holds ~340 rays per bin and
+// one of pass 2 only a few direction bins, so the scatters write runs of neighbouring positions;
+// one pass over the full key wrote single 4-byte entries all over the permutation. Per pass: count
+// the chunk's bins with LDS atomics on 16-bit halves (two bins per word; a chunk holds fewer than
+// 2^16 rays), reserve the chunk's run in every non-empty bin with one device atomic on the bin's
+// total (the chunk order inside a bin is whatever order the atomics ran in: results do not depend on
+// it), scan the totals into bin bases, and scatter each ray to base + run + its rank in the chunk
+// (LDS atomics again). Four rays per thread are in flight: a 96 KiB block per CU leaves 16 waves.
 constexpr uint32_t kSortBatch = 4;
 
-__global__ __launch_bounds__(1024) void k_ray_bins(const float4 *__restrict__ rays, uint64_t n, RaySortScratch s) {
-    __shared__ uint32_t cnt[kRaySortBins / 2];
-    for (uint32_t b = threadIdx.x; b < kRaySortBins / 2; b += 1024u) cnt[b] = 0u;
+template <int PASS>
+__global__ __launch_bounds__(1024) void k_ray_count(const float4 *__restrict__ rays, uint64_t n, RaySortScratch s) {
+    constexpr uint32_t kBins = PASS == 1 ? kRayDirBins : kRaySortBins;
+    constexpr uint32_t kChunk = PASS == 1 ? kRaySortChunk1 : kRaySortChunk;
+    __shared__ uint32_t cnt[kBins / 2];
+    for (uint32_t b = threadIdx.x; b < kBins / 2; b += 1024u) cnt[b] = 0u;
     __syncthreads();
-    const uint64_t first = (uint64_t)blockIdx.x * kRaySortChunk;
-    const uint64_t last = min(first + kRaySortChunk, n);
+    const uint64_t first = (uint64_t)blockIdx.x * kChunk;
+    const uint64_t last = min(first + kChunk, n);
     for (uint64_t base = first + threadIdx.x; base < last; base += 1024u * kSortBatch) {
-        float4 a[kSortBatch], b[kSortBatch];
+        uint32_t bin[kSortBatch];
+        if (PASS == 1) {
+            float4 a[kSortBatch], b[kSortBatch];
 #pragma unroll
-        for (uint32_t q = 0; q < kSortBatch; ++q) {
-            const uint64_t i = base + 1024u * q;
-            if (i < last) a[q] = rays[2 * i], b[q] = rays[2 * i + 1];
+            for (uint32_t q = 0; q < kSortBatch; ++q) {
+                const uint64_t i = base + 1024u * q;
+                if (i < last) a[q] = rays[2 * i], b[q] = rays[2 * i + 1];
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kSortBatch; ++q) {
+                const uint64_t i = base + 1024u * q;
+                bin[q] = 0u;
+                if (i < last) {
+                    const uint32_t key =
+                        ray_bin(RayIn{a[q].x, a[q].y, a[q].z, a[q].w, b[q].x, b[q].y, b[q].z, b[q].w}, s);
+                    s.keys[i] = key;
+                    bin[q] = key / kRayCellBins;
+                }
+            }
+        } else {
+#pragma unroll
+            for (uint32_t q = 0; q < kSortBatch; ++q) {
+                const uint64_t i = base + 1024u * q;
+                bin[q] = i < last ? s.keys1[i] : 0u;
+            }
         }
 #pragma unroll
-        for (uint32_t q = 0; q < kSortBatch; ++q) {
-            const uint64_t i = base + 1024u * q;
-            if (i >= last) continue;
-            const uint32_t key = ray_bin(RayIn{a[q].x, a[q].y, a[q].z, a[q].w, b[q].x, b[q].y, b[q].z, b[q].w}, s);
-            s.keys[i] = key;
-            atomicAdd(&cnt[key >> 1], (key & 1u) ? 0x10000u : 1u);
-        }
+        for (uint32_t q = 0; q < kSortBatch; ++q)
+            if (base + 1024u * q < last) atomicAdd(&cnt[bin[q] >> 1], (bin[q] & 1u) ? 0x10000u : 1u);
     }
     __syncthreads();
-    uint32_t *row = s.counts + (size_t)blockIdx.x * kRaySortBins;
-    for (uint32_t b = threadIdx.x; b < kRaySortBins / 2; b += 1024u) {
+    uint32_t *row = PASS == 1 ? s.offs1 + (size_t)blockIdx.x * kRayDirBins : s.offs + (size_t)blockIdx.x * kRaySortBins;
+    uint32_t *total = s.total + (PASS == 1 ? 0u : kRayDirBins);
+    for (uint32_t b = threadIdx.x; b < kBins / 2; b += 1024u) {
         const uint32_t w = cnt[b];
-        row[2 * b] = w & 0xFFFFu;
-        row[2 * b + 1] = w >> 16;
+        if (w & 0xFFFFu) row[2 * b] = atomicAdd(&total[2 * b], w & 0xFFFFu);
+        if (w >> 16) row[2 * b + 1] = atomicAdd(&total[2 * b + 1], w >> 16);
     }
 }
 
-// per bin: counts[j][bin] -> sum of counts[j' < j][bin]; hist[bin] = the bin's total
-__global__ __launch_bounds__(256) void k_ray_bin_columns(RaySortScratch s, uint32_t chunks) {
-    const uint32_t b = blockIdx.x * 256u + threadIdx.x;
-    if (b >= kRaySortBins) return;
-    uint32_t run = 0;
-    uint32_t j = 0;
-    for (; j + 8u <= chunks; j += 8u) {  // eight independent loads in flight per thread
-        uint32_t c[8];
-#pragma unroll
-        for (uint32_t q = 0; q < 8u; ++q) c[q] = s.counts[(size_t)(j + q) * kRaySortBins + b];
-#pragma unroll
-        for (uint32_t q = 0; q < 8u; ++q) {
-            s.counts[(size_t)(j + q) * kRaySortBins + b] = run;
-            run += c[q];
-        }
-    }
-    for (; j < chunks; ++j) {
-        const size_t k = (size_t)j * kRaySortBins + b;
-        const uint32_t c = s.counts[k];
-        s.counts[k] = run;
-        run += c;
-    }
-    s.hist[b] = run;
-}
-
-// exclusive scan of the bin counts in place (one block; each thread owns a contiguous run)
+// exclusive scan of kBins bin totals in place (one block; each thread owns a contiguous run)
+template <uint32_t kBins>
 __global__ __launch_bounds__(1024) void k_ray_bin_scan(uint32_t *hist) {
     __shared__ uint32_t part[1024];
-    constexpr uint32_t per = kRaySortBins / 1024u;
+    constexpr uint32_t per = (kBins + 1023u) / 1024u;
     const uint32_t t = threadIdx.x;
+    const uint32_t b0 = min(t * per, kBins), b1 = min(b0 + per, kBins);
     uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k) sum += hist[t * per + k];
+    for (uint32_t b = b0; b < b1; ++b) sum += hist[b];
     part[t] = sum;
     __syncthreads();
     for (uint32_t off = 1; off < 1024u; off <<= 1) {
@@ -234,40 +235,56 @@ __global__ __launch_bounds__(1024) void k_ray_bin_scan(uint32_t *hist) {
         __syncthreads();
     }
     uint32_t run = part[t] - sum;
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t c = hist[t * per + k];
-        hist[t * per + k] = run;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t c = hist[b];
+        hist[b] = run;
         run += c;
     }
 }
 
+// pass 1: keys1[pos] = key, perm1[pos] = i; pass 2: perm[pos] = perm1[j]
+template <int PASS>
 __global__ __launch_bounds__(1024) void k_ray_scatter(uint64_t n, RaySortScratch s) {
-    __shared__ uint32_t cnt[kRaySortBins / 2];
-    for (uint32_t b = threadIdx.x; b < kRaySortBins / 2; b += 1024u) cnt[b] = 0u;
+    constexpr uint32_t kBins = PASS == 1 ? kRayDirBins : kRaySortBins;
+    constexpr uint32_t kChunk = PASS == 1 ? kRaySortChunk1 : kRaySortChunk;
+    __shared__ uint32_t cnt[kBins / 2];
+    for (uint32_t b = threadIdx.x; b < kBins / 2; b += 1024u) cnt[b] = 0u;
     __syncthreads();
-    const uint64_t first = (uint64_t)blockIdx.x * kRaySortChunk;
-    const uint64_t last = min(first + kRaySortChunk, n);
-    const uint32_t *row = s.counts + (size_t)blockIdx.x * kRaySortBins;
+    const uint64_t first = (uint64_t)blockIdx.x * kChunk;
+    const uint64_t last = min(first + kChunk, n);
+    const uint32_t *row =
+        PASS == 1 ? s.offs1 + (size_t)blockIdx.x * kRayDirBins : s.offs + (size_t)blockIdx.x * kRaySortBins;
+    const uint32_t *bases = s.total + (PASS == 1 ? 0u : kRayDirBins);
     for (uint64_t base = first + threadIdx.x; base < last; base += 1024u * kSortBatch) {
-        uint32_t key[kSortBatch], pos[kSortBatch];
+        uint32_t key[kSortBatch], src[kSortBatch], pos[kSortBatch];
 #pragma unroll
         for (uint32_t q = 0; q < kSortBatch; ++q) {
             const uint64_t i = base + 1024u * q;
-            key[q] = i < last ? s.keys[i] : 0u;
+            key[q] = 0u;
+            src[q] = 0u;
+            if (i < last) {
+                key[q] = PASS == 1 ? s.keys[i] : s.keys1[i];
+                src[q] = PASS == 1 ? (uint32_t)i : s.perm1[i];
+            }
         }
 #pragma unroll
         for (uint32_t q = 0; q < kSortBatch; ++q) {
             pos[q] = 0u;
             if (base + 1024u * q < last) {
-                const uint32_t k = key[q];
-                const uint32_t old = atomicAdd(&cnt[k >> 1], (k & 1u) ? 0x10000u : 1u);
-                pos[q] = s.hist[k] + row[k] + ((k & 1u) ? old >> 16 : old & 0xFFFFu);
+                const uint32_t b = PASS == 1 ? key[q] / kRayCellBins : key[q];
+                const uint32_t old = atomicAdd(&cnt[b >> 1], (b & 1u) ? 0x10000u : 1u);
+                pos[q] = bases[b] + row[b] + ((b & 1u) ? old >> 16 : old & 0xFFFFu);
             }
         }
 #pragma unroll
         for (uint32_t q = 0; q < kSortBatch; ++q) {
-            const uint64_t i = base + 1024u * q;
-            if (i < last) s.perm[pos[q]] = (uint32_t)i;
+            if (base + 1024u * q >= last) continue;
+            if (PASS == 1) {
+                s.keys1[pos[q]] = key[q];
+                s.perm1[pos[q]] = src[q];
+            } else {
+                s.perm[pos[q]] = src[q];
+            }
         }
     }
 }
@@ -354,13 +371,19 @@ hipError_t launch_trace(const TraceVariant &v, const void *rays, uint64_t n, voi
         }
         if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
         const uint32_t *perm = nullptr;
-        if (v.sort) {
-            if (!sort || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+        if (v.sort && n <= 0xFFFFFFFFull) {  // 32-bit ray indices; larger calls trace in input order
+            if (!sort) return hipErrorInvalidValue;
             const uint32_t chunks = (uint32_t)((n + kRaySortChunk - 1) / kRaySortChunk);
-            hipLaunchKernelGGL(k_ray_bins, dim3(chunks), dim3(1024), 0, stream, (const float4 *)rays, n, *sort);
-            hipLaunchKernelGGL(k_ray_bin_columns, dim3(kRaySortBins / 256u), dim3(256), 0, stream, *sort, chunks);
-            hipLaunchKernelGGL(k_ray_bin_scan, dim3(1), dim3(1024), 0, stream, sort->hist);
-            hipLaunchKernelGGL(k_ray_scatter, dim3(chunks), dim3(1024), 0, stream, n, *sort);
+            const uint32_t chunks1 = (uint32_t)((n + kRaySortChunk1 - 1) / kRaySortChunk1);
+            const float4 *r4 = (const float4 *)rays;
+            hipLaunchKernelGGL(k_fill_u32, dim3((kRayDirBins + kRaySortBins + 255u) / 256u), dim3(256), 0, stream,
+                               sort->total, kRayDirBins + kRaySortBins, 0u);
+            hipLaunchKernelGGL(k_ray_count<1>, dim3(chunks1), dim3(1024), 0, stream, r4, n, *sort);
+            hipLaunchKernelGGL(k_ray_bin_scan<kRayDirBins>, dim3(1), dim3(1024), 0, stream, sort->total);
+            hipLaunchKernelGGL(k_ray_scatter<1>, dim3(chunks1), dim3(1024), 0, stream, n, *sort);
+            hipLaunchKernelGGL(k_ray_count<2>, dim3(chunks), dim3(1024), 0, stream, r4, n, *sort);
+            hipLaunchKernelGGL(k_ray_bin_scan<kRaySortBins>, dim3(1), dim3(1024), 0, stream, sort->total + kRayDirBins);
+            hipLaunchKernelGGL(k_ray_scatter<2>, dim3(chunks), dim3(1024), 0, stream, n, *sort);
             perm = sort->perm;
         }
         hipError_t te;
