@@ -957,11 +957,12 @@ int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t 
     if (e != hipSuccess) return hip_fail(ctx, e, "ray sort buffers");
     const hipEvent_t *timed = nullptr;
     if (n && 2 * (size_t)ctx->kernel_timed < ctx->kernel_events.size())
-        timed = &ctx->kernel_events[2 * (size_t)ctx->kernel_timed++];
+        timed = &ctx->kernel_events[2 * (size_t)ctx->kernel_timed];
     e = launch_trace(ds.variant, rays_dev, n, hits_dev, ds.nodes.ptr, ds.tris.ptr, 0u, ctx->invalid_counter(), nullptr,
                      nullptr, ctx->work_counter(), ds.spheres.ptr, ds.num_spheres, ds.num_tris,
                      ds.two_level ? ds.insts.ptr : nullptr, sp, ctx->stream, timed);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    if (timed) ++ctx->kernel_timed;  // a call counts once both of its events are recorded
     return SPK_OK;
 }
 

@@ -317,4 +317,7 @@ def test_kernel_timing_records_each_call(gpu_ctx):
     gpu_ctx.trace_kernel_timing(0)
     assert gpu_ctx.trace(d, hits) == spk.SPK_OK
     assert gpu_ctx.trace_kernel_times() == []
+    with pytest.raises(spk.SpkError) as e:
+        gpu_ctx.trace_kernel_timing(1 << 17)  # at most 65536 timed calls
+    assert e.value.code == spk.SPK_ERR_INVALID_ARGUMENT
     gpu_ctx.set_trace_config(w, stack, block, 0, 0)
