@@ -109,7 +109,7 @@ struct PtArgs {
     uint32_t *path_scene;    // scene of each path (written only when num_scenes > 1)
     uint32_t num_scenes, grid;
     uint32_t refill;  // extend: a wave takes new paths once this many of its lanes are idle (1..64)
-    uint32_t direct;  // 1: the wavefront extends run one queue slot per thread over a full grid
+    uint32_t direct;  // 1: wavefront extends and shadow rays run one per thread over a full grid
     uint32_t tail_from;  // segments from this index on run in k_tail (max_bounce: none)
     uint32_t npix, max_bounce, debug_mode, sky_enabled, flags, sort;
     int32_t sky_faces[6];  // cube-map faces +X, -X, +Y, -Y, +Z, -Z in the texture table, or -1
@@ -843,6 +843,20 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtArgs a, uint32_t bounce) 
     }
 }
 
+// The same over a full grid, one shadow ray per thread (a bounce holds at most one per path, so at
+// most npix): used with the direct extends (schedule refill 0), where the persistent grid of a small
+// blocks_per_cu would leave few waves per SIMD for a latency-bound any-hit traversal.
+template <int WIDTH, int STACK, bool TWO>
+__global__ __launch_bounds__(kPtBlock) void k_shadow_direct(PtArgs a, uint32_t bounce) {
+    __shared__ uint2 stack[STACK * kPtBlock];
+    const uint32_t count = a.counters[bounce * kCtrStride + CTR_SHADOWS];
+    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+    if (i >= count) return;  // no barrier below
+    const float4 s0 = a.shadow[3 * (size_t)i], s1 = a.shadow[3 * (size_t)i + 1];
+    if (shadow_unoccluded<WIDTH, STACK, TWO>(a, s0, s1, stack + threadIdx.x))
+        add_color(a, __float_as_uint(s1.w), a.shadow[3 * (size_t)i + 2]);
+}
+
 // ---- tail: the segments from `from` on, one path per lane until the path ends ----
 // Late segments hold few paths with long incoherent rays, and a bounce-synchronous frame waits for
 // the slowest ray of every bounce; here a lane carries its path through extend, shade and shadow and
@@ -1036,9 +1050,10 @@ using ExtendFn = void (*)(PtArgs, uint32_t);
 struct PtEntry {
     int width, stack;
     bool two;
-    ExtendFn extend, shadow, tail, first;
+    ExtendFn extend, shadow, tail, first, shadow_direct;
 };
-#define SPK_PT(W, S, T) {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_tail<W, S, T>, k_extend_first<W, S, T>}
+#define SPK_PT(W, S, T) \
+    {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_tail<W, S, T>, k_extend_first<W, S, T>, k_shadow_direct<W, S, T>}
 const PtEntry kPtEntries[] = {
     SPK_PT(8, 12, false), SPK_PT(4, 12, false), SPK_PT(8, 16, false), SPK_PT(8, 24, false),
     SPK_PT(8, 32, false), SPK_PT(8, 64, false), SPK_PT(4, 16, false), SPK_PT(4, 24, false),
@@ -1190,8 +1205,13 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
             hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(kPtBlock), 0, s, a, b);
         }
         hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kPtBlock), 0, s, a, b);
-        if ((a.flags & SPK_FRAME_NEE) && a.num_emitters)
-            hipLaunchKernelGGL(pe.shadow, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+        if ((a.flags & SPK_FRAME_NEE) && a.num_emitters) {
+            if (a.direct)
+                hipLaunchKernelGGL(pe.shadow_direct, dim3((a.npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, s,
+                                   a, b);
+            else
+                hipLaunchKernelGGL(pe.shadow, dim3(grid), dim3(kPtBlock), 0, s, a, b);
+        }
     }
     if (a.tail_from < a.max_bounce) hipLaunchKernelGGL(pe.tail, dim3(grid), dim3(kPtBlock), 0, s, a, a.tail_from);
     if (a.cache_on) {
