@@ -276,6 +276,7 @@ __global__ __launch_bounds__(64) void k_cache_mlp(CacheDev c, const float *__res
                                                   const uint32_t *__restrict__ n_dev, int points_are_world,
                                                   float *__restrict__ out, TrainDev td) {
     const uint32_t count = n_dev ? min(*n_dev, n) : n;
+    if (TRAIN && n_dev && count < kMinTrainSamples) return;  // the frame takes no training step
     for (uint32_t tile = blockIdx.x; tile * (uint32_t)kQ < count; tile += gridDim.x) {
         cache_tile<TRAIN>(c, points, count, points_are_world, out, td, tile * (uint32_t)kQ);
         __syncthreads();
@@ -285,7 +286,14 @@ __global__ __launch_bounds__(64) void k_cache_mlp(CacheDev c, const float *__res
 // Weight gradients dW_l[out][in] = sum_q dH_l[out][q] * A_{l-1}[in][q] as a K = n GEMM on MFMA.
 // grid.x = K chunks of 256 queries, grid.y = layer; one wave per (16-row, 16-col) output tile pair set.
 __global__ __launch_bounds__(256) void k_cache_dw(const __bf16 *__restrict__ act_t, const __bf16 *__restrict__ grad_t,
-                                                  uint32_t n_pad, uint32_t n, float *__restrict__ dw) {
+                                                  uint32_t n_pad, uint32_t n, const uint32_t *__restrict__ n_dev,
+                                                  float *__restrict__ dw) {
+    const uint32_t count = n_dev ? min(*n_dev, n) : n;
+    if (n_dev && count < kMinTrainSamples) return;
+    // columns up to the last 64-sample tile were written by the tile kernel (zeros past count)
+    const uint32_t kmax = (count + (uint32_t)kQ - 1u) / (uint32_t)kQ * (uint32_t)kQ;
+    const uint32_t k0 = blockIdx.x * 256u;
+    if (k0 >= kmax) return;
     const int layer = blockIdx.y;
     const int rows = layer == 4 ? 3 : 64;
     const int cols = layer == 0 ? 32 : 64;
@@ -297,16 +305,15 @@ __global__ __launch_bounds__(256) void k_cache_dw(const __bf16 *__restrict__ act
     const int wave = threadIdx.x >> 6;
     const int lane = __lane_id();
     const int r = lane & 15, g = lane >> 4;
-    const uint32_t k0 = blockIdx.x * 256u;
     const int mtiles = (rows + 15) / 16, ntiles = cols / 16;
+    const bf16x8 zero = {};
     for (int tile = wave; tile < mtiles * ntiles; tile += 4) {
         const int mt = tile / ntiles, nt = tile % ntiles;
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
         for (uint32_t ks = 0; ks < 256; ks += 32) {
             const uint32_t kk = k0 + ks + 8 * g;
-            // columns beyond n are zero in both operands (the buffers are cleared up to n_pad)
-            const bf16x8 a = ld8(A + (size_t)(16 * mt + r) * n_pad + kk);
-            const bf16x8 b = ld8(B + (size_t)(16 * nt + r) * n_pad + kk);
+            const bf16x8 a = kk < kmax ? ld8(A + (size_t)(16 * mt + r) * n_pad + kk) : zero;
+            const bf16x8 b = kk < kmax ? ld8(B + (size_t)(16 * nt + r) * n_pad + kk) : zero;
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         }
         // lane holds rows 4g..4g+3 (out) of column r (in) of this tile
@@ -320,11 +327,25 @@ __global__ __launch_bounds__(256) void k_cache_dw(const __bf16 *__restrict__ act
 
 // Adam (Kingma & Ba 2015) over FP32 master parameters, with bias correction; writes the low-precision
 // copies used by the kernels.
+// One thread: decides whether this step runs (enough samples) and advances the step counter.
+__global__ void k_adam_begin(AdamStep *st, const uint32_t *n_dev, uint32_t cap, float b1, float b2) {
+    const uint32_t count = n_dev ? min(*n_dev, cap) : cap;
+    const bool active = !n_dev || count >= kMinTrainSamples;
+    st->active = active ? 1u : 0u;
+    if (active) {
+        const uint32_t step = st->step + 1u;
+        st->step = step;
+        st->c1 = 1.f - powf(b1, (float)step);
+        st->c2 = 1.f - powf(b2, (float)step);
+    }
+}
+
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ grad, float *__restrict__ m,
-                       float *__restrict__ v, uint64_t count, float lr, float b1, float b2, float eps, float c1,
-                       float c2, __half *__restrict__ half_copy) {
+                       float *__restrict__ v, uint64_t count, float lr, float b1, float b2, float eps,
+                       const AdamStep *__restrict__ st, __half *__restrict__ half_copy) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+    if (i >= count || !st->active) return;
+    const float c1 = st->c1, c2 = st->c2;
     const float gi = grad[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -428,43 +449,41 @@ hipError_t cache_infer_queue(CacheState &cs, const float *points, const uint32_t
 }
 
 hipError_t cache_gradients(CacheState &cs, const float *points, const float *targets, uint32_t n, int world,
-                           float *out, hipStream_t s) {
+                           float *out, hipStream_t s, const uint32_t *n_dev) {
     hipError_t e;
     const uint32_t n_pad = ((n + 255u) / 256u) * 256u;
     if ((e = cs.act_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
     if ((e = cs.grad_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(cs.act_t.ptr, 0, (size_t)5 * 64 * n_pad * 2, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(cs.grad_t.ptr, 0, (size_t)5 * 64 * n_pad * 2, s)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(cs.gw.ptr, 0, cache_num_weights() * 4, s)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(cs.gtable.ptr, 0, cs.table_params() * 4, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(cs.loss.ptr, 0, 4, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(&cache_state(cs)->loss, 0, 4, s)) != hipSuccess) return e;
     if (n == 0) return hipSuccess;
     TrainDev td;
     td.targets = targets;
-    td.loss = (float *)cs.loss.ptr;
+    td.loss = &cache_state(cs)->loss;
     td.gtable = (float *)cs.gtable.ptr;
     td.act_t = (__bf16 *)cs.act_t.ptr;
     td.grad_t = (__bf16 *)cs.grad_t.ptr;
     td.n_pad = n_pad;
-    hipLaunchKernelGGL(k_cache_mlp<true>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, n,
-                       (const uint32_t *)nullptr, world, out, td);
+    hipLaunchKernelGGL(k_cache_mlp<true>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, n, n_dev,
+                       world, out, td);
     hipLaunchKernelGGL(k_cache_dw, dim3(n_pad / 256, 5), dim3(256), 0, s, (const __bf16 *)cs.act_t.ptr,
-                       (const __bf16 *)cs.grad_t.ptr, n_pad, n, (float *)cs.gw.ptr);
+                       (const __bf16 *)cs.grad_t.ptr, n_pad, n, n_dev, (float *)cs.gw.ptr);
     return hipGetLastError();
 }
 
-hipError_t cache_adam(CacheState &cs, hipStream_t s) {
-    cs.step += 1;
+hipError_t cache_adam(CacheState &cs, hipStream_t s, const uint32_t *n_dev, uint32_t cap) {
     const float b1 = cs.cfg_beta1, b2 = cs.cfg_beta2, eps = cs.cfg_eps, lr = cs.cfg.learning_rate;
-    const float c1 = 1.f - std::pow(b1, (float)cs.step), c2 = 1.f - std::pow(b2, (float)cs.step);
+    AdamStep *st = cache_state(cs);
+    hipLaunchKernelGGL(k_adam_begin, dim3(1), dim3(1), 0, s, st, n_dev, cap, b1, b2);
     const uint64_t nw = cache_num_weights();
     hipLaunchKernelGGL(k_adam, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, (float *)cs.w.ptr,
-                       (const float *)cs.gw.ptr, (float *)cs.mw.ptr, (float *)cs.vw.ptr, nw, lr, b1, b2, eps, c1, c2,
-                       (__half *)nullptr);
+                       (const float *)cs.gw.ptr, (float *)cs.mw.ptr, (float *)cs.vw.ptr, nw, lr, b1, b2, eps,
+                       (const AdamStep *)st, (__half *)nullptr);
     const uint64_t nt = cs.table_params();
     hipLaunchKernelGGL(k_adam, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, (float *)cs.table_f.ptr,
-                       (const float *)cs.gtable.ptr, (float *)cs.mt.ptr, (float *)cs.vt.ptr, nt, lr, b1, b2, eps, c1,
-                       c2, (__half *)cs.table_h.ptr);
+                       (const float *)cs.gtable.ptr, (float *)cs.mt.ptr, (float *)cs.vt.ptr, nt, lr, b1, b2, eps,
+                       (const AdamStep *)st, (__half *)cs.table_h.ptr);
     hipLaunchKernelGGL(k_pack_weights, dim3(16), dim3(256), 0, s, (const float *)cs.w.ptr, (__bf16 *)cs.wf.ptr,
                        (__bf16 *)cs.wb.ptr);
     return hipGetLastError();
@@ -519,11 +538,11 @@ static int cache_init_params(spk_context *ctx, CacheState &cs) {
         (e = hipMemcpy(cs.w.ptr, w.data(), nw * 4, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(cs.mt.ptr, 0, ntab * 4)) != hipSuccess || (e = hipMemset(cs.vt.ptr, 0, ntab * 4)) != hipSuccess ||
         (e = hipMemset(cs.mw.ptr, 0, nw * 4)) != hipSuccess || (e = hipMemset(cs.vw.ptr, 0, nw * 4)) != hipSuccess ||
-        (e = hipMemset(cs.wf.ptr, 0, 16384 * 2)) != hipSuccess || (e = hipMemset(cs.wb.ptr, 0, 16384 * 2)) != hipSuccess)
+        (e = hipMemset(cs.wf.ptr, 0, 16384 * 2)) != hipSuccess || (e = hipMemset(cs.wb.ptr, 0, 16384 * 2)) != hipSuccess ||
+        (e = hipMemset(cs.state.ptr, 0, sizeof(AdamStep))) != hipSuccess)
         return hip_fail(ctx, e, "cache initialisation");
     if ((e = cache_pack(cs, ctx->stream)) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
         return hip_fail(ctx, e, "cache pack");
-    cs.step = 0;
     return SPK_OK;
 }
 
@@ -572,7 +591,7 @@ int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
         (e = cs.gtable.ensure(ntab * 4)) != hipSuccess || (e = cs.w.ensure(nw * 4)) != hipSuccess ||
         (e = cs.mw.ensure(nw * 4)) != hipSuccess || (e = cs.vw.ensure(nw * 4)) != hipSuccess ||
         (e = cs.gw.ensure(nw * 4)) != hipSuccess || (e = cs.wf.ensure(16384 * 2)) != hipSuccess ||
-        (e = cs.wb.ensure(16384 * 2)) != hipSuccess || (e = cs.loss.ensure(16)) != hipSuccess)
+        (e = cs.wb.ensure(16384 * 2)) != hipSuccess || (e = cs.state.ensure(sizeof(AdamStep))) != hipSuccess)
         return hip_fail(ctx, e, "cache allocation");
     int rc = cache_init_params(ctx, cs);
     if (rc != SPK_OK) return rc;
@@ -616,7 +635,7 @@ int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *
     CacheState &cs = *ctx->cache;
     hipError_t e = cache_gradients(cs, points_dev, targets_dev, n, world, nullptr, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess && loss) e = hipMemcpy(loss, cs.loss.ptr, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && loss) e = hipMemcpy(loss, &cache_state(cs)->loss, 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && dweights) e = hipMemcpy(dweights, cs.gw.ptr, cache_num_weights() * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && dtable) e = hipMemcpy(dtable, cs.gtable.ptr, cs.table_params() * 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(ctx, e, "cache gradients");
@@ -634,7 +653,7 @@ int spk_cache_train(spk_context *ctx, const float *points_dev, const float *targ
     if (e == hipSuccess) e = cache_adam(cs, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     float l = 0.f;
-    if (e == hipSuccess) e = hipMemcpy(&l, cs.loss.ptr, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&l, &cache_state(*ctx->cache)->loss, 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(ctx, e, "cache training step");
     ctx->cache_last_loss = l;
     if (loss) *loss = l;
@@ -671,7 +690,13 @@ int spk_cache_info(spk_context *ctx, uint64_t *num_weights, uint64_t *num_table_
     if (rc != SPK_OK) return rc;
     if (num_weights) *num_weights = cache_num_weights();
     if (num_table_params) *num_table_params = ctx->cache->table_params();
-    if (steps) *steps = ctx->cache->step;
+    if (steps) {
+        AdamStep st{};
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipMemcpy(&st, ctx->cache->state.ptr, sizeof(st), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(ctx, e, "cache state");
+        *steps = st.step;
+    }
     if (last_loss) *last_loss = ctx->cache_last_loss;
     return SPK_OK;
 }

@@ -1437,28 +1437,24 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
         e = hipGraphLaunch(ps.graph, ctx->stream);
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
-    // online training of the radiance cache on this frame's samples
-    uint32_t trained = 0;
-    if (cache_on && frame->max_bounce > 0) {
-        uint32_t cnt = 0;
-        if ((e = hipMemcpyAsync(&cnt, (uint32_t *)ps.counters.ptr + kCtrGlobal + GCTR_TRAIN, 4, hipMemcpyDeviceToHost,
-                                ctx->stream)) != hipSuccess ||
-            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-            return hip_fail(ctx, e, "training sample count");
+    // online training of the radiance cache on this frame's samples, enqueued behind the frame: the
+    // sample count stays on the device (frames with fewer than kMinTrainSamples take no step)
+    const bool train = cache_on && frame->max_bounce > 0;
+    if (train) {
         CacheState &cs = *ctx->cache;
-        trained = std::min(cnt, cs.cfg.batch);
-        if (trained >= 64) {
-            if ((e = cache_gradients(cs, (const float *)cs.tr_points.ptr, (const float *)cs.tr_targets.ptr, trained, 1,
-                                     nullptr, ctx->stream)) != hipSuccess ||
-                (e = cache_adam(cs, ctx->stream)) != hipSuccess)
-                return hip_fail(ctx, e, "cache training");
-        }
+        const uint32_t *cnt_dev = (const uint32_t *)ps.counters.ptr + kCtrGlobal + GCTR_TRAIN;
+        if ((e = cache_gradients(cs, (const float *)cs.tr_points.ptr, (const float *)cs.tr_targets.ptr, cs.cfg.batch,
+                                 1, nullptr, ctx->stream, cnt_dev)) != hipSuccess ||
+            (e = cache_adam(cs, ctx->stream, cnt_dev, cs.cfg.batch)) != hipSuccess)
+            return hip_fail(ctx, e, "cache training");
     }
     if (stats) (void)hipEventRecord(t1, ctx->stream);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "frame");
-    if (cache_on && trained >= 64) {
-        float l = 0.f;
-        if ((e = hipMemcpy(&l, ctx->cache->loss.ptr, 4, hipMemcpyDeviceToHost)) == hipSuccess) ctx->cache_last_loss = l;
+    if (train) {
+        AdamStep st{};
+        if ((e = hipMemcpy(&st, ctx->cache->state.ptr, sizeof(st), hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(ctx, e, "cache state");
+        if (st.active) ctx->cache_last_loss = st.loss;
     }
     if (stats) {
         float ms = 0.f;

@@ -10,6 +10,16 @@
 
 namespace spk {
 
+// Optimiser state kept on the device, so that a training step can follow a frame without a host
+// round trip: the frame's sample count stays in device memory and decides there whether the step runs.
+struct AdamStep {
+    uint32_t step;    // Adam steps taken
+    uint32_t active;  // 1 when the current launch's step runs
+    float c1, c2;     // bias corrections 1 - beta^step of the current step
+    float loss;       // loss of the last gradient pass
+};
+constexpr uint32_t kMinTrainSamples = 64;  // a frame with fewer training samples takes no step
+
 struct CacheState {
     spk_cache_config cfg{};
     float cfg_beta1 = 0.9f, cfg_beta2 = 0.99f, cfg_eps = 1e-8f;
@@ -18,21 +28,23 @@ struct CacheState {
     float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};
     DeviceBuffer table_h, table_f, mt, vt, gtable;  // hash table (FP16 copy, FP32 master, Adam, gradient)
     DeviceBuffer w, mw, vw, gw, wf, wb;             // MLP weights (FP32 master, Adam, gradient, BF16 copies)
-    DeviceBuffer act_t, grad_t, loss;               // training scratch
+    DeviceBuffer act_t, grad_t, state;              // training scratch, AdamStep
     // per-frame records of the path tracer
     DeviceBuffer cq_points, cq_meta, cq_out;        // cache queries: point, (throughput, path), radiance
     DeviceBuffer tr_points, tr_meta0, tr_meta1, tr_targets;  // training samples
     uint32_t capacity = 0;
-    uint32_t step = 0;
     uint64_t table_params() const { return (uint64_t)cfg.levels * (1ull << cfg.log2_table) * cfg.features; }
 };
 
 size_t cache_num_weights();
 hipError_t cache_pack(CacheState &cs, hipStream_t s);
 hipError_t cache_infer(CacheState &cs, const float *points, uint32_t n, int world, float *out, hipStream_t s);
+// n_dev (device, may be null): the sample count, capped at n; with it, fewer than kMinTrainSamples
+// samples make the gradient pass and the following cache_adam (given the same n_dev) no-ops.
 hipError_t cache_gradients(CacheState &cs, const float *points, const float *targets, uint32_t n, int world,
-                           float *out, hipStream_t s);
-hipError_t cache_adam(CacheState &cs, hipStream_t s);
+                           float *out, hipStream_t s, const uint32_t *n_dev = nullptr);
+hipError_t cache_adam(CacheState &cs, hipStream_t s, const uint32_t *n_dev = nullptr, uint32_t cap = 0);
+inline AdamStep *cache_state(CacheState &cs) { return static_cast<AdamStep *>(cs.state.ptr); }
 // Inference over a queue whose length lives in device memory (capturable in a graph): `grid`
 // persistent single-wave blocks walk the 64-query tiles.
 hipError_t cache_infer_queue(CacheState &cs, const float *points, const uint32_t *count_dev, uint32_t capacity,
