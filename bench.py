@@ -112,8 +112,9 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     cache1_nee   cache from bounce 1 + next-event estimation
     For each budget b*T every estimator renders frames until b*T has passed (plain at b = 1 renders
     exactly spp frames); the cache estimators start untrained (reset after their graph-capture frame)
-    and accumulate only the frames after warm_frac of the budget. Each estimator runs on the launch
-    schedule spk_render_autotune picks for it (tuned once, before any timed region). The reference is ref_spp frames of
+    and accumulate only the frames after warm_frac of the budget. Each estimator runs on the faster of
+    the launch schedule spk_render_autotune picks for it and the default schedule (median over 8
+    frames each, before any timed region). The reference is ref_spp frames of
     the plain tracer with next-event estimation (same expectation as the reference's estimator, about
     3x lower variance per frame, so the reference's own noise adds less to every relMSE), seeds from
     1,000,000 on, summed in float64 on the device. All frames use the 64-bit seed composition: at
@@ -144,11 +145,24 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
 
     variants = [("plain", 0, None), ("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
                 ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
-    schedules = {}
+    def steady_ms(flags, sched, frames=8):
+        """Median frame time of a schedule over several frame seeds (after a capture frame)."""
+        ctx.set_schedule(*sched)
+        ctx.render_frame(frame(999_980, flags), rgba)
+        return float(np.median([ctx.render_frame(frame(999_981 + k, flags), rgba).milliseconds
+                                for k in range(frames)]))
+
+    # each estimator on the faster (median over 8 frames) of its autotuned schedule and the default
+    # schedule: a single tuning pass can pick a schedule several percent off, which would shift the
+    # equal-time budget between estimators
+    default_schedule = (8, 0, 0, 1)
+    schedules, schedule_ms = {}, {}
     for name, flags, bounce in variants:
         if bounce is not None:
             ctx.cache_create(spk.cache_config(terminate_bounce=bounce))
-        schedules[name], _ = ctx.autotune_render(frame(999_990, flags), rgba)
+        tuned, _ = ctx.autotune_render(frame(999_990, flags), rgba)
+        timed = sorted((steady_ms(flags, sc), sc) for sc in {tuned, default_schedule})
+        schedule_ms[name], schedules[name] = round(timed[0][0], 3), timed[0][1]
         if bounce is not None:
             ctx.cache_destroy()
     log(f"relMSE experiment: schedules {schedules}")
@@ -241,6 +255,7 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
         "budgets": by_budget,
         "schedules": {k: dict(zip(("blocks_per_cu", "material_sort", "refill", "tail_bounce"), v))
                       for k, v in schedules.items()},
+        "schedule_median_frame_ms": schedule_ms,
         "full_path_nee_samples_per_s": round(npx * ref_spp / t_ref, 1),
         "full_path_nee_mrays_per_s": round(rays_ref / t_ref / 1e6, 1),
         "bvh_build_s": round(t_build, 2),

@@ -15,6 +15,7 @@
 // frame seed) travel through a small device buffer written before each replay.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -1687,17 +1688,20 @@ int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule) 
 }
 
 // Times every candidate schedule on one frame (a capture frame first, then a timed one).
-// Frame time of the current schedule: one frame to capture its graph, then the fastest of three
-// (a single frame varies by a few percent, as much as neighbouring schedules differ).
+// Frame time of the current schedule: one frame to capture its graph, then the median of five
+// (single frames vary by a few percent, as much as neighbouring schedules differ; the fastest of a
+// few frames rewarded lucky frames and picked schedules slower than the default on the median).
 static int time_schedule(spk_context *ctx, int batch, const spk_frame_desc *frame, float *rgba_dev, double &ms) {
-    ms = 1e30;
-    for (int k = 0; k < 4; ++k) {
+    double t[5];
+    for (int k = 0; k < 6; ++k) {
         spk_frame_stats st{};
         const int rc = batch ? spk_batch_render(ctx, frame, rgba_dev, k ? &st : nullptr)
                              : spk_render_frame(ctx, frame, rgba_dev, nullptr, k ? &st : nullptr);
         if (rc != SPK_OK) return rc;
-        if (k) ms = std::min(ms, (double)st.milliseconds);
+        if (k) t[k - 1] = st.milliseconds;
     }
+    std::sort(t, t + 5);
+    ms = t[2];
     return SPK_OK;
 }
 
