@@ -13,6 +13,8 @@ Reference interfaces mirrored (paths relative to the Sparkle repository):
 import ctypes
 import os
 
+import math
+
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -648,3 +650,59 @@ def camera_desc(position, front, up_hint=(0.0, 0.0, 1.0), vertical_fov=None, foc
     c.lens_radius = float(aperture_radius)
     c.far_plane = float(far_plane)
     return c
+
+
+def camera_from_basis(position, right, front, up, focal_length, sensor_height, aperture, focus_distance, aspect,
+                      far_plane=1000.0):
+    """The render camera of a camera node (CameraComponent::UpdateRenderData and the focus plane of
+    CameraRenderProxy.cpp:24-27, 113-136) from its position, its (right, front, up) basis and its
+    physical attributes (metres, f-number), in float32."""
+    f32 = np.float32
+    fov = f32(2.0) * f32(math.atan(f32(sensor_height) / (f32(2.0) * f32(focal_length))))
+    lens_radius = f32(focal_length) / f32(aperture) * f32(0.5)
+    h = f32(math.tan(fov / f32(2.0)))
+    height = f32(2.0) * h * f32(focus_distance)
+    width = f32(aspect) * height
+    P, R, F, U = (np.asarray(v, f32) for v in (position, right, front, up))
+    mu = R * width
+    mv = U * height
+    ll = P + F * f32(focus_distance) - mu * f32(0.5) - mv * f32(0.5)
+    cd = CameraDesc()
+    cd.position[:] = [float(x) for x in P]
+    cd.right[:] = [float(x) for x in R]
+    cd.up[:] = [float(x) for x in U]
+    cd.front[:] = [float(x) for x in F]
+    cd.lower_left[:] = [float(x) for x in ll]
+    cd.max_u[:] = [float(x) for x in mu]
+    cd.max_v[:] = [float(x) for x in mv]
+    cd.lens_radius = float(lens_radius)
+    cd.far_plane = float(far_plane)
+    return cd
+
+
+def orbit_basis(center, radius, pitch, yaw):
+    """Position and (right, front, up) of an orbit camera (OrbitCameraComponent::Setup and
+    UpdateTransform, OrbitCameraComponent.cpp:16-30): the node rotation is
+    AngleAxis(pitch, Right) * AngleAxis(yaw, Up) (EulerRotationToRotationAxis of (pitch, 0, yaw) in
+    radians, Utilities.h:392-396, with Right = +X, Front = +Y, Up = +Z), the camera basis is its rows
+    (Transform::ExtractLocalBasis), and the camera sits `radius` behind `center` along its front.
+    Angles in degrees; positive pitch looks down."""
+    p, y = math.radians(pitch), math.radians(yaw)
+    cp, sp, cy, sy = math.cos(p), math.sin(p), math.cos(y), math.sin(y)
+    rx = np.array([[1.0, 0.0, 0.0], [0.0, cp, -sp], [0.0, sp, cp]])
+    rz = np.array([[cy, -sy, 0.0], [sy, cy, 0.0], [0.0, 0.0, 1.0]])
+    rot = rx @ rz
+    right, front, up = rot[0], rot[1], rot[2]
+    position = np.asarray(center, np.float64) - float(radius) * front
+    return position, right, front, up
+
+
+def orbit_camera(center=(0.0, 0.0, 0.0), radius=10.0, pitch=0.0, yaw=0.0, aspect=1.0, focal_length=0.035,
+                 sensor_height=0.024, aperture=22.0, far_plane=1000.0):
+    """Render camera of an orbit camera focused on its centre (SetFocusDistance(radius)), with the
+    CameraComponent defaults (35 mm, 24 mm sensor, f/22; CameraComponent.h:18-22). The defaults of
+    this function are the camera every loaded scene starts with (SceneManager.cpp:142-147:
+    Setup(Zeros, 10, 0, 0): at (0, -10, 0) looking along +Y)."""
+    position, right, front, up = orbit_basis(center, radius, pitch, yaw)
+    return camera_from_basis(position, right, front, up, focal_length, sensor_height, aperture, radius, aspect,
+                             far_plane)

@@ -45,7 +45,7 @@ import re
 import numpy as np
 
 from . import (MATERIAL_DIELECTRIC, MATERIAL_PBR, TEXTURE_RGBA32F, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA8_UNORM,
-               CameraDesc, Scene)
+               Scene, camera_from_basis)
 from .gltf import _png_decode, _png_rgba
 
 
@@ -414,31 +414,6 @@ class UsdScene:
         c = self.camera
         return camera_from_basis(c["position"], c["right"], c["front"], c["up"], c["focal_length"],
                                  c["sensor_height"], c["aperture"], c["focus_distance"], aspect, far_plane)
-
-
-def camera_from_basis(position, right, front, up, focal_length, sensor_height, aperture, focus_distance, aspect,
-                      far_plane=1000.0):
-    f32 = np.float32
-    fov = f32(2.0) * f32(math.atan(f32(sensor_height) / (f32(2.0) * f32(focal_length))))
-    lens_radius = f32(focal_length) / f32(aperture) * f32(0.5)
-    h = f32(math.tan(fov / f32(2.0)))
-    height = f32(2.0) * h * f32(focus_distance)
-    width = f32(aspect) * height
-    P, R, F, U = (np.asarray(v, f32) for v in (position, right, front, up))
-    mu = R * width
-    mv = U * height
-    ll = P + F * f32(focus_distance) - mu * f32(0.5) - mv * f32(0.5)
-    cd = CameraDesc()
-    cd.position[:] = [float(x) for x in P]
-    cd.right[:] = [float(x) for x in R]
-    cd.up[:] = [float(x) for x in U]
-    cd.front[:] = [float(x) for x in F]
-    cd.lower_left[:] = [float(x) for x in ll]
-    cd.max_u[:] = [float(x) for x in mu]
-    cd.max_v[:] = [float(x) for x in mv]
-    cd.lens_radius = float(lens_radius)
-    cd.far_plane = float(far_plane)
-    return cd
 
 
 def _major_axis(n):
