@@ -1908,6 +1908,43 @@ void orcl_adam(double *param, const double *grad, double *m, double *v, uint64_t
 
 }  // extern "C"
 
+// n samples of one lobe in the local frame (normal +z), all from one xoshiro stream seeded with `seed`:
+// kind 0 Lambertian, 1 specular (GGX VNDF, metallic / Fresnel choice), 2 dielectric, 3 the VNDF
+// microfacet normal alone. wi: n x 3; thr: n x 3 (may be null); valid: n (may be null).
+extern "C" void orcl_bxdf_sample(int kind, const float *wo, const float *base, float roughness, float metallic,
+                                 float eta, uint64_t seed, uint32_t n, float *wi, float *thr, int32_t *valid) {
+    orcl::Rng rng(seed);
+    const orcl::V3 o(wo[0], wo[1], wo[2]), b(base[0], base[1], base[2]);
+    for (uint32_t k = 0; k < n; ++k) {
+        orcl::Sample s;
+        if (kind == 0) {
+            s = orcl::lambert_sample(rng, b);
+        } else if (kind == 1) {
+            s = orcl::specular_sample(rng, o, b, roughness, metallic);
+        } else if (kind == 2) {
+            s = orcl::dielectric_sample(rng, o, b, eta);
+        } else {
+            s.valid = true;
+            s.wi = orcl::sample_vndf(rng, o, roughness);
+            s.throughput = b;
+        }
+        wi[3 * (size_t)k] = s.wi.x, wi[3 * (size_t)k + 1] = s.wi.y, wi[3 * (size_t)k + 2] = s.wi.z;
+        if (thr) thr[3 * (size_t)k] = s.throughput.x, thr[3 * (size_t)k + 1] = s.throughput.y,
+                 thr[3 * (size_t)k + 2] = s.throughput.z;
+        if (valid) valid[k] = s.valid ? 1 : 0;
+    }
+}
+
+// The shading frame SampleSurface builds (u, v, w) from a normal and a tangent
+extern "C" void orcl_local_axes(const float *normal, const float *tangent, float *u, float *v, float *w) {
+    orcl::V3 a, b, c;
+    orcl::local_axes(orcl::V3(normal[0], normal[1], normal[2]), orcl::V3(tangent[0], tangent[1], tangent[2]), a, b,
+                     c);
+    u[0] = a.x, u[1] = a.y, u[2] = a.z;
+    v[0] = b.x, v[1] = b.y, v[2] = b.z;
+    w[0] = c.x, w[1] = c.y, w[2] = c.z;
+}
+
 extern "C" void orcl_texture_sample(const orcl_scene *scene, int index, float u, float v, float *out) {
     const orcl::V3 r = orcl::texture_sample(scene->textures[index], u, v);
     out[0] = r.x;

@@ -175,6 +175,12 @@ float orcl_refine_t(const float *org, const float *dir, const float *v0, const f
 /* RNG */
 void orcl_xoshiro_stream(uint64_t seed, uint32_t *out, uint32_t n);
 void orcl_random_unit_stream(uint64_t seed, float *out, uint32_t n);
+/* n samples of one BxDF lobe in the local frame (normal +z) from one xoshiro stream: kind 0
+   Lambertian, 1 specular, 2 dielectric, 3 VNDF microfacet normal. thr / valid may be NULL. */
+void orcl_bxdf_sample(int kind, const float *wo, const float *base, float roughness, float metallic, float eta,
+                      uint64_t seed, uint32_t n, float *wi, float *thr, int32_t *valid);
+/* SampleSurface's shading frame (u, v, w) from a normal and a tangent. */
+void orcl_local_axes(const float *normal, const float *tangent, float *u, float *v, float *w);
 
 /* One frame of BasePass: rgba = (color clamped to 6, valid flag), normal = first-hit normal. */
 int orcl_render_frame(const orcl_scene *scene, const orcl_camera *camera, const orcl_frame *frame, int num_threads,

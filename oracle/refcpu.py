@@ -99,6 +99,8 @@ def lib():
             "orcl_refine_t": (F, [P, P, P, P, P]),
             "orcl_xoshiro_stream": (None, [U64, P, U32]),
             "orcl_random_unit_stream": (None, [U64, P, U32]),
+            "orcl_bxdf_sample": (None, [I, P, P, F, F, F, U64, U32, P, P, P]),
+            "orcl_local_axes": (None, [P, P, P, P, P]),
             "orcl_render_frame": (I, [P, P, P, I, P, P]),
             "orcl_accumulate": (None, [P, P, U64, U32, U32]),
             "orcl_spatial_fill": (None, [P, P, P, U32, U32]),
@@ -289,6 +291,29 @@ def random_unit_stream(seed, n):
     out = np.zeros(n, dtype=np.float32)
     lib().orcl_random_unit_stream(seed, out.ctypes.data, n)
     return out
+
+
+BXDF_LAMBERT, BXDF_SPECULAR, BXDF_DIELECTRIC, BXDF_VNDF = 0, 1, 2, 3
+
+
+def bxdf_sample(kind, wo, n, seed=1, base=(1.0, 1.0, 1.0), roughness=0.5, metallic=0.0, eta=1.5):
+    """n samples of one lobe in the local frame: (wi [n, 3], throughput [n, 3], valid [n] bool)."""
+    wo = np.ascontiguousarray(wo, dtype=np.float32)
+    base = np.ascontiguousarray(base, dtype=np.float32)
+    wi = np.zeros((n, 3), np.float32)
+    thr = np.zeros((n, 3), np.float32)
+    valid = np.zeros(n, np.int32)
+    lib().orcl_bxdf_sample(kind, wo.ctypes.data, base.ctypes.data, roughness, metallic, eta, seed, n, wi.ctypes.data,
+                           thr.ctypes.data, valid.ctypes.data)
+    return wi, thr, valid.astype(bool)
+
+
+def local_axes(normal, tangent):
+    normal = np.ascontiguousarray(normal, dtype=np.float32)
+    tangent = np.ascontiguousarray(tangent, dtype=np.float32)
+    u, v, w = (np.zeros(3, np.float32) for _ in range(3))
+    lib().orcl_local_axes(normal.ctypes.data, tangent.ctypes.data, u.ctypes.data, v.ctypes.data, w.ctypes.data)
+    return u, v, w
 
 
 # ---- learned radiance cache (FP64) ----------------------------------------------------------
