@@ -595,4 +595,16 @@ __device__ __forceinline__ float final_t(const uint4 *__restrict__ tris, const R
     return refine_t(ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, fa, fb, fc, best.t);
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8), each XCD with its own L2.
+// Within every window of 64 consecutive workgroups, XCD x runs 8 contiguous ones, so neighbouring
+// rays of the coherence order share an L2, while all XCDs still work inside the same window at the
+// same time (and share one working set in the Infinity Cache). A bijection on [0, nb). Measured on
+// 8M sorted incoherent rays: 1.89 ms against 1.90-1.91 for the plain order; a window of 512 gave
+// nothing, and one contiguous eighth of the grid per XCD was 18% slower.
+__device__ __forceinline__ uint32_t xcd_group_block(uint32_t b, uint32_t nb) {
+    const uint32_t base = b & ~63u, m = min(64u, nb - base), l = b - base;
+    const uint32_t x = l & 7u, s = l >> 3, q = m >> 3, r = m & 7u;
+    return base + x * q + min(x, r) + s;
+}
+
 }  // namespace spk
