@@ -359,7 +359,7 @@ int spk_get_schedule(spk_context *ctx, int batch, spk_schedule *out);
 int spk_set_schedule(spk_context *ctx, int batch, const spk_schedule *schedule);
 /* Schedule autotuners: time blocks_per_cu {2, 4, 8, 16} x material_sort {0, 1} x refill {0, 16, 64} on
    the median of five frames (after a capture frame per candidate), then tail_bounce {1, 2, 3, 4, 5} (below max_bounce)
-   on the fastest of those; keep and return the fastest. */
+   on the three fastest of those; keep and return the fastest. */
 int spk_render_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,
                         double *best_ms);
 int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, spk_schedule *chosen,

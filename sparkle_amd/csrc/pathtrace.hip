@@ -20,6 +20,7 @@
 #include <cstddef>
 #include <cstring>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "../../include/spk_pt.h"
@@ -1713,6 +1714,7 @@ static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *
     static const uint32_t kRefill[] = {0u, 16u, 64u};
     double best = 1e30;
     spk_schedule best_s = *target;
+    std::vector<std::pair<double, spk_schedule>> base;
     for (uint32_t srt = 0; srt < 2; ++srt) {
         for (uint32_t b : kBlocks) {
             for (uint32_t r : kRefill) {
@@ -1720,6 +1722,7 @@ static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *
                 double ms = 0.0;
                 const int rc = time_schedule(ctx, batch, frame, rgba_dev, ms);
                 if (rc != SPK_OK) return rc;
+                base.emplace_back(ms, *target);
                 if (ms < best) {
                     best = ms;
                     best_s = *target;
@@ -1727,18 +1730,25 @@ static int autotune_schedule(spk_context *ctx, int batch, const spk_frame_desc *
             }
         }
     }
-    // then the first segment of the per-path tail kernel, on the best grid / sort / refill
+    // then the first segment of the per-path tail kernel, on the three fastest grid / sort / refill
+    // choices: the grid that is best without the tail is not always best with it
+    std::sort(base.begin(), base.end(),
+              [](const std::pair<double, spk_schedule> &x, const std::pair<double, spk_schedule> &y) {
+                  return x.first < y.first;
+              });
     static const uint32_t kTail[] = {1u, 2u, 3u, 4u, 5u};
-    const spk_schedule base = best_s;
-    for (uint32_t t : kTail) {
-        if (t >= frame->max_bounce) continue;
-        *target = spk_schedule{base.blocks_per_cu, base.material_sort, base.refill, t};
-        double ms = 0.0;
-        const int rc = time_schedule(ctx, batch, frame, rgba_dev, ms);
-        if (rc != SPK_OK) return rc;
-        if (ms < best) {
-            best = ms;
-            best_s = *target;
+    for (size_t c = 0; c < std::min<size_t>(3, base.size()); ++c) {
+        const spk_schedule bs = base[c].second;
+        for (uint32_t t : kTail) {
+            if (t >= frame->max_bounce) continue;
+            *target = spk_schedule{bs.blocks_per_cu, bs.material_sort, bs.refill, t};
+            double ms = 0.0;
+            const int rc = time_schedule(ctx, batch, frame, rgba_dev, ms);
+            if (rc != SPK_OK) return rc;
+            if (ms < best) {
+                best = ms;
+                best_s = *target;
+            }
         }
     }
     *target = best_s;
