@@ -321,3 +321,26 @@ def test_kernel_timing_records_each_call(gpu_ctx):
         gpu_ctx.trace_kernel_timing(1 << 17)  # at most 65536 timed calls
     assert e.value.code == spk.SPK_ERR_INVALID_ARGUMENT
     gpu_ctx.set_trace_config(w, stack, block, 0, 0)
+
+
+def test_host_buffer_trace_matches_device_buffers(gpu_ctx):
+    """spk_trace_closest_host (rays and hits in host memory, copied in and out) gives the device-buffer
+    call's hits bit for bit, in input and coherence order, reports invalid rays the same way, and
+    treats zero rays as a no-op."""
+    sc = scenes.heightfield_scene(cells=128)
+    lo, hi = scenes.scene_aabb(sc)
+    rays = scenes.incoherent_rays(lo, hi, n=70_001, seed=41)
+    rays["dir"][12] = 0.0
+    gpu_ctx.upload_scene(sc)
+    w, stack, block = gpu_ctx.trace_config()[:3]
+    for ray_sort in (0, 1):
+        print_case("host buffers", ray_sort)
+        gpu_ctx.set_trace_config(w, stack, block, 0, ray_sort)
+        rc_d, h_d = gpu_trace(gpu_ctx, rays)
+        rc_h, h_h = gpu_ctx.trace_host(rays)
+        assert rc_d == rc_h == spk.SPK_ERR_INVALID_RAY
+        np.testing.assert_array_equal(h_h["prim"], h_d["prim"])
+        np.testing.assert_array_equal(h_h["t"].view(np.uint32), h_d["t"].view(np.uint32))
+    rc, h = gpu_ctx.trace_host(rays[:0])
+    assert rc == spk.SPK_OK and h.shape[0] == 0
+    gpu_ctx.set_trace_config(w, stack, block, 0, 0)
