@@ -24,6 +24,16 @@ def log(*a):
 
 
 def cpu_model():
+    """The host CPU's model name as lscpu prints it (falls back to /proc/cpuinfo)."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.strip().startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -33,14 +43,20 @@ def cpu_model():
     return "unknown"
 
 
+# Host cores one GPU's job may use on the benchmark machines: the CPU share of a one-GPU slot.
+HOST_CORE_SHARE = 16
+
+
 def cpu_baseline(scene, rays, seconds_target=2.0, runs=5):
     """The oracle's restatement of the reference CPU path (two-level BVH without tmax culling,
     Moller-Trumbore, one task per chunk over the host threads), timed on a bounded sample: median of
-    `runs` timed runs after a warm-up, each about `seconds_target` long, on all threads of the
-    process's share (at most 16), and on one thread. BVH build and upload are excluded."""
+    `runs` timed runs after a warm-up, each about `seconds_target` long, on every core of the
+    process's CPU share (the affinity mask, capped at the one-GPU slot's share of 16 cores), and on
+    one thread. BVH build and upload are excluded."""
     from oracle import refcpu
 
-    threads = min(os.cpu_count() or 1, 16)
+    affinity = len(os.sched_getaffinity(0))
+    threads = min(affinity, HOST_CORE_SHARE)
     o = refcpu.OracleScene(scene)
 
     def rate(nthreads, mode, budget):
@@ -60,14 +76,56 @@ def cpu_baseline(scene, rays, seconds_target=2.0, runs=5):
     mt_rate, n = rate(threads, refcpu.TRACE_REF_MT, seconds_target)
     mt1_rate, n1 = rate(1, refcpu.TRACE_REF_MT, seconds_target / 2)
     wt_rate, nw = rate(threads, refcpu.TRACE_WT, seconds_target / 2)
+    model = cpu_model()
     return {"value": round(mt_rate, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the 8,388,608 incoherent rays, reference Moller-Trumbore two-level BVH "
-                      f"without tmax culling, median of {runs} runs on {threads} threads ({cpu_model()}); "
+                      f"without tmax culling, median of {runs} runs on {threads} threads ({model}; affinity "
+                      f"mask {affinity} CPUs, capped at the one-GPU slot's share of {HOST_CORE_SHARE}); "
                       f"1 thread: {mt1_rate:.3f} Mrays/s on {n1} rays; watertight FP32 variant "
                       f"{wt_rate:.3f} Mrays/s on {nw} rays",
             "one_thread_value": round(mt1_rate, 3),
             "watertight_value": round(wt_rate, 3),
-            "cpu_model": cpu_model()}
+            "affinity_cpus": affinity,
+            "cpu_model": model}
+
+
+def oracle_check(scene, rays, hits, stride=16, fp64_stride=64):
+    """Checks the timed call's hits against the oracle: every `stride`-th ray bit for bit against the
+    watertight twin, every `fp64_stride`-th against FP64 (ids on unflagged rays, t within 1e-6
+    relative), and counts the sampled rays on which the reference's Moller-Trumbore test with its
+    |det| < 1e-6 cutoff (MeshRenderProxy.cpp:143-150) reports a different closest hit. Raises if the
+    device differs from the twin or from FP64."""
+    from oracle import refcpu
+
+    miss = 0xFFFFFFFF
+    o = refcpu.OracleScene(scene)
+    threads = min(len(os.sched_getaffinity(0)), HOST_CORE_SHARE)
+    sub = np.arange(0, rays.shape[0], stride)
+    wt = o.trace(rays[sub], refcpu.TRACE_WT, threads)
+    same = (hits["prim"][sub] == wt["prim"]) & (hits["t"][sub].view(np.uint32) == wt["t"].view(np.uint32))
+    if not same.all():
+        raise RuntimeError(f"{int((~same).sum())} of {sub.size} sampled hits differ from the oracle's twin")
+    sub64 = np.arange(0, rays.shape[0], fp64_stride)
+    h64, flags, t64 = o.trace(rays[sub64], refcpu.TRACE_FP64, threads)
+    clean = flags == 0
+    if not np.array_equal(hits["prim"][sub64][clean], h64["prim"][clean]):
+        raise RuntimeError("hit ids differ from FP64 on unflagged rays")
+    m = clean & (h64["prim"] != miss)
+    rel = float((np.abs(hits["t"][sub64][m].astype(np.float64) - t64[m]) / t64[m]).max()) if m.any() else 0.0
+    if rel > 1e-6:
+        raise RuntimeError(f"hit t differs from FP64 by {rel:.3g} relative")
+    mt = o.trace(rays[sub], refcpu.TRACE_REF_MT, threads)
+    wt_hit, mt_hit = wt["prim"] != miss, mt["prim"] != miss
+    differ = mt["prim"] != wt["prim"]
+    return {"rays_checked_bit_exact": int(sub.size), "stride": stride, "bit_exact": True,
+            "fp64_rays": int(sub64.size), "fp64_unflagged": int(clean.sum()), "fp64_ids_equal": True,
+            "fp64_max_rel_t": rel,
+            "reference_mt_disagreement": {
+                "rays": int(sub.size), "differ": int(differ.sum()), "fraction": round(float(differ.mean()), 4),
+                "twin_hits_mt_misses": int((wt_hit & ~mt_hit).sum()),
+                "mt_hits_twin_misses": int((mt_hit & ~wt_hit).sum()),
+                "both_hit_other_triangle": int((wt_hit & mt_hit & differ).sum()),
+                "twin_hits": int(wt_hit.sum())}}
 
 
 def load_traffic(config):
@@ -94,6 +152,17 @@ def load_traffic(config):
     return int(float(fields["hbm_bytes_per_launch"])), l2, "profiles/trace_incoherent_2m_pmc.md"
 
 
+def spread(ms):
+    """Median, 90th percentile and maximum of per-frame times, and how many frames took over twice
+    the median (a stall of the GPU or the host shows here)."""
+    if not ms:
+        return None
+    a = np.asarray(ms, np.float64)
+    med = float(np.median(a))
+    return {"median": round(med, 3), "p90": round(float(np.percentile(a, 90)), 3), "max": round(float(a.max()), 3),
+            "frames_over_2x_median": int((a > 2 * med).sum()), "frames": int(a.size)}
+
+
 def relmse(img, ref, eps=1e-2):
     """Relative MSE over pixels and channels: mean((x - r)^2 / (r^2 + eps))."""
     return float(((img - ref) ** 2 / (ref ** 2 + eps)).mean())
@@ -115,9 +184,9 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     and accumulate only the frames after warm_frac of the budget. Each estimator runs on the faster of
     the launch schedule spk_render_autotune picks for it and the default schedule (median over 8
     frames each, before any timed region). The reference is ref_spp frames of
-    the plain tracer with next-event estimation (same expectation as the reference's estimator, about
-    3x lower variance per frame, so the reference's own noise adds less to every relMSE), seeds from
-    1,000,000 on, summed in float64 on the device. All frames use the 64-bit seed composition: at
+    the plain FP32 tracer (the reference's estimator), seeds from 1,000,000 on, summed in float64 on the
+    device; its own noise adds about relMSE(plain at spp) * spp / ref_spp to every figure (reported as
+    reference_noise_relmse). All frames use the 64-bit seed composition: at
     1080p the reference's uint32 j*W + i + f*W*H wraps from frame 2072 on. No per-sample clamp
     anywhere: clamped estimators converge to different (biased) images, so an equal-time comparison
     against one reference would measure the clamp.
@@ -168,16 +237,23 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     log(f"relMSE experiment: schedules {schedules}")
 
     log(f"relMSE experiment: reference at {ref_spp} spp")
-    ctx.set_schedule(*schedules["plain_nee"])
-    ctx.render_frame(frame(999_997, spk.FRAME_NEE), rgba)  # graph capture
+    ctx.set_schedule(*schedules["plain"])
+    ctx.render_frame(frame(999_997), rgba)  # graph capture
     ref = torch.zeros(npx * 4, dtype=torch.float64, device="cuda")
     t0 = time.perf_counter()
     rays_ref = 0
+    ref_gpu_ms, ref_wall_ms = [], []
     for k in range(ref_spp):
-        rays_ref += ctx.render_frame(frame(1_000_000 + k, spk.FRAME_NEE), rgba).rays_traced
+        tf = time.perf_counter()
+        fs = ctx.render_frame(frame(1_000_000 + k), rgba)
+        rays_ref += fs.rays_traced
         ref += rgba.double()
+        ref_gpu_ms.append(fs.milliseconds)
+        ref_wall_ms.append((time.perf_counter() - tf) * 1e3)
         if (k + 1) % 512 == 0:
-            log(f"  reference {k + 1}/{ref_spp} spp, {time.perf_counter() - t0:.1f} s")
+            log(f"  reference {k + 1}/{ref_spp} spp, {time.perf_counter() - t0:.1f} s "
+                f"(frame kernels: median {np.median(ref_gpu_ms[-512:]):.2f} ms, max {max(ref_gpu_ms[-512:]):.2f} ms; "
+                f"wall per frame: median {np.median(ref_wall_ms[-512:]):.2f} ms, max {max(ref_wall_ms[-512:]):.2f} ms)")
     torch.cuda.synchronize()
     t_ref = time.perf_counter() - t0
     ref = (ref / ref_spp).view(height, width, 4)[..., :3]
@@ -215,11 +291,14 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
             frames = used = 0
             warm = warm_frac * budget if bounce is not None else 0.0
             t0 = time.perf_counter()
+            wall_ms = []
             while True:
                 el = time.perf_counter() - t0
                 if el >= budget:
                     break
+                tf = time.perf_counter()
                 ctx.render_frame(frame(frames, flags), rgba)
+                wall_ms.append((time.perf_counter() - tf) * 1e3)
                 frames += 1
                 if el >= warm:
                     ctx.accumulate(rgba, acc, npx, used, 1)
@@ -228,7 +307,8 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
             dt = time.perf_counter() - t0
             img = acc.view(height, width, 4)[..., :3].double()
             r = {"frames": frames, "accumulated_spp": used, "seconds": round(dt, 3),
-                 "relmse": relmse(img, ref) if used else None, "mean": float(img.mean())}
+                 "relmse": relmse(img, ref) if used else None, "mean": float(img.mean()),
+                 "frame_wall_ms": spread(wall_ms)}
             if bounce is not None:
                 r["terminate_bounce"] = bounce
                 r["last_loss"] = ctx.cache_info()["last_loss"]
@@ -247,17 +327,20 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
                  f"max {max_bounce} bounces, Russian roulette from bounce 3",
         "relmse_definition": "mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)",
         "reference_spp": ref_spp,
-        "reference_estimator": "plain + next-event estimation",
+        "reference_estimator": "plain (BSDF sampling only, the reference's estimator), FP32",
+        "reference_noise_relmse": round(first["plain"]["relmse"] * spp / ref_spp, 5),
         "reference_mean": float(ref.mean()),
         "per_sample_clamp": None,
         "reference_s": round(t_ref, 2),
+        "reference_frame_kernels_ms": spread(ref_gpu_ms),
+        "reference_frame_wall_ms": spread(ref_wall_ms),
         "equal_time_s": round(T, 3),
         "budgets": by_budget,
         "schedules": {k: dict(zip(("blocks_per_cu", "material_sort", "refill", "tail_bounce"), v))
                       for k, v in schedules.items()},
         "schedule_median_frame_ms": schedule_ms,
-        "full_path_nee_samples_per_s": round(npx * ref_spp / t_ref, 1),
-        "full_path_nee_mrays_per_s": round(rays_ref / t_ref / 1e6, 1),
+        "reference_pass_samples_per_s": round(npx * ref_spp / t_ref, 1),
+        "reference_pass_mrays_per_s": round(rays_ref / t_ref / 1e6, 1),
         "bvh_build_s": round(t_build, 2),
     }
     ctx.close()
@@ -709,6 +792,8 @@ def main():
 
     hits = d_hits.cpu().numpy().view(spk.HIT_DTYPE)
     hit_frac = float((hits["prim"] != spk.SPK_MISS).mean())
+    log("checking the timed call's hits against the oracle")
+    parity = oracle_check(sc, rays, hits)
 
     # coherent 1080p primary rays on the same scene (reported beside the metric)
     coherent_mrays = None
@@ -771,6 +856,7 @@ def main():
             "mean_nodes_per_ray": round(mean_nodes, 3),
             "mean_tris_per_ray": round(mean_tris, 3),
             "wall_s": round(wall, 4),
+            "parity": parity,
         },
         "roofline": {
             "bound": "hbm",

@@ -1179,6 +1179,16 @@ void orcl_xoshiro_stream(uint64_t seed, uint32_t *out, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
 }
 
+void orcl_xoshiro_advance(uint32_t state[4], uint64_t n, uint32_t *out) {
+    orcl::Rng r(0);
+    for (int k = 0; k < 4; ++k) r.s[k] = state[k];
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t v = r.next();
+        if (out) out[i] = v;
+    }
+    for (int k = 0; k < 4; ++k) state[k] = r.s[k];
+}
+
 void orcl_random_unit_stream(uint64_t seed, float *out, uint32_t n) {
     Rng r(seed);
     for (uint32_t i = 0; i < n; ++i) out[i] = r.unit();
@@ -1663,11 +1673,15 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
             res.valid = -1.f;
             break;
         }
+        // Russian roulette (CPURenderer.cpp:350-364)
         if (bounce >= 3) {
             float p = maxc(T);
             p = std::min(std::max(p, 0.05f), 1.0f);
-            if (rng.unit() > p) break;
-            T = T / p;
+            const float r = rng.unit();
+            if (!fr.rr_off) {
+                if (r > p) break;
+                T = T / p;
+            }
         }
         o = h.location + wi * kTolerance;
         d = wi;

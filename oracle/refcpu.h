@@ -120,6 +120,9 @@ typedef struct orcl_frame {
     uint32_t seed_hi; /* high 32 bits of a 64-bit pixel seed (0 = reference uint32 wrap) */
     uint32_t nee;      /* next-event estimation as the device's SPK_FRAME_NEE (no reference counterpart) */
     float output_limit; /* per-sample clamp; 0 = the reference's 6 */
+    uint32_t rr_off;    /* checker switch, no reference counterpart: 1 keeps every path past bounce 3
+                           (the roulette draw is still consumed, its outcome ignored, so the two
+                           estimators share every other random number); 0 = the reference's roulette */
 } orcl_frame;
 
 enum {
@@ -177,6 +180,9 @@ void orcl_xoshiro_stream(uint64_t seed, uint32_t *out, uint32_t n);
 void orcl_random_unit_stream(uint64_t seed, float *out, uint32_t n);
 /* n samples of one BxDF lobe in the local frame (normal +z) from one xoshiro stream: kind 0
    Lambertian, 1 specular, 2 dielectric, 3 VNDF microfacet normal. thr / valid may be NULL. */
+/* Advances a raw xoshiro128+ state by n steps with the same transition as the renderer's generator
+   (Sampler.h:16-41); writes the n outputs to out when out is not null. */
+void orcl_xoshiro_advance(uint32_t state[4], uint64_t n, uint32_t *out);
 void orcl_bxdf_sample(int kind, const float *wo, const float *base, float roughness, float metallic, float eta,
                       uint64_t seed, uint32_t n, float *wi, float *thr, int32_t *valid);
 /* SampleSurface's shading frame (u, v, w) from a normal and a tangent. */

@@ -60,7 +60,7 @@ class OFrame(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32),
                 ("max_bounce", ctypes.c_uint32), ("debug_mode", ctypes.c_uint32), ("sky_enabled", ctypes.c_uint32),
                 ("sky_color", ctypes.c_float * 3), ("seed_hi", ctypes.c_uint32), ("nee", ctypes.c_uint32),
-                ("output_limit", ctypes.c_float)]
+                ("output_limit", ctypes.c_float), ("rr_off", ctypes.c_uint32)]
 
 
 class OCacheConfig(ctypes.Structure):
@@ -98,6 +98,7 @@ def lib():
             "orcl_tri_wt": (I, [P, P, F, P, P, P, P, P, P]),
             "orcl_refine_t": (F, [P, P, P, P, P]),
             "orcl_xoshiro_stream": (None, [U64, P, U32]),
+            "orcl_xoshiro_advance": (None, [P, U64, P]),
             "orcl_random_unit_stream": (None, [U64, P, U32]),
             "orcl_bxdf_sample": (None, [I, P, P, F, F, F, U64, U32, P, P, P]),
             "orcl_local_axes": (None, [P, P, P, P, P]),
@@ -218,10 +219,12 @@ class OracleScene:
         return hits
 
     def render_frame(self, cam, width, height, frame_seed=0, max_bounce=4, debug_mode=0, sky=None, seed64=False,
-                     threads=None, nee=False, output_limit=0.0):
+                     threads=None, nee=False, output_limit=0.0, roulette=True):
+        """One frame of BasePass. roulette=False is a checker switch with no reference counterpart: paths
+        survive every roulette draw unweighted (the draw is still taken)."""
         fr = OFrame(width, height, frame_seed, max_bounce, debug_mode, 1 if sky is not None else 0,
                     (ctypes.c_float * 3)(*(sky if sky is not None else (0, 0, 0))), 1 if seed64 else 0,
-                    1 if nee else 0, output_limit)
+                    1 if nee else 0, output_limit, 0 if roulette else 1)
         oc = OCamera()
         for f in ("position", "right", "up", "front", "lower_left", "max_u", "max_v"):
             getattr(oc, f)[:] = list(getattr(cam, f))
@@ -285,6 +288,14 @@ def xoshiro_stream(seed, n):
     out = np.zeros(n, dtype=np.uint32)
     lib().orcl_xoshiro_stream(seed, out.ctypes.data, n)
     return out
+
+
+def xoshiro_advance(state, n=1, outputs=False):
+    """Advances a raw 4-word xoshiro128+ state n steps; returns the new state (and the outputs)."""
+    st = np.ascontiguousarray(state, dtype=np.uint32).copy()
+    out = np.zeros(n, dtype=np.uint32) if outputs else None
+    lib().orcl_xoshiro_advance(st.ctypes.data, n, None if out is None else out.ctypes.data)
+    return (st, out) if outputs else st
 
 
 def random_unit_stream(seed, n):

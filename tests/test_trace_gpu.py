@@ -234,32 +234,57 @@ def test_autotune_keeps_results(gpu_ctx):
 
 
 def test_full_heightfield_2m_subsampled(gpu_ctx):
-    """1024^2-cell heightfield (2,097,152 triangles), 8,388,608 incoherent rays traced on the GPU;
-    every 64th ray checked against the oracle; every ray checked for range properties."""
+    """1024^2-cell heightfield (2,097,152 triangles), 8,388,608 incoherent rays traced on the GPU in
+    the post-upload configuration, in the configuration the benchmark runs (BVH8, stack 12, 64
+    threads, coherence sort: 256 pass-2 sorting chunks with a permuted write-back) and in the one the
+    autotuner picks on this batch; the hit array is filled with a sentinel before each call, so a ray
+    the sort dropped would show. Every 64th ray is checked bit for bit against the oracle's
+    watertight twin and against FP64 on unflagged rays, every ray for range properties, and all
+    8,388,608 hits of the three configurations against each other."""
+    import torch
+
     sc = scenes.heightfield_scene(cells=1024)
     lo, hi = scenes.scene_aabb(sc)
     rays = scenes.incoherent_rays(lo, hi, n=8 * 1024 * 1024, seed=0)
+    n = rays.shape[0]
     print_case("full heightfield upload")
     st = gpu_ctx.upload_scene(sc)
     assert st.num_triangles == 2 * 1024 * 1024
-    print_case("full heightfield trace 8M")
-    rc, h = gpu_trace(gpu_ctx, rays)
-    assert rc == spk.SPK_OK
-    hit = h["prim"] != spk.SPK_MISS
-    assert (h["prim"][hit] < 2 * 1024 * 1024).all()
-    assert (h["t"][hit] > 0).all() and np.isfinite(h["t"][hit]).all()
-    assert np.isinf(h["t"][~hit]).all()
-    sub = np.arange(0, rays.shape[0], 64)
+    sub = np.arange(0, n, 64)
     o = refcpu.OracleScene(sc)
     wt = o.trace(rays[sub], refcpu.TRACE_WT)
-    np.testing.assert_array_equal(h["prim"][sub], wt["prim"])
-    np.testing.assert_array_equal(h["t"][sub].view(np.uint32), wt["t"].view(np.uint32))
     h64, flags, t64 = o.trace(rays[sub], refcpu.TRACE_FP64)
     clean = flags == 0
-    np.testing.assert_array_equal(h["prim"][sub][clean], h64["prim"][clean])
     m = clean & (h64["prim"] != spk.SPK_MISS)
-    rel = np.abs(h["t"][sub][m].astype(np.float64) - t64[m]) / t64[m]
-    assert rel.max() <= T_REL_TOL
+    d_rays = to_dev(rays)
+    d_hits = empty_dev(n * 8)
+    default_cfg = gpu_ctx.trace_config()
+    print_case("full heightfield autotune 8M")
+    tuned_cfg, _ = gpu_ctx.autotune(d_rays, tune_width=False)
+    first = None
+    for cfg in (default_cfg, (8, 12, 64, 0, 1), tuned_cfg):
+        print_case("full heightfield trace 8M", cfg)
+        gpu_ctx.set_trace_config(*cfg)
+        d_hits.fill_(0xAB)
+        torch.cuda.synchronize()
+        assert gpu_ctx.trace(d_rays, d_hits, n) == spk.SPK_OK
+        h = hits_from_dev(d_hits, n)
+        assert not (h["prim"].view(np.uint32) == 0xABABABAB).any()
+        hit = h["prim"] != spk.SPK_MISS
+        assert (h["prim"][hit] < 2 * 1024 * 1024).all()
+        assert (h["t"][hit] > 0).all() and np.isfinite(h["t"][hit]).all()
+        assert np.isinf(h["t"][~hit]).all()
+        np.testing.assert_array_equal(h["prim"][sub], wt["prim"])
+        np.testing.assert_array_equal(h["t"][sub].view(np.uint32), wt["t"].view(np.uint32))
+        np.testing.assert_array_equal(h["prim"][sub][clean], h64["prim"][clean])
+        rel = np.abs(h["t"][sub][m].astype(np.float64) - t64[m]) / t64[m]
+        assert rel.max() <= T_REL_TOL
+        if first is None:
+            first = h
+        else:
+            np.testing.assert_array_equal(h["prim"], first["prim"])
+            np.testing.assert_array_equal(h["t"].view(np.uint32), first["t"].view(np.uint32))
+    gpu_ctx.set_trace_config(*default_cfg)
     # the coherent 1080p set on the same scene, every 16th ray
     rays_a = scenes.coherent_rays()
     print_case("full heightfield trace coherent 1080p")
