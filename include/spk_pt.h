@@ -251,6 +251,16 @@ int spk_trace_closest(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk
 int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev);
 /* Host-array convenience wrapper (copies in and out). */
 int spk_trace_closest_host(spk_context *ctx, const spk_ray *rays, uint64_t n, spk_hit *hits);
+/* No counterpart in the reference. For callers that cannot allocate device memory during a call
+   (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+   closest hit as spk_trace_closest in the context's trace configuration, with the coherence sort's
+   scratch taken from a fixed device array of the library (44.3 MiB per GPU, held until the
+   process ends) instead of context buffers grown on demand; the call allocates nothing. At most
+   SPK_NOALLOC_MAX_RAYS rays per call (more: SPK_ERR_INVALID_ARGUMENT). Every such call on one GPU
+   shares that array: calls that overlap on one GPU (other streams, other contexts) race. Blocks
+   until done; invalid rays are reported as by spk_trace_closest. */
+#define SPK_NOALLOC_MAX_RAYS 2097152ull
+int spk_trace_closest_noalloc(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev);
 /* Traversal statistics for n rays: per ray node visits and triangle tests (device arrays of
    uint32, either may be NULL). */
 int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t *nodes_dev,

@@ -41,6 +41,7 @@ DEBUG_RAY_DIRECTION = 4
 DEBUG_ALBEDO = 9
 DEBUG_DEPTH = 11
 
+NOALLOC_MAX_RAYS = 2097152  # SPK_NOALLOC_MAX_RAYS
 FRAME_SEED64 = 1
 FRAME_NEE = 2
 
@@ -189,6 +190,7 @@ def load_library():
         "spk_trace_closest": (I, [P, P, U64, P]),
         "spk_trace_closest_async": (I, [P, P, U64, P]),
         "spk_trace_closest_host": (I, [P, P, U64, P]),
+        "spk_trace_closest_noalloc": (I, [P, P, U64, P]),
         "spk_trace_count": (I, [P, P, U64, P, P]),
         "spk_trace_kernel_timing": (I, [P, U32]),
         "spk_trace_kernel_times": (I, [P, ctypes.POINTER(F), U32, ctypes.POINTER(U32)]),
@@ -437,6 +439,13 @@ class Context:
 
     def trace_checked(self, rays_dev, hits_dev, n=None):
         return self._check(self.trace(rays_dev, hits_dev, n))
+
+    def trace_noalloc(self, rays_dev, hits_dev, n=None):
+        """spk_trace_closest_noalloc: the closest hit with the library's fixed scratch (no allocation),
+        at most NOALLOC_MAX_RAYS rays. Returns the status code."""
+        if n is None:
+            n = rays_dev.numel() * rays_dev.element_size() // 32
+        return self.lib.spk_trace_closest_noalloc(self.h, _ptr(rays_dev), n, _ptr(hits_dev))
 
     def trace_host(self, rays):
         """rays: numpy RAY_DTYPE array. Returns (rc, hits numpy HIT_DTYPE array)."""

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -21,6 +22,47 @@ using namespace spk;
 namespace spk {
 
 constexpr int kMaxVariants = 256;
+
+// ---- scratch of spk_trace_closest_noalloc ------------------------------------------------------
+// The coherence sort of up to SPK_NOALLOC_MAX_RAYS rays in one file-scope device array, so that
+// call allocates nothing: keys, keys in direction order, indices in direction order, the final
+// permutation (4 B per ray each), the pass-2 run offsets (one row of kRaySortBins per 32K-ray
+// chunk), the pass-1 run offsets (kRayDirBins per 8K-ray chunk) and the bin totals. 44.3 MiB, held
+// by every GPU the library runs on until the process ends. The host never uses the array's name as
+// an address: each device's address comes from hipGetSymbolAddress at spk_create.
+constexpr size_t align256(size_t x) { return (x + 255u) & ~size_t(255u); }
+constexpr uint64_t kNoAllocRays = SPK_NOALLOC_MAX_RAYS;
+constexpr size_t kScrKeys = 0;
+constexpr size_t kScrKeys1 = kScrKeys + align256(kNoAllocRays * 4u);
+constexpr size_t kScrPerm1 = kScrKeys1 + align256(kNoAllocRays * 4u);
+constexpr size_t kScrPerm = kScrPerm1 + align256(kNoAllocRays * 4u);
+constexpr size_t kScrOffs = kScrPerm + align256(kNoAllocRays * 4u);
+constexpr size_t kScrOffs1 =
+    kScrOffs + align256((kNoAllocRays + kRaySortChunk - 1u) / kRaySortChunk * kRaySortBins * 4u);
+constexpr size_t kScrTotal =
+    kScrOffs1 + align256((kNoAllocRays + kRaySortChunk1 - 1u) / kRaySortChunk1 * kRayDirBins * 4u);
+constexpr size_t kScrBytes = kScrTotal + align256((size_t)(kRayDirBins + kRaySortBins) * 4u);
+
+static __device__ __align__(256) uint8_t scratch[kScrBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// This device's address of `scratch`, looked up once per device with that device current.
+hipError_t noalloc_scratch_address(int device, void **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 int set_error(spk_context *ctx, int code, const char *fmt, ...) {
     if (ctx) {
@@ -636,6 +678,8 @@ int spk_create(int device, spk_context **out) {
         return SPK_ERR_HIP;
     }
     ctx->stream = ctx->own_stream;
+    // the no-allocation entry point's scratch on this device (its calls report an error if absent)
+    if (noalloc_scratch_address(device, &ctx->noalloc_scratch) != hipSuccess) ctx->noalloc_scratch = nullptr;
     if (ctx->counters.ensure(64 * sizeof(uint32_t)) != hipSuccess ||
         hipMemsetAsync(ctx->counters.ptr, 0, 64 * sizeof(uint32_t), ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -1005,6 +1049,41 @@ int spk_trace_kernel_times(spk_context *ctx, float *ms, uint32_t max, uint32_t *
 int spk_trace_closest(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev) {
     int rc = spk_trace_closest_async(ctx, rays_dev, n, hits_dev);
     if (rc != SPK_OK) return rc;
+    return spk_synchronize(ctx);
+}
+
+int spk_trace_closest_noalloc(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev) {
+    if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
+    if (n && (!rays_dev || !hits_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null ray or hit array");
+    if (!ctx->scene.valid) return set_error(ctx, SPK_ERR_NO_SCENE, "no scene uploaded");
+    if (n > kNoAllocRays)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT,
+                         "spk_trace_closest_noalloc takes at most %llu rays per call (got %llu)",
+                         (unsigned long long)kNoAllocRays, (unsigned long long)n);
+    if (!ctx->noalloc_scratch)
+        return set_error(ctx, SPK_ERR_HIP, "no-allocation scratch is not available on device %d", ctx->device);
+    const DeviceScene &ds = ctx->scene;
+    RaySortScratch ss;
+    const RaySortScratch *sp = nullptr;
+    if (ds.variant.sort) {
+        // every part of the sort scratch that must start at zero (the bin totals) is cleared by a fill
+        // kernel inside launch_trace; the rest is written before it is read
+        uint8_t *base = static_cast<uint8_t *>(ctx->noalloc_scratch);
+        ss.keys = reinterpret_cast<uint32_t *>(base + kScrKeys);
+        ss.keys1 = reinterpret_cast<uint32_t *>(base + kScrKeys1);
+        ss.perm1 = reinterpret_cast<uint32_t *>(base + kScrPerm1);
+        ss.perm = reinterpret_cast<uint32_t *>(base + kScrPerm);
+        ss.offs = reinterpret_cast<uint32_t *>(base + kScrOffs);
+        ss.offs1 = reinterpret_cast<uint32_t *>(base + kScrOffs1);
+        ss.total = reinterpret_cast<uint32_t *>(base + kScrTotal);
+        for (int a = 0; a < 3; ++a) ss.lo[a] = ds.box_lo[a], ss.inv[a] = ds.box_inv[a];
+        sp = &ss;
+    }
+    const hipError_t e = launch_trace(ds.variant, rays_dev, n, hits_dev, ds.nodes.ptr, ds.tris.ptr, 0u,
+                                      ctx->invalid_counter(), nullptr, nullptr, ctx->work_counter(), ds.spheres.ptr,
+                                      ds.num_spheres, ds.num_tris, ds.two_level ? ds.insts.ptr : nullptr, sp,
+                                      ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
     return spk_synchronize(ctx);
 }
 

@@ -96,6 +96,7 @@ struct spk_context {
     spk::DeviceBuffer counters;  // [0] invalid rays, [2..3] ray counter of the persistent kernels
     spk::DeviceBuffer tmp_rays, tmp_hits;
     spk::DeviceBuffer sort_keys, sort_keys1, sort_perm1, sort_perm, sort_offs1, sort_offs, sort_total;  // coherence order
+    void *noalloc_scratch = nullptr;  // this device's copy of the no-allocation entry point's scratch
     std::vector<hipEvent_t> kernel_events;  // spk_trace_kernel_timing: begin / end per timed call
     uint32_t kernel_timed = 0;              // calls recorded so far
     spk::DeviceScene scene;
