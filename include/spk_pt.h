@@ -159,10 +159,19 @@ typedef struct spk_camera_desc {
     float far_plane;
 } spk_camera_desc;
 
+/* RenderConfig::DebugMode (RenderConfig.h:33-47); every mode 0-11 is served. Modes 2, 5 and 6
+   rewrite the colour after the loop and take neither SPK_FRAME_NEE nor SPK_FRAME_CACHE. */
 #define SPK_DEBUG_COLOR 0u
+#define SPK_DEBUG_DEBUG 1u
+#define SPK_DEBUG_RAY_DEPTH 2u
 #define SPK_DEBUG_NORMAL 3u
 #define SPK_DEBUG_RAY_DIRECTION 4u
+#define SPK_DEBUG_INDIRECT_LIGHTING 5u
+#define SPK_DEBUG_DIRECTIONAL_LIGHTING 6u
+#define SPK_DEBUG_METALLIC 7u
+#define SPK_DEBUG_ROUGHNESS 8u
 #define SPK_DEBUG_ALBEDO 9u
+#define SPK_DEBUG_EMISSIVE 10u
 #define SPK_DEBUG_DEPTH 11u
 
 #define SPK_FRAME_SEED64 1u /* compose the pixel seed in 64 bits (no uint32 wrap) */

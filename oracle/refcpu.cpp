@@ -1585,6 +1585,14 @@ static V3 nee_sample(const orcl_scene &sc, Rng &nr, V3 loc, V3 normal, V3 base, 
     return cmul(cmul(T, base), lme.emissive) * g;
 }
 
+// VisualizeInteger (Utilities.h:93-112): a 7-colour high-contrast palette / 255, black from 7 on
+static V3 visualize_integer(unsigned number) {
+    static const float palette[7][3] = {{137, 49, 239}, {242, 202, 25}, {255, 0, 189}, {0, 87, 233},
+                                        {135, 233, 17}, {225, 24, 69},  {255, 255, 255}};
+    if (number >= 7) return kZeros;
+    return V3(palette[number][0], palette[number][1], palette[number][2]) / 255.0f;
+}
+
 static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, const orcl_frame &fr, Rng &rng,
                                 float u, float v, uint64_t &nrays, Rng *nee_rng = nullptr) {
     // SetupViewRay (CPURenderer.cpp:220-238)
@@ -1636,6 +1644,9 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
         bool diffuse = false;
         V3 f = sample_surface(rng, mat, me, d, h.normal, h.tangent, wi, &diffuse);
         switch (fr.debug_mode) {
+            case 1:  // Debug: the reference's debug colour, zero unless a debug pixel sets it
+                res.color = kZeros;
+                return res;
             case 3:  // Normal
                 res.color = (h.normal + kOnes) * 0.5f;
                 return res;
@@ -1685,6 +1696,24 @@ static PixelResult sample_pixel(const orcl_scene &sc, const orcl_camera &cam, co
         }
         o = h.location + wi * kTolerance;
         d = wi;
+    }
+    // debug modes that rewrite the colour after the loop (CPURenderer.cpp:370-391); `bounce` is the
+    // segment the path ended in, max_bounce when the loop ran out
+    switch (fr.debug_mode) {
+        case 1:  // Debug
+            res.color = kZeros;
+            break;
+        case 2:  // RayDepth: VisualizeInteger (Utilities.h:93-112)
+            res.color = visualize_integer(bounce);
+            break;
+        case 5:  // IndirectLighting
+            if (bounce <= 1) res.color = kZeros;
+            break;
+        case 6:  // DirectionalLighting
+            if (bounce > 1) res.color = kZeros;
+            break;
+        default:
+            break;
     }
     return res;
 }

@@ -36,9 +36,16 @@ MATERIAL_PBR = 0
 MATERIAL_DIELECTRIC = 1
 
 DEBUG_COLOR = 0
+DEBUG_DEBUG = 1
+DEBUG_RAY_DEPTH = 2
 DEBUG_NORMAL = 3
 DEBUG_RAY_DIRECTION = 4
+DEBUG_INDIRECT_LIGHTING = 5
+DEBUG_DIRECTIONAL_LIGHTING = 6
+DEBUG_METALLIC = 7
+DEBUG_ROUGHNESS = 8
 DEBUG_ALBEDO = 9
+DEBUG_EMISSIVE = 10
 DEBUG_DEPTH = 11
 
 NOALLOC_MAX_RAYS = 2097152  # SPK_NOALLOC_MAX_RAYS

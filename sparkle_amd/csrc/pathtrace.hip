@@ -532,6 +532,25 @@ __device__ __forceinline__ MatEval eval_material(const PtArgs &a, const Material
     return e;
 }
 
+// ---- debug modes (RenderConfig::DebugMode, RenderConfig.h:33-47) ----
+// Modes that return at the first non-emissive hit (CPURenderer.cpp:296-327): Debug, Normal,
+// RayDirection, Metallic, Roughness, Albedo, Emissive, Depth.
+__device__ __forceinline__ bool first_hit_debug(uint32_t m) { return m == 1u || m == 3u || m == 4u || (m >= 7u && m <= 11u); }
+// Modes that rewrite the colour once the loop ends (CPURenderer.cpp:370-391): Debug (paths that end
+// before a non-emissive first hit), RayDepth, IndirectLighting, DirectionalLighting.
+__device__ __forceinline__ bool post_loop_debug(uint32_t m) { return m == 1u || m == 2u || m == 5u || m == 6u; }
+__device__ __forceinline__ F3 debug_after_loop(uint32_t m, uint32_t end_bounce, F3 C) {
+    if (m == 1u) return f3(0.f, 0.f, 0.f);
+    if (m == 5u) return end_bounce <= 1u ? f3(0.f, 0.f, 0.f) : C;
+    if (m == 6u) return end_bounce > 1u ? f3(0.f, 0.f, 0.f) : C;
+    // RayDepth: VisualizeInteger (Utilities.h:93-112), a 7-colour palette / 255, black from 7 on
+    const float r[7] = {137.f, 242.f, 255.f, 0.f, 135.f, 225.f, 255.f};
+    const float g[7] = {49.f, 202.f, 0.f, 87.f, 233.f, 24.f, 255.f};
+    const float b[7] = {239.f, 25.f, 189.f, 233.f, 17.f, 69.f, 255.f};
+    if (end_bounce >= 7u) return f3(0.f, 0.f, 0.f);
+    return f3(r[end_bounce] / 255.0f, g[end_bounce] / 255.0f, b[end_bounce] / 255.0f);
+}
+
 // ---- shade: SamplePixel's loop body (CPURenderer.cpp:261-368) ----
 // What shading one path segment produced: whether the path continues, its shadow ray (NEE), and
 // the cache query or training vertex it recorded.
@@ -558,6 +577,8 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
     const float4 T4 = a.thr[path];
     F3 T = f3(T4.x, T4.y, T4.z);
     const uint32_t pf = __float_as_uint(T4.w);
+    uint32_t end_b = bounce;  // the segment the path ends in (max_bounce when the loop runs out)
+    bool first_done = false;  // a first-hit debug mode wrote the colour
     float4 C4 = a.col[path];
     F3 C = f3(C4.x, C4.y, C4.z);
     if (h.x == 0xFFFFFFFFu) {
@@ -649,8 +670,9 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
             const F3 f = sample_surface(rng, mat, me.base, me.metallic, me.roughness, d, normal, tangent, wi,
                                         diffuse);
             bool done = false;
-            if (a.debug_mode != SPK_DEBUG_COLOR && bounce == 0u) {
+            if (first_hit_debug(a.debug_mode) && bounce == 0u) {
                 switch (a.debug_mode) {
+                    case 1: C = f3(0.f, 0.f, 0.f); break;  // Debug: the reference's debug colour (zero)
                     case 3: C = (normal + f3(1.f, 1.f, 1.f)) * 0.5f; break;
                     case 4: C = (wi + f3(1.f, 1.f, 1.f)) * 0.5f; break;
                     case 7: C = f3(1.f, 1.f, 1.f) * me.metallic; break;
@@ -736,6 +758,7 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
                         p = fminf(fmaxf(p, 0.05f), 1.0f);
                         if (rng_unit(rng) > p) alive = false; else T = T / p;
                     }
+                    if (alive && last) end_b = bounce + 1u;  // the loop ran out
                     if (alive && !last) {
                         cont = true;
                         const F3 o2 = loc + wi * SPK_TOLERANCE;
@@ -748,8 +771,10 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
                     }
                 }
             }
+            first_done = done;
         }
     }
+    if (post_loop_debug(a.debug_mode) && !cont && !cache_q && !first_done) C = debug_after_loop(a.debug_mode, end_b, C);
     a.col[path] = make_float4(C.x, C.y, C.z, C4.w);
 }
 
@@ -961,7 +986,11 @@ __global__ __launch_bounds__(kPtBlock) void k_cache_targets(PtArgs a) {
 __global__ __launch_bounds__(kPtBlock) void k_finalize(PtArgs a) {
     const uint32_t stride = gridDim.x * kPtBlock;
     for (uint32_t p = blockIdx.x * kPtBlock + threadIdx.x; p < a.npix; p += stride) {
-        const float4 C = a.col[p];
+        float4 C = a.col[p];
+        if (a.max_bounce == 0u && post_loop_debug(a.debug_mode)) {  // the loop never ran: bounce 0
+            const F3 c = debug_after_loop(a.debug_mode, 0u, f3(C.x, C.y, C.z));
+            C.x = c.x, C.y = c.y, C.z = c.z;
+        }
         float4 o;
         o.x = fminf(C.x, a.output_limit);
         o.y = fminf(C.y, a.output_limit);
@@ -1315,10 +1344,12 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
                      float *normal_dev, spk_frame_stats *stats) {
     if (frame->max_bounce > (uint32_t)kMaxBounce)
         return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "max_bounce above %d", kMaxBounce);
-    // modes that rewrite the colour after the loop (RayDepth, Indirect/DirectionalLighting) are not
-    // part of the device path
-    if (frame->debug_mode == 2u || frame->debug_mode == 5u || frame->debug_mode == 6u || frame->debug_mode > 11u)
-        return set_error(ctx, SPK_ERR_UNSUPPORTED, "debug mode %u is not supported on the device", frame->debug_mode);
+    if (frame->debug_mode > 11u)
+        return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "unknown debug mode %u", frame->debug_mode);
+    // the modes that rewrite the colour after the loop see the reference's estimator only
+    if ((frame->debug_mode == 2u || frame->debug_mode == 5u || frame->debug_mode == 6u) &&
+        (frame->flags & (SPK_FRAME_NEE | SPK_FRAME_CACHE)))
+        return set_error(ctx, SPK_ERR_UNSUPPORTED, "debug mode %u takes neither NEE nor the cache", frame->debug_mode);
     const bool cache_on = (frame->flags & SPK_FRAME_CACHE) != 0;
     if (cache_on && !ctx->cache)
         return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "SPK_FRAME_CACHE needs spk_cache_create first");

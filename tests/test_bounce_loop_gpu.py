@@ -121,3 +121,61 @@ def test_dielectric_matches_oracle(gpu_ctx, bounce):
     c, _ = o.render_frame(cam, w, h, frame_seed=0, debug_mode=spk.DEBUG_ALBEDO)
     assert (np.abs(g[..., :3] - c[..., :3]).max(-1) < 1e-4).mean() > 0.99
     assert (np.abs(c[..., :3] - np.array([0.95, 0.85, 0.7], np.float32)).max(-1) == 0).sum() > 10
+
+
+# ---- debug modes that rewrite the colour after the loop (CPURenderer.cpp:370-391) -------------
+def test_post_loop_debug_modes_closed_form(gpu_ctx):
+    """RayDepth is VisualizeInteger of the segment the path ended in, IndirectLighting and
+    DirectionalLighting split the colour at segment 1, Debug is black: on the shells, where the
+    segment is fixed, every pixel equals the closed form."""
+    from tests.test_oracle_bounce_loop import PALETTE
+
+    w, h = 16, 12
+    cam = shell_camera(w, h)
+    for k_shells, max_bounce, end in ((0, 4, 0), (2, 4, 2), (2, 2, 2), (5, 8, 5), (7, 7, 7)):
+        print_case("ray depth shells", k_shells, max_bounce)
+        r = Renderer(gpu_ctx, shell_scene([(1.0, 1.0, 1.0)] * k_shells), cam, w, h)
+        g, _, _ = r.render(2, bounce=max_bounce, debug=spk.DEBUG_RAY_DEPTH)
+        expect = PALETTE[end] if end < 7 else np.zeros(3, np.float32)
+        np.testing.assert_array_equal(g[..., :3], np.broadcast_to(expect, (h, w, 3)))
+    g, _, _ = r.render(2, bounce=0, debug=spk.DEBUG_RAY_DEPTH)  # the loop never runs: segment 0
+    np.testing.assert_array_equal(g[..., :3], np.broadcast_to(PALETTE[0], (h, w, 3)))
+    for k_shells in (1, 2):
+        albedos = [(0.9, 0.8, 0.7)] * k_shells
+        r = Renderer(gpu_ctx, shell_scene(albedos), cam, w, h)
+        c, _, _ = r.render(0, bounce=6)
+        ind, _, _ = r.render(0, bounce=6, debug=spk.DEBUG_INDIRECT_LIGHTING)
+        dire, _, _ = r.render(0, bounce=6, debug=spk.DEBUG_DIRECTIONAL_LIGHTING)
+        dbg, _, _ = r.render(0, bounce=6, debug=spk.DEBUG_DEBUG)
+        assert (c[..., :3] > 0).all() and (dbg[..., :3] == 0).all()
+        kept, dropped = (dire, ind) if k_shells == 1 else (ind, dire)
+        np.testing.assert_array_equal(kept, c)
+        assert (dropped[..., :3] == 0).all()
+
+
+@pytest.mark.parametrize("mode", [spk.DEBUG_RAY_DEPTH, spk.DEBUG_INDIRECT_LIGHTING, spk.DEBUG_DIRECTIONAL_LIGHTING,
+                                  spk.DEBUG_DEBUG])
+def test_post_loop_debug_modes_match_oracle(gpu_ctx, mode):
+    """The Cornell room with glass, 8 bounces and roulette: the same paths give the same RayDepth
+    colour and the same indirect/direct split as the oracle on at least 97% of pixel samples, and the
+    means agree within 3 sigma on 99% of pixels."""
+    w = h = 40
+    frames = 16
+    sc = glass_cornell()
+    cam = scenes.cornell_camera(w, h)
+    r = Renderer(gpu_ctx, sc, cam, w, h)
+    o = refcpu.OracleScene(sc)
+    d = np.zeros((frames, h, w, 3))
+    for k in range(frames):
+        print_case("post-loop debug mode", mode, "frame", k)
+        g, _, _ = r.render(k, bounce=8, debug=mode)
+        c, _ = o.render_frame(cam, w, h, frame_seed=k, max_bounce=8, debug_mode=mode)
+        d[k] = g[..., :3] - c[..., :3]
+    same = (np.abs(d).max(-1) < 1e-5).mean()
+    print(f"debug mode {mode}: identical samples {same:.4f}")
+    assert same >= 0.97, same
+    se = d.std(0, ddof=1) / math.sqrt(frames)
+    assert (np.abs(d.mean(0)) <= 3 * se + 1e-6).mean() >= 0.99
+    with pytest.raises(spk.SpkError) as e:
+        r.render(0, bounce=4, debug=spk.DEBUG_RAY_DEPTH, flags=spk.FRAME_NEE)
+    assert e.value.code == spk.SPK_ERR_UNSUPPORTED

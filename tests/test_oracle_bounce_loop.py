@@ -224,3 +224,47 @@ def test_xoshiro_plus_output_is_first_plus_last_word():
         words.append((z ^ (z >> 31)) & 0xFFFFFFFF)
     _, raw = refcpu.xoshiro_advance(np.array(words, np.uint32), 16, outputs=True)
     np.testing.assert_array_equal(seeded, raw)
+
+
+# ---- debug modes that rewrite the colour after the loop (CPURenderer.cpp:370-391) -------------
+PALETTE = np.array([[137, 49, 239], [242, 202, 25], [255, 0, 189], [0, 87, 233], [135, 233, 17], [225, 24, 69],
+                    [255, 255, 255]], np.float32) / np.float32(255.0)  # VisualizeInteger (Utilities.h:93-112)
+
+
+@pytest.mark.parametrize("k_shells,max_bounce,end", [(0, 4, 0), (1, 4, 1), (2, 4, 2), (3, 8, 3), (2, 2, 2),
+                                                     (5, 8, 5), (6, 7, 6), (7, 7, 7), (7, 9, 7)])
+def test_ray_depth_is_the_segment_the_path_ended_in(k_shells, max_bounce, end):
+    """K shells put the emitter at segment K; a limit below K + 1 ends the loop at max_bounce. RayDepth
+    shows VisualizeInteger of that segment, black from 7 on. Roulette is on from segment 3: with
+    albedo 1 it never ends a path (survival max(T) = 1)."""
+    sc = shell_scene([(1.0, 1.0, 1.0)] * k_shells)
+    w, h = 6, 4
+    rgba, _ = refcpu.OracleScene(sc).render_frame(shell_camera(w, h), w, h, frame_seed=2, max_bounce=max_bounce,
+                                                  debug_mode=spk.DEBUG_RAY_DEPTH)
+    expect = PALETTE[end] if end < 7 else np.zeros(3, np.float32)
+    np.testing.assert_array_equal(rgba[..., :3], np.broadcast_to(expect, (h, w, 3)))
+    assert (rgba[..., 3] == 1.0).all()
+
+
+@pytest.mark.parametrize("k_shells", [0, 1, 2, 3])
+def test_indirect_and_directional_lighting_split_at_segment_one(k_shells):
+    """IndirectLighting keeps paths that ended after segment 1, DirectionalLighting those that ended
+    at segment 0 or 1; together they add up to the colour. Debug mode 1 is black everywhere."""
+    albedos = [(0.9, 0.8, 0.7)] * k_shells
+    sc = shell_scene(albedos)
+    o = refcpu.OracleScene(sc)
+    w, h = 6, 4
+    cam = shell_camera(w, h)
+    color, _ = o.render_frame(cam, w, h, frame_seed=0, max_bounce=6)
+    indirect, _ = o.render_frame(cam, w, h, frame_seed=0, max_bounce=6, debug_mode=spk.DEBUG_INDIRECT_LIGHTING)
+    direct, _ = o.render_frame(cam, w, h, frame_seed=0, max_bounce=6, debug_mode=spk.DEBUG_DIRECTIONAL_LIGHTING)
+    expect = np.broadcast_to(np.asarray(LE, np.float32) * _product(albedos), (h, w, 3))
+    np.testing.assert_allclose(color[..., :3], expect, rtol=2e-6)
+    if k_shells <= 1:
+        assert (indirect[..., :3] == 0).all()
+        np.testing.assert_array_equal(direct[..., :3], color[..., :3])
+    else:
+        assert (direct[..., :3] == 0).all()
+        np.testing.assert_array_equal(indirect[..., :3], color[..., :3])
+    dbg, _ = o.render_frame(cam, w, h, frame_seed=0, max_bounce=6, debug_mode=spk.DEBUG_DEBUG)
+    assert (dbg[..., :3] == 0).all() and (dbg[..., 3] == 1).all()
