@@ -317,6 +317,14 @@ int spk_set_camera(spk_context *ctx, const spk_camera_desc *camera);
    for the gradient sky. Missed rays see it when spk_frame_desc.sky_enabled is set. A new texture
    table clears it. */
 int spk_set_sky_cubemap(spk_context *ctx, const uint32_t faces[6]);
+/* Host-only cook of a textured sky light (SkyLightCookJob::Execute, SkyLight.cpp:83-135; no device
+   needed): samples the equirectangular map (width x height RGBA floats, row-major, as Image::Sample
+   does, at CartesianToEquirectangular of each cube texel's direction) into six faces of
+   face_size^2 RGBA floats (+X, -X, +Y, -Y, +Z, -Z, rows of face_size texels), each value rounded to
+   FP16 like the reference's RGBAFloat16 cube (alpha 1). The reference cooks 1024^2 faces. Upload the
+   faces as RGBA32F textures and pass their indices to spk_set_sky_cubemap. */
+int spk_sky_cook(const float *equirect_rgba, uint32_t width, uint32_t height, uint32_t face_size,
+                 float *faces_rgba);
 /* One BasePass: one path per pixel; rgba = clamped colour and valid flag (+1/-1); normal =
    first-hit world normal (may be NULL). Device arrays of width*height*4 / *3 floats. */
 int spk_render_frame(spk_context *ctx, const spk_frame_desc *frame, float *rgba_dev, float *normal_dev,

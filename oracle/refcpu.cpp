@@ -51,6 +51,7 @@ static inline float maxc(V3 a) { return std::max(std::max(a.x, a.y), a.z); }
 
 static const V3 kUp(0, 0, 1), kRight(1, 0, 0), kOnes(1, 1, 1), kZeros(0, 0, 0);
 static const float kPi = 3.14159265358979323846f;  // std::numbers::pi_v<float>
+static const float kInvPi = 0.318309886183790671538f;  // std::numbers::inv_pi_v<float> (Types.h:53)
 static const float kEps = 1e-6f;                    // Types.h:54
 static const float kTolerance = 1e-4f;              // Types.h:55
 static const float kOutputLimit = 6.f;              // CameraRenderProxy.h:16
@@ -1987,6 +1988,62 @@ extern "C" void orcl_local_axes(const float *normal, const float *tangent, float
     v[0] = b.x, v[1] = b.y, v[2] = b.z;
     w[0] = c.x, w[1] = c.y, w[2] = c.z;
 }
+
+// ---- textured sky light cook (SkyLightCookJob::Execute, SkyLight.cpp:83-135) ----
+// float -> binary16 -> float with round to nearest even: 11 significant bits for normal halves
+// (|x| >= 2^-14), a quantum of 2^-24 below, infinity past the largest half (65504)
+static float to_half_and_back(float x) {
+    if (!std::isfinite(x) || x == 0.f) return x;
+    const double a = std::fabs((double)x);
+    double r;
+    if (a < std::ldexp(1.0, -14)) {
+        r = std::nearbyint(std::ldexp(a, 24)) * std::ldexp(1.0, -24);
+    } else {
+        int e;
+        const double m = std::frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
+        r = std::ldexp(std::nearbyint(std::ldexp(m, 11)), e - 11);
+    }
+    if (r > 65504.0) r = std::numeric_limits<double>::infinity();
+    return static_cast<float>(std::copysign(r, (double)x));
+}
+
+extern "C" void orcl_sky_cook(const float *equirect, uint32_t w, uint32_t h, uint32_t face_size, float *faces) {
+    Texture src;
+    src.w = w;
+    src.h = h;
+    src.texels.assign(equirect, equirect + 4 * (size_t)w * h);
+    for (uint32_t face = 0; face < 6; ++face) {
+        for (uint32_t i = 0; i < face_size; ++i) {
+            for (uint32_t j = 0; j < face_size; ++j) {
+                const float u = (static_cast<float>(i) + 0.5f) / static_cast<float>(face_size) * 2.f - 1.f;
+                const float v = (static_cast<float>(j) + 0.5f) / static_cast<float>(face_size) * 2.f - 1.f;
+                // Image2DCube::TextureCoordinateToDirection (io/Image.h:287-315): components by
+                // negation, so a zero keeps its sign (atan2(-0, x < 0) = -pi picks the map's left edge)
+                V3 d;
+                switch (face) {
+                    case 0: d = V3(1.f, -v, -u); break;
+                    case 1: d = V3(-1.f, -v, u); break;
+                    case 2: d = V3(u, 1.f, v); break;
+                    case 3: d = V3(u, -1.f, -v); break;
+                    case 4: d = V3(u, -v, 1.f); break;
+                    default: d = V3(-u, -v, -1.f); break;
+                }
+                d = normalized(d);
+                // CartesianToEquirectangular (Utilities.h:510-518)
+                const float eu = std::atan2(d.y, d.x) * (kInvPi * 0.5f) + 0.5f;
+                const float ev = std::asin(-d.z) * kInvPi + 0.5f;
+                const V3 c = texture_sample(src, eu, ev);
+                float *o = faces + 4 * (((size_t)face * face_size + j) * face_size + i);
+                o[0] = to_half_and_back(c.x);
+                o[1] = to_half_and_back(c.y);
+                o[2] = to_half_and_back(c.z);
+                o[3] = 1.f;
+            }
+        }
+    }
+}
+
+extern "C" float orcl_half_round(float x) { return to_half_and_back(x); }
 
 extern "C" void orcl_texture_sample(const orcl_scene *scene, int index, float u, float v, float *out) {
     const orcl::V3 r = orcl::texture_sample(scene->textures[index], u, v);

@@ -180,6 +180,11 @@ void orcl_xoshiro_stream(uint64_t seed, uint32_t *out, uint32_t n);
 void orcl_random_unit_stream(uint64_t seed, float *out, uint32_t n);
 /* n samples of one BxDF lobe in the local frame (normal +z) from one xoshiro stream: kind 0
    Lambertian, 1 specular, 2 dielectric, 3 VNDF microfacet normal. thr / valid may be NULL. */
+/* Textured sky light cook (SkyLightCookJob::Execute, SkyLight.cpp:83-135): six face_size^2 RGBA faces
+   (+X, -X, +Y, -Y, +Z, -Z) sampled from an equirectangular RGBA float map, values rounded to FP16. */
+void orcl_sky_cook(const float *equirect, uint32_t w, uint32_t h, uint32_t face_size, float *faces);
+/* float -> binary16 (round to nearest even) -> float. */
+float orcl_half_round(float x);
 /* Advances a raw xoshiro128+ state by n steps with the same transition as the renderer's generator
    (Sampler.h:16-41); writes the n outputs to out when out is not null. */
 void orcl_xoshiro_advance(uint32_t state[4], uint64_t n, uint32_t *out);

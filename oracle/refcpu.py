@@ -99,6 +99,8 @@ def lib():
             "orcl_refine_t": (F, [P, P, P, P, P]),
             "orcl_xoshiro_stream": (None, [U64, P, U32]),
             "orcl_xoshiro_advance": (None, [P, U64, P]),
+            "orcl_sky_cook": (None, [P, U32, U32, U32, P]),
+            "orcl_half_round": (F, [F]),
             "orcl_random_unit_stream": (None, [U64, P, U32]),
             "orcl_bxdf_sample": (None, [I, P, P, F, F, F, U64, U32, P, P, P]),
             "orcl_local_axes": (None, [P, P, P, P, P]),
@@ -296,6 +298,19 @@ def xoshiro_advance(state, n=1, outputs=False):
     out = np.zeros(n, dtype=np.uint32) if outputs else None
     lib().orcl_xoshiro_advance(st.ctypes.data, n, None if out is None else out.ctypes.data)
     return (st, out) if outputs else st
+
+
+def sky_cook(equirect, face_size):
+    """The oracle's textured-sky cook: (H, W, 4) float32 texels -> (6, face_size, face_size, 4)."""
+    src = np.ascontiguousarray(equirect, dtype=np.float32)
+    h, w = src.shape[:2]
+    faces = np.zeros((6, face_size, face_size, 4), np.float32)
+    lib().orcl_sky_cook(src.ctypes.data, w, h, face_size, faces.ctypes.data)
+    return faces
+
+
+def half_round(x):
+    return float(lib().orcl_half_round(float(x)))
 
 
 def random_unit_stream(seed, n):

@@ -49,6 +49,7 @@ DEBUG_EMISSIVE = 10
 DEBUG_DEPTH = 11
 
 NOALLOC_MAX_RAYS = 2097152  # SPK_NOALLOC_MAX_RAYS
+SKY_FACE_SIZE = 1024  # the reference cooks 1024^2 cube faces (SkyLight.cpp:24)
 FRAME_SEED64 = 1
 FRAME_NEE = 2
 
@@ -193,6 +194,7 @@ def load_library():
         "spk_textures_upload": (I, [P, P, U32]),
         "spk_scene_set_spheres": (I, [P, P, U32]),
         "spk_set_sky_cubemap": (I, [P, P]),
+        "spk_sky_cook": (I, [P, U32, U32, U32, P]),
         "spk_build_check": (I, [P, U32, P, U32, P, ctypes.POINTER(SceneStats), ctypes.c_char_p, ctypes.c_size_t]),
         "spk_trace_closest": (I, [P, P, U64, P]),
         "spk_trace_closest_async": (I, [P, P, U64, P]),
@@ -290,6 +292,13 @@ class Scene:
         assert px.ndim == 3 and px.shape[2] == 4
         self.textures.append((px, fmt))
         return len(self.textures) - 1
+
+    def set_sky_map(self, equirect, face_size=SKY_FACE_SIZE):
+        """A textured sky light: cooks the equirectangular map into six cube faces (sky_cook), adds
+        them as RGBA32F textures and makes them the sky cube map. Returns the face texture indices."""
+        faces = sky_cook(equirect, face_size)
+        self.sky_faces = [self.add_texture(f, TEXTURE_RGBA32F) for f in faces]
+        return self.sky_faces
 
     def add_instance(self, mesh, material, world_from_local=None):
         M = np.eye(3, 4, dtype=np.float32) if world_from_local is None else np.asarray(world_from_local,
@@ -632,6 +641,37 @@ class Context:
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
         t = None if table is None else np.ascontiguousarray(table, dtype=np.float32)
         return self._check(self.lib.spk_cache_set_params(self.h, _ptr(w), _ptr(t)))
+
+
+_HALF_MAX = 65504.0
+
+
+def sky_source_texels(image):
+    """The texels a sky map is sampled from, as the reference's image loader keeps them
+    (Image2D::LoadFromFile, io/Image.cpp:150-197): an 8-bit image as c / 255 (R8G8B8A8Unorm), a float
+    (.hdr) image clamped to the FP16 range and rounded to FP16 (RGBAFloat16). (H, W, 4) float32."""
+    a = np.asarray(image)
+    if a.ndim != 3 or a.shape[2] not in (3, 4):
+        raise ValueError("sky map must be (H, W, 3 or 4)")
+    if a.dtype == np.uint8:
+        out = a.astype(np.float32) / np.float32(255.0)
+    else:
+        out = np.clip(a.astype(np.float32), -_HALF_MAX, _HALF_MAX).astype(np.float16).astype(np.float32)
+    if out.shape[2] == 3:
+        out = np.concatenate([out, np.ones(out.shape[:2] + (1,), np.float32)], 2)
+    return np.ascontiguousarray(out)
+
+
+def sky_cook(image, face_size=SKY_FACE_SIZE):
+    """spk_sky_cook on an equirectangular sky map (uint8 or float, (H, W, 3 or 4)): six (face_size,
+    face_size, 4) float32 faces +X, -X, +Y, -Y, +Z, -Z with FP16-rounded values."""
+    src = sky_source_texels(image)
+    h, w = src.shape[:2]
+    faces = np.empty((6, face_size, face_size, 4), np.float32)
+    rc = load_library().spk_sky_cook(src.ctypes.data, w, h, face_size, faces.ctypes.data)
+    if rc != SPK_OK:
+        raise SpkError(rc, f"spk_sky_cook failed for a {w}x{h} map and {face_size}^2 faces")
+    return list(faces)
 
 
 def camera_desc(position, front, up_hint=(0.0, 0.0, 1.0), vertical_fov=None, focus_distance=1.0,

@@ -901,37 +901,51 @@ int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, 
         const float *M = world_from_local + 12 * (size_t)i;
         if (std::memcmp(M, &hs.inst_xf[12 * (size_t)i], 48) == 0) continue;
         moved.push_back(i);
-        std::memcpy(&hs.inst_xf[12 * (size_t)i], M, 48);
     }
-    // world vertices and shading of each moved instance (independent ranges, spread over threads)
-    std::atomic<size_t> next{0};
-    auto work = [&]() {
-        for (size_t m; (m = next.fetch_add(1)) < moved.size();) {
-            const uint32_t i = moved[m];
-            const float *M = &hs.inst_xf[12 * (size_t)i];
-            const MeshCopy &mc = hs.mesh_copies[hs.inst_mesh[i]];
-            const spk_mesh_desc md = mesh_view(mc);
-            const uint32_t g0 = hs.inst_id_base[i];
-            for (uint32_t f = 0; f < mc.num_triangles; ++f) {
-                float *v = &hs.verts[9 * (size_t)(g0 + f)];
-                for (int k = 0; k < 3; ++k)
-                    transform_point(M, &mc.positions[3 * (size_t)mc.indices[3 * (size_t)f + k]], v + 3 * k);
-                shade_triangle(md, M, f, v, hs.inst_material[i], hs.shade_in[g0 + f]);
+    // world vertices and shading of each moved instance under the transforms in hs.inst_xf
+    // (independent ranges, spread over the build's thread count; 0: every hardware thread)
+    auto rebake_moved = [&]() {
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t m; (m = next.fetch_add(1)) < moved.size();) {
+                const uint32_t i = moved[m];
+                const float *M = &hs.inst_xf[12 * (size_t)i];
+                const MeshCopy &mc = hs.mesh_copies[hs.inst_mesh[i]];
+                const spk_mesh_desc md = mesh_view(mc);
+                const uint32_t g0 = hs.inst_id_base[i];
+                for (uint32_t f = 0; f < mc.num_triangles; ++f) {
+                    float *v = &hs.verts[9 * (size_t)(g0 + f)];
+                    for (int k = 0; k < 3; ++k)
+                        transform_point(M, &mc.positions[3 * (size_t)mc.indices[3 * (size_t)f + k]], v + 3 * k);
+                    shade_triangle(md, M, f, v, hs.inst_material[i], hs.shade_in[g0 + f]);
+                }
             }
-        }
+        };
+        const unsigned hw =
+            hs.input.num_threads ? hs.input.num_threads : std::max(1u, std::thread::hardware_concurrency());
+        const unsigned nt = std::min<unsigned>(hw, std::max<unsigned>(1u, (unsigned)moved.size()));
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+        work();
+        for (std::thread &t : pool) t.join();
     };
-    // the build's thread count (spk_build_options.num_threads; 0: every hardware thread)
-    const unsigned hw = hs.input.num_threads ? hs.input.num_threads : std::max(1u, std::thread::hardware_concurrency());
-    const unsigned nt = std::min<unsigned>(hw, std::max<unsigned>(1u, (unsigned)moved.size()));
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
-    work();
-    for (std::thread &t : pool) t.join();
+    for (uint32_t i : moved) std::memcpy(&hs.inst_xf[12 * (size_t)i], world_from_local + 12 * (size_t)i, 48);
+    rebake_moved();
     std::string err;
     int rc = assemble_two_level(hs, hs.input.width, hs.input.num_threads, err);
+    int stack = rc == SPK_OK ? pick_trace_stack((int)hs.depth2, true) : -1;
+    if (rc == SPK_OK && stack < 0) {
+        rc = SPK_ERR_STACK_DEPTH;
+        err = "two-level depth " + std::to_string(hs.depth2) + " exceeds the deepest compiled traversal stack";
+    }
     if (rc != SPK_OK) {
-        // keep the previous (valid) state on the device; restore the host copy of the transforms
+        // nothing has reached the device: put the host copy back as it was (the previous transforms,
+        // their world geometry and shading, and the TLAS assembled from them, which built before)
         hs.inst_xf = old_xf;
+        rebake_moved();
+        std::string err2;
+        if (assemble_two_level(hs, hs.input.width, hs.input.num_threads, err2) != SPK_OK)
+            return set_error(ctx, rc, "%s (and the previous TLAS did not rebuild: %s)", err.c_str(), err2.c_str());
         return set_error(ctx, rc, "%s", err.c_str());
     }
     DeviceScene &ds = ctx->scene;
@@ -955,9 +969,9 @@ int spk_scene_update_instances(spk_context *ctx, const float *world_from_local, 
     ds.depth = (int)hs.depth2;
     set_scene_box(hs, ds);  // the ray sort's origin cells follow the moved instances
     if (ds.variant.stack < ds.depth) {
-        const int stack = pick_trace_stack(ds.depth, true);
-        if (stack < 0) return set_error(ctx, SPK_ERR_STACK_DEPTH, "two-level depth %d too deep", ds.depth);
-        ds.variant = TraceVariant{ds.width, stack, 128};
+        // a deeper stack; the block size and the coherence order stay where a variant exists
+        const int block = stack == 64 ? 64 : (ds.variant.block == 64 || ds.variant.block == 128 ? ds.variant.block : 128);
+        ds.variant = TraceVariant{ds.width, stack, block, 0, ds.variant.sort};
     }
     fill_stats(hs, &ctx->stats,
                ds.nodes.bytes + ds.tris.bytes + ds.insts.bytes + ds.wtris.bytes + ds.shade.bytes + ds.materials.bytes);

@@ -32,7 +32,9 @@ text itself and applies the loader's rules:
     `SetupFromTransform`);
   * lights: colour x intensity x 2^exposure (:454-457); a DomeLight is the sky light (the only
     light the CPU path tracer evaluates, CPURenderer.cpp:251-268), a DistantLight the directional
-    light (reported, not used by the path); a textured dome is reported and not loaded.
+    light (reported, not used by the path); a textured dome's map (a path relative to the stage,
+    :474-497) is decoded as the image loader keeps it and cooked into the sky cube map
+    (SkyLight.cpp:83-135), which replaces the dome colour for missed rays (SkyRenderProxy.cpp:50-56).
 PNG images are decoded by `sparkle_amd.gltf`'s decoder (8-bit grey, RGB or RGBA); float textures
 travel as Radiance .hdr files (RGBE, the exporter's choice for float images, USDExporter.cpp:103-106),
 decoded as R x 2^(E - 136) with alpha 1 and encoded with a frexp mantissa scaled by 256, so a round
@@ -44,7 +46,7 @@ import re
 
 import numpy as np
 
-from . import (MATERIAL_DIELECTRIC, MATERIAL_PBR, TEXTURE_RGBA32F, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA8_UNORM,
+from . import (MATERIAL_DIELECTRIC, MATERIAL_PBR, SKY_FACE_SIZE, TEXTURE_RGBA32F, TEXTURE_RGBA8_SRGB, TEXTURE_RGBA8_UNORM,
                Scene, camera_from_basis)
 from .gltf import _png_decode, _png_rgba
 
@@ -405,6 +407,7 @@ class UsdScene:
         self.directional = None
         self.skipped = []
         self.mesh_names = []
+        self.sky_map_file = None  # the dome's texture, cooked into scene.sky_faces
 
     def camera_desc(self, aspect, far_plane=1000.0):
         """The render camera of `camera` (CameraComponent::UpdateRenderData and the focus plane of
@@ -679,15 +682,27 @@ class _Reader:
             elif t == "Camera":
                 self.camera(c, world)
             elif t == "DomeLight":
-                if c["attrs"].get("inputs:texture:file", {}).get("value"):
-                    self.out.skipped.append(f"{c['path']}: textured dome light")
                 self.out.sky = self.light_color(c)
+                tex = c["attrs"].get("inputs:texture:file", {}).get("value")
+                if tex:
+                    self.sky_map(str(tex))
             elif t == "DistantLight":
                 self.out.directional = self.light_color(c)
             elif t in ("Scope", "Material", "Shader", "GeomSubset"):
                 continue
             else:
                 self.out.skipped.append(f"{c['path']}: unsupported prim type {t or '(none)'}")
+
+    def sky_map(self, file):
+        """A textured dome (LoadSkyLight, USDLoader.cpp:474-497): the map is a path relative to the
+        stage; it is decoded as the image loader keeps it and cooked into the sky cube map
+        (SkyLightCookJob::Execute), which missed rays then sample instead of the dome colour."""
+        full = os.path.normpath(os.path.join(self.dir, file))
+        with open(full, "rb") as fh:
+            data = fh.read()
+        img = hdr_decode(data) if full.lower().endswith(".hdr") else _png_decode(data)
+        self.out.scene.set_sky_map(img, self.sky_face_size)
+        self.out.sky_map_file = full
 
     def read(self):
         default = self.meta.get("defaultPrim")
@@ -703,9 +718,12 @@ class _Reader:
         return self.out
 
 
-def read_usda(path):
-    """Reads a .usda file into a UsdScene with the reference loader's rules."""
-    return _Reader(path).read()
+def read_usda(path, sky_face_size=SKY_FACE_SIZE):
+    """Reads a .usda file into a UsdScene with the reference loader's rules. A textured dome is
+    cooked into cube faces of sky_face_size^2 texels (the reference's 1024 by default)."""
+    r = _Reader(path)
+    r.sky_face_size = sky_face_size
+    return r.read()
 
 
 # ---------------------------------------------------------------------------------------------
@@ -724,13 +742,14 @@ def _valid_name(s):
     return s if s and not s[0].isdigit() else "_" + s
 
 
-def write_usda(path, scene, camera=None, sky=None):
+def write_usda(path, scene, camera=None, sky=None, sky_map=None):
     """Writes `scene` as USDA in the exporter's layout: a root Xform "Root" holding one Xform per
     instance (its transform as `xformOp:transform`) with the mesh in object space, a `/Materials`
     scope with one UsdPreviewSurface network per material (textures as PNG, float ones as .hdr, in
     `textures/` beside the file, read through a shared `st_reader`), an optional camera Xform (`camera`: dict of
     position, right, front, up rows and focal_length / sensor_height / aperture / focus_distance in
-    metres and f-stops) and an optional DomeLight of colour `sky`."""
+    metres and f-stops) and an optional DomeLight of colour `sky`, textured by the equirectangular
+    float image `sky_map` (written as `textures/sky.hdr`) when given."""
     base = os.path.dirname(os.path.abspath(path))
     os.makedirs(os.path.join(base, "textures"), exist_ok=True)
     tex_files = {}
@@ -791,9 +810,15 @@ def write_usda(path, scene, camera=None, sky=None):
               '            token projection = "perspective"',
               f'            float verticalAperture = {_fmt(camera["sensor_height"] * 1000.0)}',
               '        }', '    }', '']
-    if sky is not None:
-        L += ['    def DomeLight "sky"', '    {', f'        color3f inputs:color = {_tuple(sky)}',
-              '        float inputs:intensity = 1', '    }']
+    if sky is not None or sky_map is not None:
+        L += ['    def DomeLight "sky"', '    {',
+              f'        color3f inputs:color = {_tuple(sky if sky is not None else (1.0, 1.0, 1.0))}',
+              '        float inputs:intensity = 1']
+        if sky_map is not None:
+            with open(os.path.join(base, "textures", "sky.hdr"), "wb") as fh:
+                fh.write(hdr_encode(np.asarray(sky_map, np.float32)))
+            L += ['        asset inputs:texture:file = @textures/sky.hdr@']
+        L += ['    }']
     L += ['}', '', 'def Scope "Materials"', '{']
     for k, m in enumerate(scene.materials):
         mp = f"/Materials/material_{k}"

@@ -182,3 +182,22 @@ def test_two_level_path_tracing(gpu_ctx):
     # 64-frame means of the two scenes: same expectation, nearly the same samples
     assert abs(ma.mean() - mb.mean()) / ma.mean() < 0.01
     assert np.median(np.abs(ma - mb)) < 1e-3
+
+
+def test_update_instances_back_and_forth(gpu_ctx):
+    """Moving instances, a rejected update, then moving them back to where they were: the hits are
+    those of the original placement (the host copy the later updates compare against stays true to
+    what the device holds)."""
+    sc, xfs = instanced_scene(seed=23, count=12)
+    rays = _rays(sc, 8000, 24)
+    gpu_ctx.upload_scene(sc, two_level=True)
+    moved = _moved(xfs, 7)
+    print_case("update instances away")
+    gpu_ctx.update_instances(moved)
+    bad = moved.copy()
+    bad[0, :, :3] = 0.0
+    with pytest.raises(spk.SpkError):
+        gpu_ctx.update_instances(bad)
+    print_case("update instances back")
+    gpu_ctx.update_instances(xfs)
+    _check_twin(gpu_ctx, sc, rays, "moved away and back")
