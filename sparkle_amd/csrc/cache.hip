@@ -83,9 +83,14 @@ __device__ __forceinline__ void encode_point(const CacheDev &c, float px, float 
     }
 }
 
-// scatter d(loss)/d(features) of one point (BF16 row g[0..31]) into the FP32 table gradient
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// scatter d(loss)/d(features) of one point (BF16 row g[0..31]) into the table gradient: BF16 pairs,
+// one packed atomic add per corner for both features (global_atomic_pk_add_bf16), half the atomics
+// of two FP32 adds; Adam normalises the step by the gradient's running RMS, so 8 mantissa bits of
+// accumulated gradient are enough
 __device__ __forceinline__ void encode_backward(const CacheDev &c, float px, float py, float pz, const __bf16 *g,
-                                                float *gtable) {
+                                                bf16x2 *gtable) {
     const uint32_t T = 1u << c.log2_table;
 #pragma unroll 1
     for (uint32_t l = 0; l < 16; ++l) {
@@ -105,9 +110,9 @@ __device__ __forceinline__ void encode_backward(const CacheDev &c, float px, flo
             const uint32_t cx = cell[0] + (k & 1), cy = cell[1] + ((k >> 1) & 1), cz = cell[2] + ((k >> 2) & 1);
             const float w = ((k & 1) ? fr[0] : 1.f - fr[0]) * (((k >> 1) & 1) ? fr[1] : 1.f - fr[1]) *
                             (((k >> 2) & 1) ? fr[2] : 1.f - fr[2]);
-            const size_t e = ((size_t)l * T + grid_index(c, l, cx, cy, cz)) * 2u;
-            atomicAdd(&gtable[e], w * g0);
-            atomicAdd(&gtable[e + 1], w * g1);
+            const size_t e = (size_t)l * T + grid_index(c, l, cx, cy, cz);
+            const bf16x2 v = {(__bf16)(w * g0), (__bf16)(w * g1)};
+            __builtin_amdgcn_global_atomic_fadd_v2bf16(gtable + e, v);
         }
     }
 }
@@ -165,7 +170,7 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ W, const __
 struct TrainDev {
     const float *targets;  // [n][3]
     float *loss;           // accumulated (sum of per-element losses / (n*3))
-    float *gtable;         // FP32 table gradient
+    bf16x2 *gtable;        // table gradient, BF16 feature pairs
     __bf16 *act_t;         // transposed activations: 5 blocks [feature][n] (X is 32 rows, H 64 rows)
     __bf16 *grad_t;        // transposed output gradients: 5 blocks [neuron][n] (dH0..dH3 64 rows, dY 16 rows)
     uint32_t n_pad;        // column stride of act_t / grad_t
@@ -340,13 +345,17 @@ __global__ void k_adam_begin(AdamStep *st, const uint32_t *n_dev, uint32_t cap, 
     }
 }
 
-__global__ void k_adam(float *__restrict__ p, const float *__restrict__ grad, float *__restrict__ m,
+__device__ __forceinline__ float grad_value(const float *g, uint64_t i) { return g[i]; }
+__device__ __forceinline__ float grad_value(const __bf16 *g, uint64_t i) { return (float)g[i]; }
+
+template <typename G>
+__global__ void k_adam(float *__restrict__ p, const G *__restrict__ grad, float *__restrict__ m,
                        float *__restrict__ v, uint64_t count, float lr, float b1, float b2, float eps,
                        const AdamStep *__restrict__ st, __half *__restrict__ half_copy) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count || !st->active) return;
     const float c1 = st->c1, c2 = st->c2;
-    const float gi = grad[i];
+    const float gi = grad_value(grad, i);
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
@@ -455,13 +464,13 @@ hipError_t cache_gradients(CacheState &cs, const float *points, const float *tar
     if ((e = cs.act_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
     if ((e = cs.grad_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(cs.gw.ptr, 0, cache_num_weights() * 4, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(cs.gtable.ptr, 0, cs.table_params() * 4, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(cs.gtable.ptr, 0, cs.table_params() * 2, s)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(&cache_state(cs)->loss, 0, 4, s)) != hipSuccess) return e;
     if (n == 0) return hipSuccess;
     TrainDev td;
     td.targets = targets;
     td.loss = &cache_state(cs)->loss;
-    td.gtable = (float *)cs.gtable.ptr;
+    td.gtable = (bf16x2 *)cs.gtable.ptr;
     td.act_t = (__bf16 *)cs.act_t.ptr;
     td.grad_t = (__bf16 *)cs.grad_t.ptr;
     td.n_pad = n_pad;
@@ -477,12 +486,12 @@ hipError_t cache_adam(CacheState &cs, hipStream_t s, const uint32_t *n_dev, uint
     AdamStep *st = cache_state(cs);
     hipLaunchKernelGGL(k_adam_begin, dim3(1), dim3(1), 0, s, st, n_dev, cap, b1, b2);
     const uint64_t nw = cache_num_weights();
-    hipLaunchKernelGGL(k_adam, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, (float *)cs.w.ptr,
+    hipLaunchKernelGGL(k_adam<float>, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, (float *)cs.w.ptr,
                        (const float *)cs.gw.ptr, (float *)cs.mw.ptr, (float *)cs.vw.ptr, nw, lr, b1, b2, eps,
                        (const AdamStep *)st, (__half *)nullptr);
     const uint64_t nt = cs.table_params();
-    hipLaunchKernelGGL(k_adam, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, (float *)cs.table_f.ptr,
-                       (const float *)cs.gtable.ptr, (float *)cs.mt.ptr, (float *)cs.vt.ptr, nt, lr, b1, b2, eps,
+    hipLaunchKernelGGL(k_adam<__bf16>, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, (float *)cs.table_f.ptr,
+                       (const __bf16 *)cs.gtable.ptr, (float *)cs.mt.ptr, (float *)cs.vt.ptr, nt, lr, b1, b2, eps,
                        (const AdamStep *)st, (__half *)cs.table_h.ptr);
     hipLaunchKernelGGL(k_pack_weights, dim3(16), dim3(256), 0, s, (const float *)cs.w.ptr, (__bf16 *)cs.wf.ptr,
                        (__bf16 *)cs.wb.ptr);
@@ -588,7 +597,7 @@ int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
     hipError_t e;
     if ((e = cs.table_h.ensure(ntab * 2)) != hipSuccess || (e = cs.table_f.ensure(ntab * 4)) != hipSuccess ||
         (e = cs.mt.ensure(ntab * 4)) != hipSuccess || (e = cs.vt.ensure(ntab * 4)) != hipSuccess ||
-        (e = cs.gtable.ensure(ntab * 4)) != hipSuccess || (e = cs.w.ensure(nw * 4)) != hipSuccess ||
+        (e = cs.gtable.ensure(ntab * 2)) != hipSuccess || (e = cs.w.ensure(nw * 4)) != hipSuccess ||
         (e = cs.mw.ensure(nw * 4)) != hipSuccess || (e = cs.vw.ensure(nw * 4)) != hipSuccess ||
         (e = cs.gw.ensure(nw * 4)) != hipSuccess || (e = cs.wf.ensure(16384 * 2)) != hipSuccess ||
         (e = cs.wb.ensure(16384 * 2)) != hipSuccess || (e = cs.state.ensure(sizeof(AdamStep))) != hipSuccess)
@@ -637,7 +646,15 @@ int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess && loss) e = hipMemcpy(loss, &cache_state(cs)->loss, 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && dweights) e = hipMemcpy(dweights, cs.gw.ptr, cache_num_weights() * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && dtable) e = hipMemcpy(dtable, cs.gtable.ptr, cs.table_params() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && dtable) {
+        // the table gradient is kept as BF16; widen it for the caller
+        std::vector<uint16_t> g(cs.table_params());
+        e = hipMemcpy(g.data(), cs.gtable.ptr, g.size() * 2, hipMemcpyDeviceToHost);
+        for (size_t k = 0; e == hipSuccess && k < g.size(); ++k) {
+            const uint32_t bits = (uint32_t)g[k] << 16;
+            std::memcpy(&dtable[k], &bits, 4);
+        }
+    }
     if (e != hipSuccess) return hip_fail(ctx, e, "cache gradients");
     return SPK_OK;
 }

@@ -26,7 +26,7 @@ struct CacheState {
     uint32_t res[16] = {0};
     uint32_t dense[16] = {0};
     float lo[3] = {0, 0, 0}, inv_ext[3] = {1, 1, 1};
-    DeviceBuffer table_h, table_f, mt, vt, gtable;  // hash table (FP16 copy, FP32 master, Adam, gradient)
+    DeviceBuffer table_h, table_f, mt, vt, gtable;  // hash table (FP16 copy, FP32 master, Adam, BF16 gradient)
     DeviceBuffer w, mw, vw, gw, wf, wb;             // MLP weights (FP32 master, Adam, gradient, BF16 copies)
     DeviceBuffer act_t, grad_t, state;              // training scratch, AdamStep
     // per-frame records of the path tracer
