@@ -347,15 +347,16 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     return out
 
 
-MLP_FLOP_FORWARD = 2 * (32 * 64 + 3 * 64 * 64 + 64 * 3)  # 29,056 per query
+MLP_FLOP_FORWARD = 2 * (64 * 64 + 3 * 64 * 64 + 64 * 3)  # 33,152 per query (64 inputs: encoding + attributes)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 
 
 def cache_kernel_bench(spk, scenes, stream, n_infer=1920 * 1080, n_train=1 << 16, reps=20):
     """The learned radiance cache's kernels alone: inference on n_infer points (one 1080p frame of
     queries) and training steps (forward, backward, Adam) on n_train samples, uniform points in the
-    unit cube, timed with HIP events on the context's stream. Algorithmic FLOPs: 29,056 per query
-    forward, 3x that per training sample (forward, input gradients, weight gradients)."""
+    unit cube with random surface attributes, timed with HIP events on the context's stream.
+    Algorithmic FLOPs: 33,152 per query forward, 3x that per training sample (forward, input
+    gradients, weight gradients)."""
     import torch
 
     ctx = spk.Context(0)
@@ -368,20 +369,27 @@ def cache_kernel_bench(spk, scenes, stream, n_infer=1920 * 1080, n_train=1 << 16
     out = torch.empty(n_infer * 3, device="cuda")
     tp = torch.rand(n_train * 3, device="cuda", generator=g)
     tt = torch.rand(n_train * 3, device="cuda", generator=g)
+
+    def attrs(n):  # unit normals, albedo, roughness, 1: what the path tracer records per query
+        nrm = torch.nn.functional.normalize(torch.randn(n, 3, device="cuda", generator=g), dim=1)
+        return torch.cat([nrm, 0.2 + 0.6 * torch.rand(n, 4, device="cuda", generator=g)[:, :3],
+                          torch.rand(n, 1, device="cuda", generator=g), torch.ones(n, 1, device="cuda")], 1).contiguous()
+
+    pa, ta = attrs(n_infer), attrs(n_train)
     for _ in range(3):
-        ctx.cache_infer(pts, out, n_infer)
-        ctx.cache_train(tp, tt, n_train)
+        ctx.cache_infer(pts, out, n_infer, attrs_dev=pa)
+        ctx.cache_train(tp, tt, n_train, attrs_dev=ta)
     torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record(stream)
     for _ in range(reps):
-        ctx.cache_infer(pts, out, n_infer)
+        ctx.cache_infer(pts, out, n_infer, attrs_dev=pa)
     b.record(stream)
     torch.cuda.synchronize()
     infer_ms = a.elapsed_time(b) / reps
     a.record(stream)
     for _ in range(reps):
-        ctx.cache_train(tp, tt, n_train)
+        ctx.cache_train(tp, tt, n_train, attrs_dev=ta)
     b.record(stream)
     torch.cuda.synchronize()
     train_ms = a.elapsed_time(b) / reps
@@ -660,6 +668,8 @@ def main():
                     help="estimator of --render-only")
     ap.add_argument("--two-level", action="store_true",
                     help="only the flattened vs two-level comparison on the instanced scene (one JSON line)")
+    ap.add_argument("--relmse-only", action="store_true",
+                    help="only the equal-time relMSE experiment (one JSON line)")
     ap.add_argument("--cache-kernels", action="store_true",
                     help="only time the learned cache's inference and training kernels (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -694,6 +704,18 @@ def main():
         stream = torch.cuda.Stream()
         torch.cuda.set_stream(stream)
         print(json.dumps({"two_level": two_level_experiment(spk, scenes, stream)}), flush=True)
+        return
+    if args.relmse_only:
+        import torch
+        import sparkle_amd as spk
+        from sparkle_amd import scenes
+
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        res, spp, ref_spp = args.relmse.split(",")
+        w_, h_ = (int(x) for x in res.split("x"))
+        print(json.dumps({"relmse": relmse_experiment(spk, scenes, stream, w_, h_, int(spp), int(ref_spp))}),
+              flush=True)
         return
     if args.cache_kernels:
         import torch

@@ -393,9 +393,15 @@ int spk_batch_autotune(spk_context *ctx, const spk_frame_desc *frame, float *rgb
                        double *best_ms);
 
 /* ---- learned radiance cache (no counterpart in the reference) ---------------------------- */
-/* A multiresolution hash-grid encoding (levels x 2 features, 2^log2_table FP16 entries per level)
-   feeding a bias-free ReLU MLP 32-64-64-64-64-3 on BF16 matrix cores with FP32 accumulation,
-   trained online with Adam on a relative L2 loss. Positions are normalised by the scene box. */
+/* A multiresolution hash-grid encoding of the position (levels x 2 features, 2^log2_table FP16
+   entries per level) and SPK_CACHE_ATTRS surface attributes per query (shading normal xyz, albedo
+   rgb, roughness, and 1.0 for the first layer's bias; zeros pad the input to 64) feeding a bias-free
+   ReLU MLP 64-64-64-64-64-3 on BF16 matrix cores with FP32 accumulation, trained online with Adam
+   on a relative L2 loss. Positions are normalised by the scene box. In the path tracer the network
+   predicts outgoing radiance divided by the vertex albedo (floored at SPK_CACHE_ALBEDO_FLOOR), which
+   the frame multiplies back. */
+#define SPK_CACHE_ATTRS 8
+#define SPK_CACHE_ALBEDO_FLOOR 0.02f
 typedef struct spk_cache_config {
     uint32_t levels;           /* 16 */
     uint32_t features;         /* 2 */
@@ -414,16 +420,18 @@ int spk_cache_create(spk_context *ctx, const spk_cache_config *config);
    keeping every buffer, so a captured frame graph stays valid. */
 int spk_cache_reset(spk_context *ctx);
 int spk_cache_destroy(spk_context *ctx);
-/* n points (device, 3 floats each; world space when world != 0, else in [0,1]^3) -> n x 3 radiance. */
-int spk_cache_infer(spk_context *ctx, const float *points_dev, uint32_t n, int world, float *out_dev);
+/* n points (device, 3 floats each; world space when world != 0, else in [0,1]^3) and their
+   attributes (device, SPK_CACHE_ATTRS floats each; NULL: all zero) -> n x 3 outputs. */
+int spk_cache_infer(spk_context *ctx, const float *points_dev, const float *attrs_dev, uint32_t n, int world,
+                    float *out_dev);
 /* One training step (forward, backward, Adam) on n samples; loss (host, may be NULL) receives the
    batch loss before the update. */
-int spk_cache_train(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n, int world,
-                    float *loss);
+int spk_cache_train(spk_context *ctx, const float *points_dev, const float *attrs_dev, const float *targets_dev,
+                    uint32_t n, int world, float *loss);
 /* Forward + backward without an update: loss, weight gradient (host, cache_num_weights floats) and
    table gradient (host, levels*2^log2_table*2 floats; may be NULL). */
-int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n,
-                        int world, float *loss, float *dweights, float *dtable);
+int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *attrs_dev, const float *targets_dev,
+                        uint32_t n, int world, float *loss, float *dweights, float *dtable);
 /* FP32 master parameters to / from host arrays (either pointer may be NULL). */
 int spk_cache_get_params(spk_context *ctx, float *weights, float *table);
 int spk_cache_set_params(spk_context *ctx, const float *weights, const float *table);

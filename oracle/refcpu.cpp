@@ -1885,7 +1885,8 @@ void orcl_hashgrid_encode(const orcl_cache_config *cfg, const float *table, cons
 
 double orcl_mlp_loss_grad(const orcl_cache_config *cfg, const double *weights, const double *features,
                           const double *targets, uint32_t n, double *outputs, double *dweights, double *dfeatures) {
-    const uint32_t in = cfg->levels * cfg->features, H = cfg->hidden, NL = cfg->hidden_layers, O = cfg->outputs;
+    const uint32_t in = cfg->inputs ? cfg->inputs : cfg->levels * cfg->features, H = cfg->hidden,
+                   NL = cfg->hidden_layers, O = cfg->outputs;
     // layer shapes: [H x in], (NL-1) x [H x H], [O x H]
     std::vector<uint32_t> rows, cols;
     rows.push_back(H);

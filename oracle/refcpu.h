@@ -217,6 +217,8 @@ typedef struct orcl_cache_config {
     uint32_t hidden;          /* 64 */
     uint32_t hidden_layers;   /* 4 */
     uint32_t outputs;         /* 3 */
+    uint32_t inputs;          /* MLP input width: the encoding, then per-query attributes and zero
+                                 padding (0: the encoding alone, levels * features) */
 } orcl_cache_config;
 
 /* Encode points (in [0,1]^3) -> features [n][levels*features] from an FP32 table. */

@@ -66,7 +66,8 @@ class OFrame(ctypes.Structure):
 class OCacheConfig(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_uint32), ("features", ctypes.c_uint32), ("log2_table", ctypes.c_uint32),
                 ("base_resolution", ctypes.c_uint32), ("per_level_scale", ctypes.c_float),
-                ("hidden", ctypes.c_uint32), ("hidden_layers", ctypes.c_uint32), ("outputs", ctypes.c_uint32)]
+                ("hidden", ctypes.c_uint32), ("hidden_layers", ctypes.c_uint32), ("outputs", ctypes.c_uint32),
+                ("inputs", ctypes.c_uint32)]
 
 
 _lib = None
@@ -343,8 +344,8 @@ def local_axes(normal, tangent):
 
 
 # ---- learned radiance cache (FP64) ----------------------------------------------------------
-def cache_cfg(log2_table=19, base_resolution=16, per_level_scale=1.3819):
-    return OCacheConfig(16, 2, log2_table, base_resolution, per_level_scale, 64, 4, 3)
+def cache_cfg(log2_table=19, base_resolution=16, per_level_scale=1.3819, inputs=0):
+    return OCacheConfig(16, 2, log2_table, base_resolution, per_level_scale, 64, 4, 3, inputs)
 
 
 def level_resolutions(cfg):

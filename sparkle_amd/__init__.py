@@ -145,6 +145,9 @@ def cache_config(log2_table=19, base_resolution=16, per_level_scale=1.3819, lear
 
 
 FRAME_CACHE = 4
+CACHE_ATTRS = 8  # SPK_CACHE_ATTRS: shading normal xyz, albedo rgb, roughness, 1
+CACHE_INPUTS = 64  # MLP input width: 32 encoding features, the attributes, zeros
+CACHE_ALBEDO_FLOOR = 0.02
 FRAME_NO_SORT = 0x80000000
 
 
@@ -225,9 +228,9 @@ def load_library():
         "spk_cache_create": (I, [P, ctypes.POINTER(CacheConfig)]),
         "spk_cache_destroy": (I, [P]),
         "spk_cache_reset": (I, [P]),
-        "spk_cache_infer": (I, [P, P, U32, I, P]),
-        "spk_cache_train": (I, [P, P, P, U32, I, ctypes.POINTER(F)]),
-        "spk_cache_gradients": (I, [P, P, P, U32, I, ctypes.POINTER(F), P, P]),
+        "spk_cache_infer": (I, [P, P, P, U32, I, P]),
+        "spk_cache_train": (I, [P, P, P, P, U32, I, ctypes.POINTER(F)]),
+        "spk_cache_gradients": (I, [P, P, P, P, U32, I, ctypes.POINTER(F), P, P]),
         "spk_cache_get_params": (I, [P, P, P]),
         "spk_cache_set_params": (I, [P, P, P]),
         "spk_cache_info": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U32), ctypes.POINTER(F)]),
@@ -612,22 +615,24 @@ class Context:
                                             ctypes.byref(loss)))
         return dict(num_weights=nw.value, num_table_params=nt.value, steps=st.value, last_loss=loss.value)
 
-    def cache_infer(self, points_dev, out_dev, n, world=False):
-        return self._check(self.lib.spk_cache_infer(self.h, _ptr(points_dev), n, 1 if world else 0, _ptr(out_dev)))
+    def cache_infer(self, points_dev, out_dev, n, world=False, attrs_dev=None):
+        """attrs_dev: CACHE_ATTRS floats per query (normal, albedo, roughness, 1) or None (zeros)."""
+        return self._check(self.lib.spk_cache_infer(self.h, _ptr(points_dev), _ptr(attrs_dev), n, 1 if world else 0,
+                                                    _ptr(out_dev)))
 
-    def cache_train(self, points_dev, targets_dev, n, world=False):
+    def cache_train(self, points_dev, targets_dev, n, world=False, attrs_dev=None):
         loss = ctypes.c_float()
-        self._check(self.lib.spk_cache_train(self.h, _ptr(points_dev), _ptr(targets_dev), n, 1 if world else 0,
-                                             ctypes.byref(loss)))
+        self._check(self.lib.spk_cache_train(self.h, _ptr(points_dev), _ptr(attrs_dev), _ptr(targets_dev), n,
+                                             1 if world else 0, ctypes.byref(loss)))
         return loss.value
 
-    def cache_gradients(self, points_dev, targets_dev, n, world=False, table=False):
+    def cache_gradients(self, points_dev, targets_dev, n, world=False, table=False, attrs_dev=None):
         info = self.cache_info()
         dw = np.zeros(info["num_weights"], dtype=np.float32)
         dt = np.zeros(info["num_table_params"], dtype=np.float32) if table else None
         loss = ctypes.c_float()
-        self._check(self.lib.spk_cache_gradients(self.h, _ptr(points_dev), _ptr(targets_dev), n, 1 if world else 0,
-                                                 ctypes.byref(loss), _ptr(dw), _ptr(dt)))
+        self._check(self.lib.spk_cache_gradients(self.h, _ptr(points_dev), _ptr(attrs_dev), _ptr(targets_dev), n,
+                                                 1 if world else 0, ctypes.byref(loss), _ptr(dw), _ptr(dt)))
         return loss.value, dw, dt
 
     def cache_get_params(self):

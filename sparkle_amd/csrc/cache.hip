@@ -1,10 +1,12 @@
 // Learned radiance cache (no counterpart in the reference): a 16-level hash-grid encoding
-// (Mueller et al., Instant NGP 2022) feeding a bias-free 32-64-64-64-64-3 ReLU MLP, trained online
-// with a relative L2 loss (Mueller et al., neural radiance caching 2021) and Adam (Kingma & Ba).
+// (Mueller et al., Instant NGP 2022) of the position, beside per-query surface attributes (shading
+// normal, albedo, roughness and a constant 1 that stands in for the first layer's bias), feeding a
+// bias-free 64-64-64-64-64-3 ReLU MLP, trained online with a relative L2 loss (Mueller et al.,
+// neural radiance caching 2021) and Adam (Kingma & Ba).
 //
 // Device layout:
 //   table      FP16 [levels][2^log2_table][2]   (+ FP32 master, Adam m/v, FP32 gradient)
-//   weights    FP32 master [out][in] per layer, concatenated (14,528 values for the default net)
+//   weights    FP32 master [out][in] per layer, concatenated (16,576 values for the default net)
 //              BF16 forward copy  Wf_l [out_pad16][in]  and backward copy Wb_l = W_l^T [in][out_pad32]
 // The MLP runs on CDNA4 matrix cores: v_mfma_f32_16x16x32_bf16 with neurons as rows (M), queries
 // as columns (N) and FP32 accumulation. One wave owns a tile of 64 queries; activations live in LDS
@@ -26,7 +28,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kQ = 64;        // queries per wave tile
-constexpr int kIn = 32;       // encoding width (levels * features)
+constexpr int kEnc = 32;      // encoding width (levels * features)
+constexpr int kAttr = 8;      // surface attributes per query (SPK_CACHE_ATTRS)
+constexpr int kIn = 64;       // MLP input: encoding, attributes, zeros
 constexpr int kH = 64;        // hidden width
 constexpr int kOutPad = 16;   // output rows padded to one MFMA tile
 constexpr int kStrideH = kH + 8;   // LDS row stride (bf16), padded against bank conflicts
@@ -183,9 +187,9 @@ struct TrainDev {
 // and gradients also go to the transposed global copies for the weight-gradient GEMM as soon as they
 // are produced. About 10 KB of LDS per wave.
 template <bool TRAIN>
-__device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__restrict__ points, uint32_t n,
-                                           int points_are_world, float *__restrict__ out, const TrainDev &td,
-                                           uint32_t base) {
+__device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__restrict__ points,
+                                           const float *__restrict__ attrs, uint32_t n, int points_are_world,
+                                           float *__restrict__ out, const TrainDev &td, uint32_t base) {
     __shared__ __bf16 P[kQ * kStrideH];
     __shared__ float Y[kQ * 3];
     __shared__ uint64_t MB[TRAIN ? 4 : 1][kQ];
@@ -202,6 +206,14 @@ __device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__res
     }
     __bf16 *p0 = P, *p1 = P;  // the same tile: every layer runs in place
     encode_point(c, p[0], p[1], p[2], valid, p0 + lane * kStrideH);
+    {
+        // the query's surface attributes after the encoding, then zeros up to the MLP's input width
+        __bf16 *row = p0 + lane * kStrideH;
+#pragma unroll
+        for (int k = 0; k < kAttr; ++k) row[kEnc + k] = (__bf16)(valid && attrs ? attrs[(size_t)kAttr * q + k] : 0.f);
+#pragma unroll
+        for (int k = kEnc + kAttr; k < kIn; ++k) row[k] = (__bf16)0.f;
+    }
     if (TRAIN)
         for (int l = 0; l < 4; ++l) MB[l][lane] = 0ull;
     __syncthreads();
@@ -216,8 +228,8 @@ __device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__res
         }
     };
     // forward: X (p0) -> H0 (p1) -> H1 (p0) -> H2 (p1) -> H3 (p0) -> Y
-    if (TRAIN) store_t(p0, kStrideH, 32, td.act_t);
-    mlp_layer<4, 1, true, false, TRAIN>(c.wf[0], p0, kStrideH, p1, kStrideH, nullptr, MB[0], nullptr, 0);
+    if (TRAIN) store_t(p0, kStrideH, kIn, td.act_t);
+    mlp_layer<4, kIn / 32, true, false, TRAIN>(c.wf[0], p0, kStrideH, p1, kStrideH, nullptr, MB[0], nullptr, 0);
     __syncthreads();
     if (TRAIN) store_t(p1, kStrideH, 64, td.act_t + blk);
     mlp_layer<4, 2, true, false, TRAIN>(c.wf[1], p1, kStrideH, p0, kStrideH, nullptr, MB[TRAIN ? 1 : 0], nullptr, 0);
@@ -270,20 +282,22 @@ __device__ __forceinline__ void cache_tile(const CacheDev &c, const float *__res
     mlp_layer<4, 2, false, true, false>(c.wb[1], p0, kStrideG, p1, kStrideG, MB[0], nullptr, nullptr, 0);  // dH0
     __syncthreads();
     store_t(p1, kStrideG, 64, td.grad_t);
-    mlp_layer<2, 2, false, false, false>(c.wb[0], p1, kStrideG, p0, kStrideG, nullptr, nullptr, nullptr, 0);  // dX
+    // dX of the encoding only (the first kEnc inputs; the attributes take no gradient)
+    mlp_layer<kEnc / 16, 2, false, false, false>(c.wb[0], p1, kStrideG, p0, kStrideG, nullptr, nullptr, nullptr, 0);
     __syncthreads();
     if (valid && td.gtable) encode_backward(c, p[0], p[1], p[2], p0 + lane * kStrideG, td.gtable);
 }
 
 // Persistent over 64-query tiles; the query count comes from n_dev (device) when given, capped at n.
 template <bool TRAIN>
-__global__ __launch_bounds__(64) void k_cache_mlp(CacheDev c, const float *__restrict__ points, uint32_t n,
+__global__ __launch_bounds__(64) void k_cache_mlp(CacheDev c, const float *__restrict__ points,
+                                                  const float *__restrict__ attrs, uint32_t n,
                                                   const uint32_t *__restrict__ n_dev, int points_are_world,
                                                   float *__restrict__ out, TrainDev td) {
     const uint32_t count = n_dev ? min(*n_dev, n) : n;
     if (TRAIN && n_dev && count < kMinTrainSamples) return;  // the frame takes no training step
     for (uint32_t tile = blockIdx.x; tile * (uint32_t)kQ < count; tile += gridDim.x) {
-        cache_tile<TRAIN>(c, points, count, points_are_world, out, td, tile * (uint32_t)kQ);
+        cache_tile<TRAIN>(c, points, attrs, count, points_are_world, out, td, tile * (uint32_t)kQ);
         __syncthreads();
     }
 }
@@ -301,12 +315,12 @@ __global__ __launch_bounds__(256) void k_cache_dw(const __bf16 *__restrict__ act
     if (k0 >= kmax) return;
     const int layer = blockIdx.y;
     const int rows = layer == 4 ? 3 : 64;
-    const int cols = layer == 0 ? 32 : 64;
+    const int cols = layer == 0 ? kIn : 64;
     const size_t blk = (size_t)64 * n_pad;
     const __bf16 *A = grad_t + layer * blk;  // [rows][n]
     const __bf16 *B = act_t + layer * blk;   // [cols][n]
     size_t woff = 0;
-    for (int l = 0; l < layer; ++l) woff += (size_t)(l == 0 ? 64 * 32 : 64 * 64);
+    for (int l = 0; l < layer; ++l) woff += (size_t)(l == 0 ? 64 * kIn : 64 * 64);
     const int wave = threadIdx.x >> 6;
     const int lane = __lane_id();
     const int r = lane & 15, g = lane >> 4;
@@ -368,7 +382,7 @@ __global__ void k_adam(float *__restrict__ p, const G *__restrict__ grad, float 
 // FP32 master weights -> BF16 forward [out_pad][in] and backward [in][out_pad32] copies
 __global__ void k_pack_weights(const float *__restrict__ w, __bf16 *__restrict__ wf, __bf16 *__restrict__ wb) {
     // layer geometry: in, out, forward out pad, backward out pad, offsets
-    const int ins[5] = {32, 64, 64, 64, 64}, outs[5] = {64, 64, 64, 64, 3};
+    const int ins[5] = {kIn, 64, 64, 64, 64}, outs[5] = {64, 64, 64, 64, 3};
     const int fpad[5] = {64, 64, 64, 64, 16}, bpad[5] = {64, 64, 64, 64, 32};
     int off = 0, foff = 0, boff = 0;
     for (int l = 0; l < 5; ++l) {
@@ -398,13 +412,23 @@ __global__ void k_f32_to_half(const float *__restrict__ f, __half *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------------------------
-static const int kLayerIn[5] = {32, 64, 64, 64, 64}, kLayerOut[5] = {64, 64, 64, 64, 3};
+static const int kLayerIn[5] = {kIn, 64, 64, 64, 64}, kLayerOut[5] = {64, 64, 64, 64, 3};
 static const int kFPad[5] = {64, 64, 64, 64, 16}, kBPad[5] = {64, 64, 64, 64, 32};
 
 size_t cache_num_weights() {
     size_t s = 0;
     for (int l = 0; l < 5; ++l) s += (size_t)kLayerIn[l] * kLayerOut[l];
     return s;
+}
+
+// BF16 values in each of the packed forward and backward weight copies (the larger of the two)
+static size_t cache_packed_bf16() {
+    size_t f = 0, b = 0;
+    for (int l = 0; l < 5; ++l) {
+        f += (size_t)kFPad[l] * kLayerIn[l];
+        b += (size_t)kLayerIn[l] * kBPad[l];
+    }
+    return f > b ? f : b;
 }
 
 static CacheDev make_dev(const CacheState &cs) {
@@ -439,26 +463,27 @@ hipError_t cache_pack(CacheState &cs, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t cache_infer(CacheState &cs, const float *points, uint32_t n, int world, float *out, hipStream_t s) {
+hipError_t cache_infer(CacheState &cs, const float *points, const float *attrs, uint32_t n, int world, float *out,
+                       hipStream_t s) {
     if (n == 0) return hipSuccess;
     TrainDev td;
     std::memset(&td, 0, sizeof(td));
-    hipLaunchKernelGGL(k_cache_mlp<false>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, n,
+    hipLaunchKernelGGL(k_cache_mlp<false>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, attrs, n,
                        (const uint32_t *)nullptr, world, out, td);
     return hipGetLastError();
 }
 
-hipError_t cache_infer_queue(CacheState &cs, const float *points, const uint32_t *count_dev, uint32_t capacity,
-                             float *out, int grid, hipStream_t s) {
+hipError_t cache_infer_queue(CacheState &cs, const float *points, const float *attrs, const uint32_t *count_dev,
+                             uint32_t capacity, float *out, int grid, hipStream_t s) {
     TrainDev td;
     std::memset(&td, 0, sizeof(td));
-    hipLaunchKernelGGL(k_cache_mlp<false>, dim3(grid), dim3(64), 0, s, make_dev(cs), points, capacity, count_dev, 1,
-                       out, td);
+    hipLaunchKernelGGL(k_cache_mlp<false>, dim3(grid), dim3(64), 0, s, make_dev(cs), points, attrs, capacity,
+                       count_dev, 1, out, td);
     return hipGetLastError();
 }
 
-hipError_t cache_gradients(CacheState &cs, const float *points, const float *targets, uint32_t n, int world,
-                           float *out, hipStream_t s, const uint32_t *n_dev) {
+hipError_t cache_gradients(CacheState &cs, const float *points, const float *attrs, const float *targets, uint32_t n,
+                           int world, float *out, hipStream_t s, const uint32_t *n_dev) {
     hipError_t e;
     const uint32_t n_pad = ((n + 255u) / 256u) * 256u;
     if ((e = cs.act_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
@@ -474,8 +499,8 @@ hipError_t cache_gradients(CacheState &cs, const float *points, const float *tar
     td.act_t = (__bf16 *)cs.act_t.ptr;
     td.grad_t = (__bf16 *)cs.grad_t.ptr;
     td.n_pad = n_pad;
-    hipLaunchKernelGGL(k_cache_mlp<true>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, n, n_dev,
-                       world, out, td);
+    hipLaunchKernelGGL(k_cache_mlp<true>, dim3((n + kQ - 1) / kQ), dim3(64), 0, s, make_dev(cs), points, attrs, n,
+                       n_dev, world, out, td);
     hipLaunchKernelGGL(k_cache_dw, dim3(n_pad / 256, 5), dim3(256), 0, s, (const __bf16 *)cs.act_t.ptr,
                        (const __bf16 *)cs.grad_t.ptr, n_pad, n, n_dev, (float *)cs.gw.ptr);
     return hipGetLastError();
@@ -538,7 +563,7 @@ static int cache_init_params(spk_context *ctx, CacheState &cs) {
     for (auto &x : tab) x = rng.uniform(-1e-4f, 1e-4f);
     size_t off = 0;
     for (int l = 0; l < 5; ++l) {
-        const int in = l == 0 ? 32 : 64, out = l == 4 ? 3 : 64;
+        const int in = kLayerIn[l], out = kLayerOut[l];
         const float lim = std::sqrt(6.f / (float)(in + out));
         for (int k = 0; k < in * out; ++k) w[off + k] = rng.uniform(-lim, lim);
         off += (size_t)in * out;
@@ -547,7 +572,8 @@ static int cache_init_params(spk_context *ctx, CacheState &cs) {
         (e = hipMemcpy(cs.w.ptr, w.data(), nw * 4, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(cs.mt.ptr, 0, ntab * 4)) != hipSuccess || (e = hipMemset(cs.vt.ptr, 0, ntab * 4)) != hipSuccess ||
         (e = hipMemset(cs.mw.ptr, 0, nw * 4)) != hipSuccess || (e = hipMemset(cs.vw.ptr, 0, nw * 4)) != hipSuccess ||
-        (e = hipMemset(cs.wf.ptr, 0, 16384 * 2)) != hipSuccess || (e = hipMemset(cs.wb.ptr, 0, 16384 * 2)) != hipSuccess ||
+        (e = hipMemset(cs.wf.ptr, 0, cache_packed_bf16() * 2)) != hipSuccess ||
+        (e = hipMemset(cs.wb.ptr, 0, cache_packed_bf16() * 2)) != hipSuccess ||
         (e = hipMemset(cs.state.ptr, 0, sizeof(AdamStep))) != hipSuccess)
         return hip_fail(ctx, e, "cache initialisation");
     if ((e = cache_pack(cs, ctx->stream)) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
@@ -599,8 +625,8 @@ int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
         (e = cs.mt.ensure(ntab * 4)) != hipSuccess || (e = cs.vt.ensure(ntab * 4)) != hipSuccess ||
         (e = cs.gtable.ensure(ntab * 2)) != hipSuccess || (e = cs.w.ensure(nw * 4)) != hipSuccess ||
         (e = cs.mw.ensure(nw * 4)) != hipSuccess || (e = cs.vw.ensure(nw * 4)) != hipSuccess ||
-        (e = cs.gw.ensure(nw * 4)) != hipSuccess || (e = cs.wf.ensure(16384 * 2)) != hipSuccess ||
-        (e = cs.wb.ensure(16384 * 2)) != hipSuccess || (e = cs.state.ensure(sizeof(AdamStep))) != hipSuccess)
+        (e = cs.gw.ensure(nw * 4)) != hipSuccess || (e = cs.wf.ensure(cache_packed_bf16() * 2)) != hipSuccess ||
+        (e = cs.wb.ensure(cache_packed_bf16() * 2)) != hipSuccess || (e = cs.state.ensure(sizeof(AdamStep))) != hipSuccess)
         return hip_fail(ctx, e, "cache allocation");
     int rc = cache_init_params(ctx, cs);
     if (rc != SPK_OK) return rc;
@@ -626,23 +652,24 @@ int spk_cache_destroy(spk_context *ctx) {
     return SPK_OK;
 }
 
-int spk_cache_infer(spk_context *ctx, const float *points_dev, uint32_t n, int world, float *out_dev) {
+int spk_cache_infer(spk_context *ctx, const float *points_dev, const float *attrs_dev, uint32_t n, int world,
+                    float *out_dev) {
     int rc = cache_check(ctx);
     if (rc != SPK_OK) return rc;
     if (n && (!points_dev || !out_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null array");
-    hipError_t e = cache_infer(*ctx->cache, points_dev, n, world, out_dev, ctx->stream);
+    hipError_t e = cache_infer(*ctx->cache, points_dev, attrs_dev, n, world, out_dev, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "cache inference");
     return SPK_OK;
 }
 
-int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n, int world,
-                        float *loss, float *dweights, float *dtable) {
+int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *attrs_dev, const float *targets_dev,
+                        uint32_t n, int world, float *loss, float *dweights, float *dtable) {
     int rc = cache_check(ctx);
     if (rc != SPK_OK) return rc;
     if (n && (!points_dev || !targets_dev)) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null array");
     CacheState &cs = *ctx->cache;
-    hipError_t e = cache_gradients(cs, points_dev, targets_dev, n, world, nullptr, ctx->stream);
+    hipError_t e = cache_gradients(cs, points_dev, attrs_dev, targets_dev, n, world, nullptr, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess && loss) e = hipMemcpy(loss, &cache_state(cs)->loss, 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && dweights) e = hipMemcpy(dweights, cs.gw.ptr, cache_num_weights() * 4, hipMemcpyDeviceToHost);
@@ -659,14 +686,14 @@ int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *
     return SPK_OK;
 }
 
-int spk_cache_train(spk_context *ctx, const float *points_dev, const float *targets_dev, uint32_t n, int world,
-                    float *loss) {
+int spk_cache_train(spk_context *ctx, const float *points_dev, const float *attrs_dev, const float *targets_dev,
+                    uint32_t n, int world, float *loss) {
     int rc = cache_check(ctx);
     if (rc != SPK_OK) return rc;
     if (n == 0) return SPK_OK;
     if (!points_dev || !targets_dev) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null array");
     CacheState &cs = *ctx->cache;
-    hipError_t e = cache_gradients(cs, points_dev, targets_dev, n, world, nullptr, ctx->stream);
+    hipError_t e = cache_gradients(cs, points_dev, attrs_dev, targets_dev, n, world, nullptr, ctx->stream);
     if (e == hipSuccess) e = cache_adam(cs, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     float l = 0.f;

@@ -121,9 +121,11 @@ struct PtArgs {
     uint32_t cache_on, cache_bounce, train_cap;
     float train_prob;
     float *cq_points;  // 3 per query
+    float4 *cq_attrs;  // 2 per query: (normal, albedo.r), (albedo.gb, roughness, 1)
     float4 *cq_meta;   // (throughput, path)
     float *cq_out;     // 3 per query
     float *tr_points;  // 3 per sample
+    float4 *tr_attrs;  // 2 per sample, as cq_attrs
     float4 *tr_meta0;  // (colour before the vertex, path)
     float4 *tr_meta1;  // (throughput at the vertex, 0)
     float *tr_targets; // 3 per sample
@@ -558,6 +560,7 @@ struct ShadeOut {
     bool cont, shadow, cache_q, train;
     float4 sh0, sh1, sh2;
     F3 loc, Tv, Cv;
+    float4 at0, at1;  // the cache attributes of a query or training vertex
 };
 
 // Shades path `path` at hit record h of segment `bounce`: adds emission/sky to its colour, samples the
@@ -572,6 +575,7 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
     cont = shadow = cache_q = train = false;
     sh0 = sh1 = sh2 = make_float4(0.f, 0.f, 0.f, 0.f);
     loc = Tv = Cv = f3(0.f, 0.f, 0.f);
+    o.at0 = o.at1 = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 rd4 = a.rays[2 * path + 1], ro4 = a.rays[2 * path];
     const F3 d = f3(rd4.x, rd4.y, rd4.z);
     const float4 T4 = a.thr[path];
@@ -657,10 +661,14 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
             // terminate into the cache: radiance leaving this vertex comes from the network
             cache_q = true;
             Tv = T;
+            o.at0 = make_float4(normal.x, normal.y, normal.z, me.base.x);
+            o.at1 = make_float4(me.base.y, me.base.z, me.roughness, 1.f);
         } else {
             if (cache_vertex) {  // a training path: record the vertex, its target follows
                 train = true;
                 Tv = T;
+                o.at0 = make_float4(normal.x, normal.y, normal.z, me.base.x);
+                o.at1 = make_float4(me.base.y, me.base.z, me.roughness, 1.f);
                 Cv = C;
             }
             uint4 rs = a.rng[path];
@@ -786,6 +794,8 @@ __device__ __forceinline__ void append_cache_records(const PtArgs &a, uint32_t p
         a.cq_points[3 * (size_t)cs + 1] = o.loc.y;
         a.cq_points[3 * (size_t)cs + 2] = o.loc.z;
         a.cq_meta[cs] = make_float4(o.Tv.x, o.Tv.y, o.Tv.z, __uint_as_float(path));
+        a.cq_attrs[2 * (size_t)cs] = o.at0;
+        a.cq_attrs[2 * (size_t)cs + 1] = o.at1;
     }
     const uint32_t ts = wave_append(a.counters + kCtrGlobal + GCTR_TRAIN, o.train);
     if (o.train && ts < a.train_cap) {
@@ -794,6 +804,8 @@ __device__ __forceinline__ void append_cache_records(const PtArgs &a, uint32_t p
         a.tr_points[3 * (size_t)ts + 2] = o.loc.z;
         a.tr_meta0[ts] = make_float4(o.Cv.x, o.Cv.y, o.Cv.z, __uint_as_float(path));
         a.tr_meta1[ts] = make_float4(o.Tv.x, o.Tv.y, o.Tv.z, 0.f);
+        a.tr_attrs[2 * (size_t)ts] = o.at0;
+        a.tr_attrs[2 * (size_t)ts + 1] = o.at1;
     }
 }
 
@@ -952,6 +964,13 @@ __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
 }
 
 // ---- cache: add throughput x predicted radiance; build training targets ----
+// The network predicts outgoing radiance divided by the vertex albedo (floored), which varies less
+// across materials and textures than the radiance itself; the albedo is multiplied back here.
+__device__ __forceinline__ F3 demod_albedo(const float4 at0, const float4 at1) {
+    return f3(fmaxf(at0.w, SPK_CACHE_ALBEDO_FLOOR), fmaxf(at1.x, SPK_CACHE_ALBEDO_FLOOR),
+              fmaxf(at1.y, SPK_CACHE_ALBEDO_FLOOR));
+}
+
 __global__ __launch_bounds__(kPtBlock) void k_cache_apply(PtArgs a) {
     const uint32_t count = min(a.counters[kCtrGlobal + GCTR_CACHE_Q], a.npix);
     for (uint32_t base = grid_wave() * 64u; base < count; base += grid_waves() * 64u) {
@@ -959,15 +978,17 @@ __global__ __launch_bounds__(kPtBlock) void k_cache_apply(PtArgs a) {
         if (i >= count) continue;
         const float4 m = a.cq_meta[i];
         const uint32_t path = __float_as_uint(m.w);
+        const F3 alb = demod_albedo(a.cq_attrs[2 * (size_t)i], a.cq_attrs[2 * (size_t)i + 1]);
         float4 C = a.col[path];
-        C.x += m.x * fmaxf(a.cq_out[3 * (size_t)i], 0.f);
-        C.y += m.y * fmaxf(a.cq_out[3 * (size_t)i + 1], 0.f);
-        C.z += m.z * fmaxf(a.cq_out[3 * (size_t)i + 2], 0.f);
+        C.x += m.x * (fmaxf(a.cq_out[3 * (size_t)i], 0.f) * alb.x);
+        C.y += m.y * (fmaxf(a.cq_out[3 * (size_t)i + 1], 0.f) * alb.y);
+        C.z += m.z * (fmaxf(a.cq_out[3 * (size_t)i + 2], 0.f) * alb.z);
         a.col[path] = C;
     }
 }
 
-// target = radiance gathered after the vertex / throughput at the vertex (clamped to [0, 64])
+// target = radiance gathered after the vertex / throughput at the vertex (clamped to [0, 64]), divided
+// by the vertex albedo (floored)
 __global__ __launch_bounds__(kPtBlock) void k_cache_targets(PtArgs a) {
     const uint32_t count = min(a.counters[kCtrGlobal + GCTR_TRAIN], a.train_cap);
     for (uint32_t base = grid_wave() * 64u; base < count; base += grid_waves() * 64u) {
@@ -977,8 +998,11 @@ __global__ __launch_bounds__(kPtBlock) void k_cache_targets(PtArgs a) {
         const float4 C = a.col[__float_as_uint(m0.w)];
         const float g[3] = {C.x - m0.x, C.y - m0.y, C.z - m0.z};
         const float tt[3] = {m1.x, m1.y, m1.z};
+        const F3 alb = demod_albedo(a.tr_attrs[2 * (size_t)i], a.tr_attrs[2 * (size_t)i + 1]);
+        const float al[3] = {alb.x, alb.y, alb.z};
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.tr_targets[3 * (size_t)i + k] = fminf(fmaxf(g[k] / fmaxf(tt[k], 1e-4f), 0.f), 64.f);
+        for (int k = 0; k < 3; ++k)
+            a.tr_targets[3 * (size_t)i + k] = fminf(fmaxf(g[k] / fmaxf(tt[k], 1e-4f), 0.f), 64.f) / al[k];
     }
 }
 
@@ -1212,6 +1236,7 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
         if (cs.capacity < npix) {
             const size_t n = npix, tcap = cs.cfg.batch;
             if ((e = cs.cq_points.ensure(n * 12)) != hipSuccess || (e = cs.cq_meta.ensure(n * 16)) != hipSuccess ||
+                (e = cs.cq_attrs.ensure(n * 32)) != hipSuccess || (e = cs.tr_attrs.ensure(tcap * 32)) != hipSuccess ||
                 (e = cs.cq_out.ensure(n * 12)) != hipSuccess || (e = cs.tr_points.ensure(tcap * 12)) != hipSuccess ||
                 (e = cs.tr_meta0.ensure(tcap * 16)) != hipSuccess || (e = cs.tr_meta1.ensure(tcap * 16)) != hipSuccess ||
                 (e = cs.tr_targets.ensure(tcap * 12)) != hipSuccess)
@@ -1246,7 +1271,8 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
     }
     if (a.tail_from < a.max_bounce) hipLaunchKernelGGL(pe.tail, dim3(grid), dim3(kPtBlock), 0, s, a, a.tail_from);
     if (a.cache_on) {
-        hipError_t e = cache_infer_queue(*cache, a.cq_points, a.counters + kCtrGlobal + GCTR_CACHE_Q, a.npix, a.cq_out,
+        hipError_t e = cache_infer_queue(*cache, a.cq_points, (const float *)a.cq_attrs,
+                                         a.counters + kCtrGlobal + GCTR_CACHE_Q, a.npix, a.cq_out,
                                          cus * 8, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(kPtBlock), 0, s, a);
@@ -1428,6 +1454,8 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
                                                    : std::min(1.f, (float)cs.cfg.batch / (float)npix / 2.f);
         a.cq_points = (float *)cs.cq_points.ptr;
         a.cq_meta = (float4 *)cs.cq_meta.ptr;
+        a.cq_attrs = (float4 *)cs.cq_attrs.ptr;
+        a.tr_attrs = (float4 *)cs.tr_attrs.ptr;
         a.cq_out = (float *)cs.cq_out.ptr;
         a.tr_points = (float *)cs.tr_points.ptr;
         a.tr_meta0 = (float4 *)cs.tr_meta0.ptr;
@@ -1476,7 +1504,8 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     if (train) {
         CacheState &cs = *ctx->cache;
         const uint32_t *cnt_dev = (const uint32_t *)ps.counters.ptr + kCtrGlobal + GCTR_TRAIN;
-        if ((e = cache_gradients(cs, (const float *)cs.tr_points.ptr, (const float *)cs.tr_targets.ptr, cs.cfg.batch,
+        if ((e = cache_gradients(cs, (const float *)cs.tr_points.ptr, (const float *)cs.tr_attrs.ptr,
+                                 (const float *)cs.tr_targets.ptr, cs.cfg.batch,
                                  1, nullptr, ctx->stream, cnt_dev)) != hipSuccess ||
             (e = cache_adam(cs, ctx->stream, cnt_dev, cs.cfg.batch)) != hipSuccess)
             return hip_fail(ctx, e, "cache training");

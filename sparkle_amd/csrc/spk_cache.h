@@ -30,24 +30,27 @@ struct CacheState {
     DeviceBuffer w, mw, vw, gw, wf, wb;             // MLP weights (FP32 master, Adam, gradient, BF16 copies)
     DeviceBuffer act_t, grad_t, state;              // training scratch, AdamStep
     // per-frame records of the path tracer
-    DeviceBuffer cq_points, cq_meta, cq_out;        // cache queries: point, (throughput, path), radiance
-    DeviceBuffer tr_points, tr_meta0, tr_meta1, tr_targets;  // training samples
+    DeviceBuffer cq_points, cq_attrs, cq_meta, cq_out;  // cache queries: point, attributes, (throughput, path),
+                                                        // radiance / albedo
+    DeviceBuffer tr_points, tr_attrs, tr_meta0, tr_meta1, tr_targets;  // training samples
     uint32_t capacity = 0;
     uint64_t table_params() const { return (uint64_t)cfg.levels * (1ull << cfg.log2_table) * cfg.features; }
 };
 
 size_t cache_num_weights();
 hipError_t cache_pack(CacheState &cs, hipStream_t s);
-hipError_t cache_infer(CacheState &cs, const float *points, uint32_t n, int world, float *out, hipStream_t s);
+// attrs (device, may be null): SPK_CACHE_ATTRS floats per query (normal, albedo, roughness, 1); null: zeros
+hipError_t cache_infer(CacheState &cs, const float *points, const float *attrs, uint32_t n, int world, float *out,
+                       hipStream_t s);
 // n_dev (device, may be null): the sample count, capped at n; with it, fewer than kMinTrainSamples
 // samples make the gradient pass and the following cache_adam (given the same n_dev) no-ops.
-hipError_t cache_gradients(CacheState &cs, const float *points, const float *targets, uint32_t n, int world,
-                           float *out, hipStream_t s, const uint32_t *n_dev = nullptr);
+hipError_t cache_gradients(CacheState &cs, const float *points, const float *attrs, const float *targets, uint32_t n,
+                           int world, float *out, hipStream_t s, const uint32_t *n_dev = nullptr);
 hipError_t cache_adam(CacheState &cs, hipStream_t s, const uint32_t *n_dev = nullptr, uint32_t cap = 0);
 inline AdamStep *cache_state(CacheState &cs) { return static_cast<AdamStep *>(cs.state.ptr); }
 // Inference over a queue whose length lives in device memory (capturable in a graph): `grid`
 // persistent single-wave blocks walk the 64-query tiles.
-hipError_t cache_infer_queue(CacheState &cs, const float *points, const uint32_t *count_dev, uint32_t capacity,
-                             float *out, int grid, hipStream_t s);
+hipError_t cache_infer_queue(CacheState &cs, const float *points, const float *attrs, const uint32_t *count_dev,
+                             uint32_t capacity, float *out, int grid, hipStream_t s);
 
 }  // namespace spk

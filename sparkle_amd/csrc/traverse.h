@@ -344,6 +344,11 @@ struct Traversal {
         g_hits = WIDTH == 8 ? 1u << oct : 1u;  // 4-wide: rank order is resolved per node below
         g_oct = WIDTH == 8 ? oct : 0u;
         sp = 0;
+        // a ray with a NaN or infinite component or a zero direction hits nothing (the triangle test
+        // rejects it everywhere); it ends here rather than walking the whole tree, which its NaN slab
+        // distances would otherwise open (a refraction past the critical angle gives such a
+        // direction when the Fresnel draw is exactly 1, BxDF.h:112-141)
+        if (!ray_valid(ray)) g_hits = 0u;
         best.t = ray.tmax;
         best.prim = 0xFFFFFFFFu;
         best.tri = 0xFFFFFFFFu;

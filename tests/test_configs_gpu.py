@@ -127,7 +127,7 @@ def test_batch_of_64_scenes_at_size():
 def test_cache_at_benchmark_size_matches_fp64(gpu_ctx):
     import torch
 
-    from tests.test_cache_gpu import oracle_forward, to_dev
+    from tests.test_cache_gpu import oracle_forward, random_attrs, to_dev
 
     w, h = 240, 136
     sc, cam = scenes.many_instance_scene(seed=0, aspect=w / h)
@@ -148,22 +148,47 @@ def test_cache_at_benchmark_size_matches_fp64(gpu_ctx):
     assert info["steps"] >= 20 and np.isfinite(losses).all()
     assert np.isfinite(rgba.cpu().numpy()).all()
     wts, tab = gpu_ctx.cache_get_params()
-    cfg = refcpu.cache_cfg(log2_table=19)
+    cfg = refcpu.cache_cfg(log2_table=19, inputs=spk.CACHE_INPUTS)
     rng = np.random.Generator(np.random.PCG64(11))
     n = 256
     pts = rng.random((n, 3)).astype(np.float32)  # the cache's own [0,1]^3 frame of the scene box
+    at = random_attrs(rng, n)
     out = torch.zeros(n * 3, device="cuda")
     print_case("cache config forward")
-    gpu_ctx.cache_infer(to_dev(pts.reshape(-1)), out, n)
+    gpu_ctx.cache_infer(to_dev(pts.reshape(-1)), out, n, attrs_dev=to_dev(at.reshape(-1)))
     g = out.cpu().numpy().reshape(n, 3)
     targets = rng.random((n, 3)).astype(np.float32)
-    ref_loss, ref, ref_dw, ref_dx, _ = oracle_forward(cfg, wts, tab, pts, targets)
+    ref_loss, ref, ref_dw, ref_dx, _ = oracle_forward(cfg, wts, tab, pts, targets, at)
     err = np.abs(g - ref)
     assert (err <= 0.03 * np.abs(ref) + 2e-3).mean() > 0.99, err.max()
     print_case("cache config gradients")
-    loss, dw, dtab = gpu_ctx.cache_gradients(to_dev(pts.reshape(-1)), to_dev(targets.reshape(-1)), n, table=True)
+    loss, dw, dtab = gpu_ctx.cache_gradients(to_dev(pts.reshape(-1)), to_dev(targets.reshape(-1)), n, table=True,
+                                             attrs_dev=to_dev(at.reshape(-1)))
     assert abs(loss - ref_loss) <= 0.03 * abs(ref_loss)
     assert np.linalg.norm(dw - ref_dw) / np.linalg.norm(ref_dw) < 0.05
     ref_tab = refcpu.table_backward(cfg, pts, ref_dx)
     assert np.linalg.norm(dtab - ref_tab) / np.linalg.norm(ref_tab) < 0.05
     gpu_ctx.cache_destroy()
+
+
+def test_path_with_a_nan_direction_does_not_walk_the_tree(gpu_ctx):
+    """Frame seed 1,000,559 of the 1080p many-instance render holds a path whose refraction past the
+    critical angle met a Fresnel draw of exactly 1 (RandomUnit reaches 1.0), so its next direction is
+    NaN, as in the reference. Such a ray hits nothing; it used to open every box of the tree (NaN slab
+    distances) and cost ~10 s of one lane. The frame now takes as long as its neighbours."""
+    import torch
+
+    w, h = 1920, 1080
+    sc, cam = scenes.many_instance_scene(seed=0, aspect=w / h)
+    print_case("nan-direction frame upload")
+    gpu_ctx.upload_scene(sc)
+    gpu_ctx.set_camera(cam)
+    gpu_ctx.set_schedule(8, 0, 0, 1)
+    rgba = torch.zeros(w * h * 4, dtype=torch.float32, device="cuda")
+    times = []
+    for seed in (999_997, 1_000_558, 1_000_559, 1_000_560):
+        print_case("nan-direction frame", seed)
+        st = gpu_ctx.render_frame(frame(w, h, seed, bounce=8, flags=spk.FRAME_SEED64, limit=1e30), rgba)
+        times.append(st.milliseconds)
+    print("frame ms", times)
+    assert times[2] < 5 * max(times[1], times[3]) and times[2] < 1000.0
