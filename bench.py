@@ -410,7 +410,7 @@ RENDER_MODES = {"plain": (0, None), "plain_nee": ("NEE", None), "cache_nee": ("N
 
 
 def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_bounce=8, autotune=True,
-                      mode="plain"):
+                      mode="plain", schedule=None):
     """The full path trace of the many-instance scene (about 5.5M triangles, 400 instances,
     3 area lights): spp frames of one path per pixel, max_bounce bounces, Russian roulette from
     bounce 3, material-sorted queues, the frame graph replayed; schedule from spk_render_autotune.
@@ -441,7 +441,10 @@ def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_
     if cache_bounce is not None:
         ctx.cache_create(spk.cache_config(terminate_bounce=cache_bounce))
     sched = ctx.schedule()
-    if autotune:
+    if schedule is not None:
+        ctx.set_schedule(*schedule)
+        sched = tuple(schedule)
+    elif autotune:
         sched, _ = ctx.autotune_render(frame(10 ** 6), rgba)
     ctx.render_frame(frame(10 ** 6 + 1), rgba)
     torch.cuda.synchronize()
@@ -666,6 +669,8 @@ def main():
                     help="only run the full path trace of the many-instance scene for N spp (one JSON line)")
     ap.add_argument("--render-mode", choices=sorted(RENDER_MODES), default="plain",
                     help="estimator of --render-only")
+    ap.add_argument("--render-schedule", type=str, default=None,
+                    help="blocks_per_cu,material_sort,refill,tail_bounce for --render-only (default: autotuned)")
     ap.add_argument("--two-level", action="store_true",
                     help="only the flattened vs two-level comparison on the instanced scene (one JSON line)")
     ap.add_argument("--relmse-only", action="store_true",
@@ -693,8 +698,9 @@ def main():
 
         stream = torch.cuda.Stream()
         torch.cuda.set_stream(stream)
+        fixed = tuple(int(x) for x in args.render_schedule.split(",")) if args.render_schedule else None
         print(json.dumps({"full_path": render_experiment(spk, scenes, stream, spp=args.render_only,
-                                                         mode=args.render_mode)}), flush=True)
+                                                         mode=args.render_mode, schedule=fixed)}), flush=True)
         return
     if args.two_level:
         import torch

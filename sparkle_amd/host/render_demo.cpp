@@ -1,10 +1,12 @@
 // Minimal C++ host of the drop-in path: builds a small Cornell-style room, renders it through
 // GpuPathRenderer (CPURenderer's role over the C ABI) and writes the tone-mapped image as PFM.
-//   render_demo <out.pfm> [width height frames [spatial_denoise 0/1]]
+//   render_demo <out.pfm> [width height frames [spatial_denoise 0/1 [accumulated.pfm]]]
+// The last line of stdout is a JSON record with the frame statistics and the camera record used.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "gpu_renderer.h"
@@ -113,6 +115,16 @@ int main(int argc, char **argv) {
         ms += r.LastFrameMilliseconds();
         rays += r.LastFrameRays();
     }
+    if (argc > 6) {
+        // the accumulated frame buffer before tone mapping, row j = 0 (v = 0) first
+        std::vector<float> acc;
+        if ((rc = r.ReadbackFrameBuffer(acc)) != SPK_OK) return 1;
+        FILE *fa = std::fopen(argv[6], "wb");
+        if (!fa) return 1;
+        std::fprintf(fa, "PF\n%u %u\n-1.0\n", W, H);
+        for (size_t k = 0; k < (size_t)W * H; ++k) std::fwrite(&acc[4 * k], 4, 3, fa);
+        std::fclose(fa);
+    }
     std::vector<float> img;
     if ((rc = r.ReadbackImage(img)) != SPK_OK) return 1;
     FILE *fp = std::fopen(out, "wb");
@@ -124,8 +136,17 @@ int main(int argc, char **argv) {
     double mean = 0.0;
     for (size_t k = 0; k < img.size(); k += 4) mean += img[k] + img[k + 1] + img[k + 2];
     mean /= 3.0 * W * H;
+    auto vec = [](const float *v) {
+        char b[96];
+        std::snprintf(b, sizeof(b), "[%.9g, %.9g, %.9g]", v[0], v[1], v[2]);
+        return std::string(b);
+    };
     std::printf("{\"frames\": %d, \"width\": %u, \"height\": %u, \"device_ms\": %.3f, \"rays\": %llu, "
-                "\"mean\": %.6f, \"cumulated\": %u}\n",
-                frames, W, H, ms, (unsigned long long)rays, mean, r.CumulatedSampleCount());
+                "\"mean\": %.6f, \"cumulated\": %u, \"camera\": {\"position\": %s, \"right\": %s, \"up\": %s, "
+                "\"front\": %s, \"lower_left\": %s, \"max_u\": %s, \"max_v\": %s, \"lens_radius\": %.9g, "
+                "\"far_plane\": %.9g}}\n",
+                frames, W, H, ms, (unsigned long long)rays, mean, r.CumulatedSampleCount(), vec(cam.position).c_str(),
+                vec(cam.right).c_str(), vec(cam.up).c_str(), vec(cam.front).c_str(), vec(cam.lower_left).c_str(),
+                vec(cam.max_u).c_str(), vec(cam.max_v).c_str(), cam.lens_radius, cam.far_plane);
     return 0;
 }
