@@ -402,6 +402,13 @@ __global__ void k_pack_weights(const float *__restrict__ w, __bf16 *__restrict__
     }
 }
 
+// clears n 32-bit words (a kernel rather than hipMemsetAsync, so the training step can sit inside a
+// captured frame graph)
+__global__ void k_zero_words(uint32_t *__restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 __global__ void k_half_to_f32(const __half *__restrict__ h, float *__restrict__ f, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) f[i] = __half2float(h[i]);
@@ -482,16 +489,22 @@ hipError_t cache_infer_queue(CacheState &cs, const float *points, const float *a
     return hipGetLastError();
 }
 
+hipError_t cache_prepare(CacheState &cs, uint32_t n) {
+    const uint32_t n_pad = ((n + 255u) / 256u) * 256u;
+    hipError_t e = cs.act_t.ensure((size_t)5 * 64 * n_pad * 2);
+    if (e == hipSuccess) e = cs.grad_t.ensure((size_t)5 * 64 * n_pad * 2);
+    return e;
+}
+
 hipError_t cache_gradients(CacheState &cs, const float *points, const float *attrs, const float *targets, uint32_t n,
                            int world, float *out, hipStream_t s, const uint32_t *n_dev) {
     hipError_t e;
     const uint32_t n_pad = ((n + 255u) / 256u) * 256u;
-    if ((e = cs.act_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
-    if ((e = cs.grad_t.ensure((size_t)5 * 64 * n_pad * 2)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(cs.gw.ptr, 0, cache_num_weights() * 4, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(cs.gtable.ptr, 0, cs.table_params() * 2, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(&cache_state(cs)->loss, 0, 4, s)) != hipSuccess) return e;
-    if (n == 0) return hipSuccess;
+    if ((e = cache_prepare(cs, n)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, (uint32_t *)cs.gw.ptr, (uint64_t)cache_num_weights());
+    hipLaunchKernelGGL(k_zero_words, dim3(1024), dim3(256), 0, s, (uint32_t *)cs.gtable.ptr, cs.table_params() / 2);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (uint32_t *)&cache_state(cs)->loss, (uint64_t)1);
+    if (n == 0) return hipGetLastError();
     TrainDev td;
     td.targets = targets;
     td.loss = &cache_state(cs)->loss;
@@ -574,7 +587,8 @@ static int cache_init_params(spk_context *ctx, CacheState &cs) {
         (e = hipMemset(cs.mw.ptr, 0, nw * 4)) != hipSuccess || (e = hipMemset(cs.vw.ptr, 0, nw * 4)) != hipSuccess ||
         (e = hipMemset(cs.wf.ptr, 0, cache_packed_bf16() * 2)) != hipSuccess ||
         (e = hipMemset(cs.wb.ptr, 0, cache_packed_bf16() * 2)) != hipSuccess ||
-        (e = hipMemset(cs.state.ptr, 0, sizeof(AdamStep))) != hipSuccess)
+        (e = hipMemset(cs.state.ptr, 0, sizeof(AdamStep))) != hipSuccess ||
+        (e = hipMemset(cs.st_count.ptr, 0, 4)) != hipSuccess)  // no staged samples to train on
         return hip_fail(ctx, e, "cache initialisation");
     if ((e = cache_pack(cs, ctx->stream)) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
         return hip_fail(ctx, e, "cache pack");
@@ -592,6 +606,7 @@ int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
         return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "invalid cache resolution or learning rate");
     delete ctx->cache;
     ctx->cache = new CacheState();
+    ctx->cache_last_loss = 0.f;
     CacheState &cs = *ctx->cache;
     cs.cfg = *config;
     if (cs.cfg.batch == 0) cs.cfg.batch = 1u << 16;
@@ -626,7 +641,8 @@ int spk_cache_create(spk_context *ctx, const spk_cache_config *config) {
         (e = cs.gtable.ensure(ntab * 2)) != hipSuccess || (e = cs.w.ensure(nw * 4)) != hipSuccess ||
         (e = cs.mw.ensure(nw * 4)) != hipSuccess || (e = cs.vw.ensure(nw * 4)) != hipSuccess ||
         (e = cs.gw.ensure(nw * 4)) != hipSuccess || (e = cs.wf.ensure(cache_packed_bf16() * 2)) != hipSuccess ||
-        (e = cs.wb.ensure(cache_packed_bf16() * 2)) != hipSuccess || (e = cs.state.ensure(sizeof(AdamStep))) != hipSuccess)
+        (e = cs.wb.ensure(cache_packed_bf16() * 2)) != hipSuccess || (e = cs.state.ensure(sizeof(AdamStep))) != hipSuccess ||
+        (e = cs.st_count.ensure(4)) != hipSuccess)
         return hip_fail(ctx, e, "cache allocation");
     int rc = cache_init_params(ctx, cs);
     if (rc != SPK_OK) return rc;

@@ -129,6 +129,10 @@ struct PtArgs {
     float4 *tr_meta0;  // (colour before the vertex, path)
     float4 *tr_meta1;  // (throughput at the vertex, 0)
     float *tr_targets; // 3 per sample
+    float *st_points;  // the frame's training samples handed to the next frame's training step
+    float4 *st_attrs;
+    float *st_targets;
+    uint32_t *st_count;
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -987,13 +991,25 @@ __global__ __launch_bounds__(kPtBlock) void k_cache_apply(PtArgs a) {
     }
 }
 
-// target = radiance gathered after the vertex / throughput at the vertex (clamped to [0, 64]), divided
-// by the vertex albedo (floored)
+// Largest training target (radiance after the vertex). Roulette survivors carry weights up to 20, so
+// a single sample past a vertex can reach 20x the brightest emitter; clamping them low biases the
+// cache dark (at 64, -0.6% on the many-instance scene's mean).
+constexpr float kTargetClamp = 1024.f;
+
+// target = radiance gathered after the vertex / throughput at the vertex (clamped to [0, kTargetClamp]), divided
+// by the vertex albedo (floored); written with the sample's point and attributes to the staging set
+// the next frame's training step reads
 __global__ __launch_bounds__(kPtBlock) void k_cache_targets(PtArgs a) {
     const uint32_t count = min(a.counters[kCtrGlobal + GCTR_TRAIN], a.train_cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.st_count = count;
     for (uint32_t base = grid_wave() * 64u; base < count; base += grid_waves() * 64u) {
         const uint32_t i = base + lane_id();
         if (i >= count) continue;
+        a.st_points[3 * (size_t)i] = a.tr_points[3 * (size_t)i];
+        a.st_points[3 * (size_t)i + 1] = a.tr_points[3 * (size_t)i + 1];
+        a.st_points[3 * (size_t)i + 2] = a.tr_points[3 * (size_t)i + 2];
+        a.st_attrs[2 * (size_t)i] = a.tr_attrs[2 * (size_t)i];
+        a.st_attrs[2 * (size_t)i + 1] = a.tr_attrs[2 * (size_t)i + 1];
         const float4 m0 = a.tr_meta0[i], m1 = a.tr_meta1[i];
         const float4 C = a.col[__float_as_uint(m0.w)];
         const float g[3] = {C.x - m0.x, C.y - m0.y, C.z - m0.z};
@@ -1002,7 +1018,7 @@ __global__ __launch_bounds__(kPtBlock) void k_cache_targets(PtArgs a) {
         const float al[3] = {alb.x, alb.y, alb.z};
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            a.tr_targets[3 * (size_t)i + k] = fminf(fmaxf(g[k] / fmaxf(tt[k], 1e-4f), 0.f), 64.f) / al[k];
+            a.st_targets[3 * (size_t)i + k] = fminf(fmaxf(g[k] / fmaxf(tt[k], 1e-4f), 0.f), kTargetClamp) / al[k];
     }
 }
 
@@ -1132,10 +1148,17 @@ struct PathState {
     hipGraph_t graph_def = nullptr;
     PtArgs graph_args{};
     hipStream_t graph_stream = nullptr;
+    // the branch of a cache frame's graph that trains on the previous frame's samples beside the
+    // frame's first kernels (default priority, like every stream here)
+    hipStream_t train_stream = nullptr;
+    hipEvent_t train_fork = nullptr, train_join = nullptr;
     ~PathState() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_def) (void)hipGraphDestroy(graph_def);
         if (frame_host) (void)hipHostFree(frame_host);
+        if (train_fork) (void)hipEventDestroy(train_fork);
+        if (train_join) (void)hipEventDestroy(train_join);
+        if (train_stream) (void)hipStreamDestroy(train_stream);
     }
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -1237,6 +1260,8 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
             const size_t n = npix, tcap = cs.cfg.batch;
             if ((e = cs.cq_points.ensure(n * 12)) != hipSuccess || (e = cs.cq_meta.ensure(n * 16)) != hipSuccess ||
                 (e = cs.cq_attrs.ensure(n * 32)) != hipSuccess || (e = cs.tr_attrs.ensure(tcap * 32)) != hipSuccess ||
+                (e = cs.st_points.ensure(tcap * 12)) != hipSuccess || (e = cs.st_attrs.ensure(tcap * 32)) != hipSuccess ||
+                (e = cs.st_targets.ensure(tcap * 12)) != hipSuccess || (e = cache_prepare(cs, (uint32_t)tcap)) != hipSuccess ||
                 (e = cs.cq_out.ensure(n * 12)) != hipSuccess || (e = cs.tr_points.ensure(tcap * 12)) != hipSuccess ||
                 (e = cs.tr_meta0.ensure(tcap * 16)) != hipSuccess || (e = cs.tr_meta1.ensure(tcap * 16)) != hipSuccess ||
                 (e = cs.tr_targets.ensure(tcap * 12)) != hipSuccess)
@@ -1244,12 +1269,34 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
             cs.capacity = npix;
             ps.drop_graph();
         }
+        if (!ps.train_stream) {
+            if ((e = hipStreamCreateWithFlags(&ps.train_stream, hipStreamNonBlocking)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&ps.train_fork, hipEventDisableTiming)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&ps.train_join, hipEventDisableTiming)) != hipSuccess)
+                return hip_fail(ctx, e, "cache training stream");
+        }
     }
     return SPK_OK;
 }
 
+// One frame as a sequence of launches on s (captured into the frame graph). A cache frame forks a
+// branch onto the training stream that takes one training step on the previous frame's samples
+// (staged by k_cache_targets; frames with fewer than kMinTrainSamples take none) while the frame's
+// first kernels run, and joins it before the frame's cache queries, which then see the updated
+// network, as they would with the step run between the frames.
 static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t s, int grid, CacheState *cache,
-                                int cus) {
+                                int cus, const PathState &ps) {
+    hipError_t e;
+    if (a.cache_on) {
+        CacheState &cs = *cache;
+        if ((e = hipEventRecord(ps.train_fork, s)) != hipSuccess ||
+            (e = hipStreamWaitEvent(ps.train_stream, ps.train_fork, 0)) != hipSuccess ||
+            (e = cache_gradients(cs, a.st_points, (const float *)a.st_attrs, a.st_targets, cs.cfg.batch, 1, nullptr,
+                                 ps.train_stream, a.st_count)) != hipSuccess ||
+            (e = cache_adam(cs, ps.train_stream, a.st_count, cs.cfg.batch)) != hipSuccess ||
+            (e = hipEventRecord(ps.train_join, ps.train_stream)) != hipSuccess)
+            return e;
+    }
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kPtBlock), 0, s, a);
     for (uint32_t b = 0; b < a.tail_from; ++b) {
         if (b == 0u || a.direct)  // a full grid, one queue slot per thread
@@ -1271,7 +1318,8 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
     }
     if (a.tail_from < a.max_bounce) hipLaunchKernelGGL(pe.tail, dim3(grid), dim3(kPtBlock), 0, s, a, a.tail_from);
     if (a.cache_on) {
-        hipError_t e = cache_infer_queue(*cache, a.cq_points, (const float *)a.cq_attrs,
+        if ((e = hipStreamWaitEvent(s, ps.train_join, 0)) != hipSuccess) return e;
+        e = cache_infer_queue(*cache, a.cq_points, (const float *)a.cq_attrs,
                                          a.counters + kCtrGlobal + GCTR_CACHE_Q, a.npix, a.cq_out,
                                          cus * 8, s);
         if (e != hipSuccess) return e;
@@ -1461,6 +1509,10 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
         a.tr_meta0 = (float4 *)cs.tr_meta0.ptr;
         a.tr_meta1 = (float4 *)cs.tr_meta1.ptr;
         a.tr_targets = (float *)cs.tr_targets.ptr;
+        a.st_points = (float *)cs.st_points.ptr;
+        a.st_attrs = (float4 *)cs.st_attrs.ptr;
+        a.st_targets = (float *)cs.st_targets.ptr;
+        a.st_count = (uint32_t *)cs.st_count.ptr;
     }
 
     hipError_t e;
@@ -1474,7 +1526,7 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
         ps.drop_graph();
         if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return hip_fail(ctx, e, "begin capture");
-        hipError_t le = enqueue_frame(a, *pe, ctx->stream, grid, ctx->cache, cus);
+        hipError_t le = enqueue_frame(a, *pe, ctx->stream, grid, ctx->cache, cus, ps);
         e = hipStreamEndCapture(ctx->stream, &ps.graph_def);
         if (le != hipSuccess) return hip_fail(ctx, le, "frame launch during capture");
         if (e != hipSuccess) return hip_fail(ctx, e, "end capture");
@@ -1498,18 +1550,9 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
         e = hipGraphLaunch(ps.graph, ctx->stream);
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
-    // online training of the radiance cache on this frame's samples, enqueued behind the frame: the
-    // sample count stays on the device (frames with fewer than kMinTrainSamples take no step)
+    // the online training of the radiance cache runs inside the graph (on the previous frame's
+    // samples); its loss is read back below
     const bool train = cache_on && frame->max_bounce > 0;
-    if (train) {
-        CacheState &cs = *ctx->cache;
-        const uint32_t *cnt_dev = (const uint32_t *)ps.counters.ptr + kCtrGlobal + GCTR_TRAIN;
-        if ((e = cache_gradients(cs, (const float *)cs.tr_points.ptr, (const float *)cs.tr_attrs.ptr,
-                                 (const float *)cs.tr_targets.ptr, cs.cfg.batch,
-                                 1, nullptr, ctx->stream, cnt_dev)) != hipSuccess ||
-            (e = cache_adam(cs, ctx->stream, cnt_dev, cs.cfg.batch)) != hipSuccess)
-            return hip_fail(ctx, e, "cache training");
-    }
     if (stats) (void)hipEventRecord(t1, ctx->stream);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "frame");
     if (train) {

@@ -33,6 +33,9 @@ struct CacheState {
     DeviceBuffer cq_points, cq_attrs, cq_meta, cq_out;  // cache queries: point, attributes, (throughput, path),
                                                         // radiance / albedo
     DeviceBuffer tr_points, tr_attrs, tr_meta0, tr_meta1, tr_targets;  // training samples
+    // the last frame's training samples, handed to the training step that runs inside the next
+    // frame's graph beside its first kernels (points, attributes, targets, count)
+    DeviceBuffer st_points, st_attrs, st_targets, st_count;
     uint32_t capacity = 0;
     uint64_t table_params() const { return (uint64_t)cfg.levels * (1ull << cfg.log2_table) * cfg.features; }
 };
@@ -47,6 +50,8 @@ hipError_t cache_infer(CacheState &cs, const float *points, const float *attrs, 
 hipError_t cache_gradients(CacheState &cs, const float *points, const float *attrs, const float *targets, uint32_t n,
                            int world, float *out, hipStream_t s, const uint32_t *n_dev = nullptr);
 hipError_t cache_adam(CacheState &cs, hipStream_t s, const uint32_t *n_dev = nullptr, uint32_t cap = 0);
+// Sizes the training scratch for n samples (call before capturing a graph that trains).
+hipError_t cache_prepare(CacheState &cs, uint32_t n);
 inline AdamStep *cache_state(CacheState &cs) { return static_cast<AdamStep *>(cs.state.ptr); }
 // Inference over a queue whose length lives in device memory (capturable in a graph): `grid`
 // persistent single-wave blocks walk the 64-query tiles.

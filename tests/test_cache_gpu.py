@@ -198,9 +198,11 @@ def test_cache_render_trains_online_and_stays_close_to_plain(gpu_ctx):
         cache_imgs = np.array(cache_imgs)
         mc, mp = cache_imgs.mean(0), plain.mean(0)
         vc, vp = cache_imgs.var(0).mean(), plain.var(0).mean()
-        print("bounce", bounce, "loss first/median late", losses[0], np.median(losses[-100:]),
+        print("bounce", bounce, "loss first/median late", losses[1], np.median(losses[-100:]),
               "flux cache/plain", mc.mean(), mp.mean(), "variance cache/plain", vc, vp)
-        assert losses[0] > 3 * np.median(losses[-100:])
+        # each frame trains on the previous frame's samples, so the first frame takes no step
+        assert losses[1] > 0.0
+        assert losses[1] > 3 * np.median(losses[-100:])
         assert abs(mc.mean() - mp.mean()) / mp.mean() < 0.1
         assert gpu_ctx.cache_info()["steps"] >= 290
         # terminating earlier removes more of the tail's variance; what remains is the direct hits
