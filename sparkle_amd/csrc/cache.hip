@@ -54,6 +54,8 @@ __device__ __forceinline__ uint32_t grid_index(const CacheDev &c, uint32_t l, ui
     return ((x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u)) & ((1u << c.log2_table) - 1u);
 }
 
+__device__ __forceinline__ float half_bits(uint32_t b) { return __half2float(__ushort_as_half((unsigned short)b)); }
+
 // trilinear hash-grid features of one point in [0,1]^3 (16 levels x 2 features), written as BF16 to
 // row[0..31] (zeros when !valid)
 __device__ __forceinline__ void encode_point(const CacheDev &c, float px, float py, float pz, bool valid,
@@ -73,14 +75,23 @@ __device__ __forceinline__ void encode_point(const CacheDev &c, float px, float 
             fr[a] = pos[a] - cf;
         }
         float f0 = 0.f, f1 = 0.f;
+        const uint32_t *lv = reinterpret_cast<const uint32_t *>(c.table) + (size_t)l * T;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t cx = cell[0] + (k & 1), cy = cell[1] + ((k >> 1) & 1), cz = cell[2] + ((k >> 2) & 1);
-            const float w = ((k & 1) ? fr[0] : 1.f - fr[0]) * (((k >> 1) & 1) ? fr[1] : 1.f - fr[1]) *
-                            (((k >> 2) & 1) ? fr[2] : 1.f - fr[2]);
-            const __half2 v = c.table[(size_t)l * T + grid_index(c, l, cx, cy, cz)];
-            f0 += w * __low2float(v);
-            f1 += w * __high2float(v);
+        for (int yz = 0; yz < 4; ++yz) {
+            const uint32_t cy = cell[1] + (yz & 1), cz = cell[2] + (yz >> 1);
+            const uint32_t i0 = grid_index(c, l, cell[0], cy, cz), i1 = grid_index(c, l, cell[0] + 1u, cy, cz);
+            // the two x-neighbours share an aligned pair of entries when i1 = i0 ^ 1 (an even i0 on a dense
+            // level, an even cell x on a hashed one): one 8-byte load brings both, a second load the
+            // other neighbour otherwise. The gathers are bound by L2 requests, not bytes.
+            const uint2 pair = *reinterpret_cast<const uint2 *>(lv + (i0 & ~1u));
+            const uint32_t e0 = (i0 & 1u) ? pair.y : pair.x;
+            const uint32_t e1 = i1 == (i0 ^ 1u) ? ((i0 & 1u) ? pair.x : pair.y) : lv[i1];
+            const float wyz = ((yz & 1) ? fr[1] : 1.f - fr[1]) * ((yz >> 1) ? fr[2] : 1.f - fr[2]);
+            const float w0 = (1.f - fr[0]) * wyz, w1 = fr[0] * wyz;
+            f0 += w0 * half_bits(e0 & 0xFFFFu);
+            f1 += w0 * half_bits(e0 >> 16);
+            f0 += w1 * half_bits(e1 & 0xFFFFu);
+            f1 += w1 * half_bits(e1 >> 16);
         }
         row[2 * l] = (__bf16)(valid ? f0 : 0.f);
         row[2 * l + 1] = (__bf16)(valid ? f1 : 0.f);
