@@ -178,6 +178,7 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
     plain_nee    plain + next-event estimation
     cache        learned radiance cache (terminate at bounce 2, online training every frame)
     cache_nee    cache + next-event estimation
+    cache1       cache from bounce 1 (a path ends at its first non-metal vertex after the camera hit)
     cache1_nee   cache from bounce 1 + next-event estimation
     For each budget b*T every estimator renders frames until b*T has passed (plain at b = 1 renders
     exactly spp frames); the cache estimators start untrained (reset after their graph-capture frame)
@@ -213,7 +214,8 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
         return f
 
     variants = [("plain", 0, None), ("plain_nee", spk.FRAME_NEE, None), ("cache", spk.FRAME_CACHE, 2),
-                ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
+                ("cache_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 2), ("cache1", spk.FRAME_CACHE, 1),
+                ("cache1_nee", spk.FRAME_CACHE | spk.FRAME_NEE, 1)]
     def steady_ms(flags, sched, frames=8):
         """Median frame time of a schedule over several frame seeds (after a capture frame)."""
         ctx.set_schedule(*sched)
@@ -406,7 +408,7 @@ def cache_kernel_bench(spk, scenes, stream, n_infer=1920 * 1080, n_train=1 << 16
 
 
 RENDER_MODES = {"plain": (0, None), "plain_nee": ("NEE", None), "cache_nee": ("NEE", 2), "cache1_nee": ("NEE", 1),
-                "cache": (0, 2)}
+                "cache": (0, 2), "cache1": (0, 1)}
 
 
 def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_bounce=8, autotune=True,
@@ -415,7 +417,7 @@ def render_experiment(spk, scenes, stream, width=1920, height=1080, spp=64, max_
     3 area lights): spp frames of one path per pixel, max_bounce bounces, Russian roulette from
     bounce 3, material-sorted queues, the frame graph replayed; schedule from spk_render_autotune.
     mode "plain" is the reference's estimator; the others add next-event estimation and/or the
-    learned cache (terminating at bounce 2, or 1 for cache1_nee, trained every frame)."""
+    learned cache (terminating at bounce 2, or 1 for cache1 and cache1_nee, trained every frame)."""
     import torch
 
     nee, cache_bounce = RENDER_MODES[mode]

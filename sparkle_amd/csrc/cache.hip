@@ -80,12 +80,15 @@ __device__ __forceinline__ void encode_point(const CacheDev &c, float px, float 
         for (int yz = 0; yz < 4; ++yz) {
             const uint32_t cy = cell[1] + (yz & 1), cz = cell[2] + (yz >> 1);
             const uint32_t i0 = grid_index(c, l, cell[0], cy, cz), i1 = grid_index(c, l, cell[0] + 1u, cy, cz);
-            // the two x-neighbours share an aligned pair of entries when i1 = i0 ^ 1 (an even i0 on a dense
-            // level, an even cell x on a hashed one): one 8-byte load brings both, a second load the
-            // other neighbour otherwise. The gathers are bound by L2 requests, not bytes.
-            const uint2 pair = *reinterpret_cast<const uint2 *>(lv + (i0 & ~1u));
-            const uint32_t e0 = (i0 & 1u) ? pair.y : pair.x;
-            const uint32_t e1 = i1 == (i0 ^ 1u) ? ((i0 & 1u) ? pair.x : pair.y) : lv[i1];
+            // the two x-neighbours fall into one aligned quad of entries when i0 ^ i1 < 4: on a dense level
+            // unless i0 % 4 == 3, on a hashed one (x has hash prime 1, so i1 = i0 ^ (x ^ (x + 1))) unless
+            // x % 4 == 3. One 16-byte load brings the quad, a second load the other neighbour otherwise
+            // (1.25 loads per pair on average). The gathers are bound by L2 requests, not bytes. The
+            // quad stays inside the level: T is a multiple of 4.
+            const uint4 quad = *reinterpret_cast<const uint4 *>(lv + (i0 & ~3u));
+            const uint32_t e0 = (i0 & 2u) ? ((i0 & 1u) ? quad.w : quad.z) : ((i0 & 1u) ? quad.y : quad.x);
+            const uint32_t q1 = (i1 & 2u) ? ((i1 & 1u) ? quad.w : quad.z) : ((i1 & 1u) ? quad.y : quad.x);
+            const uint32_t e1 = (i0 ^ i1) < 4u ? q1 : lv[i1];
             const float wyz = ((yz & 1) ? fr[1] : 1.f - fr[1]) * ((yz >> 1) ? fr[2] : 1.f - fr[2]);
             const float w0 = (1.f - fr[0]) * wyz, w1 = fr[0] * wyz;
             f0 += w0 * half_bits(e0 & 0xFFFFu);

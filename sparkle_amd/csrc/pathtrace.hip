@@ -1280,24 +1280,31 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
 }
 
 // One frame as a sequence of launches on s (captured into the frame graph). A cache frame forks a
-// branch onto the training stream that takes one training step on the previous frame's samples
-// (staged by k_cache_targets; frames with fewer than kMinTrainSamples take none) while the frame's
-// first kernels run, and joins it before the frame's cache queries, which then see the updated
-// network, as they would with the step run between the frames.
+// branch onto the training stream after k_generate that takes one training step on the previous
+// frame's samples (staged by k_cache_targets; frames with fewer than kMinTrainSamples take none)
+// beside the bounce-0 traversal, and joins it before the frame's cache queries, which then see the
+// updated network, as they would with the step run between the frames.
 static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t s, int grid, CacheState *cache,
                                 int cus, const PathState &ps) {
     hipError_t e;
-    if (a.cache_on) {
+    auto fork_training = [&]() -> hipError_t {
+        if (!a.cache_on) return hipSuccess;
         CacheState &cs = *cache;
-        if ((e = hipEventRecord(ps.train_fork, s)) != hipSuccess ||
-            (e = hipStreamWaitEvent(ps.train_stream, ps.train_fork, 0)) != hipSuccess ||
-            (e = cache_gradients(cs, a.st_points, (const float *)a.st_attrs, a.st_targets, cs.cfg.batch, 1, nullptr,
+        hipError_t f;
+        if ((f = hipEventRecord(ps.train_fork, s)) != hipSuccess ||
+            (f = hipStreamWaitEvent(ps.train_stream, ps.train_fork, 0)) != hipSuccess ||
+            (f = cache_gradients(cs, a.st_points, (const float *)a.st_attrs, a.st_targets, cs.cfg.batch, 1, nullptr,
                                  ps.train_stream, a.st_count)) != hipSuccess ||
-            (e = cache_adam(cs, ps.train_stream, a.st_count, cs.cfg.batch)) != hipSuccess ||
-            (e = hipEventRecord(ps.train_join, ps.train_stream)) != hipSuccess)
-            return e;
-    }
+            (f = cache_adam(cs, ps.train_stream, a.st_count, cs.cfg.batch)) != hipSuccess ||
+            (f = hipEventRecord(ps.train_join, ps.train_stream)) != hipSuccess)
+            return f;
+        return hipSuccess;
+    };
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kPtBlock), 0, s, a);
+    // forked after the camera rays: forked before them, the training kernels took the CUs first and
+    // k_generate ran 5x longer (1080p cache frame: 2.86-2.89 ms; forked here 2.80 ms; forked before
+    // the per-path tail, where the training fills the CUs the tail's finished waves leave, 2.81 ms)
+    if ((e = fork_training()) != hipSuccess) return e;
     for (uint32_t b = 0; b < a.tail_from; ++b) {
         if (b == 0u || a.direct)  // a full grid, one queue slot per thread
             hipLaunchKernelGGL(pe.first, dim3((a.npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, s, a, b);
