@@ -393,6 +393,33 @@ __global__ void k_adam(float *__restrict__ p, const G *__restrict__ grad, float 
     if (half_copy) half_copy[i] = __float2half(pi);
 }
 
+// Adam over the hash table, lazily (the rule of torch.optim.SparseAdam): an entry (two features,
+// one BF16 gradient pair) whose gradient is zero this step keeps its parameters and moments; a
+// touched one is updated and its gradient word cleared for the next step, so the table gradient is
+// all zero again after the step without a separate clear. A step touches a fraction of the 16 x 2^19
+// entries (few on the coarse levels, where the samples share corners), and an untouched entry costs
+// a 4-byte read instead of 28 bytes of parameters, moments and copies.
+__global__ void k_adam_table(float2 *__restrict__ p, uint32_t *__restrict__ grad, float2 *__restrict__ m,
+                             float2 *__restrict__ v, uint64_t entries, float lr, float b1, float b2, float eps,
+                             const AdamStep *__restrict__ st, __half2 *__restrict__ half_copy) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= entries || !st->active) return;
+    const uint32_t gw = grad[i];
+    if (gw == 0u) return;
+    grad[i] = 0u;
+    const float c1 = st->c1, c2 = st->c2;
+    const float g0 = __uint_as_float(gw << 16), g1 = __uint_as_float(gw & 0xFFFF0000u);
+    const float2 mi = m[i], vi = v[i], pi = p[i];
+    const float m0 = b1 * mi.x + (1.f - b1) * g0, m1 = b1 * mi.y + (1.f - b1) * g1;
+    const float v0 = b2 * vi.x + (1.f - b2) * g0 * g0, v1 = b2 * vi.y + (1.f - b2) * g1 * g1;
+    const float p0 = pi.x - lr * (m0 / c1) / (sqrtf(v0 / c2) + eps);
+    const float p1 = pi.y - lr * (m1 / c1) / (sqrtf(v1 / c2) + eps);
+    m[i] = make_float2(m0, m1);
+    v[i] = make_float2(v0, v1);
+    p[i] = make_float2(p0, p1);
+    half_copy[i] = __floats2half2_rn(p0, p1);
+}
+
 // FP32 master weights -> BF16 forward [out_pad][in] and backward [in][out_pad32] copies
 __global__ void k_pack_weights(const float *__restrict__ w, __bf16 *__restrict__ wf, __bf16 *__restrict__ wb) {
     // layer geometry: in, out, forward out pad, backward out pad, offsets
@@ -516,8 +543,10 @@ hipError_t cache_gradients(CacheState &cs, const float *points, const float *att
     const uint32_t n_pad = ((n + 255u) / 256u) * 256u;
     if ((e = cache_prepare(cs, n)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_zero_words, dim3(64), dim3(256), 0, s, (uint32_t *)cs.gw.ptr, (uint64_t)cache_num_weights());
-    hipLaunchKernelGGL(k_zero_words, dim3(1024), dim3(256), 0, s, (uint32_t *)cs.gtable.ptr, cs.table_params() / 2);
+    if (!cs.gtable_clean)  // the table Adam clears what it consumes; otherwise clear it here
+        hipLaunchKernelGGL(k_zero_words, dim3(1024), dim3(256), 0, s, (uint32_t *)cs.gtable.ptr, cs.table_params() / 2);
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (uint32_t *)&cache_state(cs)->loss, (uint64_t)1);
+    cs.gtable_clean = n == 0;
     if (n == 0) return hipGetLastError();
     TrainDev td;
     td.targets = targets;
@@ -541,10 +570,13 @@ hipError_t cache_adam(CacheState &cs, hipStream_t s, const uint32_t *n_dev, uint
     hipLaunchKernelGGL(k_adam<float>, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, (float *)cs.w.ptr,
                        (const float *)cs.gw.ptr, (float *)cs.mw.ptr, (float *)cs.vw.ptr, nw, lr, b1, b2, eps,
                        (const AdamStep *)st, (__half *)nullptr);
-    const uint64_t nt = cs.table_params();
-    hipLaunchKernelGGL(k_adam<__bf16>, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, (float *)cs.table_f.ptr,
-                       (const __bf16 *)cs.gtable.ptr, (float *)cs.mt.ptr, (float *)cs.vt.ptr, nt, lr, b1, b2, eps,
-                       (const AdamStep *)st, (__half *)cs.table_h.ptr);
+    const uint64_t entries = cs.table_params() / 2;
+    hipLaunchKernelGGL(k_adam_table, dim3((uint32_t)((entries + 255) / 256)), dim3(256), 0, s,
+                       (float2 *)cs.table_f.ptr, (uint32_t *)cs.gtable.ptr, (float2 *)cs.mt.ptr, (float2 *)cs.vt.ptr,
+                       entries, lr, b1, b2, eps, (const AdamStep *)st, (__half2 *)cs.table_h.ptr);
+    // an active step cleared every gradient word it read; an inactive one (fewer than
+    // kMinTrainSamples samples through n_dev) followed a gradient pass that wrote nothing
+    cs.gtable_clean = true;
     hipLaunchKernelGGL(k_pack_weights, dim3(16), dim3(256), 0, s, (const float *)cs.w.ptr, (__bf16 *)cs.wf.ptr,
                        (__bf16 *)cs.wb.ptr);
     return hipGetLastError();
@@ -602,10 +634,12 @@ static int cache_init_params(spk_context *ctx, CacheState &cs) {
         (e = hipMemset(cs.wf.ptr, 0, cache_packed_bf16() * 2)) != hipSuccess ||
         (e = hipMemset(cs.wb.ptr, 0, cache_packed_bf16() * 2)) != hipSuccess ||
         (e = hipMemset(cs.state.ptr, 0, sizeof(AdamStep))) != hipSuccess ||
+        (e = hipMemset(cs.gtable.ptr, 0, ntab * 2)) != hipSuccess ||
         (e = hipMemset(cs.st_count.ptr, 0, 4)) != hipSuccess)  // no staged samples to train on
         return hip_fail(ctx, e, "cache initialisation");
     if ((e = cache_pack(cs, ctx->stream)) != hipSuccess || (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
         return hip_fail(ctx, e, "cache pack");
+    cs.gtable_clean = true;
     return SPK_OK;
 }
 
@@ -711,6 +745,12 @@ int spk_cache_gradients(spk_context *ctx, const float *points_dev, const float *
             const uint32_t bits = (uint32_t)g[k] << 16;
             std::memcpy(&dtable[k], &bits, 4);
         }
+    }
+    if (e == hipSuccess) {  // no Adam step consumes this gradient: clear it for the next pass
+        hipLaunchKernelGGL(k_zero_words, dim3(1024), dim3(256), 0, ctx->stream, (uint32_t *)cs.gtable.ptr,
+                           cs.table_params() / 2);
+        e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) cs.gtable_clean = true;
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "cache gradients");
     return SPK_OK;

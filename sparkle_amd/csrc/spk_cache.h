@@ -37,6 +37,9 @@ struct CacheState {
     // frame's graph beside its first kernels (points, attributes, targets, count)
     DeviceBuffer st_points, st_attrs, st_targets, st_count;
     uint32_t capacity = 0;
+    // the table gradient is all zero (k_adam_table clears what it consumes); false between a gradient
+    // pass and the step that consumes it
+    bool gtable_clean = false;
     uint64_t table_params() const { return (uint64_t)cfg.levels * (1ull << cfg.log2_table) * cfg.features; }
 };
 
