@@ -412,7 +412,11 @@ typedef struct spk_cache_config {
     uint32_t batch;            /* training samples per frame (65536) */
     uint32_t terminate_bounce; /* paths terminate into the cache at this hit index (2) */
     uint32_t seed;             /* initialisation seed */
-    float train_fraction;      /* fraction of paths traced in full for training, 0 = batch/pixels */
+    float train_fraction;      /* fraction of paths that are training paths, 0 = batch / (2 pixels) */
+    uint32_t train_suffix;     /* a training path ends in the cache from hit index terminate_bounce +
+                                  train_suffix on (self-training: its targets include the network's
+                                  prediction there), except one in 16 traced to its end; 0: every
+                                  training path is traced to its end (2) */
 } spk_cache_config;
 
 int spk_cache_create(spk_context *ctx, const spk_cache_config *config);

@@ -134,14 +134,16 @@ class CacheConfig(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_uint32), ("features", ctypes.c_uint32), ("log2_table", ctypes.c_uint32),
                 ("base_resolution", ctypes.c_uint32), ("per_level_scale", ctypes.c_float),
                 ("learning_rate", ctypes.c_float), ("batch", ctypes.c_uint32), ("terminate_bounce", ctypes.c_uint32),
-                ("seed", ctypes.c_uint32), ("train_fraction", ctypes.c_float)]
+                ("seed", ctypes.c_uint32), ("train_fraction", ctypes.c_float), ("train_suffix", ctypes.c_uint32)]
 
 
 def cache_config(log2_table=19, base_resolution=16, per_level_scale=1.3819, learning_rate=1e-2, batch=1 << 16,
-                 terminate_bounce=2, seed=0, train_fraction=0.0):
-    """The configuration of the learned radiance cache: 16 levels x 2 features, FP16 table."""
+                 terminate_bounce=2, seed=0, train_fraction=0.0, train_suffix=2):
+    """The configuration of the learned radiance cache: 16 levels x 2 features, FP16 table.
+    train_suffix: training paths end in the cache train_suffix hit indices after terminate_bounce
+    (self-training), one in 16 excepted; 0 traces every training path to its end."""
     return CacheConfig(16, 2, log2_table, base_resolution, per_level_scale, learning_rate, batch, terminate_bounce,
-                       seed, train_fraction)
+                       seed, train_fraction, train_suffix)
 
 
 FRAME_CACHE = 4

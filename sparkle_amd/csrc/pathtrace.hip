@@ -57,6 +57,7 @@ enum : uint32_t {
 enum : uint32_t {
     PF_NEE_PREV = 1u,  // the previous vertex sampled the emitters directly (MIS-weight emission hits)
     PF_TRAIN = 2u,          // a training path of the radiance cache
+    PF_TRAIN_FULL = 4u,     // a training path traced to its end (its targets never see the cache)
 };
 
 // Per-scene record of a frame: BVH root, image size, first pixel of the scene's image, camera.
@@ -119,6 +120,7 @@ struct PtArgs {
     float output_limit;
     // learned radiance cache
     uint32_t cache_on, cache_bounce, train_cap;
+    uint32_t train_suffix;  // hit indices a training path runs past cache_bounce before it ends in the cache
     float train_prob;
     float *cq_points;  // 3 per query
     float4 *cq_attrs;  // 2 per query: (normal, albedo.r), (albedo.gb, roughness, 1)
@@ -277,6 +279,7 @@ __global__ __launch_bounds__(kPtBlock) void k_generate(PtArgs a) {
             // training paths: an independent hash of pixel and frame, so the path itself is unchanged
             const float h = (float)(hash32(p * 0x9E3779B9u ^ hash32(frame_seed + 0x632BE5ABu)) >> 8) * (1.f / 16777216.f);
             if (h < a.train_prob) pf |= PF_TRAIN;
+            if (h < a.train_prob * (1.f / 16.f)) pf |= PF_TRAIN_FULL;
         }
         a.rays[2 * p] = make_float4(o.x, o.y, o.z, 0.f);
         a.rays[2 * p + 1] = make_float4(d.x, d.y, d.z, INFINITY);
@@ -661,8 +664,11 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
             } else {
                 C = C + cmul(me.emissive, T);
             }
-        } else if (cache_vertex && !(pf & PF_TRAIN)) {
-            // terminate into the cache: radiance leaving this vertex comes from the network
+        } else if (cache_vertex && (!(pf & PF_TRAIN) ||
+                                    (!(pf & PF_TRAIN_FULL) && bounce >= a.cache_bounce + a.train_suffix))) {
+            // terminate into the cache: radiance leaving this vertex comes from the network (a
+            // training path past its suffix too: its targets then include the network's prediction
+            // here, which carries light over more bounces than the suffix traces)
             cache_q = true;
             Tv = T;
             o.at0 = make_float4(normal.x, normal.y, normal.z, me.base.x);
@@ -774,7 +780,7 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
                     if (alive && !last) {
                         cont = true;
                         const F3 o2 = loc + wi * SPK_TOLERANCE;
-                        const uint32_t npf = (pf & PF_TRAIN) | ((nee && diffuse) ? PF_NEE_PREV : 0u);
+                        const uint32_t npf = (pf & (PF_TRAIN | PF_TRAIN_FULL)) | ((nee && diffuse) ? PF_NEE_PREV : 0u);
                         if (nee && diffuse) a.prev_pdf[path] = fmaxf(dot(wi, normal), 0.f) / SPK_PI;
                         a.rays[2 * path] = make_float4(o2.x, o2.y, o2.z, SPK_EPS);
                         a.rays[2 * path + 1] = make_float4(wi.x, wi.y, wi.z, INFINITY);
@@ -1505,6 +1511,7 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
         a.cache_on = 1u;
         a.cache_bounce = cs.cfg.terminate_bounce;
         a.train_cap = cs.cfg.batch;
+        a.train_suffix = cs.cfg.train_suffix ? cs.cfg.train_suffix : 0xFFFFu;  // 0: traced to the end
         a.train_prob = cs.cfg.train_fraction > 0.f ? cs.cfg.train_fraction
                                                    : std::min(1.f, (float)cs.cfg.batch / (float)npix / 2.f);
         a.cq_points = (float *)cs.cq_points.ptr;

@@ -29,17 +29,30 @@ def test_header_has_no_cpp_or_torch_types():
         assert bad not in text
 
 
-def test_struct_sizes_match_header_layout():
-    assert ctypes.sizeof(spk.MeshDesc) == 56
-    assert ctypes.sizeof(spk.InstanceDesc) == 56
-    assert ctypes.sizeof(spk.MaterialDesc) == 56
-    assert ctypes.sizeof(spk.TextureDesc) == 24
-    assert ctypes.sizeof(spk.CameraDesc) == 92
-    assert ctypes.sizeof(spk.FrameDesc) == 44
-    assert ctypes.sizeof(spk.CacheConfig) == 40
-    assert ctypes.sizeof(spk.BuildOptions) == 16
-    assert ctypes.sizeof(spk.TraceConfig) == 20
-    assert ctypes.sizeof(spk.Schedule) == 16
+# ctypes mirror of each struct the header declares
+STRUCTS = {"spk_mesh_desc": "MeshDesc", "spk_instance_desc": "InstanceDesc", "spk_material_desc": "MaterialDesc",
+           "spk_sphere_desc": "SphereDesc", "spk_texture_desc": "TextureDesc", "spk_build_options": "BuildOptions",
+           "spk_scene_stats": "SceneStats", "spk_trace_config": "TraceConfig", "spk_camera_desc": "CameraDesc",
+           "spk_frame_desc": "FrameDesc", "spk_cache_config": "CacheConfig", "spk_frame_stats": "FrameStats",
+           "spk_schedule": "Schedule", "spk_batch_scene": "BatchScene"}
+
+
+def test_struct_sizes_match_header_layout(tmp_path):
+    """A C program compiled against the header prints every struct's size; the ctypes mirrors
+    must agree (a field added on one side only fails here)."""
+    import subprocess
+
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "spk_pt.h"\nint main(void) {\n' +
+                   "".join(f'    printf("%s %zu\\n", "{c}", sizeof({c}));\n' for c in STRUCTS) + "    return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    sizes = dict(zip(out[0::2], map(int, out[1::2])))
+    assert set(sizes) == set(STRUCTS)
+    for c, py in STRUCTS.items():
+        assert ctypes.sizeof(getattr(spk, py)) == sizes[c], (c, ctypes.sizeof(getattr(spk, py)), sizes[c])
+    assert sizes["spk_cache_config"] == 44
     assert spk.RAY_DTYPE.itemsize == 32 and spk.HIT_DTYPE.itemsize == 8
 
 
