@@ -373,24 +373,20 @@ __global__ void k_adam_begin(AdamStep *st, const uint32_t *n_dev, uint32_t cap, 
     }
 }
 
-__device__ __forceinline__ float grad_value(const float *g, uint64_t i) { return g[i]; }
-__device__ __forceinline__ float grad_value(const __bf16 *g, uint64_t i) { return (float)g[i]; }
-
-template <typename G>
-__global__ void k_adam(float *__restrict__ p, const G *__restrict__ grad, float *__restrict__ m,
+// Adam over the MLP weights (dense: every weight has a gradient every step)
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ grad, float *__restrict__ m,
                        float *__restrict__ v, uint64_t count, float lr, float b1, float b2, float eps,
-                       const AdamStep *__restrict__ st, __half *__restrict__ half_copy) {
+                       const AdamStep *__restrict__ st) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count || !st->active) return;
     const float c1 = st->c1, c2 = st->c2;
-    const float gi = grad_value(grad, i);
+    const float gi = grad[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
     const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
-    if (half_copy) half_copy[i] = __float2half(pi);
 }
 
 // Adam over the hash table, lazily (the rule of torch.optim.SparseAdam): an entry (two features,
@@ -567,9 +563,9 @@ hipError_t cache_adam(CacheState &cs, hipStream_t s, const uint32_t *n_dev, uint
     AdamStep *st = cache_state(cs);
     hipLaunchKernelGGL(k_adam_begin, dim3(1), dim3(1), 0, s, st, n_dev, cap, b1, b2);
     const uint64_t nw = cache_num_weights();
-    hipLaunchKernelGGL(k_adam<float>, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, (float *)cs.w.ptr,
+    hipLaunchKernelGGL(k_adam, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, (float *)cs.w.ptr,
                        (const float *)cs.gw.ptr, (float *)cs.mw.ptr, (float *)cs.vw.ptr, nw, lr, b1, b2, eps,
-                       (const AdamStep *)st, (__half *)nullptr);
+                       (const AdamStep *)st);
     const uint64_t entries = cs.table_params() / 2;
     hipLaunchKernelGGL(k_adam_table, dim3((uint32_t)((entries + 255) / 256)), dim3(256), 0, s,
                        (float2 *)cs.table_f.ptr, (uint32_t *)cs.gtable.ptr, (float2 *)cs.mt.ptr, (float2 *)cs.vt.ptr,

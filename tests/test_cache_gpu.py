@@ -125,6 +125,39 @@ def test_adam_first_step_is_sign_of_gradient(gpu_ctx):
     assert np.array_equal(t1[dtab == 0], t0[dtab == 0])
 
 
+def test_table_adam_is_lazy_and_consumes_its_gradient(gpu_ctx):
+    """The table's Adam skips entries (feature pairs) whose gradient is zero (torch.optim.SparseAdam's
+    rule): after a step on points A and a step on disjoint points B, an entry A touched and B did not
+    keeps its value, although its first moment is non-zero (plain Adam would move it again). The step also
+    clears the gradient it read: a gradient pass on points C afterwards is non-zero only where C's
+    own gradient is, with no trace of B's."""
+    print_case("cache lazy table Adam")
+    setup_cache(gpu_ctx, seed=5, lr=1e-3)
+    rng = np.random.Generator(np.random.PCG64(11))
+    n = 512
+    pa = rng.uniform(0.05, 0.45, (n, 3))
+    pb = rng.uniform(0.55, 0.95, (n, 3))
+    pc = rng.uniform(0.05, 0.95, (64, 3))
+    targets = to_dev(rng.random((n, 3)).reshape(-1))
+    _, _, ga = gpu_ctx.cache_gradients(to_dev(pa.reshape(-1)), targets, n, table=True)
+    _, _, gb = gpu_ctx.cache_gradients(to_dev(pb.reshape(-1)), targets, n, table=True)
+    _, _, gc_fresh = gpu_ctx.cache_gradients(to_dev(pc.reshape(-1)), targets, 64, table=True)
+    gpu_ctx.cache_train(to_dev(pa.reshape(-1)), targets, n)
+    _, t1 = gpu_ctx.cache_get_params()
+    gpu_ctx.cache_train(to_dev(pb.reshape(-1)), targets, n)
+    _, t2 = gpu_ctx.cache_get_params()
+    # an entry is its two features (one BF16 gradient pair); it is skipped when both are zero
+    ga2, gb2 = ga.reshape(-1, 2), gb.reshape(-1, 2)
+    only_a = (ga2 != 0).any(1) & (gb2 == 0).all(1)
+    assert only_a.sum() > 10000
+    np.testing.assert_array_equal(t2.reshape(-1, 2)[only_a], t1.reshape(-1, 2)[only_a])
+    moved = np.abs(gb) > 1e-5
+    assert (t2[moved] != t1[moved]).mean() > 0.99
+    _, _, gc = gpu_ctx.cache_gradients(to_dev(pc.reshape(-1)), targets, 64, table=True)
+    assert (gc != 0).sum() > 0
+    assert not ((gc != 0) & (gc_fresh == 0)).any()
+
+
 def test_online_training_fits_an_analytic_field(gpu_ctx):
     """Fitting f(x) = a smooth positive RGB field: the relative L2 loss falls tenfold and the
     trained network beats the untrained one on held-out points."""
