@@ -5,7 +5,7 @@
 // neural radiance caching 2021) and Adam (Kingma & Ba).
 //
 // Device layout:
-//   table      FP16 [levels][2^log2_table][2]   (+ FP32 master, Adam m/v, FP32 gradient)
+//   table      FP16 [levels][2^log2_table][2]   (+ FP32 master, Adam m/v, BF16-pair gradient)
 //   weights    FP32 master [out][in] per layer, concatenated (16,576 values for the default net)
 //              BF16 forward copy  Wf_l [out_pad16][in]  and backward copy Wb_l = W_l^T [in][out_pad32]
 // The MLP runs on CDNA4 matrix cores: v_mfma_f32_16x16x32_bf16 with neurons as rows (M), queries

@@ -8,11 +8,14 @@
 //   shadow  - occlusion rays of next-event estimation (opt-in, not in the reference)
 // With the learned radiance cache enabled, non-training paths stop at the first non-metal vertex
 // from the cache bounce on and take the cache's prediction of the reflected radiance; a fraction
-// of paths is traced in full and records, at each such vertex, the radiance gathered after that
-// vertex as a training target.
-// All kernels are persistent: the grid is sized to the device, waves fetch work from counters in
-// device memory, so a frame is captured once as a hipGraph and replayed; per-frame values (the
-// frame seed) travel through a small device buffer written before each replay.
+// of paths (training paths) run on and record, at each such vertex, the radiance gathered after
+// that vertex as a training target, until they too end in the cache a few hit indices later
+// (self-training; one in 16 is traced to its end).
+// The queue kernels are persistent (the grid is sized to the device and waves fetch work from
+// counters in device memory); camera rays, and every bounce under a schedule without lane refill,
+// run one ray per thread over a grid that covers every pixel. Work sizes live in device memory, so a
+// frame is captured once as a hipGraph and replayed; per-frame values (the frame seed) travel through
+// a small device buffer written before each replay.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
