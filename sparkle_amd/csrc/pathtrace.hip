@@ -57,6 +57,9 @@ enum : uint32_t {
     GCTR_TAIL_RAYS = 4,     // segments traced by the tail kernel
     GCTR_TAIL_SHADOWS = 5,  // shadow rays traced by the tail kernel
 };
+// feature sets a kernel variant is compiled for
+enum PtFeat : uint32_t { kFeatNee = 1u, kFeatCache = 2u, kFeatAll = 3u };
+
 enum : uint32_t {
     PF_NEE_PREV = 1u,  // the previous vertex sampled the emitters directly (MIS-weight emission hits)
     PF_TRAIN = 2u,          // a training path of the radiance cache
@@ -569,23 +572,50 @@ __device__ __forceinline__ F3 debug_after_loop(uint32_t m, uint32_t end_bounce, 
 struct ShadeOut {
     bool cont, shadow, cache_q, train;
     float4 sh0, sh1, sh2;
-    F3 loc, Tv, Cv;
-    float4 at0, at1;  // the cache attributes of a query or training vertex
 };
+
+// Cache query and training vertex records, written where shading decides on them (wave-aggregated over
+// the lanes taking that branch), so their values need no registers past that point.
+__device__ __forceinline__ void record_cache_query(const PtArgs &a, uint32_t path, const F3 loc, const F3 T,
+                                                   const float4 at0, const float4 at1) {
+    const uint32_t cs = wave_append(a.counters + kCtrGlobal + GCTR_CACHE_Q, true);
+    a.cq_points[3 * (size_t)cs] = loc.x;
+    a.cq_points[3 * (size_t)cs + 1] = loc.y;
+    a.cq_points[3 * (size_t)cs + 2] = loc.z;
+    a.cq_meta[cs] = make_float4(T.x, T.y, T.z, __uint_as_float(path));
+    a.cq_attrs[2 * (size_t)cs] = at0;
+    a.cq_attrs[2 * (size_t)cs + 1] = at1;
+}
+
+// C: the path's colour before this vertex adds anything (the target is what the path gathers after it)
+__device__ __forceinline__ void record_training(const PtArgs &a, uint32_t path, const F3 loc, const F3 C, const F3 T,
+                                                const float4 at0, const float4 at1) {
+    const uint32_t ts = wave_append(a.counters + kCtrGlobal + GCTR_TRAIN, true);
+    if (ts >= a.train_cap) return;
+    a.tr_points[3 * (size_t)ts] = loc.x;
+    a.tr_points[3 * (size_t)ts + 1] = loc.y;
+    a.tr_points[3 * (size_t)ts + 2] = loc.z;
+    a.tr_meta0[ts] = make_float4(C.x, C.y, C.z, __uint_as_float(path));
+    a.tr_meta1[ts] = make_float4(T.x, T.y, T.z, 0.f);
+    a.tr_attrs[2 * (size_t)ts] = at0;
+    a.tr_attrs[2 * (size_t)ts + 1] = at1;
+}
 
 // Shades path `path` at hit record h of segment `bounce`: adds emission/sky to its colour, samples the
 // next direction (rays/thr/rng/prev_pdf written when it continues) and prepares the NEE shadow ray.
 // The colour is written back before the shadow ray's contribution is added, so the sum order per
-// path is the same whichever kernel runs the segment.
-__device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, bool last, bool nee, uint32_t path,
+// path is the same whichever kernel runs the segment. FEAT (PtFeat bits) compiles in next-event
+// estimation and the cache; a frame that uses neither runs the same arithmetic with fewer registers.
+template <uint32_t FEAT = kFeatAll>
+__device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, bool last, bool nee_on, uint32_t path,
                                            const uint4 h, ShadeOut &o) {
+    const bool nee = (FEAT & kFeatNee) && nee_on;
+    const bool cache_on = (FEAT & kFeatCache) && a.cache_on;
     bool &cont = o.cont, &shadow = o.shadow, &cache_q = o.cache_q, &train = o.train;
     float4 &sh0 = o.sh0, &sh1 = o.sh1, &sh2 = o.sh2;
-    F3 &loc = o.loc, &Tv = o.Tv, &Cv = o.Cv;
+    F3 loc = f3(0.f, 0.f, 0.f);
     cont = shadow = cache_q = train = false;
     sh0 = sh1 = sh2 = make_float4(0.f, 0.f, 0.f, 0.f);
-    loc = Tv = Cv = f3(0.f, 0.f, 0.f);
-    o.at0 = o.at1 = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 rd4 = a.rays[2 * path + 1], ro4 = a.rays[2 * path];
     const F3 d = f3(rd4.x, rd4.y, rd4.z);
     const float4 T4 = a.thr[path];
@@ -649,10 +679,10 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
         // the cache is a function of position only, so it stands in for the path tail only at
         // vertices whose reflected radiance barely depends on direction (non-metal PBR); paths
         // continue through metal and glass vertices to the next such vertex
-        const bool cache_vertex = a.cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
+        const bool cache_vertex = cache_on && bounce >= a.cache_bounce && mat.type == 0u &&
                                   me.metallic < 0.5f;
         if (me.emits) {
-            if ((pf & PF_NEE_PREV) && !is_sphere) {  // spheres are not among the sampled emitters
+            if ((FEAT & kFeatNee) && (pf & PF_NEE_PREV) && !is_sphere) {  // spheres are not sampled emitters
                 // the previous vertex also sampled this emitter: balance-heuristic weight of
                 // the BSDF-sampled direction against light sampling of this point
                 const uint4 *tp = a.wtris + (size_t)h.x * 3u;
@@ -673,16 +703,13 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
             // training path past its suffix too: its targets then include the network's prediction
             // here, which carries light over more bounces than the suffix traces)
             cache_q = true;
-            Tv = T;
-            o.at0 = make_float4(normal.x, normal.y, normal.z, me.base.x);
-            o.at1 = make_float4(me.base.y, me.base.z, me.roughness, 1.f);
+            record_cache_query(a, path, loc, T, make_float4(normal.x, normal.y, normal.z, me.base.x),
+                               make_float4(me.base.y, me.base.z, me.roughness, 1.f));
         } else {
             if (cache_vertex) {  // a training path: record the vertex, its target follows
                 train = true;
-                Tv = T;
-                o.at0 = make_float4(normal.x, normal.y, normal.z, me.base.x);
-                o.at1 = make_float4(me.base.y, me.base.z, me.roughness, 1.f);
-                Cv = C;
+                record_training(a, path, loc, C, T, make_float4(normal.x, normal.y, normal.z, me.base.x),
+                                make_float4(me.base.y, me.base.z, me.roughness, 1.f));
             }
             uint4 rs = a.rng[path];
             Rng rng{rs.x, rs.y, rs.z, rs.w};
@@ -800,28 +827,6 @@ __device__ __forceinline__ void shade_path(const PtArgs &a, uint32_t bounce, boo
 }
 
 // cache query / training vertex records of a shaded segment (wave-aggregated over the active lanes)
-__device__ __forceinline__ void append_cache_records(const PtArgs &a, uint32_t path, const ShadeOut &o) {
-    const uint32_t cs = wave_append(a.counters + kCtrGlobal + GCTR_CACHE_Q, o.cache_q);
-    if (o.cache_q) {
-        a.cq_points[3 * (size_t)cs] = o.loc.x;
-        a.cq_points[3 * (size_t)cs + 1] = o.loc.y;
-        a.cq_points[3 * (size_t)cs + 2] = o.loc.z;
-        a.cq_meta[cs] = make_float4(o.Tv.x, o.Tv.y, o.Tv.z, __uint_as_float(path));
-        a.cq_attrs[2 * (size_t)cs] = o.at0;
-        a.cq_attrs[2 * (size_t)cs + 1] = o.at1;
-    }
-    const uint32_t ts = wave_append(a.counters + kCtrGlobal + GCTR_TRAIN, o.train);
-    if (o.train && ts < a.train_cap) {
-        a.tr_points[3 * (size_t)ts] = o.loc.x;
-        a.tr_points[3 * (size_t)ts + 1] = o.loc.y;
-        a.tr_points[3 * (size_t)ts + 2] = o.loc.z;
-        a.tr_meta0[ts] = make_float4(o.Cv.x, o.Cv.y, o.Cv.z, __uint_as_float(path));
-        a.tr_meta1[ts] = make_float4(o.Tv.x, o.Tv.y, o.Tv.z, 0.f);
-        a.tr_attrs[2 * (size_t)ts] = o.at0;
-        a.tr_attrs[2 * (size_t)ts + 1] = o.at1;
-    }
-}
-
 __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
     uint32_t *ctr = a.counters + bounce * kCtrStride;
     const uint32_t count = ctr[CTR_QUEUE];
@@ -847,7 +852,6 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtArgs a, uint32_t bounce) {
             a.shadow[3 * (size_t)ss + 1] = o.sh1;
             a.shadow[3 * (size_t)ss + 2] = o.sh2;
         }
-        if (a.cache_on) append_cache_records(a, path, o);
     }
 }
 
@@ -915,13 +919,17 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow_direct(PtArgs a, uint32_t b
 // takes the next queued path when it ends, so the frame waits for the slowest path instead. Every
 // segment runs the wavefront kernels' per-path code in the same order (closest hit, shading with
 // the colour written back, then the shadow contribution), so images do not depend on `from`.
-template <int WIDTH, int STACK, bool TWO>
+// One variant per feature set (FEAT, PtFeat bits) a frame uses, with the other branches compiled out:
+// the per-path state of NEE and of the cache's records costs registers, and the tail is bound by the
+// occupancy they leave (k_tail<8, 12>: 109 VGPRs and 4 waves/SIMD for the reference's estimator,
+// 138 and 3 with NEE and the cache).
+template <int WIDTH, int STACK, bool TWO, uint32_t FEAT>
 __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
     __shared__ uint2 stack[STACK * kPtBlock];
     uint32_t *ctr = a.counters + from * kCtrStride;
     const uint32_t count = ctr[CTR_QUEUE];
     const uint32_t *queue = (from & 1u) ? a.queue1 : a.queue0;
-    const bool nee = (a.flags & SPK_FRAME_NEE) && a.num_emitters > 0;
+    const bool nee = (FEAT & kFeatNee) && (a.flags & SPK_FRAME_NEE) && a.num_emitters > 0;
     const uint32_t lane = lane_id();
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
     uint32_t path = 0u, bounce = from, rays = 0u, shadows = 0u;
@@ -956,13 +964,12 @@ __global__ __launch_bounds__(kPtBlock) void k_tail(PtArgs a, uint32_t from) {
             }
             const uint4 h = hit_record<TWO>(a, r, tr.best);
             ShadeOut o;
-            shade_path(a, bounce, bounce + 1u >= a.max_bounce, nee, path, h, o);
+            shade_path<FEAT>(a, bounce, bounce + 1u >= a.max_bounce, nee, path, h, o);
             ++rays;
             if (o.shadow) {
                 ++shadows;
                 if (shadow_unoccluded<WIDTH, STACK, TWO>(a, o.sh0, o.sh1, stack + threadIdx.x)) add_color(a, path, o.sh2);
             }
-            if (a.cache_on) append_cache_records(a, path, o);
             if (o.cont) ++bounce; else active = false;
         }
     }
@@ -1130,10 +1137,12 @@ using ExtendFn = void (*)(PtArgs, uint32_t);
 struct PtEntry {
     int width, stack;
     bool two;
-    ExtendFn extend, shadow, tail, first, shadow_direct;
+    ExtendFn extend, shadow, first, shadow_direct;
+    ExtendFn tail[4];  // by feature set (PtFeat bits)
 };
-#define SPK_PT(W, S, T) \
-    {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_tail<W, S, T>, k_extend_first<W, S, T>, k_shadow_direct<W, S, T>}
+#define SPK_PT(W, S, T)                                                                                  \
+    {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_extend_first<W, S, T>, k_shadow_direct<W, S, T>, \
+     {k_tail<W, S, T, 0u>, k_tail<W, S, T, 1u>, k_tail<W, S, T, 2u>, k_tail<W, S, T, 3u>}}
 const PtEntry kPtEntries[] = {
     SPK_PT(8, 12, false), SPK_PT(4, 12, false), SPK_PT(8, 16, false), SPK_PT(8, 24, false),
     SPK_PT(8, 32, false), SPK_PT(8, 64, false), SPK_PT(4, 16, false), SPK_PT(4, 24, false),
@@ -1332,7 +1341,11 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
                 hipLaunchKernelGGL(pe.shadow, dim3(grid), dim3(kPtBlock), 0, s, a, b);
         }
     }
-    if (a.tail_from < a.max_bounce) hipLaunchKernelGGL(pe.tail, dim3(grid), dim3(kPtBlock), 0, s, a, a.tail_from);
+    if (a.tail_from < a.max_bounce) {
+        const uint32_t feat = (((a.flags & SPK_FRAME_NEE) && a.num_emitters > 0) ? kFeatNee : 0u) |
+                              (a.cache_on ? kFeatCache : 0u);
+        hipLaunchKernelGGL(pe.tail[feat], dim3(grid), dim3(kPtBlock), 0, s, a, a.tail_from);
+    }
     if (a.cache_on) {
         if ((e = hipStreamWaitEvent(s, ps.train_join, 0)) != hipSuccess) return e;
         e = cache_infer_queue(*cache, a.cq_points, (const float *)a.cq_attrs,
