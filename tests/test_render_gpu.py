@@ -174,21 +174,23 @@ def test_per_path_tail_matches_the_wavefront(gpu_ctx):
                     assert st.rays_traced == ref_st.rays_traced and st.shadow_rays == ref_st.shadow_rays
                     assert ref_st.rays_traced > w * h
                     assert flags == 0 or ref_st.shadow_rays > 0
-        # the cache's first frame after a reset: the same queries, so the same image
+        # the cache's first frame after a reset: the same queries, so the same image (with and
+        # without NEE: the tail kernel is compiled per feature set)
         gpu_ctx.cache_create(spk.cache_config(terminate_bounce=2))
-        f = frame(w, h, 5, bounce=8, flags=spk.FRAME_CACHE | spk.FRAME_NEE)
-        imgs = []
-        for tail in (0, 1, 3):
-            print_case("cache tail", tail)
-            gpu_ctx.set_schedule(8, 1, 16, tail)
-            gpu_ctx.render_frame(f, r.rgba)  # capture
-            gpu_ctx.cache_reset()
-            gpu_ctx.render_frame(f, r.rgba)
-            imgs.append(r.rgba.cpu().numpy().copy())
+        for cflags in (spk.FRAME_CACHE | spk.FRAME_NEE, spk.FRAME_CACHE):
+            f = frame(w, h, 5, bounce=8, flags=cflags)
+            imgs = []
+            for tail in (0, 1, 3):
+                print_case("cache tail", tail, "flags", cflags)
+                gpu_ctx.set_schedule(8, 1, 16, tail)
+                gpu_ctx.render_frame(f, r.rgba)  # capture
+                gpu_ctx.cache_reset()
+                gpu_ctx.render_frame(f, r.rgba)
+                imgs.append(r.rgba.cpu().numpy().copy())
+            for img in imgs[1:]:
+                np.testing.assert_array_equal(img, imgs[0])
+            assert np.isfinite(imgs[0]).all()
         gpu_ctx.cache_destroy()
-        for img in imgs[1:]:
-            np.testing.assert_array_equal(img, imgs[0])
-        assert np.isfinite(imgs[0]).all()
     finally:
         gpu_ctx.set_schedule(8, 1, 16)
     torch.cuda.synchronize()
