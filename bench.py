@@ -595,10 +595,11 @@ def c1_experiment(spk, scenes, stream, width=256, height=256, spp=16, max_bounce
             "relmse_gpu_vs_cpu": relmse(gimg, cimg[..., :3].astype(np.float64))}
 
 
-def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
+def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6, schedule=None):
     """Many scenes of uneven size in one launch: `count` many-instance scenes (triangle budgets
     log-uniform in [1k, 500k], images 64^2..512^2), `spp` frames of `max_bounce` bounces each, all
-    scenes' paths in the same persistent kernels; schedule picked by spk_batch_autotune."""
+    scenes' paths in the same persistent kernels; schedule picked by spk_batch_autotune, or the
+    given (blocks_per_cu, material_sort, refill, tail_bounce)."""
     import torch
 
     log(f"batch experiment: generating {count} scenes")
@@ -618,7 +619,12 @@ def batch_experiment(spk, scenes, stream, count=64, spp=32, max_bounce=6):
         f.debug_mode, f.sky_enabled, f.flags, f.output_limit = 0, 0, 0, 0.0
         return f
 
-    (blocks, srt, refill, tail), tune_ms = ctx.autotune_batch(frame(0), rgba)
+    if schedule is None:
+        (blocks, srt, refill, tail), tune_ms = ctx.autotune_batch(frame(0), rgba)
+    else:
+        (blocks, srt, refill, tail), tune_ms = schedule, 0.0
+        ctx.set_schedule(*schedule, batch=True)
+        ctx.render_batch(frame(10 ** 6), rgba)  # graph capture
     torch.cuda.synchronize()
     rays = 0
     t0 = time.perf_counter()
@@ -672,13 +678,16 @@ def main():
     ap.add_argument("--render-mode", choices=sorted(RENDER_MODES), default="plain",
                     help="estimator of --render-only")
     ap.add_argument("--render-schedule", type=str, default=None,
-                    help="blocks_per_cu,material_sort,refill,tail_bounce for --render-only (default: autotuned)")
+                    help="blocks_per_cu,material_sort,refill,tail_bounce for --render-only and --batch-only "
+                         "(default: autotuned)")
     ap.add_argument("--two-level", action="store_true",
                     help="only the flattened vs two-level comparison on the instanced scene (one JSON line)")
     ap.add_argument("--relmse-only", action="store_true",
                     help="only the equal-time relMSE experiment (one JSON line)")
     ap.add_argument("--cache-kernels", action="store_true",
                     help="only time the learned cache's inference and training kernels (one JSON line)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 64-scene batch measurement (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -733,6 +742,16 @@ def main():
         stream = torch.cuda.Stream()
         torch.cuda.set_stream(stream)
         print(json.dumps({"cache_kernels": cache_kernel_bench(spk, scenes, stream)}), flush=True)
+        return
+    if args.batch_only:
+        import torch
+        import sparkle_amd as spk
+        from sparkle_amd import scenes
+
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        fixed = tuple(int(x) for x in args.render_schedule.split(",")) if args.render_schedule else None
+        print(json.dumps({"batch": batch_experiment(spk, scenes, stream, schedule=fixed)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by image tiles or scenes, "
