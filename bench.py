@@ -317,12 +317,21 @@ def relmse_experiment(spk, scenes, stream, width, height, spp, ref_spp, max_boun
                 ctx.cache_destroy()
             results[name] = r
         e_plain, e_nee = results["plain"]["relmse"], results["plain_nee"]["relmse"]
+        # the reference's own noise adds about the same relMSE to every estimator; taking it off
+        # estimates each one's error against the converged image
+        noise = first["plain"]["relmse"] * spp / ref_spp
         by_budget.append({
             "seconds": round(budget, 3), "estimators": results,
             "ratio_plain_over": {k: round(e_plain / v["relmse"], 3) for k, v in results.items()
                                  if k != "plain" and v["relmse"]},
             "ratio_plain_nee_over": {k: round(e_nee / v["relmse"], 3) for k, v in results.items()
-                                     if k.startswith("cache") and v["relmse"]}})
+                                     if k.startswith("cache") and v["relmse"]},
+            "ratio_plain_over_less_reference_noise": {
+                k: round((e_plain - noise) / (v["relmse"] - noise), 3) for k, v in results.items()
+                if k != "plain" and v["relmse"] and v["relmse"] > noise},
+            "ratio_plain_nee_over_less_reference_noise": {
+                k: round((e_nee - noise) / (v["relmse"] - noise), 3) for k, v in results.items()
+                if k.startswith("cache") and v["relmse"] and v["relmse"] > noise}})
 
     out = {
         "scene": f"many-instance generator seed 0: {st.num_triangles} triangles, 3 area lights, {width}x{height}, "
