@@ -6,7 +6,9 @@ attributes (shading normal, albedo, roughness, 1), then zeros up to 64.
 
 Tolerances: the device computes the MLP on BF16 operands (8-bit mantissa) with FP32 accumulation,
 so outputs are compared to the FP64 oracle (fed the same FP16 table and BF16-rounded weights) within
-3% relative + 2e-3 absolute, and gradients by relative vector error below 5%.
+3% relative + 2e-3 absolute, and gradients by relative vector error: 5% over all weights, 8% per
+layer, 2% for the table gradient of 4 samples and 8% for that of 300 (where the BF16 accumulation
+of many contributions per coarse entry dominates).
 """
 import numpy as np
 import pytest
@@ -101,9 +103,32 @@ def test_gradients_match_fp64(gpu_ctx):
     for a, b in zip(offs[:-1], offs[1:]):
         r = np.linalg.norm(dw[a:b] - ref_dw[a:b]) / max(np.linalg.norm(ref_dw[a:b]), 1e-30)
         assert r < 0.08, (a, r)
+    # At 300 samples the coarse levels' entries sum hundreds of mixed-sign contributions in BF16
+    # (8-bit mantissa, by design: Adam normalises by the gradient's RMS), which dominates the table
+    # gradient's error (4.7% here; element-wise the median is 0.7%).
     ref_tab = refcpu.table_backward(cfg, pts, ref_dx)
     rel_t = np.linalg.norm(dtab - ref_tab) / np.linalg.norm(ref_tab)
-    assert rel_t < 0.05, rel_t
+    assert rel_t < 0.08, rel_t
+
+
+def test_table_gradient_of_few_samples_matches_fp64(gpu_ctx):
+    """With 4 samples almost every table entry takes one contribution, so the BF16 accumulation does
+    not enter: the table gradient pins d(loss)/d(features) of the backward pass, the trilinear
+    weights and the corner indices of the scatter (0.45% measured; every touched entry must be the
+    oracle's)."""
+    cfg, w, t = setup_cache(gpu_ctx, seed=1)
+    rng = np.random.Generator(np.random.PCG64(6))
+    n = 4
+    pts = rng.random((n, 3)).astype(np.float32)
+    targets = rng.random((n, 3)).astype(np.float32)
+    at = random_attrs(rng, n)
+    print_case("cache table gradient", n)
+    _, _, dtab = gpu_ctx.cache_gradients(to_dev(pts.reshape(-1)), to_dev(targets.reshape(-1)), n, table=True,
+                                         attrs_dev=to_dev(at.reshape(-1)))
+    _, _, _, ref_dx, _ = oracle_forward(cfg, w, t, pts, targets, at)
+    ref_tab = refcpu.table_backward(cfg, pts, ref_dx)
+    np.testing.assert_array_equal(dtab != 0, ref_tab != 0)
+    assert np.linalg.norm(dtab - ref_tab) / np.linalg.norm(ref_tab) < 0.02
 
 
 def test_adam_first_step_is_sign_of_gradient(gpu_ctx):
