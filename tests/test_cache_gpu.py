@@ -273,3 +273,30 @@ def test_cache_render_trains_online_and_stays_close_to_plain(gpu_ctx):
         gpu_ctx.cache_reset()
         assert gpu_ctx.cache_info()["steps"] == 0
         gpu_ctx.cache_destroy()
+
+
+def test_training_paths_end_in_the_cache_after_their_suffix(gpu_ctx):
+    """Self-training: a training path ends in the cache train_suffix hit indices after
+    terminate_bounce (one in 16 excepted), so a frame with a suffix traces fewer rays than one whose
+    training paths run to their end, and fewer with a short suffix than with a long one (the same
+    paths otherwise: same seeds)."""
+    import torch
+
+    w = h = 64
+    sc = scenes.cornell_scene()
+    gpu_ctx.upload_scene(sc)
+    gpu_ctx.set_camera(scenes.cornell_camera(w, h))
+    rgba = torch.zeros(w * h * 4, device="cuda")
+    f = spk.FrameDesc()
+    f.width, f.height, f.frame_seed, f.max_bounce, f.flags = w, h, 7, 8, spk.FRAME_CACHE
+    rays = {}
+    for suffix in (0, 1, 3):
+        print_case("cache train suffix", suffix)
+        # every path a training path, so the suffix decides how far most paths run
+        gpu_ctx.cache_create(spk.cache_config(log2_table=14, batch=1 << 14, seed=4, terminate_bounce=1,
+                                              train_fraction=1.0, train_suffix=suffix))
+        rays[suffix] = gpu_ctx.render_frame(f, rgba).rays_traced
+        assert np.isfinite(rgba.cpu().numpy()).all()
+        gpu_ctx.cache_destroy()
+    print("rays by suffix", rays)
+    assert rays[1] < rays[3] < rays[0]
