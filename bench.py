@@ -589,7 +589,7 @@ def c1_experiment(spk, scenes, stream, width=256, height=256, spp=16, max_bounce
     t_gpu = time.perf_counter() - t0
     gimg = acc.view(height, width, 4)[..., :3].double().cpu().numpy()
     ctx.close()
-    threads = min(os.cpu_count() or 1, 16)
+    threads = min(len(os.sched_getaffinity(0)), HOST_CORE_SHARE)
     o = refcpu.OracleScene(sc)
     cimg = np.zeros((height, width, 4), np.float32)
     t0 = time.perf_counter()
@@ -871,11 +871,13 @@ def main():
         coherent_mrays = round(rays_a.shape[0] * 10 / (ea.elapsed_time(eb) / 1e3) / 1e6, 1)
 
     # the same rays handed over in pageable host memory (spk_trace_closest_host: 256 MiB of rays in
-    # over PCIe, the traversal, 64 MiB of hits out); reported beside the metric, never as it
+    # over PCIe through the pinned staging ring, the traversal, 64 MiB of hits out into a hit array
+    # the caller reuses); reported beside the metric, never as it. The first call sets the ring up.
     host_ms = []
-    for _ in range(3):
+    hits_h = np.zeros(n, dtype=spk.HIT_DTYPE)
+    for _ in range(5):
         t_h = time.perf_counter()
-        rc_h, hits_h = ctx.trace_host(rays)
+        rc_h, hits_h = ctx.trace_host(rays, hits_h)
         host_ms.append((time.perf_counter() - t_h) * 1e3)
         if rc_h != spk.SPK_OK:
             raise RuntimeError(f"host-buffer trace failed {rc_h}")

@@ -258,7 +258,9 @@ int spk_trace_closest(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk
 /* Same, enqueued on the context stream without waiting; invalid-ray counts are reported by the
    next blocking call. */
 int spk_trace_closest_async(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, spk_hit *hits_dev);
-/* Host-array convenience wrapper (copies in and out). */
+/* Same with rays and hits in host memory (pageable is fine): the batch goes through pinned staging
+   buffers in chunks, with host copies, uploads, traversal and downloads overlapped. Blocks until
+   done. The hits equal those of spk_trace_closest over the same rays. */
 int spk_trace_closest_host(spk_context *ctx, const spk_ray *rays, uint64_t n, spk_hit *hits);
 /* No counterpart in the reference. For callers that cannot allocate device memory during a call
    (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same

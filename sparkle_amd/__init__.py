@@ -468,10 +468,15 @@ class Context:
             n = rays_dev.numel() * rays_dev.element_size() // 32
         return self.lib.spk_trace_closest_noalloc(self.h, _ptr(rays_dev), n, _ptr(hits_dev))
 
-    def trace_host(self, rays):
-        """rays: numpy RAY_DTYPE array. Returns (rc, hits numpy HIT_DTYPE array)."""
+    def trace_host(self, rays, hits=None):
+        """rays: numpy RAY_DTYPE array; hits: optional numpy HIT_DTYPE array of the same length to
+        write into (a caller that reuses it saves the page faults of a fresh array). Returns (rc,
+        hits)."""
         rays = np.ascontiguousarray(rays)
-        hits = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        if hits is None:
+            hits = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        elif hits.dtype != HIT_DTYPE or hits.shape != (rays.shape[0],) or not hits.flags.c_contiguous:
+            raise ValueError("hits must be a contiguous HIT_DTYPE array with one record per ray")
         rc = self.lib.spk_trace_closest_host(self.h, rays.ctypes.data, rays.shape[0], hits.ctypes.data)
         return rc, hits
 

@@ -1105,19 +1105,7 @@ int spk_trace_closest_host(spk_context *ctx, const spk_ray *rays, uint64_t n, sp
     if (!ctx) return SPK_ERR_INVALID_ARGUMENT;
     if (n == 0) return SPK_OK;
     if (!rays || !hits) return set_error(ctx, SPK_ERR_INVALID_ARGUMENT, "null ray or hit array");
-    hipError_t e;
-    if ((e = ctx->tmp_rays.ensure(n * sizeof(spk_ray))) != hipSuccess ||
-        (e = ctx->tmp_hits.ensure(n * sizeof(spk_hit))) != hipSuccess)
-        return hip_fail(ctx, e, "ray buffers");
-    if ((e = hipMemcpyAsync(ctx->tmp_rays.ptr, rays, n * sizeof(spk_ray), hipMemcpyHostToDevice, ctx->stream)) !=
-        hipSuccess)
-        return hip_fail(ctx, e, "ray upload");
-    int rc = spk_trace_closest_async(ctx, (const spk_ray *)ctx->tmp_rays.ptr, n, (spk_hit *)ctx->tmp_hits.ptr);
-    if (rc != SPK_OK) return rc;
-    if ((e = hipMemcpyAsync(hits, ctx->tmp_hits.ptr, n * sizeof(spk_hit), hipMemcpyDeviceToHost, ctx->stream)) !=
-        hipSuccess)
-        return hip_fail(ctx, e, "hit download");
-    return spk_synchronize(ctx);
+    return trace_closest_host_staged(ctx, rays, n, hits);
 }
 
 int spk_trace_count(spk_context *ctx, const spk_ray *rays_dev, uint64_t n, uint32_t *nodes_dev, uint32_t *tris_dev) {

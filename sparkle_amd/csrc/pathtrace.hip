@@ -1391,6 +1391,8 @@ void spk_context::release_all() {
     cache = nullptr;
     delete batch;
     batch = nullptr;
+    spk::destroy_host_staging(staging);
+    staging = nullptr;
     scene.nodes.release();
     scene.tris.release();
     scene.shade.release();

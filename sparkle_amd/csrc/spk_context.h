@@ -76,6 +76,11 @@ struct DeviceScene {
 struct PathState;   // path-tracer buffers, defined with the path-tracer kernels
 struct CacheState;  // learned radiance cache (spk_cache.h)
 struct BatchState;  // batch of independent scenes, defined with the path-tracer kernels
+struct HostStaging; // pinned ring buffers and copy streams of host-buffer closest-hit calls
+
+void destroy_host_staging(HostStaging *hs);
+// spk_trace_closest_host over the pinned staging pipeline (n > 0, arrays checked by the caller).
+int trace_closest_host_staged(spk_context *ctx, const spk_ray *rays, uint64_t n, spk_hit *hits);
 
 // Validates and flattens one scene, builds its BVH, converts shading attributes (input order) and
 // materials. Returns an SPK_* code and a message in err.
@@ -112,6 +117,7 @@ struct spk_context {
     spk::PathState *paths = nullptr;
     spk::CacheState *cache = nullptr;
     spk::BatchState *batch = nullptr;
+    spk::HostStaging *staging = nullptr;
     float cache_last_loss = 0.f;
 
     uint32_t *invalid_counter() { return static_cast<uint32_t *>(counters.ptr); }

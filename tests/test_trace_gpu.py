@@ -349,23 +349,31 @@ def test_kernel_timing_records_each_call(gpu_ctx):
 
 
 def test_host_buffer_trace_matches_device_buffers(gpu_ctx):
-    """spk_trace_closest_host (rays and hits in host memory, copied in and out) gives the device-buffer
-    call's hits bit for bit, in input and coherence order, reports invalid rays the same way, and
-    treats zero rays as a no-op."""
+    """spk_trace_closest_host (rays and hits in host memory, staged through a ring of three pinned
+    buffers in chunks of 524,288 rays at these sizes) gives the device-buffer call's hits bit for bit,
+    in input and coherence order, reports invalid rays the same way (one in the last, ragged chunk),
+    and treats zero rays as a no-op. 2,100,003 rays make five chunks, so ring slots are reused; 600,001
+    make two; 70,001 one partial chunk. A caller's hit array is written in place."""
     sc = scenes.heightfield_scene(cells=128)
     lo, hi = scenes.scene_aabb(sc)
-    rays = scenes.incoherent_rays(lo, hi, n=70_001, seed=41)
-    rays["dir"][12] = 0.0
+    rays = scenes.incoherent_rays(lo, hi, n=2_100_003, seed=41)
+    rays["dir"][2_099_000] = 0.0
     gpu_ctx.upload_scene(sc)
     w, stack, block = gpu_ctx.trace_config()[:3]
     for ray_sort in (0, 1):
-        print_case("host buffers", ray_sort)
         gpu_ctx.set_trace_config(w, stack, block, 0, ray_sort)
-        rc_d, h_d = gpu_trace(gpu_ctx, rays)
-        rc_h, h_h = gpu_ctx.trace_host(rays)
-        assert rc_d == rc_h == spk.SPK_ERR_INVALID_RAY
-        np.testing.assert_array_equal(h_h["prim"], h_d["prim"])
-        np.testing.assert_array_equal(h_h["t"].view(np.uint32), h_d["t"].view(np.uint32))
+        for n in (70_001, 2_100_003, 600_001):  # the first call sets the ring up, the second grows it
+            print_case("host buffers", ray_sort, n)
+            sub = rays[:n]
+            rc_d, h_d = gpu_trace(gpu_ctx, sub)
+            out = np.full(n, 0xFF, dtype=np.uint8).repeat(8).view(spk.HIT_DTYPE) if n == 600_001 else None
+            rc_h, h_h = gpu_ctx.trace_host(sub, out)
+            if out is not None:
+                assert h_h is out
+            bad = spk.SPK_ERR_INVALID_RAY if n > 2_099_000 else spk.SPK_OK
+            assert rc_d == rc_h == bad, (rc_d, rc_h)
+            np.testing.assert_array_equal(h_h["prim"], h_d["prim"])
+            np.testing.assert_array_equal(h_h["t"].view(np.uint32), h_d["t"].view(np.uint32))
     rc, h = gpu_ctx.trace_host(rays[:0])
     assert rc == spk.SPK_OK and h.shape[0] == 0
     gpu_ctx.set_trace_config(w, stack, block, 0, 0)
