@@ -674,6 +674,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-coherent", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-buffer (spk_trace_closest_host) measurement")
     ap.add_argument("--config", type=str, default=None,
                     help="traversal variant width,stack,block[,refill[,ray_sort]]")
     ap.add_argument("--leaf", type=int, default=3, help="triangles per BVH leaf slot (1..3)")
@@ -873,17 +875,21 @@ def main():
     # the same rays handed over in pageable host memory (spk_trace_closest_host: 256 MiB of rays in
     # over PCIe through the pinned staging ring, the traversal, 64 MiB of hits out into a hit array
     # the caller reuses); reported beside the metric, never as it. The first call sets the ring up.
+    # (--no-host skips it: its chunked traversal calls would mix with the full-size launches in a
+    # profiler's per-kernel averages)
     host_ms = []
-    hits_h = np.zeros(n, dtype=spk.HIT_DTYPE)
-    for _ in range(5):
-        t_h = time.perf_counter()
-        rc_h, hits_h = ctx.trace_host(rays, hits_h)
-        host_ms.append((time.perf_counter() - t_h) * 1e3)
-        if rc_h != spk.SPK_OK:
-            raise RuntimeError(f"host-buffer trace failed {rc_h}")
-    host_mrays = round(n / (float(np.median(host_ms)) / 1e3) / 1e6, 1)
-    if not np.array_equal(hits_h.view(np.uint8).reshape(-1), d_hits.cpu().numpy()):
-        raise RuntimeError("host-buffer hits differ from the device-buffer hits")
+    host_mrays = None
+    if not args.no_host:
+        hits_h = np.zeros(n, dtype=spk.HIT_DTYPE)
+        for _ in range(5):
+            t_h = time.perf_counter()
+            rc_h, hits_h = ctx.trace_host(rays, hits_h)
+            host_ms.append((time.perf_counter() - t_h) * 1e3)
+            if rc_h != spk.SPK_OK:
+                raise RuntimeError(f"host-buffer trace failed {rc_h}")
+        host_mrays = round(n / (float(np.median(host_ms)) / 1e3) / 1e6, 1)
+        if not np.array_equal(hits_h.view(np.uint8).reshape(-1), d_hits.cpu().numpy()):
+            raise RuntimeError("host-buffer hits differ from the device-buffer hits")
 
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     cur = ctx.trace_config()[:4] if not ctx.trace_config()[4] else ctx.trace_config()
@@ -912,7 +918,7 @@ def main():
             "hit_fraction": round(hit_frac, 4),
             "coherent_1080p_mrays": coherent_mrays,
             "host_buffers_mrays": host_mrays,
-            "host_buffers_ms": round(float(np.median(host_ms)), 3),
+            "host_buffers_ms": round(float(np.median(host_ms)), 3) if host_ms else None,
             "mean_nodes_per_ray": round(mean_nodes, 3),
             "mean_tris_per_ray": round(mean_tris, 3),
             "wall_s": round(wall, 4),
