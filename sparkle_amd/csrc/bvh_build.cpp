@@ -329,11 +329,13 @@ std::string build_wide_bvh(const BuildInput &in, BuildOutput &out) {
         }
     };
 
-    // SAH weights of the collapse: 1 per wide node visited (its eight child boxes), 1 per triangle
-    // tested. Equal weights measured best on MI355X against 0.3 and 0.5 per triangle; against a greedy
-    // collapse (open the largest child until W) the 1080p frame of the 5.5M-triangle many-instance
-    // scene went from 2.82 to 2.70-2.76 ms, with 20% fewer nodes.
-    const Collapse collapse(bn, W, L, 1.0f, 1.0f);
+    // SAH weights of the collapse: 1 per wide node visited (its eight child boxes), 3 per triangle
+    // tested (fetch, watertight test and, in the path tracer, the divergence of lanes in leaves).
+    // Measured on MI355X, 1080p frames of the 5.5M-triangle many-instance scene: greedy collapse
+    // (open the largest child until W slots) 2.82 ms; triangle weight 0.3, 0.5 and 1 down to
+    // 2.70-2.75 ms; 2 2.67-2.69; 3 2.66; 4 2.64-2.66 (with cache + NEE: 1 3.42-3.43, 3 3.32-3.37 ms).
+    // Incoherent closest hit on the 2M-triangle heightfield stays at 1.88-1.89 ms for every weight.
+    const Collapse collapse(bn, W, L, 1.0f, 3.0f);
     const double root_area = std::max(bn[0].box.area(), 1e-300);
     struct Pending {
         uint32_t bin;
