@@ -1310,7 +1310,7 @@ static int ensure_paths(spk_context *ctx, uint32_t npix) {
 // beside the bounce-0 traversal, and joins it before the frame's cache queries, which then see the
 // updated network, as they would with the step run between the frames.
 static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t s, int grid, CacheState *cache,
-                                int cus, const PathState &ps) {
+                                const PathState &ps) {
     hipError_t e;
     auto fork_training = [&]() -> hipError_t {
         if (!a.cache_on) return hipSuccess;
@@ -1356,9 +1356,12 @@ static hipError_t enqueue_frame(const PtArgs &a, const PtEntry &pe, hipStream_t 
     }
     if (a.cache_on) {
         if ((e = hipStreamWaitEvent(s, ps.train_join, 0)) != hipSuccess) return e;
+        // one 64-query tile per block over the largest possible count (one query per path), blocks past
+        // the frame's count exit at once: the gather-bound kernel gets every wave the chip holds (a
+        // persistent grid of 8 one-wave blocks per CU, 2 waves/SIMD, made cache frames 0.6-2.4% longer)
         e = cache_infer_queue(*cache, a.cq_points, (const float *)a.cq_attrs,
                                          a.counters + kCtrGlobal + GCTR_CACHE_Q, a.npix, a.cq_out,
-                                         cus * 8, s);
+                                         (int)((a.npix + 63u) / 64u), s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(kPtBlock), 0, s, a);
         hipLaunchKernelGGL(k_cache_targets, dim3(grid), dim3(kPtBlock), 0, s, a);
@@ -1566,7 +1569,7 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
         ps.drop_graph();
         if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return hip_fail(ctx, e, "begin capture");
-        hipError_t le = enqueue_frame(a, *pe, ctx->stream, grid, ctx->cache, cus, ps);
+        hipError_t le = enqueue_frame(a, *pe, ctx->stream, grid, ctx->cache, ps);
         e = hipStreamEndCapture(ctx->stream, &ps.graph_def);
         if (le != hipSuccess) return hip_fail(ctx, le, "frame launch during capture");
         if (e != hipSuccess) return hip_fail(ctx, e, "end capture");
