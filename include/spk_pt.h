@@ -377,7 +377,7 @@ typedef struct spk_schedule {
     uint32_t refill;        /* extend after the camera rays (which run one path per thread): a wave
                                takes new paths once this many of its 64 lanes are idle, 1..64
                                (default 16); 0: one path (and one NEE shadow ray) per thread over a
-                               full grid, like the camera rays (the per-path tail then refills at 16) */
+                               full grid, like the camera rays (the per-path tail then refills at 8) */
     uint32_t tail_bounce;   /* segments from this index on run one path per lane to its end (closest
                                hit, shading and shadow ray in one persistent kernel) instead of one
                                bounce-synchronous launch per segment; 0 = none (default), <= 64 */
