@@ -1534,7 +1534,8 @@ static int run_frame(spk_context *ctx, const spk_frame_desc *frame, const FrameS
     // refill 0: direct extends; the per-path tail then refills at 8 idle lanes (1080p frames of the
     // many-instance scene against 16: plain 2.59-2.65 ms against 2.63-2.69, cache + NEE 3.32-3.33
     // against 3.35-3.36, the 64-scene batch 1419 against 1414 M samples/s; cache from bounce 1
-    // 2.45-2.47 against 2.44-2.45; 4, 6 and 12 within noise of 8, 32 and 48 slower)
+    // 2.45-2.47 against 2.44-2.45; 4, 6 and 12 within noise of 8, 32 and 48 slower; measured with
+    // 128 queue slots per atomic, with 16 the thresholds 4, 8 and 16 are within noise of each other)
     a.refill = src.refill ? std::min(src.refill, 64u) : 8u;
     a.direct = src.refill == 0u ? 1u : 0u;
     a.npix = npix;
